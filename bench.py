@@ -1,0 +1,141 @@
+#!/usr/bin/env python3
+"""Closure throughput on one MI355X.
+
+Workload: the in-core pointer-alias closure -- 1,000,000 vertices from the
+seeded alias generator (graspan_g.gen.alias, seed 0: procedures of U[20,200]
+vertices, Poisson(1.5) assign edges, 10% deref edges, call sites to Zipf(1.1)
+callees inside modules of 8 procedures, reverse A/D edges; ~3.2M input edges)
+under the alias grammar (L ::= eps | L A | L M; R ::= eps | R a | R M;
+V ::= L R; T ::= D V; M ::= T d), filter 0.
+
+One step = one full fixpoint of that graph with its input already resident in
+HBM. value = new closure edges per second = (|closure| - |dedup input|) / t,
+summed over the timed steps.
+
+roofline: the join/dedup/difference kernel (k_expand), timed live with HIP
+events on the engine's stream. Its algorithmic bytes per round are
+4*P + 4*O + 4*N (P probe reads: key elements scanned plus every candidate-list
+element read; O old/delta row elements read for the difference; N new edges
+written), from the engine's counters, which the parity tests hold equal to the
+oracle's.
+
+cpu_baseline: the C++/OpenMP oracle on a bounded sample of the same workload
+(the same generator at 100,000 vertices, run to its fixpoint), new closure
+edges per second on the host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "graspan-g_amd"))
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD_N = 1_000_000
+SAMPLE_N = 100_000
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per k_expand launch measured by rocprofv3 PMC counters, if a
+    summary for this workload is committed."""
+    p = os.path.join(ROOT, "profiles", "expand_alias1m_traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # checker / baseline only
+    from graspan_g import gen
+
+    g = gen.alias(SAMPLE_N, seed=0)
+    threads = os.cpu_count() or 1
+    r = oracle.closure_of(g, want_triples=False, threads=threads)
+    new = r.total - r.input_unique
+    return {"value": new / r.seconds, "unit": "closure edges/s", "cores": threads, "kind": "port",
+            "sample": f"alias generator n={SAMPLE_N} seed 0 run to its fixpoint ({r.total} closure edges, "
+                      f"{r.seconds:.2f} s, OpenMP threads={threads})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=WORKLOAD_N, help="vertex count (default: the 1M-vertex workload)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import graspan_g as gg
+    from graspan_g import gen
+
+    t = time.time()
+    g = gen.alias(args.n, seed=0)
+    log(f"generated alias n={g.n} m={g.m} in {time.time() - t:.1f}s")
+    ctx = gg.Context()
+    ctx.set_spec(g)
+    for i in range(args.warmup):
+        st = ctx.run(0)
+        log(f"warmup {i}: {st.fixpoint_ms:.1f} ms, closure {st.total_edges}, rounds {st.rounds}")
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = ctx.run(0)
+        stats.append(st.as_dict())
+        log(f"step {i}: {st.fixpoint_ms:.1f} ms (expand {st.expand_ms:.1f}, merge {st.merge_ms:.1f}, "
+            f"compact {st.compact_ms:.1f}, other {st.other_ms:.1f}) closure {st.total_edges}")
+    wall = time.perf_counter() - t0
+    new_edges = sum(s["total_edges"] - s["input_unique"] for s in stats)
+    value = new_edges / wall
+    s0 = stats[-1]
+    exp_ms = sum(s["expand_ms"] for s in stats)
+    launches = sum(s["expand_launches"] for s in stats)
+    alg = sum(4 * s["probe_reads"] + 4 * s["old_reads"] + 4 * s["new_edges"] for s in stats)
+    achieved = alg / (exp_ms / 1e3) / 1e9 if exp_ms > 0 else 0.0
+    traffic = traffic_from_profiles()
+    line = {
+        "metric": "closure edges/sec",
+        "value": value,
+        "unit": "closure edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"alias closure, n={g.n}, {g.m} input edges, alias grammar, filter 0",
+                   "closure_edges": s0["total_edges"], "input_unique": s0["input_unique"],
+                   "rounds": s0["rounds"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_expand (join+dedup+difference)",
+                     "alg_bytes_per_launch": alg / max(1, launches),
+                     "avg_launch_ms": exp_ms / max(1, launches)},
+        "breakdown_ms": {k: sum(s[k] for s in stats) / len(stats)
+                         for k in ("fixpoint_ms", "expand_ms", "compact_ms", "merge_ms", "other_ms")},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle on the n=100000 sample)")
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

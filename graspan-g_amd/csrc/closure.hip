@@ -1,0 +1,1340 @@
+// Device engine of the CFL-reachability closure for MI355X (gfx950, CDNA4).
+//
+// Replaces the reference's device code and its launch drivers
+// (src/kernel.cu:85-985: compute00/compute10/compute11 with
+// apply/decode/unionAll/union2, merge/diff/mergeAndDiff, computeDegree, and
+// spagpu_s/spagpu_b/spagpu) with a device-resident semi-naive fixpoint over
+// sorted uint32 CSR rows.
+//
+// Data layout in HBM (one context = one graph):
+//   row id r = l * n + u for label l (0-based) and vertex u; L*n < 2^32.
+//   old, delta : rowptr uint64[L*n + 1] + col uint32[], rows sorted/unique.
+//   staging    : uint32[] bump-allocated output of the expand kernel.
+//   seg_off/seg_cnt : per-row (offset, count) of that output.
+//   nonempty   : one bit per row, set when the delta row is not empty.
+//
+// One round (rule semantics of src/kernel.cu:403-464; end-of-step update of
+// src/kernel.cu:943-985):
+//   cand(A,u) = U_{A::=B} delta_B(u)
+//             U_{A::=B C} ( U_{v in delta_B(u)} old_C(v) U delta_C(v) )
+//             U           ( U_{v in old_B(u)}   delta_C(v) )
+//   delta'(A,u) = cand(A,u) - old_A(u) - delta_A(u);  old' = old U delta.
+//   Kernels: k_activate (which rows can receive candidates) -> k_expand (join
+//   + dedup + difference, fused, output sorted) -> one host read of the round
+//   counters -> k_scan_* + k_gather (compact delta') -> k_merge (old U delta).
+//
+// k_expand is the hot kernel. One wave64 owns one (A,u) row: it seeds an LDS
+// bitmap over the row's candidate window with old_A(u) U delta_A(u), streams
+// every candidate list through it with LDS atomicOr (a candidate whose bit
+// was clear is new and also sets a bit in a second "new" bitmap), then scans
+// the new bitmap to emit the new edges already in ascending order.
+// Candidates never go to HBM. Lists of >= 64 elements are read by the whole
+// wave (coalesced); shorter ones are packed across lanes with a wave prefix
+// sum and an LDS binary search. Windows wider than kChunkBits are processed
+// in chunks (count pass, then write pass).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+#include "hash.h"
+
+namespace gg {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr uint32_t kChunkBits = 32768;  // LDS window per wave: 4 KiB seen + 4 KiB new
+constexpr uint32_t kChunkWords = kChunkBits / 32;
+constexpr uint32_t kLongList = 64;      // lists this long are read by the whole wave
+constexpr int kMaxRules = 64;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+struct DevGrammar {
+  int nu = 0, nb = 0, ne = 0;
+  uint8_t u_lhs[kMaxRules], u_rhs[kMaxRules];
+  uint8_t b_lhs[kMaxRules], b_rhs1[kMaxRules], b_rhs2[kMaxRules];
+  uint8_t e_lhs[kMaxRules];
+};
+
+// Round counters, one device copy per context (zeroed by a fill kernel).
+struct Counters {
+  unsigned long long cursor;   // staging bump pointer (= new edges emitted)
+  unsigned long long cand;     // K: candidates streamed
+  unsigned long long pkeys;    // key elements scanned (binary rules)
+  unsigned long long oreads;   // O: old/delta row elements read for the difference
+  unsigned long long newe;     // N: new edges
+  unsigned long long checksum; // closure checksum accumulator
+  unsigned int active;         // rows in the active list
+  unsigned int work;           // work-stealing index
+  unsigned int overflow;       // staging too small
+  unsigned int label_flags[8]; // bit l: delta of label l is not empty
+  unsigned int pad;
+};
+
+struct DCsr {
+  const unsigned long long* ptr;
+  const uint32_t* col;
+};
+
+// ---------------------------------------------------------------------------
+// wave helpers
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t x, int lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint32_t y = __shfl_up(x, d, kWave);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+__device__ inline unsigned long long wave_sum_u64(unsigned long long x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+  return x;
+}
+
+__device__ inline uint32_t wave_min_u32(uint32_t x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x = min(x, (uint32_t)__shfl_xor(x, d, kWave));
+  return x;
+}
+
+__device__ inline uint32_t wave_max_u32(uint32_t x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x = max(x, (uint32_t)__shfl_xor(x, d, kWave));
+  return x;
+}
+
+__device__ inline bool label_nonempty(const Counters* c, uint32_t l) { return (c->label_flags[l >> 5] >> (l & 31)) & 1u; }
+
+// ---------------------------------------------------------------------------
+// small utility kernels
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t count) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, uint64_t count) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// out = a + b elementwise (rowptr of old U delta = rowptr(old) + rowptr(delta)).
+__global__ void k_add_ptr(unsigned long long* out, const unsigned long long* a, const unsigned long long* b, uint64_t count) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+    out[i] = a[i] + b[i];
+}
+
+// Exclusive scan of uint32 counts into uint64 offsets, out[count] = total.
+// Three phases: per-tile sums, scan of tile sums (one block), per-tile scan.
+__device__ inline unsigned long long block_excl_scan_u64(unsigned long long x, unsigned long long* s_tmp,
+                                                         unsigned long long& total) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  unsigned long long incl = x;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    unsigned long long y = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += y;
+  }
+  if (lane == kWave - 1) s_tmp[w] = incl;
+  __syncthreads();
+  unsigned long long off = 0, tot = 0;
+  for (int i = 0; i < kWavesPerBlock; ++i) {
+    if (i < w) off += s_tmp[i];
+    tot += s_tmp[i];
+  }
+  __syncthreads();
+  total = tot;
+  return off + incl - x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* in, uint64_t count, unsigned long long* tile_sums) {
+  __shared__ unsigned long long s_tmp[kWavesPerBlock];
+  const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+  unsigned long long s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < count) s += in[base + i];
+  unsigned long long tot;
+  block_excl_scan_u64(s, s_tmp, tot);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_sums(unsigned long long* tile_sums, uint64_t ntiles, unsigned long long* out_total) {
+  __shared__ unsigned long long s_tmp[kWavesPerBlock];
+  unsigned long long carry = 0;
+  for (uint64_t b = 0; b < ntiles; b += kBlock) {
+    uint64_t i = b + threadIdx.x;
+    unsigned long long x = i < ntiles ? tile_sums[i] : 0;
+    unsigned long long tot;
+    unsigned long long ex = block_excl_scan_u64(x, s_tmp, tot);
+    if (i < ntiles) tile_sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* in, uint64_t count, const unsigned long long* tile_off,
+                                                      unsigned long long* out) {
+  __shared__ unsigned long long s_tmp[kWavesPerBlock];
+  const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+  uint32_t v[kScanItems];
+  unsigned long long s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = base + i < count ? in[base + i] : 0;
+    s += v[i];
+  }
+  unsigned long long tot;
+  unsigned long long ex = block_excl_scan_u64(s, s_tmp, tot) + tile_off[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < count) out[base + i] = ex;
+    ex += v[i];
+  }
+}
+
+// Bit r of `bits` = delta row r is not empty.
+__global__ void k_nonempty(const unsigned long long* ptr, uint64_t rows, uint32_t* bits) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (uint64_t base = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) & ~uint64_t(kWave - 1); base < rows;
+       base += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t r = base + lane;
+    const bool ne = r < rows && ptr[r + 1] > ptr[r];
+    const unsigned long long m = __ballot(ne);
+    if (lane == 0) {
+      bits[base / 32] = uint32_t(m);
+      bits[base / 32 + 1] = uint32_t(m >> 32);
+    }
+  }
+}
+
+// label_flags bit l = the delta of label l is not empty.
+__global__ void k_label_flags(const unsigned long long* ptr, uint32_t n, uint32_t L, Counters* ctr) {
+  for (uint32_t l = threadIdx.x; l < L; l += blockDim.x)
+    if (ptr[uint64_t(l + 1) * n] > ptr[uint64_t(l) * n]) atomicOr(&ctr->label_flags[l >> 5], 1u << (l & 31));
+}
+
+__device__ inline bool row_nonempty(const uint32_t* bits, uint32_t r) { return (bits[r >> 5] >> (r & 31)) & 1u; }
+
+// ---------------------------------------------------------------------------
+// initial delta: raw rows from input COO + eps loops
+
+__global__ void k_count_raw(const uint32_t* src, const uint8_t* lab, uint64_t m, uint32_t n, const DevGrammar g,
+                            uint32_t* cnt) {
+  const uint64_t tid = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = tid; i < m; i += stride) atomicAdd(&cnt[uint32_t(lab[i] - 1) * n + src[i]], 1u);
+  const uint64_t neps = uint64_t(g.ne) * n;
+  for (uint64_t i = tid; i < neps; i += stride) {
+    uint32_t e = uint32_t(i / n), v = uint32_t(i % n);
+    atomicAdd(&cnt[uint32_t(g.e_lhs[e]) * n + v], 1u);
+  }
+}
+
+__global__ void k_scatter_raw(const uint32_t* src, const uint32_t* dst, const uint8_t* lab, uint64_t m, uint32_t n,
+                              const DevGrammar g, const unsigned long long* ptr, uint32_t* fill, uint32_t* col) {
+  const uint64_t tid = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = tid; i < m; i += stride) {
+    uint32_t r = uint32_t(lab[i] - 1) * n + src[i];
+    col[ptr[r] + atomicAdd(&fill[r], 1u)] = dst[i];
+  }
+  const uint64_t neps = uint64_t(g.ne) * n;
+  for (uint64_t i = tid; i < neps; i += stride) {
+    uint32_t e = uint32_t(i / n), v = uint32_t(i % n);
+    uint32_t r = uint32_t(g.e_lhs[e]) * n + v;
+    col[ptr[r] + atomicAdd(&fill[r], 1u)] = v;
+  }
+}
+
+// Active list of the initial dedup: rows with raw entries.
+__global__ void k_activate_raw(const unsigned long long* ptr, uint32_t rows, uint32_t* active, Counters* ctr) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < rows;
+       base += gridDim.x * blockDim.x) {
+    uint32_t r = base + lane;
+    bool act = r < rows && ptr[r + 1] > ptr[r];
+    unsigned long long m = __ballot(act);
+    if (!m) continue;
+    uint32_t pos = 0;
+    if (lane == 0) pos = atomicAdd(&ctr->active, uint32_t(__popcll(m)));
+    pos = __shfl(pos, 0, kWave);
+    if (act) active[pos + __popcll(m & ((1ull << lane) - 1))] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// activation: which (A,u) rows can receive candidates this round, plus the
+// key-scan part of the probe counter (same definition as the oracle).
+
+struct ActArgs {
+  DCsr old, dl;
+  uint32_t n;
+  uint32_t nlhs;
+  uint8_t lhs[256];
+  DevGrammar g;
+  uint32_t* active;
+  Counters* ctr;
+};
+
+__global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t total = uint64_t(a.nlhs) * a.n;
+  unsigned long long pk = 0;
+  for (uint64_t base = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) & ~uint64_t(kWave - 1); base < total;
+       base += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t idx = base + lane;
+    bool act = false;
+    uint32_t r = 0;
+    if (idx < total) {
+      const uint32_t A = a.lhs[idx / a.n];
+      const uint32_t u = uint32_t(idx % a.n);
+      r = A * a.n + u;
+      bool any = false;
+      for (int i = 0; i < a.g.nu; ++i)
+        if (a.g.u_lhs[i] == A) any |= label_nonempty(a.ctr, a.g.u_rhs[i]);
+      for (int i = 0; i < a.g.nb; ++i)
+        if (a.g.b_lhs[i] == A) any |= label_nonempty(a.ctr, a.g.b_rhs1[i]) || label_nonempty(a.ctr, a.g.b_rhs2[i]);
+      if (any) {
+        for (int i = 0; i < a.g.nu; ++i) {
+          if (a.g.u_lhs[i] != A) continue;
+          const uint32_t rb = uint32_t(a.g.u_rhs[i]) * a.n + u;
+          act |= a.dl.ptr[rb + 1] > a.dl.ptr[rb];
+        }
+        for (int i = 0; i < a.g.nb; ++i) {
+          if (a.g.b_lhs[i] != A) continue;
+          const uint32_t rb = uint32_t(a.g.b_rhs1[i]) * a.n + u;
+          const unsigned long long kl = a.dl.ptr[rb + 1] - a.dl.ptr[rb];
+          pk += kl;
+          act |= kl > 0;
+          if (label_nonempty(a.ctr, a.g.b_rhs2[i])) {
+            const unsigned long long ol = a.old.ptr[rb + 1] - a.old.ptr[rb];
+            pk += ol;
+            act |= ol > 0;
+          }
+        }
+      }
+    }
+    unsigned long long m = __ballot(act);
+    if (m) {
+      uint32_t pos = 0;
+      if (lane == 0) pos = atomicAdd(&a.ctr->active, uint32_t(__popcll(m)));
+      pos = __shfl(pos, 0, kWave);
+      if (act) a.active[pos + __popcll(m & ((1ull << lane) - 1))] = r;
+    }
+  }
+  pk = wave_sum_u64(pk);
+  if (lane == 0 && pk) atomicAdd(&a.ctr->pkeys, pk);
+}
+
+// ---------------------------------------------------------------------------
+// expand: fused join + dedup + difference, one wave per active row
+
+struct ExpandArgs {
+  DCsr old, dl, raw;
+  const uint32_t* nonempty;
+  uint32_t n;
+  DevGrammar g;
+  const uint32_t* active;
+  Counters* ctr;
+  uint32_t* staging;
+  unsigned long long staging_cap;
+  unsigned long long* seg_off;
+  uint32_t* seg_cnt;
+};
+
+struct WaveLds {
+  uint32_t seen[kChunkWords];
+  uint32_t fresh[kChunkWords];
+  uint32_t pref[kWave];
+  unsigned long long base[kWave];
+};
+
+// Calls elem(x) for every element of the lists held by the lanes (lane i
+// holds col[lb .. lb+len)). Long lists: whole wave per list. Short lists:
+// packed across lanes by a wave prefix sum and an LDS binary search.
+template <class F>
+__device__ inline void expand_lists(const uint32_t* __restrict__ col, unsigned long long lb, uint32_t len, WaveLds& w,
+                                    int lane, F&& elem) {
+  unsigned long long longm = __ballot(len >= kLongList);
+  while (longm) {
+    const int j = __builtin_ctzll(longm);
+    longm &= longm - 1;
+    const unsigned long long b = __shfl(lb, j, kWave);
+    const uint32_t l = __shfl(len, j, kWave);
+    for (uint32_t t = lane; t < l; t += kWave) elem(col[b + t]);
+  }
+  const uint32_t slen = len >= kLongList ? 0u : len;
+  const uint32_t incl = wave_incl_scan(slen, lane);
+  const uint32_t total = __shfl(incl, kWave - 1, kWave);
+  if (total == 0) return;
+  wave_sync();
+  w.pref[lane] = incl;
+  w.base[lane] = lb;
+  wave_sync();
+  for (uint32_t t = lane; t < total; t += kWave) {
+    int k = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+      if (w.pref[k + step - 1] <= t) k += step;
+    const uint32_t prev = k ? w.pref[k - 1] : 0u;
+    elem(col[w.base[k] + (t - prev)]);
+  }
+  wave_sync();
+}
+
+// Visits every candidate list of row (A,u): calls visit(col, lb, len) once
+// per batch with one list per lane (len 0 = no list on that lane).
+template <bool INIT, class V>
+__device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A, uint32_t u, int lane, V&& visit) {
+  const uint32_t n = a.n;
+  if (INIT) {
+    const unsigned long long b = a.raw.ptr[r], e = a.raw.ptr[r + 1];
+    visit(a.raw.col, lane == 0 ? b : 0ull, lane == 0 ? uint32_t(e - b) : 0u);
+    return;
+  }
+  for (int i = 0; i < a.g.nu; ++i) {
+    if (a.g.u_lhs[i] != A) continue;
+    const uint32_t rb = uint32_t(a.g.u_rhs[i]) * n + u;
+    const unsigned long long b = a.dl.ptr[rb], e = a.dl.ptr[rb + 1];
+    visit(a.dl.col, lane == 0 ? b : 0ull, lane == 0 ? uint32_t(e - b) : 0u);
+  }
+  for (int i = 0; i < a.g.nb; ++i) {
+    if (a.g.b_lhs[i] != A) continue;
+    const uint32_t B = a.g.b_rhs1[i], C = a.g.b_rhs2[i];
+    // delta_B(u) x (old_C U delta_C)
+    {
+      const uint32_t rb = B * n + u;
+      const unsigned long long kb = a.dl.ptr[rb], ke = a.dl.ptr[rb + 1];
+      for (unsigned long long k0 = kb; k0 < ke; k0 += kWave) {
+        const unsigned long long k = k0 + lane;
+        unsigned long long ob = 0, db = 0;
+        uint32_t ol = 0, dlen = 0;
+        if (k < ke) {
+          const uint32_t rc = C * n + a.dl.col[k];
+          ob = a.old.ptr[rc];
+          ol = uint32_t(a.old.ptr[rc + 1] - ob);
+          db = a.dl.ptr[rc];
+          dlen = uint32_t(a.dl.ptr[rc + 1] - db);
+        }
+        visit(a.old.col, ob, ol);
+        visit(a.dl.col, db, dlen);
+      }
+    }
+    // old_B(u) x delta_C
+    if (label_nonempty(a.ctr, C)) {
+      const uint32_t rb = B * n + u;
+      const unsigned long long kb = a.old.ptr[rb], ke = a.old.ptr[rb + 1];
+      for (unsigned long long k0 = kb; k0 < ke; k0 += kWave) {
+        const unsigned long long k = k0 + lane;
+        unsigned long long db = 0;
+        uint32_t dlen = 0;
+        if (k < ke) {
+          const uint32_t rc = C * n + a.old.col[k];
+          if (row_nonempty(a.nonempty, rc)) {
+            db = a.dl.ptr[rc];
+            dlen = uint32_t(a.dl.ptr[rc + 1] - db);
+          }
+        }
+        visit(a.dl.col, db, dlen);
+      }
+    }
+  }
+}
+
+// Clears the window [c0, c0+span), seeds it with old_A(u) U delta_A(u) and
+// streams every candidate through it. Returns the number of new edges (set
+// bits of w.fresh) and stores the candidate count in kcount.
+template <bool INIT>
+__device__ __attribute__((always_inline)) inline uint32_t process_chunk(const ExpandArgs& a, uint32_t r, uint32_t A,
+                                                                        uint32_t u, uint32_t c0, uint32_t span,
+                                                                        WaveLds& w, int lane, unsigned long long& kcount) {
+  const uint32_t words = (span + 31) >> 5;
+  for (uint32_t i = lane; i < words; i += kWave) {
+    w.seen[i] = 0;
+    w.fresh[i] = 0;
+  }
+  wave_sync();
+  if (!INIT) {
+    auto seed = [&](uint32_t x) {
+      const uint32_t off = x - c0;
+      if (off < span) atomicOr(&w.seen[off >> 5], 1u << (off & 31));
+    };
+    const unsigned long long ob = a.old.ptr[r], oe = a.old.ptr[r + 1];
+    for (unsigned long long i = ob + lane; i < oe; i += kWave) seed(a.old.col[i]);
+    const unsigned long long db = a.dl.ptr[r], de = a.dl.ptr[r + 1];
+    for (unsigned long long i = db + lane; i < de; i += kWave) seed(a.dl.col[i]);
+    wave_sync();
+  }
+  volatile uint32_t* vseen = w.seen;
+  auto mark = [&](uint32_t x) {
+    const uint32_t off = x - c0;
+    if (off < span) {
+      const uint32_t wi = off >> 5, m = 1u << (off & 31);
+      if (!(vseen[wi] & m)) {
+        const uint32_t prev = atomicOr(&w.seen[wi], m);
+        if (!(prev & m)) atomicOr(&w.fresh[wi], m);
+      }
+    }
+  };
+  unsigned long long kc = 0;
+  for_each_list<INIT>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
+    kc += len;
+    expand_lists(col, lb, len, w, lane, mark);
+  });
+  kcount = wave_sum_u64(kc);
+  wave_sync();
+  uint32_t c = 0;
+  for (uint32_t i = lane; i < words; i += kWave) c += __popc(w.fresh[i]);
+  return uint32_t(wave_sum_u64(c));
+}
+
+// Writes the set bits of w.fresh (window base c0) in ascending order at
+// staging[pos ..].
+__device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t span, unsigned long long pos, WaveLds& w,
+                                  int lane) {
+  const uint32_t words = (span + 31) >> 5;
+  for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
+    const uint32_t wi = w0 + lane;
+    uint32_t bits = wi < words ? w.fresh[wi] : 0u;
+    const uint32_t pc = __popc(bits);
+    const uint32_t incl = wave_incl_scan(pc, lane);
+    unsigned long long p = pos + (incl - pc);
+    while (bits) {
+      const int b = __ffs(bits) - 1;
+      bits &= bits - 1;
+      a.staging[p++] = c0 + wi * 32 + uint32_t(b);
+    }
+    pos += __shfl(incl, kWave - 1, kWave);
+  }
+}
+
+template <bool INIT>
+__global__ __launch_bounds__(kBlock) void k_expand(const ExpandArgs a) {
+  __shared__ WaveLds s_lds[kWavesPerBlock];
+  const int lane = threadIdx.x & (kWave - 1);
+  WaveLds& w = s_lds[threadIdx.x / kWave];
+  const uint32_t nactive = a.ctr->active;
+  unsigned long long kc = 0, orr = 0, ne = 0;
+  for (;;) {
+    uint32_t idx = 0;
+    if (lane == 0) idx = atomicAdd(&a.ctr->work, 1u);
+    idx = __shfl(idx, 0, kWave);
+    if (idx >= nactive) break;
+    const uint32_t r = a.active[idx];
+    const uint32_t A = r / a.n, u = r - A * a.n;
+    // candidate window [lo, hi]
+    uint32_t lo = 0, hi = a.n - 1;
+    if (a.n > kChunkBits) {
+      uint32_t mn = 0xFFFFFFFFu, mx = 0;
+      unsigned long long kk = 0;
+      for_each_list<INIT>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
+        if (INIT) {
+          // raw rows are unsorted: reduce over every element
+          const unsigned long long b = __shfl(lb, 0, kWave);
+          const uint32_t l = __shfl(len, 0, kWave);
+          for (uint32_t t = lane; t < l; t += kWave) {
+            const uint32_t x = col[b + t];
+            mn = min(mn, x);
+            mx = max(mx, x);
+          }
+          kk += len;
+        } else if (len) {
+          mn = min(mn, col[lb]);
+          mx = max(mx, col[lb + len - 1]);
+          kk += len;
+        }
+      });
+      kk = wave_sum_u64(kk);
+      if (kk == 0) continue;
+      lo = wave_min_u32(mn);
+      hi = wave_max_u32(mx);
+    }
+    const uint32_t nchunks = (hi - lo) / kChunkBits + 1;
+    unsigned long long krow = 0;
+    uint32_t cnt = 0;
+    unsigned long long pos = 0;
+    if (nchunks == 1) {
+      cnt = process_chunk<INIT>(a, r, A, u, lo, hi - lo + 1, w, lane, krow);
+      if (cnt) {
+        if (lane == 0) pos = atomicAdd(&a.ctr->cursor, (unsigned long long)cnt);
+        pos = __shfl(pos, 0, kWave);
+        if (pos + cnt <= a.staging_cap)
+          emit_chunk(a, lo, hi - lo + 1, pos, w, lane);
+        else if (lane == 0)
+          atomicExch(&a.ctr->overflow, 1u);
+      }
+    } else {
+      for (uint32_t ch = 0; ch < nchunks; ++ch) {
+        const uint32_t c0 = lo + ch * kChunkBits;
+        const uint32_t span = min(kChunkBits, hi - c0 + 1);
+        unsigned long long kch = 0;
+        cnt += process_chunk<INIT>(a, r, A, u, c0, span, w, lane, kch);
+        if (ch == 0) krow = kch;
+      }
+      if (cnt) {
+        if (lane == 0) pos = atomicAdd(&a.ctr->cursor, (unsigned long long)cnt);
+        pos = __shfl(pos, 0, kWave);
+        if (pos + cnt <= a.staging_cap) {
+          unsigned long long p = pos;
+          for (uint32_t ch = 0; ch < nchunks; ++ch) {
+            const uint32_t c0 = lo + ch * kChunkBits;
+            const uint32_t span = min(kChunkBits, hi - c0 + 1);
+            unsigned long long kch = 0;
+            const uint32_t c = process_chunk<INIT>(a, r, A, u, c0, span, w, lane, kch);
+            emit_chunk(a, c0, span, p, w, lane);
+            p += c;
+          }
+        } else if (lane == 0) {
+          atomicExch(&a.ctr->overflow, 1u);
+        }
+      }
+    }
+    if (lane == 0) {
+      a.seg_off[r] = pos;
+      a.seg_cnt[r] = cnt;
+      kc += krow;
+      ne += cnt;
+      if (!INIT && krow) orr += (a.old.ptr[r + 1] - a.old.ptr[r]) + (a.dl.ptr[r + 1] - a.dl.ptr[r]);
+    }
+  }
+  if (lane == 0) {
+    if (kc) atomicAdd(&a.ctr->cand, kc);
+    if (orr) atomicAdd(&a.ctr->oreads, orr);
+    if (ne) atomicAdd(&a.ctr->newe, ne);
+  }
+}
+
+// Copies each active row's staged output into the delta' CSR.
+__global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* active, const Counters* ctr, const uint32_t* seg_cnt,
+                                                   const unsigned long long* seg_off, const uint32_t* staging,
+                                                   const unsigned long long* out_ptr, uint32_t* out_col) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t nactive = ctr->active;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  for (uint32_t i = wave; i < nactive; i += nwaves) {
+    const uint32_t r = active[i];
+    const uint32_t c = seg_cnt[r];
+    if (!c) continue;
+    const unsigned long long s = seg_off[r], d = out_ptr[r];
+    for (uint32_t t = lane; t < c; t += kWave) out_col[d + t] = staging[s + t];
+  }
+}
+
+// Number of elements of sorted row [b, e) smaller than x.
+__device__ inline uint32_t lower_rank(const uint32_t* col, unsigned long long b, unsigned long long e, uint32_t x) {
+  unsigned long long lo = b, hi = e;
+  while (lo < hi) {
+    const unsigned long long mid = (lo + hi) >> 1;
+    if (col[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return uint32_t(lo - b);
+}
+
+// old' = old U delta, row by row; rowptr(old') = rowptr(old) + rowptr(delta).
+__global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t rows, uint32_t* out_col) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  for (uint32_t r = wave; r < rows; r += nwaves) {
+    const unsigned long long ob = old.ptr[r], oe = old.ptr[r + 1];
+    const unsigned long long db = dl.ptr[r], de = dl.ptr[r + 1];
+    uint32_t* out = out_col + ob + db;
+    const uint32_t ol = uint32_t(oe - ob), dln = uint32_t(de - db);
+    if (dln == 0) {
+      for (uint32_t t = lane; t < ol; t += kWave) out[t] = old.col[ob + t];
+    } else if (ol == 0) {
+      for (uint32_t t = lane; t < dln; t += kWave) out[t] = dl.col[db + t];
+    } else {
+      for (uint32_t t = lane; t < ol; t += kWave) {
+        const uint32_t x = old.col[ob + t];
+        out[t + lower_rank(dl.col, db, de, x)] = x;
+      }
+      for (uint32_t t = lane; t < dln; t += kWave) {
+        const uint32_t x = dl.col[db + t];
+        out[t + lower_rank(old.col, ob, oe, x)] = x;
+      }
+    }
+  }
+}
+
+// Sum of triple_hash over the closure.
+__global__ __launch_bounds__(kBlock) void k_checksum(DCsr c, uint32_t rows, uint32_t n, Counters* ctr) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  unsigned long long s = 0;
+  for (uint32_t r = wave; r < rows; r += nwaves) {
+    const uint32_t l = r / n, u = r - l * n;
+    const unsigned long long b = c.ptr[r], e = c.ptr[r + 1];
+    for (unsigned long long i = b + lane; i < e; i += kWave) s += triple_hash(u, c.col[i], l + 1);
+  }
+  s = wave_sum_u64(s);
+  if (lane == 0 && s) atomicAdd(&ctr->checksum, s);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+#define GG_HIP(expr)                                                                           \
+  do {                                                                                         \
+    hipError_t e__ = (expr);                                                                   \
+    if (e__ != hipSuccess) {                                                                   \
+      err = std::string(#expr) + ": " + hipGetErrorString(e__);                                \
+      return GG_E_DEVICE;                                                                      \
+    }                                                                                          \
+  } while (0)
+
+#define GG_TRY(expr)        \
+  do {                      \
+    int s__ = (expr);       \
+    if (s__ != GG_OK) return s__; \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+class Engine {
+ public:
+  int device = 0;
+  int cus = 256;
+  uint64_t budget = 0;
+  int verbose = 0;
+  hipStream_t stream = nullptr;
+  size_t used = 0, peak = 0;
+
+  DevGrammar g{};
+  int L = 0;
+  std::vector<uint8_t> lhs;  // labels that are the lhs of a unary/binary rule
+  bool has_grammar = false;
+
+  uint32_t n = 0;
+  uint64_t m = 0;
+  DevBuf in_src, in_dst, in_lab;
+  bool has_graph = false;
+
+  // closure state
+  DevBuf old_ptr, old_col, dl_ptr, dl_col, nx_ptr, nx_col, o2_ptr, o2_col;
+  DevBuf raw_ptr, raw_col, staging, seg_off, seg_cnt, active, nonempty, tiles, ctr;
+  uint64_t old_nnz = 0, dl_nnz = 0;
+  bool has_closure = false;
+  Counters* h_ctr = nullptr;  // pinned
+
+  ~Engine() {
+    DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col, &nx_ptr, &nx_col, &o2_ptr,
+                     &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active, &nonempty, &tiles, &ctr};
+    for (DevBuf* b : all)
+      if (b->p) (void)hipFree(b->p);
+    if (h_ctr) (void)hipHostFree(h_ctr);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int alloc(DevBuf& b, size_t bytes, std::string& err) {
+    if (b.p && b.bytes >= bytes) return GG_OK;
+    if (b.p) {
+      (void)hipFree(b.p);
+      used -= b.bytes;
+      b.p = nullptr;
+      b.bytes = 0;
+    }
+    bytes = std::max<size_t>(bytes, 256);
+    if (budget && used + bytes > budget) {
+      err = "device budget exceeded: need " + std::to_string(used + bytes) + " bytes, budget " + std::to_string(budget);
+      return GG_E_NOMEM;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      b.p = nullptr;
+      err = "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e);
+      return GG_E_NOMEM;
+    }
+    b.bytes = bytes;
+    used += bytes;
+    peak = std::max(peak, used);
+    return GG_OK;
+  }
+
+  // Grows geometrically (x1.5) so per-round reallocations stay rare.
+  int grow(DevBuf& b, size_t bytes, std::string& err) {
+    if (b.p && b.bytes >= bytes) return GG_OK;
+    size_t want = std::max(bytes, b.bytes + b.bytes / 2);
+    int s = alloc(b, want, err);
+    if (s == GG_E_NOMEM && want > bytes) s = alloc(b, bytes, err);
+    return s;
+  }
+
+  void release(DevBuf& b) {
+    if (b.p) {
+      (void)hipFree(b.p);
+      used -= b.bytes;
+      b.p = nullptr;
+      b.bytes = 0;
+    }
+  }
+
+  uint32_t grid_for(uint64_t work, uint32_t per_block = kBlock) const {
+    uint64_t g = (work + per_block - 1) / per_block;
+    uint64_t cap = uint64_t(cus) * 16;
+    return uint32_t(std::max<uint64_t>(1, std::min(g, cap)));
+  }
+
+  template <class T>
+  T* P(DevBuf& b) {
+    return static_cast<T*>(b.p);
+  }
+  Counters* C() { return static_cast<Counters*>(ctr.p); }
+
+  void fill_u32(DevBuf& b, uint32_t v, uint64_t count) {
+    k_fill_u32<<<grid_for(count), kBlock, 0, stream>>>(P<uint32_t>(b), v, count);
+  }
+  void fill_u64(DevBuf& b, unsigned long long v, uint64_t count) {
+    k_fill_u64<<<grid_for(count), kBlock, 0, stream>>>(P<unsigned long long>(b), v, count);
+  }
+  void zero_ctr() {
+    k_fill_u32<<<1, kBlock, 0, stream>>>(P<uint32_t>(ctr), 0u, sizeof(Counters) / 4);
+  }
+
+  // counts (rows entries) -> out_ptr (rows+1 entries, last = total)
+  int scan(DevBuf& counts, uint64_t rows, DevBuf& out_ptr, std::string& err) {
+    const uint64_t ntiles = (rows + kScanTile - 1) / kScanTile;
+    GG_TRY(grow(tiles, (ntiles + 1) * 8, err));
+    k_scan_tiles<<<uint32_t(ntiles), kBlock, 0, stream>>>(P<uint32_t>(counts), rows, P<unsigned long long>(tiles));
+    k_scan_sums<<<1, kBlock, 0, stream>>>(P<unsigned long long>(tiles), ntiles, P<unsigned long long>(out_ptr) + rows);
+    k_scan_final<<<uint32_t(ntiles), kBlock, 0, stream>>>(P<uint32_t>(counts), rows, P<unsigned long long>(tiles),
+                                                         P<unsigned long long>(out_ptr));
+    GG_HIP(hipGetLastError());
+    return GG_OK;
+  }
+
+  int read_ctr(std::string& err) {
+    GG_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, stream));
+    GG_HIP(hipStreamSynchronize(stream));
+    return GG_OK;
+  }
+
+  ExpandArgs expand_args(bool init) {
+    ExpandArgs a{};
+    a.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
+    a.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
+    a.raw = {init ? P<unsigned long long>(raw_ptr) : nullptr, init ? P<uint32_t>(raw_col) : nullptr};
+    a.nonempty = P<uint32_t>(nonempty);
+    a.n = n;
+    a.g = g;
+    a.active = P<uint32_t>(active);
+    a.ctr = C();
+    a.staging = P<uint32_t>(staging);
+    a.staging_cap = staging.bytes / 4;
+    a.seg_off = P<unsigned long long>(seg_off);
+    a.seg_cnt = P<uint32_t>(seg_cnt);
+    return a;
+  }
+
+  uint32_t expand_grid() const { return uint32_t(cus) * 4; }
+  uint32_t rows() const { return uint32_t(uint64_t(L) * n); }
+
+  int run(int filter, gg_stats* st, std::string& err);
+};
+
+namespace {
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+}  // namespace
+
+int Engine::run(int filter, gg_stats* st, std::string& err) {
+  if (!has_grammar || !has_graph) {
+    err = "grammar and graph must be loaded before gg_closure_run";
+    return GG_E_STATE;
+  }
+  if (filter != 0) {
+    err = "filter_flag != 0 (the reference's dataflow-only schedule) is not supported by this build";
+    return GG_E_UNSUPPORTED;
+  }
+  if (uint64_t(L) * n >= 0xFFFFFFFFull) {
+    err = "num_labels * num_vertices must be < 2^32";
+    return GG_E_UNSUPPORTED;
+  }
+  GG_HIP(hipSetDevice(device));
+  gg_stats S{};
+  const uint32_t R = rows();
+  const uint64_t words = (uint64_t(R) + 63) / 64 * 2;
+  std::vector<EventPair> ev_expand, ev_compact, ev_merge, ev_other;
+  auto mk = [&](std::vector<EventPair>& v) -> EventPair& {
+    v.emplace_back();
+    (void)hipEventCreate(&v.back().a);
+    (void)hipEventCreate(&v.back().b);
+    return v.back();
+  };
+  struct EvGuard {
+    std::vector<EventPair>* vs[4];
+    ~EvGuard() {
+      for (auto* v : vs)
+        for (auto& e : *v) {
+          if (e.a) (void)hipEventDestroy(e.a);
+          if (e.b) (void)hipEventDestroy(e.b);
+        }
+    }
+  } guard{{&ev_expand, &ev_compact, &ev_merge, &ev_other}};
+
+  GG_TRY(alloc(ctr, sizeof(Counters), err));
+  GG_TRY(alloc(old_ptr, (uint64_t(R) + 1) * 8, err));
+  GG_TRY(alloc(dl_ptr, (uint64_t(R) + 1) * 8, err));
+  GG_TRY(alloc(nx_ptr, (uint64_t(R) + 1) * 8, err));
+  GG_TRY(alloc(o2_ptr, (uint64_t(R) + 1) * 8, err));
+  GG_TRY(alloc(seg_off, uint64_t(R) * 8, err));
+  GG_TRY(alloc(seg_cnt, uint64_t(R) * 4, err));
+  GG_TRY(alloc(active, uint64_t(R) * 4, err));
+  GG_TRY(alloc(nonempty, words * 4, err));
+  if (!h_ctr) GG_HIP(hipHostMalloc((void**)&h_ctr, sizeof(Counters), hipHostMallocDefault));
+
+  const auto t0 = std::chrono::steady_clock::now();
+
+  // ---- initial delta: dedup(input U eps loops) ---------------------------
+  {
+    EventPair& e = mk(ev_other);
+    GG_HIP(hipEventRecord(e.a, stream));
+    GG_TRY(alloc(raw_ptr, (uint64_t(R) + 1) * 8, err));
+    fill_u32(seg_cnt, 0, R);
+    k_count_raw<<<grid_for(std::max<uint64_t>(m, uint64_t(g.ne) * n)), kBlock, 0, stream>>>(
+        P<uint32_t>(in_src), P<uint8_t>(in_lab), m, n, g, P<uint32_t>(seg_cnt));
+    GG_TRY(scan(seg_cnt, R, raw_ptr, err));
+    const uint64_t raw_n = m + uint64_t(g.ne) * n;
+    GG_TRY(alloc(raw_col, raw_n * 4, err));
+    fill_u32(seg_cnt, 0, R);
+    k_scatter_raw<<<grid_for(std::max<uint64_t>(m, uint64_t(g.ne) * n)), kBlock, 0, stream>>>(
+        P<uint32_t>(in_src), P<uint32_t>(in_dst), P<uint8_t>(in_lab), m, n, g, P<unsigned long long>(raw_ptr),
+        P<uint32_t>(seg_cnt), P<uint32_t>(raw_col));
+    fill_u64(old_ptr, 0, uint64_t(R) + 1);
+    fill_u64(dl_ptr, 0, uint64_t(R) + 1);
+    // GG_STAGING_MIN_WORDS sets a smaller first staging area (tests use it to
+    // exercise the grow-and-retry path).
+    uint64_t stage_words = std::max<uint64_t>(raw_n, 1 << 20);
+    if (const char* sw = std::getenv("GG_STAGING_MIN_WORDS")) stage_words = std::max<uint64_t>(1, std::strtoull(sw, nullptr, 10));
+    release(staging);
+    GG_TRY(alloc(staging, stage_words * 4, err));
+    GG_HIP(hipEventRecord(e.b, stream));
+    for (;;) {
+      EventPair& x = mk(ev_expand);
+      zero_ctr();
+      fill_u32(seg_cnt, 0, R);
+      k_activate_raw<<<grid_for(R), kBlock, 0, stream>>>(P<unsigned long long>(raw_ptr), R, P<uint32_t>(active), C());
+      GG_HIP(hipEventRecord(x.a, stream));
+      k_expand<true><<<expand_grid(), kBlock, 0, stream>>>(expand_args(true));
+      GG_HIP(hipEventRecord(x.b, stream));
+      GG_HIP(hipGetLastError());
+      GG_TRY(read_ctr(err));
+      S.expand_launches++;
+      if (!h_ctr->overflow) break;
+      S.retries++;
+      GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
+    }
+    const uint64_t N0 = h_ctr->newe;
+    EventPair& c = mk(ev_compact);
+    GG_HIP(hipEventRecord(c.a, stream));
+    GG_TRY(scan(seg_cnt, R, dl_ptr, err));
+    GG_TRY(grow(dl_col, std::max<uint64_t>(N0, 1) * 4, err));
+    k_gather<<<expand_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
+                                                   P<unsigned long long>(seg_off), P<uint32_t>(staging),
+                                                   P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col));
+    GG_HIP(hipEventRecord(c.b, stream));
+    release(raw_col);
+    release(raw_ptr);
+    dl_nnz = N0;
+    old_nnz = 0;
+    S.input_unique = N0;
+  }
+
+  // ---- semi-naive rounds -------------------------------------------------
+  ActArgs act{};
+  act.n = n;
+  act.nlhs = uint32_t(lhs.size());
+  for (size_t i = 0; i < lhs.size(); ++i) act.lhs[i] = lhs[i];
+  act.g = g;
+  act.active = P<uint32_t>(active);
+  act.ctr = C();
+  const uint64_t act_work = uint64_t(act.nlhs) * n;
+
+  while (dl_nnz > 0) {
+    uint64_t N = 0;
+    for (;;) {
+      EventPair& o = mk(ev_other);
+      GG_HIP(hipEventRecord(o.a, stream));
+      zero_ctr();
+      fill_u32(seg_cnt, 0, R);
+      k_nonempty<<<grid_for(R), kBlock, 0, stream>>>(P<unsigned long long>(dl_ptr), R, P<uint32_t>(nonempty));
+      k_label_flags<<<1, kBlock, 0, stream>>>(P<unsigned long long>(dl_ptr), n, uint32_t(L), C());
+      act.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
+      act.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
+      if (act_work) k_activate<<<grid_for(act_work), kBlock, 0, stream>>>(act);
+      GG_HIP(hipEventRecord(o.b, stream));
+      EventPair& x = mk(ev_expand);
+      GG_HIP(hipEventRecord(x.a, stream));
+      k_expand<false><<<expand_grid(), kBlock, 0, stream>>>(expand_args(false));
+      GG_HIP(hipEventRecord(x.b, stream));
+      GG_HIP(hipGetLastError());
+      GG_TRY(read_ctr(err));
+      S.expand_launches++;
+      if (!h_ctr->overflow) break;
+      S.retries++;
+      GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
+    }
+    N = h_ctr->newe;
+    S.rounds++;
+    S.probe_reads += h_ctr->pkeys + h_ctr->cand;
+    S.candidates += h_ctr->cand;
+    S.old_reads += h_ctr->oreads;
+    S.new_edges += N;
+    if (verbose)
+      std::fprintf(stderr, "[gg] round %u: active %u cand %llu new %llu\n", S.rounds, h_ctr->active,
+                   (unsigned long long)h_ctr->cand, (unsigned long long)N);
+    // delta' from staging
+    EventPair& c = mk(ev_compact);
+    GG_HIP(hipEventRecord(c.a, stream));
+    GG_TRY(scan(seg_cnt, R, nx_ptr, err));
+    GG_TRY(grow(nx_col, std::max<uint64_t>(N, 1) * 4, err));
+    if (N)
+      k_gather<<<expand_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
+                                                     P<unsigned long long>(seg_off), P<uint32_t>(staging),
+                                                     P<unsigned long long>(nx_ptr), P<uint32_t>(nx_col));
+    GG_HIP(hipEventRecord(c.b, stream));
+    // old' = old U delta
+    EventPair& mg = mk(ev_merge);
+    GG_HIP(hipEventRecord(mg.a, stream));
+    const uint64_t o2n = old_nnz + dl_nnz;
+    GG_TRY(grow(o2_col, std::max<uint64_t>(o2n, 1) * 4, err));
+    k_add_ptr<<<grid_for(uint64_t(R) + 1), kBlock, 0, stream>>>(P<unsigned long long>(o2_ptr), P<unsigned long long>(old_ptr),
+                                                               P<unsigned long long>(dl_ptr), uint64_t(R) + 1);
+    k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>({P<unsigned long long>(old_ptr), P<uint32_t>(old_col)},
+                                                      {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)}, R,
+                                                      P<uint32_t>(o2_col));
+    GG_HIP(hipEventRecord(mg.b, stream));
+    GG_HIP(hipGetLastError());
+    std::swap(old_ptr, o2_ptr);
+    std::swap(old_col, o2_col);
+    std::swap(dl_ptr, nx_ptr);
+    std::swap(dl_col, nx_col);
+    old_nnz = o2n;
+    dl_nnz = N;
+  }
+  GG_HIP(hipStreamSynchronize(stream));
+  const auto t1 = std::chrono::steady_clock::now();
+  S.fixpoint_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  auto sum = [](std::vector<EventPair>& v) {
+    double t = 0;
+    for (auto& e : v) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) t += ms;
+    }
+    return t;
+  };
+  S.expand_ms = sum(ev_expand);
+  S.compact_ms = sum(ev_compact);
+  S.merge_ms = sum(ev_merge);
+  S.other_ms = sum(ev_other);
+  S.total_edges = old_nnz;
+  S.device_bytes_peak = peak;
+  // drop per-round scratch; keep the closure (old) for queries
+  release(nx_col);
+  release(o2_col);
+  release(staging);
+  has_closure = true;
+  if (st) *st = S;
+  return GG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// engine API
+
+Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std::string& err) {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    status = GG_E_DEVICE;
+    err = "no HIP device available";
+    return nullptr;
+  }
+  if (device < 0) {
+    int best = -1;
+    for (int d = 0; d < count; ++d) {
+      hipDeviceProp_t p;
+      if (hipGetDeviceProperties(&p, d) == hipSuccess && p.multiProcessorCount > best) {
+        best = p.multiProcessorCount;
+        device = d;
+      }
+    }
+  }
+  if (device < 0 || device >= count) {
+    status = GG_E_ARG;
+    err = "device ordinal out of range";
+    return nullptr;
+  }
+  auto* en = new Engine();
+  en->device = device;
+  en->budget = budget;
+  en->verbose = verbose;
+  hipDeviceProp_t prop;
+  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreate(&en->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    delete en;
+    status = GG_E_DEVICE;
+    err = "cannot initialise HIP device " + std::to_string(device);
+    return nullptr;
+  }
+  en->cus = prop.multiProcessorCount;
+  status = GG_OK;
+  return en;
+}
+
+void engine_destroy(Engine* e) { delete e; }
+
+int engine_set_grammar(Engine* e, const Grammar& gr, std::string& err) {
+  DevGrammar g{};
+  std::vector<bool> is_lhs(size_t(gr.num_labels()) + 1, false);
+  for (auto& r : gr.rules) {
+    if (r[1] == 0) {
+      if (g.ne >= kMaxRules) {
+        err = "more than 64 eps rules";
+        return GG_E_UNSUPPORTED;
+      }
+      g.e_lhs[g.ne++] = uint8_t(r[0] - 1);
+    } else if (r[2] == 0) {
+      if (g.nu >= kMaxRules) {
+        err = "more than 64 unary rules";
+        return GG_E_UNSUPPORTED;
+      }
+      g.u_lhs[g.nu] = uint8_t(r[0] - 1);
+      g.u_rhs[g.nu++] = uint8_t(r[1] - 1);
+      is_lhs[r[0]] = true;
+    } else {
+      if (g.nb >= kMaxRules) {
+        err = "more than 64 binary rules";
+        return GG_E_UNSUPPORTED;
+      }
+      g.b_lhs[g.nb] = uint8_t(r[0] - 1);
+      g.b_rhs1[g.nb] = uint8_t(r[1] - 1);
+      g.b_rhs2[g.nb++] = uint8_t(r[2] - 1);
+      is_lhs[r[0]] = true;
+    }
+  }
+  e->g = g;
+  e->L = gr.num_labels();
+  e->lhs.clear();
+  for (int l = 1; l <= e->L; ++l)
+    if (is_lhs[l]) e->lhs.push_back(uint8_t(l - 1));
+  e->has_grammar = true;
+  e->has_closure = false;
+  return GG_OK;
+}
+
+int engine_set_graph(Engine* e, const Graph& gr, std::string& err) {
+  GG_HIP(hipSetDevice(e->device));
+  e->n = gr.n;
+  e->m = gr.m();
+  GG_TRY(e->alloc(e->in_src, e->m * 4, err));
+  GG_TRY(e->alloc(e->in_dst, e->m * 4, err));
+  GG_TRY(e->alloc(e->in_lab, e->m, err));
+  if (e->m) {
+    GG_HIP(hipMemcpyAsync(e->in_src.p, gr.src.data(), e->m * 4, hipMemcpyHostToDevice, e->stream));
+    GG_HIP(hipMemcpyAsync(e->in_dst.p, gr.dst.data(), e->m * 4, hipMemcpyHostToDevice, e->stream));
+    GG_HIP(hipMemcpyAsync(e->in_lab.p, gr.lab.data(), e->m, hipMemcpyHostToDevice, e->stream));
+  }
+  GG_HIP(hipStreamSynchronize(e->stream));
+  e->has_graph = true;
+  e->has_closure = false;
+  return GG_OK;
+}
+
+int engine_run(Engine* e, int filter, gg_stats* stats, std::string& err) {
+  e->has_closure = false;
+  return e->run(filter, stats, err);
+}
+
+namespace {
+int need_closure(Engine* e, std::string& err) {
+  if (!e->has_closure) {
+    err = "no closure: call gg_closure_run first";
+    return GG_E_STATE;
+  }
+  return GG_OK;
+}
+}  // namespace
+
+int engine_counts(Engine* e, uint64_t* per_label, uint64_t* total, std::string& err) {
+  GG_TRY(need_closure(e, err));
+  GG_HIP(hipSetDevice(e->device));
+  std::vector<unsigned long long> b(size_t(e->L) + 1);
+  for (int l = 0; l <= e->L; ++l)
+    GG_HIP(hipMemcpy(&b[l], static_cast<unsigned long long*>(e->old_ptr.p) + uint64_t(l) * e->n, 8,
+                     hipMemcpyDeviceToHost));
+  if (per_label)
+    for (int l = 0; l < e->L; ++l) per_label[l] = b[l + 1] - b[l];
+  if (total) *total = b[e->L];
+  return GG_OK;
+}
+
+int engine_degrees(Engine* e, uint64_t* deg, std::string& err) {
+  GG_TRY(need_closure(e, err));
+  GG_HIP(hipSetDevice(e->device));
+  const uint64_t R = e->rows();
+  std::vector<unsigned long long> ptr(R + 1);
+  GG_HIP(hipMemcpy(ptr.data(), e->old_ptr.p, (R + 1) * 8, hipMemcpyDeviceToHost));
+  for (uint32_t u = 0; u < e->n; ++u) {
+    uint64_t s = 0;
+    for (int l = 0; l < e->L; ++l) {
+      const uint64_t r = uint64_t(l) * e->n + u;
+      s += ptr[r + 1] - ptr[r];
+    }
+    deg[u] = s;
+  }
+  return GG_OK;
+}
+
+int engine_export(Engine* e, uint32_t* buf, uint64_t cap, uint64_t* count, std::string& err) {
+  GG_TRY(need_closure(e, err));
+  GG_HIP(hipSetDevice(e->device));
+  const uint64_t R = e->rows();
+  const uint32_t n = e->n;
+  const int L = e->L;
+  std::vector<unsigned long long> ptr(R + 1);
+  GG_HIP(hipMemcpy(ptr.data(), e->old_ptr.p, (R + 1) * 8, hipMemcpyDeviceToHost));
+  const uint64_t total = ptr[R];
+  if (count) *count = total;
+  if (buf == nullptr || cap < total) {
+    err = "export buffer too small: need " + std::to_string(total) + " triples";
+    return GG_E_ARG;
+  }
+  std::vector<uint32_t> col(total);
+  if (total) GG_HIP(hipMemcpy(col.data(), e->old_col.p, total * 4, hipMemcpyDeviceToHost));
+  std::vector<uint64_t> off(size_t(n) + 1, 0);
+  for (uint32_t u = 0; u < n; ++u) {
+    uint64_t s = 0;
+    for (int l = 0; l < L; ++l) s += ptr[uint64_t(l) * n + u + 1] - ptr[uint64_t(l) * n + u];
+    off[u + 1] = off[u] + s;
+  }
+  // per vertex: merge the L sorted label rows by (dst, label)
+  auto work = [&](uint32_t u0, uint32_t u1) {
+    std::vector<uint64_t> pos(L), end(L);
+    for (uint32_t u = u0; u < u1; ++u) {
+      for (int l = 0; l < L; ++l) {
+        pos[l] = ptr[uint64_t(l) * n + u];
+        end[l] = ptr[uint64_t(l) * n + u + 1];
+      }
+      uint32_t* o = buf + 3 * off[u];
+      for (;;) {
+        int best = -1;
+        uint32_t bd = 0;
+        for (int l = 0; l < L; ++l) {
+          if (pos[l] < end[l] && (best < 0 || col[pos[l]] < bd)) {
+            best = l;
+            bd = col[pos[l]];
+          }
+        }
+        if (best < 0) break;
+        o[0] = u;
+        o[1] = bd;
+        o[2] = uint32_t(best + 1);
+        o += 3;
+        ++pos[best];
+      }
+    }
+  };
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  const uint32_t per = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; ++t) {
+    const uint32_t a = std::min<uint64_t>(n, uint64_t(t) * per), b = std::min<uint64_t>(n, uint64_t(t + 1) * per);
+    if (a < b) th.emplace_back(work, a, b);
+  }
+  for (auto& t : th) t.join();
+  return GG_OK;
+}
+
+int engine_checksum(Engine* e, uint64_t* sum, std::string& err) {
+  GG_TRY(need_closure(e, err));
+  GG_HIP(hipSetDevice(e->device));
+  e->zero_ctr();
+  k_checksum<<<uint32_t(e->cus) * 8, kBlock, 0, e->stream>>>(
+      {static_cast<unsigned long long*>(e->old_ptr.p), static_cast<uint32_t*>(e->old_col.p)}, e->rows(), e->n, e->C());
+  GG_HIP(hipGetLastError());
+  GG_TRY(e->read_ctr(err));
+  *sum = e->h_ctr->checksum;
+  return GG_OK;
+}
+
+int engine_release(Engine* e) {
+  DevBuf* all[] = {&e->old_ptr, &e->old_col, &e->dl_ptr, &e->dl_col, &e->nx_ptr, &e->nx_col, &e->o2_ptr,
+                   &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt, &e->active, &e->nonempty, &e->tiles};
+  for (DevBuf* b : all) e->release(*b);
+  e->has_closure = false;
+  return GG_OK;
+}
+
+int device_query(int* count, int preferred, int* selected, char* name, size_t name_cap, uint64_t* total,
+                 uint64_t* free_bytes, std::string& err) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess || c == 0) {
+    (void)hipGetLastError();
+    if (count) *count = 0;
+    err = "no HIP device available";
+    return GG_E_DEVICE;
+  }
+  int d = preferred;
+  if (d < 0) {
+    int best = -1;
+    for (int i = 0; i < c; ++i) {
+      hipDeviceProp_t p;
+      if (hipGetDeviceProperties(&p, i) == hipSuccess && p.multiProcessorCount > best) {
+        best = p.multiProcessorCount;
+        d = i;
+      }
+    }
+  }
+  hipDeviceProp_t p;
+  GG_HIP(hipGetDeviceProperties(&p, d));
+  GG_HIP(hipSetDevice(d));
+  size_t fr = 0, tot = 0;
+  GG_HIP(hipMemGetInfo(&fr, &tot));
+  if (count) *count = c;
+  if (selected) *selected = d;
+  if (name && name_cap) {
+    std::snprintf(name, name_cap, "%s", p.name);
+  }
+  if (total) *total = tot;
+  if (free_bytes) *free_bytes = fr;
+  return GG_OK;
+}
+
+}  // namespace gg

@@ -1,0 +1,36 @@
+// Interface between the C ABI (gg_api.cpp) and the device engine
+// (closure.hip).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "formats.h"
+#include "graspan_g.h"
+
+namespace gg {
+
+class Engine;
+
+// Creates the device engine on `device` (-1 = most CUs). Returns nullptr and
+// sets status/err on failure.
+Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std::string& err);
+void engine_destroy(Engine* e);
+
+// Uploads the grammar rule table and the input edges (host copies). The
+// edges stay resident on the device across runs.
+int engine_set_grammar(Engine* e, const Grammar& g, std::string& err);
+int engine_set_graph(Engine* e, const Graph& g, std::string& err);
+
+int engine_run(Engine* e, int filter, gg_stats* stats, std::string& err);
+int engine_counts(Engine* e, uint64_t* per_label, uint64_t* total, std::string& err);
+int engine_export(Engine* e, uint32_t* buf, uint64_t cap, uint64_t* count, std::string& err);
+int engine_degrees(Engine* e, uint64_t* deg, std::string& err);
+int engine_checksum(Engine* e, uint64_t* sum, std::string& err);
+int engine_release(Engine* e);
+
+int device_query(int* count, int preferred, int* selected, char* name, size_t name_cap, uint64_t* total,
+                 uint64_t* free_bytes, std::string& err);
+
+}  // namespace gg

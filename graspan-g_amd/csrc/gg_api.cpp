@@ -1,0 +1,233 @@
+// C ABI of the closure engine (declarations and reference citations in
+// include/graspan_g.h). Host C++; the device work lives in closure.hip.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.h"
+#include "formats.h"
+#include "graspan_g.h"
+#include "hash.h"
+
+struct gg_ctx {
+  gg_options opts{-1, 0, 0};
+  std::string err;
+  gg::Grammar grammar;
+  bool has_grammar = false;
+  gg::Graph graph;
+  bool has_graph = false;
+  bool graph_on_device = false;
+  gg::Engine* engine = nullptr;
+};
+
+namespace {
+
+int fail(gg_ctx* c, int status, const std::string& msg) {
+  c->err = msg;
+  return status;
+}
+
+// Creates the engine on first use (the only place that touches the GPU).
+int ensure_engine(gg_ctx* c) {
+  if (c->engine) return GG_OK;
+  int status = GG_OK;
+  c->engine = gg::engine_create(c->opts.device, c->opts.device_budget, c->opts.verbose, status, c->err);
+  if (!c->engine) return status;
+  if (c->has_grammar) {
+    int s = gg::engine_set_grammar(c->engine, c->grammar, c->err);
+    if (s != GG_OK) return s;
+  }
+  return GG_OK;
+}
+
+int push_graph(gg_ctx* c) {
+  int s = ensure_engine(c);
+  if (s != GG_OK) return s;
+  s = gg::engine_set_graph(c->engine, c->graph, c->err);
+  if (s == GG_OK) c->graph_on_device = true;
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gg_version(void) { return "graspan-g-amd 0.1 (gfx950)"; }
+
+int gg_ctx_create(const gg_options* opts, gg_ctx** out) {
+  if (!out) return GG_E_ARG;
+  gg_ctx* c = new (std::nothrow) gg_ctx();
+  if (!c) return GG_E_NOMEM;
+  if (opts) c->opts = *opts;
+  *out = c;
+  return GG_OK;
+}
+
+int gg_ctx_destroy(gg_ctx* c) {
+  if (!c) return GG_E_ARG;
+  if (c->engine) gg::engine_destroy(c->engine);
+  delete c;
+  return GG_OK;
+}
+
+const char* gg_last_error(const gg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gg_grammar_load_file(gg_ctx* c, const char* path) {
+  if (!c || !path) return c ? fail(c, GG_E_ARG, "null path") : GG_E_ARG;
+  gg::Grammar g;
+  int s = gg::parse_grammar_file(path, g, c->err);
+  if (s != GG_OK) return s;
+  c->grammar = std::move(g);
+  c->has_grammar = true;
+  c->has_graph = false;  // label ids changed: the graph must be reloaded
+  c->graph_on_device = false;
+  if (c->engine) return gg::engine_set_grammar(c->engine, c->grammar, c->err);
+  return GG_OK;
+}
+
+int gg_grammar_set(gg_ctx* c, int num_rules, const int32_t* rules, int num_labels) {
+  if (!c) return GG_E_ARG;
+  gg::Grammar g;
+  int s = gg::grammar_from_rules(num_rules, rules, num_labels, g, c->err);
+  if (s != GG_OK) return s;
+  c->grammar = std::move(g);
+  c->has_grammar = true;
+  c->has_graph = false;
+  c->graph_on_device = false;
+  if (c->engine) return gg::engine_set_grammar(c->engine, c->grammar, c->err);
+  return GG_OK;
+}
+
+int gg_grammar_info(const gg_ctx* c, int* num_rules, int* num_labels) {
+  if (!c) return GG_E_ARG;
+  if (!c->has_grammar) return GG_E_STATE;
+  if (num_rules) *num_rules = int(c->grammar.rules.size());
+  if (num_labels) *num_labels = c->grammar.num_labels();
+  return GG_OK;
+}
+
+int gg_grammar_rules(const gg_ctx* c, int32_t* out) {
+  if (!c || !out) return GG_E_ARG;
+  if (!c->has_grammar) return GG_E_STATE;
+  for (size_t i = 0; i < c->grammar.rules.size(); ++i)
+    for (int j = 0; j < 3; ++j) out[3 * i + j] = c->grammar.rules[i][j];
+  return GG_OK;
+}
+
+const char* gg_label_name(const gg_ctx* c, int id) {
+  if (!c || !c->has_grammar || id < 1 || id > c->grammar.num_labels()) return nullptr;
+  return c->grammar.names[size_t(id - 1)].c_str();
+}
+
+int gg_label_id(const gg_ctx* c, const char* name) {
+  if (!c || !name || !c->has_grammar) return 0;
+  auto it = c->grammar.ids.find(name);
+  return it == c->grammar.ids.end() ? 0 : it->second;
+}
+
+int gg_graph_load_file(gg_ctx* c, const char* path) {
+  if (!c || !path) return c ? fail(c, GG_E_ARG, "null path") : GG_E_ARG;
+  if (!c->has_grammar) return fail(c, GG_E_STATE, "load a grammar before the graph (labels are resolved by name)");
+  gg::Graph g;
+  int s = gg::parse_graph_file(path, c->grammar, g, c->err);
+  if (s != GG_OK) return s;
+  if (g.n == 0) return fail(c, GG_E_FORMAT, "graph has no vertices");
+  c->graph = std::move(g);
+  c->has_graph = true;
+  c->graph_on_device = false;
+  if (c->engine) return push_graph(c);
+  return GG_OK;
+}
+
+int gg_graph_set_coo(gg_ctx* c, uint32_t nv, uint64_t m, const uint32_t* src, const uint32_t* dst, const uint8_t* lab) {
+  if (!c) return GG_E_ARG;
+  if (!c->has_grammar) return fail(c, GG_E_STATE, "set a grammar before the graph");
+  if (nv == 0) return fail(c, GG_E_ARG, "num_vertices must be > 0");
+  if (m && (!src || !dst || !lab)) return fail(c, GG_E_ARG, "null edge array");
+  const int L = c->grammar.num_labels();
+  for (uint64_t i = 0; i < m; ++i) {
+    if (src[i] >= nv || dst[i] >= nv)
+      return fail(c, GG_E_ARG, "edge " + std::to_string(i) + ": vertex id out of range");
+    if (lab[i] < 1 || lab[i] > L) return fail(c, GG_E_ARG, "edge " + std::to_string(i) + ": label id out of range");
+  }
+  gg::Graph g;
+  g.n = nv;
+  g.src.assign(src, src + m);
+  g.dst.assign(dst, dst + m);
+  g.lab.assign(lab, lab + m);
+  c->graph = std::move(g);
+  c->has_graph = true;
+  c->graph_on_device = false;
+  if (c->engine) return push_graph(c);
+  return GG_OK;
+}
+
+int gg_graph_info(const gg_ctx* c, uint32_t* nv, uint64_t* m) {
+  if (!c) return GG_E_ARG;
+  if (!c->has_graph) return GG_E_STATE;
+  if (nv) *nv = c->graph.n;
+  if (m) *m = c->graph.m();
+  return GG_OK;
+}
+
+int gg_graph_edges(const gg_ctx* c, uint32_t* src, uint32_t* dst, uint8_t* lab) {
+  if (!c) return GG_E_ARG;
+  if (!c->has_graph) return GG_E_STATE;
+  const size_t m = c->graph.m();
+  if (src) std::memcpy(src, c->graph.src.data(), m * 4);
+  if (dst) std::memcpy(dst, c->graph.dst.data(), m * 4);
+  if (lab) std::memcpy(lab, c->graph.lab.data(), m);
+  return GG_OK;
+}
+
+int gg_closure_run(gg_ctx* c, int filter_flag, gg_stats* stats) {
+  if (!c) return GG_E_ARG;
+  if (!c->has_grammar || !c->has_graph) return fail(c, GG_E_STATE, "load a grammar and a graph first");
+  int s = ensure_engine(c);
+  if (s != GG_OK) return s;
+  if (!c->graph_on_device) {
+    s = push_graph(c);
+    if (s != GG_OK) return s;
+  }
+  return gg::engine_run(c->engine, filter_flag, stats, c->err);
+}
+
+int gg_closure_counts(gg_ctx* c, uint64_t* per_label, uint64_t* total) {
+  if (!c) return GG_E_ARG;
+  if (!c->engine) return fail(c, GG_E_STATE, "no closure: call gg_closure_run first");
+  return gg::engine_counts(c->engine, per_label, total, c->err);
+}
+
+int gg_closure_export(gg_ctx* c, uint32_t* buf, uint64_t cap, uint64_t* count) {
+  if (!c) return GG_E_ARG;
+  if (!c->engine) return fail(c, GG_E_STATE, "no closure: call gg_closure_run first");
+  return gg::engine_export(c->engine, buf, cap, count, c->err);
+}
+
+int gg_closure_degrees(gg_ctx* c, uint64_t* deg) {
+  if (!c || !deg) return GG_E_ARG;
+  if (!c->engine) return fail(c, GG_E_STATE, "no closure: call gg_closure_run first");
+  return gg::engine_degrees(c->engine, deg, c->err);
+}
+
+int gg_closure_checksum(gg_ctx* c, uint64_t* sum) {
+  if (!c || !sum) return GG_E_ARG;
+  if (!c->engine) return fail(c, GG_E_STATE, "no closure: call gg_closure_run first");
+  return gg::engine_checksum(c->engine, sum, c->err);
+}
+
+uint64_t gg_triple_hash(uint32_t s, uint32_t d, uint32_t l) { return gg::triple_hash(s, d, l); }
+
+int gg_closure_release(gg_ctx* c) {
+  if (!c) return GG_E_ARG;
+  if (c->engine) gg::engine_release(c->engine);
+  return GG_OK;
+}
+
+int gg_device_info(gg_ctx* c, int* count, int* selected, char* name, size_t name_cap, uint64_t* total,
+                   uint64_t* free_bytes) {
+  if (!c) return GG_E_ARG;
+  return gg::device_query(count, c->opts.device, selected, name, name_cap, total, free_bytes, c->err);
+}
+
+}  // extern "C"

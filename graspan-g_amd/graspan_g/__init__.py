@@ -1,0 +1,255 @@
+"""Python host mirror of the closure engine's C ABI (include/graspan_g.h).
+
+A thin ctypes layer over the in-tree ``libgraspan_g.so``: it exposes the same
+operations as the reference's driver (load grammar, load graph, run the
+closure, report FINAL NUM EDGES; src/main.cpp:860-921) plus the sorted-triple
+export, per-label counts and checksum used for parity. The compute path is
+the HIP library; this module never computes a closure itself and raises if the
+library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libgraspan_g.so")
+COMP_PATH = os.path.join(PKG_DIR, "comp")
+
+GG_OK = 0
+GG_E_ARG, GG_E_IO, GG_E_FORMAT, GG_E_STATE, GG_E_DEVICE, GG_E_NOMEM, GG_E_UNSUPPORTED = -1, -2, -3, -4, -5, -6, -7
+_STATUS = {GG_E_ARG: "GG_E_ARG", GG_E_IO: "GG_E_IO", GG_E_FORMAT: "GG_E_FORMAT", GG_E_STATE: "GG_E_STATE",
+           GG_E_DEVICE: "GG_E_DEVICE", GG_E_NOMEM: "GG_E_NOMEM", GG_E_UNSUPPORTED: "GG_E_UNSUPPORTED"}
+
+# Every function declared in include/graspan_g.h.
+EXPORTED = (
+    "gg_ctx_create", "gg_ctx_destroy", "gg_last_error", "gg_version",
+    "gg_grammar_load_file", "gg_grammar_set", "gg_grammar_info", "gg_grammar_rules", "gg_label_name", "gg_label_id",
+    "gg_graph_load_file", "gg_graph_set_coo", "gg_graph_info", "gg_graph_edges",
+    "gg_closure_run", "gg_closure_counts", "gg_closure_export", "gg_closure_degrees", "gg_closure_checksum",
+    "gg_triple_hash", "gg_closure_release", "gg_device_info",
+)
+
+
+class GGError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("device_budget", ctypes.c_uint64), ("verbose", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("rounds", ctypes.c_uint32), ("retries", ctypes.c_uint32),
+        ("input_unique", ctypes.c_uint64), ("total_edges", ctypes.c_uint64),
+        ("fixpoint_ms", ctypes.c_double), ("expand_ms", ctypes.c_double), ("compact_ms", ctypes.c_double),
+        ("merge_ms", ctypes.c_double), ("other_ms", ctypes.c_double),
+        ("expand_launches", ctypes.c_uint32),
+        ("probe_reads", ctypes.c_uint64), ("candidates", ctypes.c_uint64), ("old_reads", ctypes.c_uint64),
+        ("new_edges", ctypes.c_uint64), ("device_bytes_peak", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def build() -> None:
+    """Compiles libgraspan_g.so and comp in-tree (hipcc, gfx950)."""
+    subprocess.run(["make", "-s", "-C", PKG_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run make -C {PKG_DIR}")
+        L = ctypes.CDLL(LIB_PATH)
+        P, I, U32, U64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64
+        sig = {
+            "gg_ctx_create": (I, [P, P]), "gg_ctx_destroy": (I, [P]), "gg_last_error": (ctypes.c_char_p, [P]),
+            "gg_version": (ctypes.c_char_p, []),
+            "gg_grammar_load_file": (I, [P, ctypes.c_char_p]), "gg_grammar_set": (I, [P, I, P, I]),
+            "gg_grammar_info": (I, [P, P, P]), "gg_grammar_rules": (I, [P, P]),
+            "gg_label_name": (ctypes.c_char_p, [P, I]), "gg_label_id": (I, [P, ctypes.c_char_p]),
+            "gg_graph_load_file": (I, [P, ctypes.c_char_p]), "gg_graph_set_coo": (I, [P, U32, U64, P, P, P]),
+            "gg_graph_info": (I, [P, P, P]), "gg_graph_edges": (I, [P, P, P, P]),
+            "gg_closure_run": (I, [P, I, P]), "gg_closure_counts": (I, [P, P, P]),
+            "gg_closure_export": (I, [P, P, U64, P]), "gg_closure_degrees": (I, [P, P]),
+            "gg_closure_checksum": (I, [P, P]), "gg_triple_hash": (U64, [U32, U32, U32]),
+            "gg_closure_release": (I, [P]),
+            "gg_device_info": (I, [P, P, P, ctypes.c_char_p, ctypes.c_size_t, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def triple_hash(s: int, d: int, l: int) -> int:
+    return int(lib().gg_triple_hash(s, d, l))
+
+
+@dataclass
+class Closure:
+    total: int
+    counts: np.ndarray
+    stats: dict
+
+
+class Context:
+    """One graph on one GPU (gg_ctx)."""
+
+    def __init__(self, device: int = -1, device_budget: int = 0, verbose: bool = False):
+        self._L = lib()
+        self._h = ctypes.c_void_p()
+        opts = Options(device, device_budget, 1 if verbose else 0)
+        self._check(self._L.gg_ctx_create(ctypes.byref(opts), ctypes.byref(self._h)), "gg_ctx_create")
+
+    def close(self):
+        if self._h:
+            self._L.gg_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, status: int, what: str):
+        if status != GG_OK:
+            msg = self._L.gg_last_error(self._h).decode() if self._h else ""
+            raise GGError(status, f"{what}: {msg}")
+
+    @property
+    def last_error(self) -> str:
+        return self._L.gg_last_error(self._h).decode()
+
+    # grammar / graph
+    def load_grammar(self, path: str):
+        self._check(self._L.gg_grammar_load_file(self._h, os.fsencode(path)), "gg_grammar_load_file")
+
+    def set_grammar(self, rules: np.ndarray, num_labels: int):
+        rules = np.ascontiguousarray(rules, dtype=np.int32).reshape(-1, 3)
+        self._check(self._L.gg_grammar_set(self._h, rules.shape[0], _ptr(rules), num_labels), "gg_grammar_set")
+
+    def grammar_info(self):
+        r, l = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.gg_grammar_info(self._h, ctypes.byref(r), ctypes.byref(l)), "gg_grammar_info")
+        return r.value, l.value
+
+    def rules(self) -> np.ndarray:
+        nr, _ = self.grammar_info()
+        out = np.zeros((nr, 3), dtype=np.int32)
+        self._check(self._L.gg_grammar_rules(self._h, _ptr(out)), "gg_grammar_rules")
+        return out
+
+    def label_name(self, label_id: int) -> Optional[str]:
+        s = self._L.gg_label_name(self._h, label_id)
+        return None if s is None else s.decode()
+
+    def label_id(self, name: str) -> int:
+        return int(self._L.gg_label_id(self._h, name.encode()))
+
+    def load_graph(self, path: str):
+        self._check(self._L.gg_graph_load_file(self._h, os.fsencode(path)), "gg_graph_load_file")
+
+    def set_graph(self, n: int, src, dst, lab):
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        lab = np.ascontiguousarray(lab, dtype=np.uint8)
+        self._check(self._L.gg_graph_set_coo(self._h, n, src.shape[0], _ptr(src), _ptr(dst), _ptr(lab)),
+                    "gg_graph_set_coo")
+
+    def set_spec(self, g):
+        """Grammar + graph from a generator GraphSpec."""
+        self.set_grammar(g.rules, g.num_labels)
+        self.set_graph(g.n, g.src, g.dst, g.lab)
+
+    def graph_info(self):
+        n, m = ctypes.c_uint32(), ctypes.c_uint64()
+        self._check(self._L.gg_graph_info(self._h, ctypes.byref(n), ctypes.byref(m)), "gg_graph_info")
+        return n.value, m.value
+
+    def graph_edges(self):
+        _, m = self.graph_info()
+        s = np.empty(m, np.uint32)
+        d = np.empty(m, np.uint32)
+        l = np.empty(m, np.uint8)
+        self._check(self._L.gg_graph_edges(self._h, _ptr(s), _ptr(d), _ptr(l)), "gg_graph_edges")
+        return s, d, l
+
+    # closure
+    def run(self, filter_flag: int = 0) -> Stats:
+        st = Stats()
+        self._check(self._L.gg_closure_run(self._h, filter_flag, ctypes.byref(st)), "gg_closure_run")
+        return st
+
+    def counts(self):
+        _, L = self.grammar_info()
+        per = np.zeros(L, dtype=np.uint64)
+        tot = ctypes.c_uint64()
+        self._check(self._L.gg_closure_counts(self._h, _ptr(per), ctypes.byref(tot)), "gg_closure_counts")
+        return per, tot.value
+
+    def export(self) -> np.ndarray:
+        cnt = ctypes.c_uint64()
+        _, tot = self.counts()
+        buf = np.empty((tot, 3), dtype=np.uint32)
+        self._check(self._L.gg_closure_export(self._h, _ptr(buf), tot, ctypes.byref(cnt)), "gg_closure_export")
+        return buf[: cnt.value]
+
+    def degrees(self) -> np.ndarray:
+        n, _ = self.graph_info()
+        out = np.zeros(n, dtype=np.uint64)
+        self._check(self._L.gg_closure_degrees(self._h, _ptr(out)), "gg_closure_degrees")
+        return out
+
+    def checksum(self) -> int:
+        s = ctypes.c_uint64()
+        self._check(self._L.gg_closure_checksum(self._h, ctypes.byref(s)), "gg_closure_checksum")
+        return s.value
+
+    def release(self):
+        self._check(self._L.gg_closure_release(self._h), "gg_closure_release")
+
+    def device_info(self) -> dict:
+        cnt, sel = ctypes.c_int(), ctypes.c_int()
+        name = ctypes.create_string_buffer(256)
+        tot, fr = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self._L.gg_device_info(self._h, ctypes.byref(cnt), ctypes.byref(sel), name, 256,
+                                           ctypes.byref(tot), ctypes.byref(fr)), "gg_device_info")
+        return {"count": cnt.value, "selected": sel.value, "name": name.value.decode(), "total": tot.value,
+                "free": fr.value}
+
+
+def closure_of(g, filter_flag: int = 0, device: int = -1) -> tuple:
+    """Closure of a generator GraphSpec on the GPU: (Context, Stats). The
+    context keeps the closure for counts/export/checksum."""
+    ctx = Context(device=device)
+    ctx.set_spec(g)
+    st = ctx.run(filter_flag)
+    return ctx, st

@@ -1,0 +1,168 @@
+/*
+ * graspan_g.h -- C ABI of the MI355X (gfx950) CFL-reachability closure engine.
+ *
+ * This is the library boundary that replaces the host API of the reference
+ * (Graspan-G, src/kernel.h:115-129: transferRules, allocateElementPool,
+ * transferElements, spagpu_s, spagpu_b, spagpu, getDegree, mergeAndDiff) and
+ * the host parsers that drive it (src/main.cpp:60-139, 177-221). The
+ * reference keeps one graph per process in __constant__/__device__ globals
+ * (src/kernel.cu:3-20); here every graph lives in an explicit context.
+ *
+ * Conventions
+ *   - Every function returns GG_OK (0) or a negative GG_E* status; none of
+ *     them throws or exits. gg_last_error() holds a message for the last
+ *     failure on that context.
+ *   - Buffers are owned by the caller; the library copies what it keeps.
+ *   - Label ids are 1..L in order of first appearance in the grammar file
+ *     (src/main.cpp:77-88); 0 means "absent" inside a rule triple.
+ *   - A context is used by one host thread at a time. Independent contexts
+ *     may run concurrently.
+ *   - Functions that only parse or inspect host data (grammar, graph, label
+ *     names) work without a GPU; the closure functions need one.
+ */
+#ifndef GRASPAN_G_H
+#define GRASPAN_G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GG_OK 0
+#define GG_E_ARG -1       /* bad argument (null pointer, size out of range) */
+#define GG_E_IO -2        /* file could not be opened or read */
+#define GG_E_FORMAT -3    /* malformed grammar or graph file */
+#define GG_E_STATE -4     /* call out of order (e.g. run before load) */
+#define GG_E_DEVICE -5    /* HIP runtime error or no usable GPU */
+#define GG_E_NOMEM -6     /* device or host allocation failed / over budget */
+#define GG_E_UNSUPPORTED -7
+
+typedef struct gg_ctx gg_ctx;
+
+typedef struct gg_options {
+  int device;               /* HIP device ordinal; -1 = the device with most CUs
+                               (src/main.cpp:30-51) */
+  uint64_t device_budget;   /* bytes of device memory the context may hold;
+                               0 = no limit beyond the device's free memory */
+  int verbose;              /* 1 = progress lines on stderr */
+} gg_options;
+
+/* Statistics of one closure run. Times are wall-clock milliseconds measured
+ * with HIP events on the context's stream. */
+typedef struct gg_stats {
+  uint32_t rounds;          /* semi-naive rounds until delta was empty */
+  uint32_t retries;         /* rounds re-run after growing the staging area */
+  uint64_t input_unique;    /* |dedup(input) U eps loops| */
+  uint64_t total_edges;     /* |closure| (the reference's FINAL NUM EDGES) */
+  double fixpoint_ms;       /* first round launch .. termination */
+  double expand_ms;         /* sum of the join/dedup/difference kernel */
+  double compact_ms;        /* sum of the delta compaction kernels */
+  double merge_ms;          /* sum of the old U delta merge kernels */
+  double other_ms;          /* activation, scans, bitmaps */
+  uint32_t expand_launches; /* launches of the join/dedup/difference kernel */
+  /* algorithmic counters summed over rounds (same definitions as the
+   * oracle): probe reads, candidates, old-row reads, new edges */
+  uint64_t probe_reads;
+  uint64_t candidates;
+  uint64_t old_reads;
+  uint64_t new_edges;
+  uint64_t device_bytes_peak; /* high-water mark of the context's device arena */
+} gg_stats;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* opts may be NULL for defaults. Does not touch the GPU. */
+int gg_ctx_create(const gg_options *opts, gg_ctx **out);
+int gg_ctx_destroy(gg_ctx *ctx);
+const char *gg_last_error(const gg_ctx *ctx);
+const char *gg_version(void);
+
+/* ---- grammar ------------------------------------------------------------ */
+
+/* Parses a grammar file (src/main.cpp:60-97): '#' lines skipped, first data
+ * line = rule count, then one line per rule of 1-3 tab-separated label names.
+ * Replaces readAndTransferRules + transferRules (src/kernel.cu:22-31). More
+ * than 21 rules is accepted (the reference exits); more than 3 names on a
+ * line, an empty line or a short file is GG_E_FORMAT. */
+int gg_grammar_load_file(gg_ctx *ctx, const char *path);
+
+/* Sets the grammar from rule triples (lhs, rhs1, rhs2), ids in 1..num_labels,
+ * 0 = absent. Label names default to "1".."L". */
+int gg_grammar_set(gg_ctx *ctx, int num_rules, const int32_t *rules, int num_labels);
+
+int gg_grammar_info(const gg_ctx *ctx, int *num_rules, int *num_labels);
+/* Writes the rule triples (3 * num_rules int32). */
+int gg_grammar_rules(const gg_ctx *ctx, int32_t *rules_out);
+/* Name of label id (1..L); NULL if out of range. */
+const char *gg_label_name(const gg_ctx *ctx, int label_id);
+/* Id of a label name, 0 if unknown. */
+int gg_label_id(const gg_ctx *ctx, const char *name);
+
+/* ---- graph -------------------------------------------------------------- */
+
+/* Parses a graph file (src/main.cpp:106-139, 177-221): '#' lines skipped,
+ * header "numEdges<TAB>numVars", then numEdges lines "src<TAB>dst<TAB>label"
+ * with labels by name. Needs a grammar first. Unlike the reference, edges do
+ * not have to be grouped by src; ids >= numVars, unknown labels and short or
+ * blank lines are GG_E_FORMAT. Replaces getGraphInfo + readEdges +
+ * createGraph + transferElements (src/main.cpp:106-333, src/kernel.cu:42-52).
+ * The edges are copied to device memory here if a GPU is present, else at
+ * the first gg_closure_run. */
+int gg_graph_load_file(gg_ctx *ctx, const char *path);
+
+/* Sets the graph from host COO arrays (labels are ids 1..L). */
+int gg_graph_set_coo(gg_ctx *ctx, uint32_t num_vertices, uint64_t num_edges, const uint32_t *src,
+                     const uint32_t *dst, const uint8_t *label);
+
+int gg_graph_info(const gg_ctx *ctx, uint32_t *num_vertices, uint64_t *num_edges);
+/* Copies the parsed input edges (host copy) into caller buffers of
+ * num_edges entries each. */
+int gg_graph_edges(const gg_ctx *ctx, uint32_t *src, uint32_t *dst, uint8_t *label);
+
+/* ---- closure ------------------------------------------------------------ */
+
+/* Runs the fixpoint on the GPU. filter_flag follows the reference's third
+ * argument (src/main.cpp:890, src/kernel.cu:490-780): 0 computes the least
+ * fixpoint; a non-zero value is GG_E_UNSUPPORTED in this build. stats may be
+ * NULL. The input edges stay resident on the device across runs. */
+int gg_closure_run(gg_ctx *ctx, int filter_flag, gg_stats *stats);
+
+/* Per-label edge counts of the last closure (num_labels uint64, index l-1 for
+ * label l) and their sum (the reference's FINAL NUM EDGES, src/main.cpp:909-917,
+ * widened to 64 bits). Either pointer may be NULL. */
+int gg_closure_counts(gg_ctx *ctx, uint64_t *per_label, uint64_t *total);
+
+/* Writes the closure as uint32 (src, dst, label) triples sorted
+ * lexicographically into buf (3 * cap words). *count receives the number of
+ * triples; if cap is smaller nothing is written and GG_E_ARG is returned. */
+int gg_closure_export(gg_ctx *ctx, uint32_t *buf, uint64_t cap, uint64_t *count);
+
+/* Per-vertex out-degree of the closure over all labels (num_vertices
+ * uint64), the reference's degree[old] column (src/kernel.cu:818-839). */
+int gg_closure_degrees(gg_ctx *ctx, uint64_t *deg);
+
+/* Order-independent checksum of the closure: the sum modulo 2^64 of
+ * gg_triple_hash(src, dst, label) over every closure edge. It lets two
+ * closures too large to export be compared. No counterpart in the
+ * reference, which reports only the edge count. */
+int gg_closure_checksum(gg_ctx *ctx, uint64_t *sum);
+
+/* The per-edge hash summed by gg_closure_checksum (a splitmix64 finaliser of a
+ * linear mix of the three ids). */
+uint64_t gg_triple_hash(uint32_t src, uint32_t dst, uint32_t label);
+
+/* Releases the device state of the last closure (the input stays). */
+int gg_closure_release(gg_ctx *ctx);
+
+/* Device description used for the report lines of src/main.cpp:34,54,57.
+ * name_cap bytes are written to name. */
+int gg_device_info(gg_ctx *ctx, int *device_count, int *selected, char *name, size_t name_cap,
+                   uint64_t *total_bytes, uint64_t *free_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRASPAN_G_H */

@@ -1,0 +1,153 @@
+"""GPU closure (HIP engine through the C ABI) against the oracle and the
+golden fixtures. Integer work: every comparison is bit-exact."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden_files, load_golden
+
+import graspan_g as gg
+import oracle
+from graspan_g import gen
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, file=sys.stderr, flush=True)
+
+
+def gpu_closure(n, rules, src, dst, lab, L, **kw):
+    c = gg.Context(**kw)
+    c.set_grammar(rules, L)
+    c.set_graph(n, src, dst, lab)
+    st = c.run(0)
+    return c, st
+
+
+@pytest.mark.parametrize("path", golden_files(), ids=lambda p: p.rsplit("/", 1)[-1])
+def test_gpu_matches_golden(built, path):
+    d = load_golden(path)
+    say("golden", d["name"])
+    c, st = gpu_closure(d["n"], d["rules_arr"], d["src"], d["dst"], d["lab"], d["num_labels"])
+    np.testing.assert_array_equal(c.export(), d["expected_arr"])
+    per, tot = c.counts()
+    assert tot == len(d["expected"]) == st.total_edges
+    np.testing.assert_array_equal(per, np.asarray(d["counts"], np.uint64))
+
+
+def check_against_oracle(g, full_triples=True, **ctx_kw):
+    r = oracle.closure_of(g, want_triples=full_triples)
+    c = gg.Context(**ctx_kw)
+    c.set_spec(g)
+    st = c.run(0)
+    per, tot = c.counts()
+    assert tot == r.total
+    np.testing.assert_array_equal(per, r.counts)
+    assert c.checksum() == r.checksum
+    if full_triples:
+        np.testing.assert_array_equal(c.export(), r.triples)
+    # algorithmic counters and round count agree with the oracle's
+    P, K, O, N = (int(x) for x in r.rounds.sum(0))
+    assert st.rounds == r.rounds.shape[0]
+    assert (st.probe_reads, st.candidates, st.old_reads, st.new_edges) == (P, K, O, N)
+    assert st.input_unique == r.input_unique
+    return c, st, r
+
+
+def test_gpu_dataflow_small(built):
+    say("dataflow n=2000")
+    check_against_oracle(gen.dataflow(2000, seed=1))
+
+
+def test_gpu_alias_small(built):
+    say("alias n=5000")
+    check_against_oracle(gen.alias(5000, seed=2))
+
+
+def test_gpu_dataflow_10k(built):
+    """The small plain-dataflow workload (n = 10,000, filter 0)."""
+    say("dataflow n=10000")
+    check_against_oracle(gen.dataflow(10_000, seed=0), full_triples=False)
+
+
+def test_gpu_alias_50k_wide_window(built):
+    """n > 32768: rows get a computed candidate window."""
+    say("alias n=50000")
+    check_against_oracle(gen.alias(50_000, seed=3), full_triples=True)
+
+
+def test_gpu_multichunk_rows(built):
+    """Rows whose candidate window is wider than one LDS chunk (32768 ids)."""
+    say("wide rows n=200000")
+    rng = np.random.default_rng(4)
+    n = 200_000
+    # a few hubs with out-edges spread over the whole id range, chained
+    src = np.concatenate([np.zeros(300, np.uint32), np.arange(1, 400, dtype=np.uint32)])
+    dst = np.concatenate([rng.integers(0, n, 300).astype(np.uint32), rng.integers(0, n, 399).astype(np.uint32)])
+    lab = np.full(src.shape[0], 2, np.uint8)
+    g = gen.GraphSpec(n, gen.DATAFLOW_RULES, src, dst, lab, "wide")
+    check_against_oracle(g)
+
+
+def test_gpu_staging_retry(built, monkeypatch):
+    """A tiny first staging area forces the grow-and-retry path."""
+    say("staging retry")
+    monkeypatch.setenv("GG_STAGING_MIN_WORDS", "64")
+    c, st, _ = check_against_oracle(gen.alias(3000, seed=6))
+    assert st.retries > 0
+
+
+def test_gpu_deterministic_and_rerunnable(built):
+    say("rerun")
+    g = gen.alias(8000, seed=8)
+    c = gg.Context()
+    c.set_spec(g)
+    c.run(0)
+    a = (c.counts()[1], c.checksum())
+    c.run(0)
+    assert (c.counts()[1], c.checksum()) == a
+
+
+def test_gpu_empty_graph_eps_only(built):
+    say("empty graph")
+    c, st = gpu_closure(5, np.array([[1, 0, 0], [2, 1, 1]], np.int32), np.zeros(0, np.uint32),
+                        np.zeros(0, np.uint32), np.zeros(0, np.uint8), 2)
+    t = c.export()
+    exp = [[v, v, l] for v in range(5) for l in (1, 2)]
+    np.testing.assert_array_equal(t, np.asarray(exp, np.uint32))
+
+
+def test_gpu_filter_flag_unsupported(built):
+    say("filter=1")
+    d = load_golden(golden_files()[0])
+    c = gg.Context()
+    c.set_grammar(d["rules_arr"], d["num_labels"])
+    c.set_graph(d["n"], d["src"], d["dst"], d["lab"])
+    with pytest.raises(gg.GGError) as ei:
+        c.run(1)
+    assert ei.value.status == gg.GG_E_UNSUPPORTED
+
+
+def test_comp_cli_report_and_dump(built, tmp_path):
+    say("comp cli sample")
+    dump = tmp_path / "out.bin"
+    p = subprocess.run([gg.COMP_PATH, os.path.join(GOLDEN, "sample_graph.txt"),
+                        os.path.join(GOLDEN, "sample_grammar.txt"), "0", "--dump", str(dump)],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    d = load_golden(os.path.join(GOLDEN, "sample.json"))
+    lines = p.stdout.splitlines()
+    for want in ["===== PREPROCESSING INFO =====", "NUM RULES: 12", "NUM LABELS: 13", "NUM VERTICES: 530",
+                 "INITIAL NUM EDGES: 2000", "===== SPAGPU FINISHED =====",
+                 f"FINAL NUM EDGES: {len(d['expected'])}"]:
+        assert want in lines, want
+    got = np.fromfile(dump, dtype=np.uint32).reshape(-1, 3)
+    np.testing.assert_array_equal(got, d["expected_arr"])
+    # code order (grammar first) is detected and swapped
+    p2 = subprocess.run([gg.COMP_PATH, os.path.join(GOLDEN, "sample_grammar.txt"),
+                         os.path.join(GOLDEN, "sample_graph.txt")], capture_output=True, text=True, timeout=120)
+    assert p2.returncode == 0 and f"FINAL NUM EDGES: {len(d['expected'])}" in p2.stdout
