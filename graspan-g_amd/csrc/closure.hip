@@ -53,9 +53,9 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr uint32_t kChunkBits = 32768;  // LDS window per wave: 4 KiB seen + 4 KiB new
+constexpr uint32_t kChunkBits = 16384;  // LDS window per wave: 2 KiB seen + 2 KiB new
 constexpr uint32_t kChunkWords = kChunkBits / 32;
-constexpr uint32_t kLongList = 64;      // lists this long are read by the whole wave
+constexpr int kUnroll = 8;              // loads in flight per lane in the candidate stream
 constexpr int kMaxRules = 64;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -65,6 +65,15 @@ struct DevGrammar {
   uint8_t u_lhs[kMaxRules], u_rhs[kMaxRules];
   uint8_t b_lhs[kMaxRules], b_rhs1[kMaxRules], b_rhs2[kMaxRules];
   uint8_t e_lhs[kMaxRules];
+  // old_B x delta_C is driven from the delta_C frontier through the transpose
+  // of old_B. b_slot[i] = pair-list slot of binary rule i (0xFF when C is
+  // never derived, so delta_C is empty after the first round);
+  // slot_c/slot_t = C label and transposed-label index of each slot;
+  // t_label = label of each transposed index.
+  int nslot = 0, nt = 0;
+  uint8_t b_slot[kMaxRules];
+  uint8_t slot_c[kMaxRules], slot_t[kMaxRules];
+  uint8_t t_label[kMaxRules];
 };
 
 // Round counters, one device copy per context (zeroed by a fill kernel).
@@ -289,6 +298,7 @@ __global__ void k_activate_raw(const unsigned long long* ptr, uint32_t rows, uin
 
 struct ActArgs {
   DCsr old, dl;
+  const unsigned long long* b2ptr;  // may be null (no pair lists this round)
   uint32_t n;
   uint32_t nlhs;
   uint8_t lhs[256];
@@ -327,10 +337,11 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
           const unsigned long long kl = a.dl.ptr[rb + 1] - a.dl.ptr[rb];
           pk += kl;
           act |= kl > 0;
-          if (label_nonempty(a.ctr, a.g.b_rhs2[i])) {
-            const unsigned long long ol = a.old.ptr[rb + 1] - a.old.ptr[rb];
-            pk += ol;
-            act |= ol > 0;
+          if (label_nonempty(a.ctr, a.g.b_rhs2[i])) pk += a.old.ptr[rb + 1] - a.old.ptr[rb];
+          const uint32_t slot = a.g.b_slot[i];
+          if (slot != 0xFFu && a.b2ptr) {
+            const uint32_t rs = slot * a.n + u;
+            act |= a.b2ptr[rs + 1] > a.b2ptr[rs];
           }
         }
       }
@@ -348,10 +359,77 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// frontier pair lists for old_B x delta_C and the transpose of old_B
+
+// For every slot j and frontier vertex v (delta_C(v) non-empty), every u in
+// old_B^T(v) gets key v in pair list (j, u). COUNT pass: cnt[j*n+u]++; fill
+// pass: col[ptr[j*n+u] + fill[j*n+u]++] = v.
+template <bool FILL>
+__global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr dl, DCsr ot, uint32_t* cnt,
+                                                  const unsigned long long* ptr, uint32_t* col) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / kWave;
+  const uint64_t nwaves = uint64_t(gridDim.x) * blockDim.x / kWave;
+  const uint64_t items = uint64_t(g.nslot) * n;
+  for (uint64_t it = wave; it < items; it += nwaves) {
+    const uint32_t j = uint32_t(it / n), v = uint32_t(it % n);
+    const uint32_t rc = uint32_t(g.slot_c[j]) * n + v;
+    if (dl.ptr[rc + 1] == dl.ptr[rc]) continue;
+    const uint32_t rt = uint32_t(g.slot_t[j]) * n + v;
+    const unsigned long long b = ot.ptr[rt], e = ot.ptr[rt + 1];
+    for (unsigned long long k = b + lane; k < e; k += kWave) {
+      const uint32_t idx = j * n + ot.col[k];
+      if (FILL)
+        col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = v;
+      else
+        atomicAdd(&cnt[idx], 1u);
+    }
+  }
+}
+
+// Transpose of delta restricted to the transposed labels: row t*n + v lists
+// every u with delta_{t_label[t]}(u, v).
+template <bool FILL>
+__global__ __launch_bounds__(kBlock) void k_dtrans(const DevGrammar g, uint32_t n, DCsr dl, uint32_t* cnt,
+                                                   const unsigned long long* ptr, uint32_t* col) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / kWave;
+  const uint64_t nwaves = uint64_t(gridDim.x) * blockDim.x / kWave;
+  const uint64_t items = uint64_t(g.nt) * n;
+  for (uint64_t it = wave; it < items; it += nwaves) {
+    const uint32_t t = uint32_t(it / n), u = uint32_t(it % n);
+    const uint32_t r = uint32_t(g.t_label[t]) * n + u;
+    const unsigned long long b = dl.ptr[r], e = dl.ptr[r + 1];
+    for (unsigned long long k = b + lane; k < e; k += kWave) {
+      const uint32_t idx = t * n + dl.col[k];
+      if (FILL)
+        col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = u;
+      else
+        atomicAdd(&cnt[idx], 1u);
+    }
+  }
+}
+
+// out row r = a row r followed by b row r; rowptr(out) = rowptr(a) + rowptr(b).
+__global__ __launch_bounds__(kBlock) void k_concat(DCsr a, DCsr b, uint32_t rows, uint32_t* out_col) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  for (uint32_t r = wave; r < rows; r += nwaves) {
+    const unsigned long long ab = a.ptr[r], ae = a.ptr[r + 1], bb = b.ptr[r], be = b.ptr[r + 1];
+    uint32_t* out = out_col + ab + bb;
+    const uint32_t al = uint32_t(ae - ab), bl = uint32_t(be - bb);
+    for (uint32_t t = lane; t < al; t += kWave) out[t] = a.col[ab + t];
+    for (uint32_t t = lane; t < bl; t += kWave) out[al + t] = b.col[bb + t];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // expand: fused join + dedup + difference, one wave per active row
 
 struct ExpandArgs {
   DCsr old, dl, raw;
+  DCsr b2;  // rows slot*n + u: keys v with old_B(u,v) and delta_C(v) non-empty
   const uint32_t* nonempty;
   uint32_t n;
   DevGrammar g;
@@ -366,39 +444,48 @@ struct ExpandArgs {
 struct WaveLds {
   uint32_t seen[kChunkWords];
   uint32_t fresh[kChunkWords];
-  uint32_t pref[kWave];
+  unsigned long long pref[kWave];
   unsigned long long base[kWave];
 };
 
 // Calls elem(x) for every element of the lists held by the lanes (lane i
-// holds col[lb .. lb+len)). Long lists: whole wave per list. Short lists:
-// packed across lanes by a wave prefix sum and an LDS binary search.
+// holds col[lb .. lb+len)). All lists of the batch are flattened into one
+// element stream: a wave prefix sum of the lengths goes to LDS and each
+// element index is mapped back to its list by a 6-step binary search. Every
+// lane keeps kUnroll loads in flight before it consumes them.
 template <class F>
 __device__ inline void expand_lists(const uint32_t* __restrict__ col, unsigned long long lb, uint32_t len, WaveLds& w,
                                     int lane, F&& elem) {
-  unsigned long long longm = __ballot(len >= kLongList);
-  while (longm) {
-    const int j = __builtin_ctzll(longm);
-    longm &= longm - 1;
-    const unsigned long long b = __shfl(lb, j, kWave);
-    const uint32_t l = __shfl(len, j, kWave);
-    for (uint32_t t = lane; t < l; t += kWave) elem(col[b + t]);
+  unsigned long long incl = len;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const unsigned long long y = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += y;
   }
-  const uint32_t slen = len >= kLongList ? 0u : len;
-  const uint32_t incl = wave_incl_scan(slen, lane);
-  const uint32_t total = __shfl(incl, kWave - 1, kWave);
+  const unsigned long long total = __shfl(incl, kWave - 1, kWave);
   if (total == 0) return;
   wave_sync();
   w.pref[lane] = incl;
   w.base[lane] = lb;
   wave_sync();
-  for (uint32_t t = lane; t < total; t += kWave) {
-    int k = 0;
+  for (unsigned long long t0 = 0; t0 < total; t0 += kWave * kUnroll) {
+    uint32_t x[kUnroll];
 #pragma unroll
-    for (int step = 32; step > 0; step >>= 1)
-      if (w.pref[k + step - 1] <= t) k += step;
-    const uint32_t prev = k ? w.pref[k - 1] : 0u;
-    elem(col[w.base[k] + (t - prev)]);
+    for (int q = 0; q < kUnroll; ++q) {
+      const unsigned long long t = t0 + q * kWave + lane;
+      x[q] = 0;
+      if (t < total) {
+        int k = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+          if (w.pref[k + step - 1] <= t) k += step;
+        const unsigned long long prev = k ? w.pref[k - 1] : 0ull;
+        x[q] = col[w.base[k] + (t - prev)];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kUnroll; ++q)
+      if (t0 + q * kWave + lane < total) elem(x[q]);
   }
   wave_sync();
 }
@@ -441,20 +528,19 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
         visit(a.dl.col, db, dlen);
       }
     }
-    // old_B(u) x delta_C
-    if (label_nonempty(a.ctr, C)) {
-      const uint32_t rb = B * n + u;
-      const unsigned long long kb = a.old.ptr[rb], ke = a.old.ptr[rb + 1];
+    // old_B(u) x delta_C, keys from the frontier pair list
+    const uint32_t slot = a.g.b_slot[i];
+    if (slot != 0xFFu && a.b2.ptr) {
+      const uint32_t rb = slot * n + u;
+      const unsigned long long kb = a.b2.ptr[rb], ke = a.b2.ptr[rb + 1];
       for (unsigned long long k0 = kb; k0 < ke; k0 += kWave) {
         const unsigned long long k = k0 + lane;
         unsigned long long db = 0;
         uint32_t dlen = 0;
         if (k < ke) {
-          const uint32_t rc = C * n + a.old.col[k];
-          if (row_nonempty(a.nonempty, rc)) {
-            db = a.dl.ptr[rc];
-            dlen = uint32_t(a.dl.ptr[rc + 1] - db);
-          }
+          const uint32_t rc = C * n + a.b2.col[k];
+          db = a.dl.ptr[rc];
+          dlen = uint32_t(a.dl.ptr[rc + 1] - db);
         }
         visit(a.dl.col, db, dlen);
       }
@@ -744,16 +830,23 @@ class Engine {
   // closure state
   DevBuf old_ptr, old_col, dl_ptr, dl_col, nx_ptr, nx_col, o2_ptr, o2_col;
   DevBuf raw_ptr, raw_col, staging, seg_off, seg_cnt, active, nonempty, tiles, ctr;
-  uint64_t old_nnz = 0, dl_nnz = 0;
+  // transpose of old over the transposed labels (+ ping-pong), transpose of
+  // delta, frontier pair lists
+  DevBuf ot_ptr, ot_col, ot2_ptr, ot2_col, dt_ptr, dt_col, dt_cnt, pr_ptr, pr_col, pr_cnt;
+  uint64_t old_nnz = 0, dl_nnz = 0, ot_nnz = 0;
+  bool pairs_ready = false;
   bool has_closure = false;
   Counters* h_ctr = nullptr;  // pinned
 
   ~Engine() {
-    DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col, &nx_ptr, &nx_col, &o2_ptr,
-                     &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active, &nonempty, &tiles, &ctr};
+    DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
+                     &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active, &nonempty,
+                     &tiles,  &ctr,    &ot_ptr,  &ot_col, &ot2_ptr, &ot2_col, &dt_ptr,  &dt_col, &dt_cnt,
+                     &pr_ptr, &pr_col, &pr_cnt};
     for (DevBuf* b : all)
       if (b->p) (void)hipFree(b->p);
     if (h_ctr) (void)hipHostFree(h_ctr);
+    if (h_u64) (void)hipHostFree(h_u64);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -835,6 +928,17 @@ class Engine {
     return GG_OK;
   }
 
+  // Reads one uint64 from device memory (synchronises the stream).
+  int read_u64(const void* dptr, uint64_t& out, std::string& err) {
+    unsigned long long v = 0;
+    GG_HIP(hipMemcpyAsync(h_u64, dptr, 8, hipMemcpyDeviceToHost, stream));
+    GG_HIP(hipStreamSynchronize(stream));
+    v = *h_u64;
+    out = v;
+    return GG_OK;
+  }
+  unsigned long long* h_u64 = nullptr;  // pinned
+
   int read_ctr(std::string& err) {
     GG_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, stream));
     GG_HIP(hipStreamSynchronize(stream));
@@ -847,6 +951,7 @@ class Engine {
     a.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
     a.raw = {init ? P<unsigned long long>(raw_ptr) : nullptr, init ? P<uint32_t>(raw_col) : nullptr};
     a.nonempty = P<uint32_t>(nonempty);
+    a.b2 = {pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr, pairs_ready ? P<uint32_t>(pr_col) : nullptr};
     a.n = n;
     a.g = g;
     a.active = P<uint32_t>(active);
@@ -858,7 +963,8 @@ class Engine {
     return a;
   }
 
-  uint32_t expand_grid() const { return uint32_t(cus) * 4; }
+  int expand_blocks_per_cu = 4;
+  uint32_t expand_grid() const { return uint32_t(cus) * uint32_t(expand_blocks_per_cu); }
   uint32_t rows() const { return uint32_t(uint64_t(L) * n); }
 
   int run(int filter, gg_stats* st, std::string& err);
@@ -881,8 +987,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     err = "filter_flag != 0 (the reference's dataflow-only schedule) is not supported by this build";
     return GG_E_UNSUPPORTED;
   }
-  if (uint64_t(L) * n >= 0xFFFFFFFFull) {
-    err = "num_labels * num_vertices must be < 2^32";
+  if (uint64_t(L) * n >= 0xFFFFFFFFull || uint64_t(g.nslot) * n >= 0xFFFFFFFFull) {
+    err = "num_labels * num_vertices (and binary rules with a derived second label * num_vertices) must be < 2^32";
     return GG_E_UNSUPPORTED;
   }
   GG_HIP(hipSetDevice(device));
@@ -917,6 +1023,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   GG_TRY(alloc(active, uint64_t(R) * 4, err));
   GG_TRY(alloc(nonempty, words * 4, err));
   if (!h_ctr) GG_HIP(hipHostMalloc((void**)&h_ctr, sizeof(Counters), hipHostMallocDefault));
+  if (!h_u64) GG_HIP(hipHostMalloc((void**)&h_u64, 8, hipHostMallocDefault));
 
   const auto t0 = std::chrono::steady_clock::now();
 
@@ -984,18 +1091,50 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   act.active = P<uint32_t>(active);
   act.ctr = C();
   const uint64_t act_work = uint64_t(act.nlhs) * n;
+  const uint64_t NT = uint64_t(g.nt) * n, NS = uint64_t(g.nslot) * n;
+  if (NT) {
+    GG_TRY(alloc(ot_ptr, (NT + 1) * 8, err));
+    GG_TRY(alloc(ot2_ptr, (NT + 1) * 8, err));
+    GG_TRY(alloc(dt_ptr, (NT + 1) * 8, err));
+    GG_TRY(alloc(dt_cnt, NT * 4, err));
+    fill_u64(ot_ptr, 0, NT + 1);
+  }
+  if (NS) {
+    GG_TRY(alloc(pr_ptr, (NS + 1) * 8, err));
+    GG_TRY(alloc(pr_cnt, NS * 4, err));
+  }
+  ot_nnz = 0;
 
   while (dl_nnz > 0) {
     uint64_t N = 0;
+    // frontier pair lists for old_B x delta_C (old is empty in the first round)
+    pairs_ready = false;
+    if (NS && ot_nnz > 0) {
+      EventPair& o = mk(ev_other);
+      GG_HIP(hipEventRecord(o.a, stream));
+      const DCsr dlc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
+      const DCsr otc{P<unsigned long long>(ot_ptr), P<uint32_t>(ot_col)};
+      fill_u32(pr_cnt, 0, NS);
+      k_pairs<false><<<grid_for(NS * kWave), kBlock, 0, stream>>>(g, n, dlc, otc, P<uint32_t>(pr_cnt), nullptr, nullptr);
+      GG_TRY(scan(pr_cnt, NS, pr_ptr, err));
+      uint64_t npairs = 0;
+      GG_TRY(read_u64(P<unsigned long long>(pr_ptr) + NS, npairs, err));
+      GG_TRY(grow(pr_col, std::max<uint64_t>(npairs, 1) * 4, err));
+      fill_u32(pr_cnt, 0, NS);
+      k_pairs<true><<<grid_for(NS * kWave), kBlock, 0, stream>>>(g, n, dlc, otc, P<uint32_t>(pr_cnt),
+                                                                P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
+      GG_HIP(hipEventRecord(o.b, stream));
+      pairs_ready = true;
+    }
     for (;;) {
       EventPair& o = mk(ev_other);
       GG_HIP(hipEventRecord(o.a, stream));
       zero_ctr();
       fill_u32(seg_cnt, 0, R);
-      k_nonempty<<<grid_for(R), kBlock, 0, stream>>>(P<unsigned long long>(dl_ptr), R, P<uint32_t>(nonempty));
       k_label_flags<<<1, kBlock, 0, stream>>>(P<unsigned long long>(dl_ptr), n, uint32_t(L), C());
       act.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
       act.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
+      act.b2ptr = pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr;
       if (act_work) k_activate<<<grid_for(act_work), kBlock, 0, stream>>>(act);
       GG_HIP(hipEventRecord(o.b, stream));
       EventPair& x = mk(ev_expand);
@@ -1028,7 +1167,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
                                                      P<unsigned long long>(seg_off), P<uint32_t>(staging),
                                                      P<unsigned long long>(nx_ptr), P<uint32_t>(nx_col));
     GG_HIP(hipEventRecord(c.b, stream));
-    // old' = old U delta
+    // old' = old U delta, and its transpose old'^T = old^T ++ delta^T
     EventPair& mg = mk(ev_merge);
     GG_HIP(hipEventRecord(mg.a, stream));
     const uint64_t o2n = old_nnz + dl_nnz;
@@ -1038,6 +1177,27 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>({P<unsigned long long>(old_ptr), P<uint32_t>(old_col)},
                                                       {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)}, R,
                                                       P<uint32_t>(o2_col));
+    if (NT && N) {
+      const DCsr dlc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
+      fill_u32(dt_cnt, 0, NT);
+      k_dtrans<false><<<grid_for(NT * kWave), kBlock, 0, stream>>>(g, n, dlc, P<uint32_t>(dt_cnt), nullptr, nullptr);
+      GG_TRY(scan(dt_cnt, NT, dt_ptr, err));
+      uint64_t ndt = 0;
+      GG_TRY(read_u64(P<unsigned long long>(dt_ptr) + NT, ndt, err));
+      GG_TRY(grow(dt_col, std::max<uint64_t>(ndt, 1) * 4, err));
+      fill_u32(dt_cnt, 0, NT);
+      k_dtrans<true><<<grid_for(NT * kWave), kBlock, 0, stream>>>(g, n, dlc, P<uint32_t>(dt_cnt),
+                                                                 P<unsigned long long>(dt_ptr), P<uint32_t>(dt_col));
+      GG_TRY(grow(ot2_col, std::max<uint64_t>(ot_nnz + ndt, 1) * 4, err));
+      k_add_ptr<<<grid_for(NT + 1), kBlock, 0, stream>>>(P<unsigned long long>(ot2_ptr), P<unsigned long long>(ot_ptr),
+                                                        P<unsigned long long>(dt_ptr), NT + 1);
+      k_concat<<<uint32_t(cus) * 8, kBlock, 0, stream>>>({P<unsigned long long>(ot_ptr), P<uint32_t>(ot_col)},
+                                                         {P<unsigned long long>(dt_ptr), P<uint32_t>(dt_col)},
+                                                         uint32_t(NT), P<uint32_t>(ot2_col));
+      std::swap(ot_ptr, ot2_ptr);
+      std::swap(ot_col, ot2_col);
+      ot_nnz += ndt;
+    }
     GG_HIP(hipEventRecord(mg.b, stream));
     GG_HIP(hipGetLastError());
     std::swap(old_ptr, o2_ptr);
@@ -1114,6 +1274,11 @@ Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std
     return nullptr;
   }
   en->cus = prop.multiProcessorCount;
+  int bpc = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_expand<false>), kBlock, 0) ==
+          hipSuccess &&
+      bpc > 0)
+    en->expand_blocks_per_cu = bpc;
   status = GG_OK;
   return en;
 }
@@ -1148,6 +1313,20 @@ int engine_set_grammar(Engine* e, const Grammar& gr, std::string& err) {
       g.b_rhs2[g.nb++] = uint8_t(r[2] - 1);
       is_lhs[r[0]] = true;
     }
+  }
+  // pair-list slots: binary rules whose C is derived by a unary/binary rule
+  std::vector<int> tslot(size_t(gr.num_labels()) + 1, -1);
+  for (int i = 0; i < g.nb; ++i) {
+    g.b_slot[i] = 0xFF;
+    const int C = g.b_rhs2[i] + 1, B = g.b_rhs1[i] + 1;
+    if (!is_lhs[C]) continue;
+    if (tslot[B] < 0) {
+      tslot[B] = g.nt;
+      g.t_label[g.nt++] = uint8_t(B - 1);
+    }
+    g.slot_c[g.nslot] = uint8_t(C - 1);
+    g.slot_t[g.nslot] = uint8_t(tslot[B]);
+    g.b_slot[i] = uint8_t(g.nslot++);
   }
   e->g = g;
   e->L = gr.num_labels();
@@ -1295,8 +1474,10 @@ int engine_checksum(Engine* e, uint64_t* sum, std::string& err) {
 }
 
 int engine_release(Engine* e) {
-  DevBuf* all[] = {&e->old_ptr, &e->old_col, &e->dl_ptr, &e->dl_col, &e->nx_ptr, &e->nx_col, &e->o2_ptr,
-                   &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt, &e->active, &e->nonempty, &e->tiles};
+  DevBuf* all[] = {&e->old_ptr, &e->old_col, &e->dl_ptr,  &e->dl_col,   &e->nx_ptr, &e->nx_col,   &e->o2_ptr,
+                   &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt,  &e->active, &e->nonempty, &e->tiles,
+                   &e->ot_ptr,  &e->ot_col,  &e->ot2_ptr, &e->ot2_col,  &e->dt_ptr, &e->dt_col,   &e->dt_cnt,
+                   &e->pr_ptr,  &e->pr_col,  &e->pr_cnt};
   for (DevBuf* b : all) e->release(*b);
   e->has_closure = false;
   return GG_OK;
