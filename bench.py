@@ -75,6 +75,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=WORKLOAD_N, help="vertex count (default: the 1M-vertex workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verbose", action="store_true", help="per-round engine statistics on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -86,7 +87,7 @@ def main():
     t = time.time()
     g = gen.alias(args.n, seed=0)
     log(f"generated alias n={g.n} m={g.m} in {time.time() - t:.1f}s")
-    ctx = gg.Context()
+    ctx = gg.Context(verbose=args.verbose)
     ctx.set_spec(g)
     for i in range(args.warmup):
         st = ctx.run(0)

@@ -56,6 +56,8 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kChunkBits = 16384;  // LDS window per wave: 2 KiB seen + 2 KiB new
 constexpr uint32_t kChunkWords = kChunkBits / 32;
 constexpr int kUnroll = 8;              // loads in flight per lane in the candidate stream
+constexpr uint32_t kGrab = 16;          // active rows taken per work-counter atomic
+constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
 constexpr int kMaxRules = 64;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -78,7 +80,7 @@ struct DevGrammar {
 
 // Round counters, one device copy per context (zeroed by a fill kernel).
 struct Counters {
-  unsigned long long cursor;   // staging bump pointer (= new edges emitted)
+  unsigned long long cursor;   // staging bump pointer (new edges + unused tails of per-wave blocks)
   unsigned long long cand;     // K: candidates streamed
   unsigned long long pkeys;    // key elements scanned (binary rules)
   unsigned long long oreads;   // O: old/delta row elements read for the difference
@@ -427,8 +429,19 @@ __global__ __launch_bounds__(kBlock) void k_concat(DCsr a, DCsr b, uint32_t rows
 // ---------------------------------------------------------------------------
 // expand: fused join + dedup + difference, one wave per active row
 
+// Packed form of a sorted row: one entry per non-empty 32-id word,
+// low 32 bits = word index (id >> 5), high 32 bits = the word's bit mask.
+struct DPacked {
+  const unsigned long long* ptr;  // rows+1, null when absent
+  const unsigned long long* ent;
+  uint32_t labels[8];             // bit l: rows of label l are packed
+};
+
+__device__ inline bool packed_label(const DPacked& p, uint32_t l) { return (p.labels[l >> 5] >> (l & 31)) & 1u; }
+
 struct ExpandArgs {
   DCsr old, dl, raw;
+  DPacked pk;  // packed old rows
   DCsr b2;  // rows slot*n + u: keys v with old_B(u,v) and delta_C(v) non-empty
   const uint32_t* nonempty;
   uint32_t n;
@@ -490,10 +503,52 @@ __device__ inline void expand_lists(const uint32_t* __restrict__ col, unsigned l
   wave_sync();
 }
 
+// expand_lists for packed lists: calls elem(entry) for every packed word of
+// the lists held by the lanes (lane i holds ent[pb .. pb+plen)).
+template <class F>
+__device__ inline void expand_words(const unsigned long long* __restrict__ ent, unsigned long long pb, uint32_t plen,
+                                    WaveLds& w, int lane, F&& elem) {
+  unsigned long long incl = plen;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const unsigned long long y = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += y;
+  }
+  const unsigned long long total = __shfl(incl, kWave - 1, kWave);
+  if (total == 0) return;
+  wave_sync();
+  w.pref[lane] = incl;
+  w.base[lane] = pb;
+  wave_sync();
+  for (unsigned long long t0 = 0; t0 < total; t0 += kWave * kUnroll) {
+    unsigned long long x[kUnroll];
+#pragma unroll
+    for (int q = 0; q < kUnroll; ++q) {
+      const unsigned long long t = t0 + q * kWave + lane;
+      x[q] = 0;
+      if (t < total) {
+        int k = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+          if (w.pref[k + step - 1] <= t) k += step;
+        const unsigned long long prev = k ? w.pref[k - 1] : 0ull;
+        x[q] = ent[w.base[k] + (t - prev)];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kUnroll; ++q)
+      if (t0 + q * kWave + lane < total) elem(x[q]);
+  }
+  wave_sync();
+}
+
 // Visits every candidate list of row (A,u): calls visit(col, lb, len) once
-// per batch with one list per lane (len 0 = no list on that lane).
-template <bool INIT, class V>
-__device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A, uint32_t u, int lane, V&& visit) {
+// per batch with one list per lane (len 0 = no list on that lane). With
+// PACKED, old_C rows of packed labels go to visitw(pb, plen, len) instead
+// (plen packed words standing for len ids).
+template <bool INIT, bool PACKED, class V, class W>
+__device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A, uint32_t u, int lane, V&& visit,
+                                     W&& visitw) {
   const uint32_t n = a.n;
   if (INIT) {
     const unsigned long long b = a.raw.ptr[r], e = a.raw.ptr[r + 1];
@@ -511,20 +566,28 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
     const uint32_t B = a.g.b_rhs1[i], C = a.g.b_rhs2[i];
     // delta_B(u) x (old_C U delta_C)
     {
+      const bool packed = PACKED && a.pk.ptr && packed_label(a.pk, C);
       const uint32_t rb = B * n + u;
       const unsigned long long kb = a.dl.ptr[rb], ke = a.dl.ptr[rb + 1];
       for (unsigned long long k0 = kb; k0 < ke; k0 += kWave) {
         const unsigned long long k = k0 + lane;
-        unsigned long long ob = 0, db = 0;
-        uint32_t ol = 0, dlen = 0;
+        unsigned long long ob = 0, db = 0, pb = 0;
+        uint32_t ol = 0, dlen = 0, pl = 0;
         if (k < ke) {
           const uint32_t rc = C * n + a.dl.col[k];
           ob = a.old.ptr[rc];
           ol = uint32_t(a.old.ptr[rc + 1] - ob);
           db = a.dl.ptr[rc];
           dlen = uint32_t(a.dl.ptr[rc + 1] - db);
+          if (packed) {
+            pb = a.pk.ptr[rc];
+            pl = uint32_t(a.pk.ptr[rc + 1] - pb);
+          }
         }
-        visit(a.old.col, ob, ol);
+        if (packed)
+          visitw(pb, pl, ol);
+        else
+          visit(a.old.col, ob, ol);
         visit(a.dl.col, db, dlen);
       }
     }
@@ -566,8 +629,19 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
       const uint32_t off = x - c0;
       if (off < span) atomicOr(&w.seen[off >> 5], 1u << (off & 31));
     };
-    const unsigned long long ob = a.old.ptr[r], oe = a.old.ptr[r + 1];
-    for (unsigned long long i = ob + lane; i < oe; i += kWave) seed(a.old.col[i]);
+    if (a.pk.ptr && packed_label(a.pk, A)) {
+      // seed old_A(u) word by word (c0 is a multiple of 32)
+      const uint32_t w0 = c0 >> 5;
+      const unsigned long long pb = a.pk.ptr[r], pe = a.pk.ptr[r + 1];
+      for (unsigned long long i = pb + lane; i < pe; i += kWave) {
+        const unsigned long long e = a.pk.ent[i];
+        const uint32_t wi = uint32_t(e) - w0;
+        if (wi < words) atomicOr(&w.seen[wi], uint32_t(e >> 32));
+      }
+    } else {
+      const unsigned long long ob = a.old.ptr[r], oe = a.old.ptr[r + 1];
+      for (unsigned long long i = ob + lane; i < oe; i += kWave) seed(a.old.col[i]);
+    }
     const unsigned long long db = a.dl.ptr[r], de = a.dl.ptr[r + 1];
     for (unsigned long long i = db + lane; i < de; i += kWave) seed(a.dl.col[i]);
     wave_sync();
@@ -583,11 +657,28 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
       }
     }
   };
+  const uint32_t w0 = c0 >> 5;
+  auto mark_word = [&](unsigned long long e) {
+    const uint32_t wi = uint32_t(e) - w0;
+    if (wi < words) {
+      const uint32_t bits = uint32_t(e >> 32);
+      if (bits & ~vseen[wi]) {
+        const uint32_t nb = bits & ~atomicOr(&w.seen[wi], bits);
+        if (nb) atomicOr(&w.fresh[wi], nb);
+      }
+    }
+  };
   unsigned long long kc = 0;
-  for_each_list<INIT>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
-    kc += len;
-    expand_lists(col, lb, len, w, lane, mark);
-  });
+  for_each_list<INIT, true>(
+      a, r, A, u, lane,
+      [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
+        kc += len;
+        expand_lists(col, lb, len, w, lane, mark);
+      },
+      [&](unsigned long long pb, uint32_t pl, uint32_t len) {
+        kc += len;
+        expand_words(a.pk.ent, pb, pl, w, lane, mark_word);
+      });
   kcount = wave_sum_u64(kc);
   wave_sync();
   uint32_t c = 0;
@@ -622,11 +713,33 @@ __global__ __launch_bounds__(kBlock) void k_expand(const ExpandArgs a) {
   WaveLds& w = s_lds[threadIdx.x / kWave];
   const uint32_t nactive = a.ctr->active;
   unsigned long long kc = 0, orr = 0, ne = 0;
+  // rows are taken kGrab at a time and staging space kStageBlock words at a
+  // time, so the two global counters see one atomic per many rows
+  uint32_t next = 0, end = 0;
+  unsigned long long blk_pos = 0, blk_end = 0;
+  auto reserve = [&](uint32_t cnt) -> unsigned long long {
+    if (blk_pos + cnt > blk_end) {
+      const unsigned long long grab = max<unsigned long long>(cnt, kStageBlock);
+      unsigned long long b = 0;
+      if (lane == 0) b = atomicAdd(&a.ctr->cursor, grab);
+      b = __shfl(b, 0, kWave);
+      blk_pos = b;
+      blk_end = b + grab;
+    }
+    const unsigned long long p = blk_pos;
+    blk_pos += cnt;
+    return p;
+  };
   for (;;) {
-    uint32_t idx = 0;
-    if (lane == 0) idx = atomicAdd(&a.ctr->work, 1u);
-    idx = __shfl(idx, 0, kWave);
-    if (idx >= nactive) break;
+    if (next >= end) {
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(&a.ctr->work, kGrab);
+      b = __shfl(b, 0, kWave);
+      if (b >= nactive) break;
+      next = b;
+      end = min(b + kGrab, nactive);
+    }
+    const uint32_t idx = next++;
     const uint32_t r = a.active[idx];
     const uint32_t A = r / a.n, u = r - A * a.n;
     // candidate window [lo, hi]
@@ -634,7 +747,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(const ExpandArgs a) {
     if (a.n > kChunkBits) {
       uint32_t mn = 0xFFFFFFFFu, mx = 0;
       unsigned long long kk = 0;
-      for_each_list<INIT>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
+      for_each_list<INIT, false>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
         if (INIT) {
           // raw rows are unsorted: reduce over every element
           const unsigned long long b = __shfl(lb, 0, kWave);
@@ -650,10 +763,10 @@ __global__ __launch_bounds__(kBlock) void k_expand(const ExpandArgs a) {
           mx = max(mx, col[lb + len - 1]);
           kk += len;
         }
-      });
+      }, [](unsigned long long, uint32_t, uint32_t) {});
       kk = wave_sum_u64(kk);
       if (kk == 0) continue;
-      lo = wave_min_u32(mn);
+      lo = wave_min_u32(mn) & ~31u;  // word-aligned, for packed rows
       hi = wave_max_u32(mx);
     }
     const uint32_t nchunks = (hi - lo) / kChunkBits + 1;
@@ -663,8 +776,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(const ExpandArgs a) {
     if (nchunks == 1) {
       cnt = process_chunk<INIT>(a, r, A, u, lo, hi - lo + 1, w, lane, krow);
       if (cnt) {
-        if (lane == 0) pos = atomicAdd(&a.ctr->cursor, (unsigned long long)cnt);
-        pos = __shfl(pos, 0, kWave);
+        pos = reserve(cnt);
         if (pos + cnt <= a.staging_cap)
           emit_chunk(a, lo, hi - lo + 1, pos, w, lane);
         else if (lane == 0)
@@ -679,8 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(const ExpandArgs a) {
         if (ch == 0) krow = kch;
       }
       if (cnt) {
-        if (lane == 0) pos = atomicAdd(&a.ctr->cursor, (unsigned long long)cnt);
-        pos = __shfl(pos, 0, kWave);
+        pos = reserve(cnt);
         if (pos + cnt <= a.staging_cap) {
           unsigned long long p = pos;
           for (uint32_t ch = 0; ch < nchunks; ++ch) {
@@ -708,6 +819,54 @@ __global__ __launch_bounds__(kBlock) void k_expand(const ExpandArgs a) {
     if (kc) atomicAdd(&a.ctr->cand, kc);
     if (orr) atomicAdd(&a.ctr->oreads, orr);
     if (ne) atomicAdd(&a.ctr->newe, ne);
+  }
+}
+
+// Packs the rows of the labels in `labels` into (word index, mask) entries.
+// COUNT pass writes cnt[r] (0 for rows of other labels); FILL pass writes the
+// entries at ptr[r]. One wave per row; ids are gathered 64 at a time into a
+// 64-word LDS window (2048 ids) that is flushed in word order.
+template <bool FILL>
+__global__ __launch_bounds__(kBlock) void k_pack(DCsr c, uint32_t n, uint32_t rows, DPacked lab, uint32_t* cnt,
+                                                 const unsigned long long* ptr, unsigned long long* ent) {
+  __shared__ uint32_t s_win[kWavesPerBlock][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  uint32_t* win = s_win[wv];
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  const unsigned long long below = (1ull << lane) - 1;
+  for (uint32_t r = wave; r < rows; r += nwaves) {
+    if (!packed_label(lab, r / n)) {
+      if (!FILL && lane == 0) cnt[r] = 0;
+      continue;
+    }
+    unsigned long long i = c.ptr[r];
+    const unsigned long long e = c.ptr[r + 1];
+    uint32_t written = 0;
+    while (i < e) {
+      const uint32_t c0 = c.col[i] & ~31u;
+      const unsigned long long c1 = (unsigned long long)c0 + 32u * kWave;
+      win[lane] = 0;
+      wave_sync();
+      for (;;) {
+        const unsigned long long t = i + lane;
+        const uint32_t x = t < e ? c.col[t] : 0u;
+        const bool in = t < e && x < c1;
+        if (in) atomicOr(&win[(x - c0) >> 5], 1u << (x & 31));
+        const uint32_t used = uint32_t(__popcll(__ballot(in)));
+        i += used;
+        if (used < kWave) break;
+      }
+      wave_sync();
+      const uint32_t word = win[lane];
+      const unsigned long long nz = __ballot(word != 0);
+      if (FILL && word)
+        ent[ptr[r] + written + __popcll(nz & below)] = ((unsigned long long)word << 32) | ((c0 >> 5) + uint32_t(lane));
+      written += uint32_t(__popcll(nz));
+      wave_sync();
+    }
+    if (!FILL && lane == 0) cnt[r] = written;
   }
 }
 
@@ -833,6 +992,10 @@ class Engine {
   // transpose of old over the transposed labels (+ ping-pong), transpose of
   // delta, frontier pair lists
   DevBuf ot_ptr, ot_col, ot2_ptr, ot2_col, dt_ptr, dt_col, dt_cnt, pr_ptr, pr_col, pr_cnt;
+  // packed old rows (labels in pack_labels)
+  DevBuf pk_ptr, pk_ent, pk_cnt;
+  uint32_t pack_labels[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool pk_ready = false;
   uint64_t old_nnz = 0, dl_nnz = 0, ot_nnz = 0;
   bool pairs_ready = false;
   bool has_closure = false;
@@ -842,7 +1005,7 @@ class Engine {
     DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
                      &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active, &nonempty,
                      &tiles,  &ctr,    &ot_ptr,  &ot_col, &ot2_ptr, &ot2_col, &dt_ptr,  &dt_col, &dt_cnt,
-                     &pr_ptr, &pr_col, &pr_cnt};
+                     &pr_ptr, &pr_col, &pr_cnt, &pk_ptr, &pk_ent, &pk_cnt};
     for (DevBuf* b : all)
       if (b->p) (void)hipFree(b->p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -952,6 +1115,9 @@ class Engine {
     a.raw = {init ? P<unsigned long long>(raw_ptr) : nullptr, init ? P<uint32_t>(raw_col) : nullptr};
     a.nonempty = P<uint32_t>(nonempty);
     a.b2 = {pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr, pairs_ready ? P<uint32_t>(pr_col) : nullptr};
+    a.pk.ptr = (!init && pk_ready) ? P<unsigned long long>(pk_ptr) : nullptr;
+    a.pk.ent = P<unsigned long long>(pk_ent);
+    for (int i = 0; i < 8; ++i) a.pk.labels[i] = pack_labels[i];
     a.n = n;
     a.g = g;
     a.active = P<uint32_t>(active);
@@ -1104,6 +1270,13 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     GG_TRY(alloc(pr_cnt, NS * 4, err));
   }
   ot_nnz = 0;
+  pk_ready = false;
+  bool pack_any = false;
+  for (int i = 0; i < 8; ++i) pack_any |= pack_labels[i] != 0;
+  if (pack_any) {
+    GG_TRY(alloc(pk_ptr, (uint64_t(R) + 1) * 8, err));
+    GG_TRY(alloc(pk_cnt, uint64_t(R) * 4, err));
+  }
 
   while (dl_nnz > 0) {
     uint64_t N = 0;
@@ -1126,6 +1299,9 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_HIP(hipEventRecord(o.b, stream));
       pairs_ready = true;
     }
+    // staging: room for every wave's reservation block plus a guess of the
+    // output; an overflow grows it and re-runs the round
+    GG_TRY(grow(staging, (uint64_t(expand_grid()) * kWavesPerBlock * kStageBlock * 2 + 4 * dl_nnz) * 4, err));
     for (;;) {
       EventPair& o = mk(ev_other);
       GG_HIP(hipEventRecord(o.a, stream));
@@ -1154,9 +1330,12 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     S.candidates += h_ctr->cand;
     S.old_reads += h_ctr->oreads;
     S.new_edges += N;
-    if (verbose)
-      std::fprintf(stderr, "[gg] round %u: active %u cand %llu new %llu\n", S.rounds, h_ctr->active,
-                   (unsigned long long)h_ctr->cand, (unsigned long long)N);
+    if (verbose) {
+      float xms = 0;
+      (void)hipEventElapsedTime(&xms, ev_expand.back().a, ev_expand.back().b);
+      std::fprintf(stderr, "[gg] round %u: active %u cand %llu new %llu expand %.2f ms\n", S.rounds, h_ctr->active,
+                   (unsigned long long)h_ctr->cand, (unsigned long long)N, xms);
+    }
     // delta' from staging
     EventPair& c = mk(ev_compact);
     GG_HIP(hipEventRecord(c.a, stream));
@@ -1197,6 +1376,21 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       std::swap(ot_ptr, ot2_ptr);
       std::swap(ot_col, ot2_col);
       ot_nnz += ndt;
+    }
+    // packed form of old' for the next round
+    pk_ready = false;
+    if (pack_any && N) {
+      DPacked lab{};
+      for (int i = 0; i < 8; ++i) lab.labels[i] = pack_labels[i];
+      const DCsr oc{P<unsigned long long>(o2_ptr), P<uint32_t>(o2_col)};
+      k_pack<false><<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, n, R, lab, P<uint32_t>(pk_cnt), nullptr, nullptr);
+      GG_TRY(scan(pk_cnt, R, pk_ptr, err));
+      uint64_t npk = 0;
+      GG_TRY(read_u64(P<unsigned long long>(pk_ptr) + R, npk, err));
+      GG_TRY(grow(pk_ent, std::max<uint64_t>(npk, 1) * 8, err));
+      k_pack<true><<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, n, R, lab, P<uint32_t>(pk_cnt),
+                                                            P<unsigned long long>(pk_ptr), P<unsigned long long>(pk_ent));
+      pk_ready = true;
     }
     GG_HIP(hipEventRecord(mg.b, stream));
     GG_HIP(hipGetLastError());
@@ -1328,6 +1522,14 @@ int engine_set_grammar(Engine* e, const Grammar& gr, std::string& err) {
     g.slot_t[g.nslot] = uint8_t(tslot[B]);
     g.b_slot[i] = uint8_t(g.nslot++);
   }
+  for (int i = 0; i < 8; ++i) e->pack_labels[i] = 0;
+  auto mark_pack = [&](int l0) { e->pack_labels[l0 >> 5] |= 1u << (l0 & 31); };
+  for (int i = 0; i < g.nb; ++i) mark_pack(g.b_rhs2[i]);
+  for (int l = 1; l <= gr.num_labels(); ++l)
+    if (is_lhs[l]) mark_pack(l - 1);
+  if (const char* np = std::getenv("GG_NO_PACK"))
+    if (np[0] == '1')
+      for (int i = 0; i < 8; ++i) e->pack_labels[i] = 0;
   e->g = g;
   e->L = gr.num_labels();
   e->lhs.clear();
@@ -1477,7 +1679,7 @@ int engine_release(Engine* e) {
   DevBuf* all[] = {&e->old_ptr, &e->old_col, &e->dl_ptr,  &e->dl_col,   &e->nx_ptr, &e->nx_col,   &e->o2_ptr,
                    &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt,  &e->active, &e->nonempty, &e->tiles,
                    &e->ot_ptr,  &e->ot_col,  &e->ot2_ptr, &e->ot2_col,  &e->dt_ptr, &e->dt_col,   &e->dt_cnt,
-                   &e->pr_ptr,  &e->pr_col,  &e->pr_cnt};
+                   &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_ptr, &e->pk_ent, &e->pk_cnt};
   for (DevBuf* b : all) e->release(*b);
   e->has_closure = false;
   return GG_OK;

@@ -151,3 +151,11 @@ def test_comp_cli_report_and_dump(built, tmp_path):
     p2 = subprocess.run([gg.COMP_PATH, os.path.join(GOLDEN, "sample_grammar.txt"),
                          os.path.join(GOLDEN, "sample_graph.txt")], capture_output=True, text=True, timeout=120)
     assert p2.returncode == 0 and f"FINAL NUM EDGES: {len(d['expected'])}" in p2.stdout
+
+
+def test_gpu_unpacked_path_matches(built, monkeypatch):
+    """The same closure with packed old rows switched off (every candidate as a
+    separate id)."""
+    say("unpacked path alias n=20000")
+    monkeypatch.setenv("GG_NO_PACK", "1")
+    check_against_oracle(gen.alias(20_000, seed=12))
