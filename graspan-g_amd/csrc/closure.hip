@@ -57,6 +57,8 @@ constexpr uint32_t kChunkBits = 16384;  // LDS window per wave: 2 KiB seen + 2 K
 constexpr uint32_t kChunkWords = kChunkBits / 32;
 constexpr int kUnroll = 8;              // loads in flight per lane in the candidate stream
 constexpr uint32_t kGrab = 16;          // active rows taken per work-counter atomic
+constexpr uint32_t kXcds = 8;           // MI355X: 8 XCDs, each with its own L2
+constexpr int kExpandWaves = 4;         // k_expand occupancy target (waves per SIMD)
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
 constexpr int kMaxRules = 64;
 constexpr int kScanItems = 8;
@@ -88,7 +90,9 @@ struct Counters {
   unsigned long long checksum; // closure checksum accumulator
   unsigned int active;         // rows in the active list
   unsigned int work;           // work-stealing index
+  unsigned int work_xcd[8];    // per-XCD work indices into the active list
   unsigned int overflow;       // staging too small
+  unsigned int misses;         // rows whose guessed window missed a candidate
   unsigned int label_flags[8]; // bit l: delta of label l is not empty
   unsigned int pad;
 };
@@ -309,25 +313,27 @@ struct ActArgs {
   Counters* ctr;
 };
 
+// One thread per vertex u, looping over the lhs labels, so the active list
+// comes out vertex-major (rows of one vertex range stay together, which keeps
+// the rows they join against in one XCD's L2 when the expand kernel splits
+// the list by XCD).
 __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t total = uint64_t(a.nlhs) * a.n;
   unsigned long long pk = 0;
-  for (uint64_t base = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) & ~uint64_t(kWave - 1); base < total;
-       base += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t idx = base + lane;
-    bool act = false;
-    uint32_t r = 0;
-    if (idx < total) {
-      const uint32_t A = a.lhs[idx / a.n];
-      const uint32_t u = uint32_t(idx % a.n);
-      r = A * a.n + u;
+  for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < a.n;
+       base += gridDim.x * blockDim.x) {
+    const uint32_t u = base + lane;
+    const bool valid = u < a.n;
+    for (uint32_t li = 0; li < a.nlhs; ++li) {
+      const uint32_t A = a.lhs[li];
       bool any = false;
       for (int i = 0; i < a.g.nu; ++i)
         if (a.g.u_lhs[i] == A) any |= label_nonempty(a.ctr, a.g.u_rhs[i]);
       for (int i = 0; i < a.g.nb; ++i)
         if (a.g.b_lhs[i] == A) any |= label_nonempty(a.ctr, a.g.b_rhs1[i]) || label_nonempty(a.ctr, a.g.b_rhs2[i]);
-      if (any) {
+      if (!any) continue;
+      bool act = false;
+      if (valid) {
         for (int i = 0; i < a.g.nu; ++i) {
           if (a.g.u_lhs[i] != A) continue;
           const uint32_t rb = uint32_t(a.g.u_rhs[i]) * a.n + u;
@@ -347,13 +353,13 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
           }
         }
       }
-    }
-    unsigned long long m = __ballot(act);
-    if (m) {
-      uint32_t pos = 0;
-      if (lane == 0) pos = atomicAdd(&a.ctr->active, uint32_t(__popcll(m)));
-      pos = __shfl(pos, 0, kWave);
-      if (act) a.active[pos + __popcll(m & ((1ull << lane) - 1))] = r;
+      const unsigned long long m = __ballot(act);
+      if (m) {
+        uint32_t pos = 0;
+        if (lane == 0) pos = atomicAdd(&a.ctr->active, uint32_t(__popcll(m)));
+        pos = __shfl(pos, 0, kWave);
+        if (act) a.active[pos + __popcll(m & ((1ull << lane) - 1))] = A * a.n + u;
+      }
     }
   }
   pk = wave_sum_u64(pk);
@@ -370,21 +376,26 @@ template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr dl, DCsr ot, uint32_t* cnt,
                                                   const unsigned long long* ptr, uint32_t* col) {
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t wave = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / kWave;
-  const uint64_t nwaves = uint64_t(gridDim.x) * blockDim.x / kWave;
-  const uint64_t items = uint64_t(g.nslot) * n;
-  for (uint64_t it = wave; it < items; it += nwaves) {
-    const uint32_t j = uint32_t(it / n), v = uint32_t(it % n);
-    const uint32_t rc = uint32_t(g.slot_c[j]) * n + v;
-    if (dl.ptr[rc + 1] == dl.ptr[rc]) continue;
-    const uint32_t rt = uint32_t(g.slot_t[j]) * n + v;
-    const unsigned long long b = ot.ptr[rt], e = ot.ptr[rt + 1];
-    for (unsigned long long k = b + lane; k < e; k += kWave) {
-      const uint32_t idx = j * n + ot.col[k];
-      if (FILL)
-        col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = v;
-      else
-        atomicAdd(&cnt[idx], 1u);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (int j = 0; j < g.nslot; ++j) {
+    const uint32_t cb = uint32_t(g.slot_c[j]) * n, tb = uint32_t(g.slot_t[j]) * n;
+    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < n; base += stride) {
+      // each lane tests one vertex; the wave then walks the frontier ones
+      const uint32_t v0 = base + lane;
+      const bool fr = v0 < n && dl.ptr[cb + v0 + 1] > dl.ptr[cb + v0];
+      unsigned long long m = __ballot(fr);
+      while (m) {
+        const uint32_t v = base + uint32_t(__builtin_ctzll(m));
+        m &= m - 1;
+        const unsigned long long b = ot.ptr[tb + v], e = ot.ptr[tb + v + 1];
+        for (unsigned long long k = b + lane; k < e; k += kWave) {
+          const uint32_t idx = uint32_t(j) * n + ot.col[k];
+          if (FILL)
+            col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = v;
+          else
+            atomicAdd(&cnt[idx], 1u);
+        }
+      }
     }
   }
 }
@@ -395,19 +406,25 @@ template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_dtrans(const DevGrammar g, uint32_t n, DCsr dl, uint32_t* cnt,
                                                    const unsigned long long* ptr, uint32_t* col) {
   const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t wave = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / kWave;
-  const uint64_t nwaves = uint64_t(gridDim.x) * blockDim.x / kWave;
-  const uint64_t items = uint64_t(g.nt) * n;
-  for (uint64_t it = wave; it < items; it += nwaves) {
-    const uint32_t t = uint32_t(it / n), u = uint32_t(it % n);
-    const uint32_t r = uint32_t(g.t_label[t]) * n + u;
-    const unsigned long long b = dl.ptr[r], e = dl.ptr[r + 1];
-    for (unsigned long long k = b + lane; k < e; k += kWave) {
-      const uint32_t idx = t * n + dl.col[k];
-      if (FILL)
-        col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = u;
-      else
-        atomicAdd(&cnt[idx], 1u);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (int t = 0; t < g.nt; ++t) {
+    const uint32_t lb = uint32_t(g.t_label[t]) * n;
+    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < n; base += stride) {
+      const uint32_t u0 = base + lane;
+      const bool ne = u0 < n && dl.ptr[lb + u0 + 1] > dl.ptr[lb + u0];
+      unsigned long long m = __ballot(ne);
+      while (m) {
+        const uint32_t u = base + uint32_t(__builtin_ctzll(m));
+        m &= m - 1;
+        const unsigned long long b = dl.ptr[lb + u], e = dl.ptr[lb + u + 1];
+        for (unsigned long long k = b + lane; k < e; k += kWave) {
+          const uint32_t idx = uint32_t(t) * n + dl.col[k];
+          if (FILL)
+            col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = u;
+          else
+            atomicAdd(&cnt[idx], 1u);
+        }
+      }
     }
   }
 }
@@ -431,8 +448,10 @@ __global__ __launch_bounds__(kBlock) void k_concat(DCsr a, DCsr b, uint32_t rows
 
 // Packed form of a sorted row: one entry per non-empty 32-id word,
 // low 32 bits = word index (id >> 5), high 32 bits = the word's bit mask.
+// The packed entries of row r start at the row's own CSR offset (the entry
+// count never exceeds the id count) and len[r] says how many there are.
 struct DPacked {
-  const unsigned long long* ptr;  // rows+1, null when absent
+  const uint32_t* len;            // per row; null when absent
   const unsigned long long* ent;
   uint32_t labels[8];             // bit l: rows of label l are packed
 };
@@ -452,6 +471,7 @@ struct ExpandArgs {
   unsigned long long staging_cap;
   unsigned long long* seg_off;
   uint32_t* seg_cnt;
+  int guess;  // try a window centred on the row's vertex first
 };
 
 struct WaveLds {
@@ -566,7 +586,7 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
     const uint32_t B = a.g.b_rhs1[i], C = a.g.b_rhs2[i];
     // delta_B(u) x (old_C U delta_C)
     {
-      const bool packed = PACKED && a.pk.ptr && packed_label(a.pk, C);
+      const bool packed = PACKED && a.pk.len && packed_label(a.pk, C);
       const uint32_t rb = B * n + u;
       const unsigned long long kb = a.dl.ptr[rb], ke = a.dl.ptr[rb + 1];
       for (unsigned long long k0 = kb; k0 < ke; k0 += kWave) {
@@ -580,8 +600,8 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
           db = a.dl.ptr[rc];
           dlen = uint32_t(a.dl.ptr[rc + 1] - db);
           if (packed) {
-            pb = a.pk.ptr[rc];
-            pl = uint32_t(a.pk.ptr[rc + 1] - pb);
+            pb = ob;
+            pl = a.pk.len[rc];
           }
         }
         if (packed)
@@ -617,7 +637,9 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
 template <bool INIT>
 __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const ExpandArgs& a, uint32_t r, uint32_t A,
                                                                         uint32_t u, uint32_t c0, uint32_t span,
-                                                                        WaveLds& w, int lane, unsigned long long& kcount) {
+                                                                        WaveLds& w, int lane, unsigned long long& kcount,
+                                                                        bool& miss) {
+  bool miss_l = false;
   const uint32_t words = (span + 31) >> 5;
   for (uint32_t i = lane; i < words; i += kWave) {
     w.seen[i] = 0;
@@ -629,10 +651,10 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
       const uint32_t off = x - c0;
       if (off < span) atomicOr(&w.seen[off >> 5], 1u << (off & 31));
     };
-    if (a.pk.ptr && packed_label(a.pk, A)) {
+    if (a.pk.len && packed_label(a.pk, A)) {
       // seed old_A(u) word by word (c0 is a multiple of 32)
       const uint32_t w0 = c0 >> 5;
-      const unsigned long long pb = a.pk.ptr[r], pe = a.pk.ptr[r + 1];
+      const unsigned long long pb = a.old.ptr[r], pe = pb + a.pk.len[r];
       for (unsigned long long i = pb + lane; i < pe; i += kWave) {
         const unsigned long long e = a.pk.ent[i];
         const uint32_t wi = uint32_t(e) - w0;
@@ -655,6 +677,8 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
         const uint32_t prev = atomicOr(&w.seen[wi], m);
         if (!(prev & m)) atomicOr(&w.fresh[wi], m);
       }
+    } else {
+      miss_l = true;
     }
   };
   const uint32_t w0 = c0 >> 5;
@@ -666,6 +690,8 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
         const uint32_t nb = bits & ~atomicOr(&w.seen[wi], bits);
         if (nb) atomicOr(&w.fresh[wi], nb);
       }
+    } else {
+      miss_l = true;
     }
   };
   unsigned long long kc = 0;
@@ -680,6 +706,7 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
         expand_words(a.pk.ent, pb, pl, w, lane, mark_word);
       });
   kcount = wave_sum_u64(kc);
+  miss = __ballot(miss_l) != 0;
   wave_sync();
   uint32_t c = 0;
   for (uint32_t i = lane; i < words; i += kWave) c += __popc(w.fresh[i]);
@@ -707,7 +734,8 @@ __device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t spa
 }
 
 template <bool INIT>
-__global__ __launch_bounds__(kBlock) void k_expand(const ExpandArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandWaves, kExpandWaves))) void k_expand(
+    const ExpandArgs a) {
   __shared__ WaveLds s_lds[kWavesPerBlock];
   const int lane = threadIdx.x & (kWave - 1);
   WaveLds& w = s_lds[threadIdx.x / kWave];
@@ -730,83 +758,130 @@ __global__ __launch_bounds__(kBlock) void k_expand(const ExpandArgs a) {
     blk_pos += cnt;
     return p;
   };
+  // Workgroups are dispatched to the XCDs round-robin, so blockIdx % 8 names
+  // this block's XCD. Each XCD first drains its own eighth of the (vertex-
+  // major) active list, then helps the others.
+  uint32_t seg = blockIdx.x % kXcds, visited = 0;
   for (;;) {
     if (next >= end) {
-      uint32_t b = 0;
-      if (lane == 0) b = atomicAdd(&a.ctr->work, kGrab);
-      b = __shfl(b, 0, kWave);
-      if (b >= nactive) break;
-      next = b;
-      end = min(b + kGrab, nactive);
+      bool got = false;
+      while (visited < kXcds) {
+        const uint32_t sb = uint32_t((unsigned long long)nactive * seg / kXcds);
+        const uint32_t se = uint32_t((unsigned long long)nactive * (seg + 1) / kXcds);
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(&a.ctr->work_xcd[seg], kGrab);
+        b = sb + __shfl(b, 0, kWave);
+        if (b < se) {
+          next = b;
+          end = min(b + kGrab, se);
+          got = true;
+          break;
+        }
+        seg = (seg + 1) % kXcds;
+        ++visited;
+      }
+      if (!got) break;
     }
     const uint32_t idx = next++;
     const uint32_t r = a.active[idx];
     const uint32_t A = r / a.n, u = r - A * a.n;
-    // candidate window [lo, hi]
-    uint32_t lo = 0, hi = a.n - 1;
-    if (a.n > kChunkBits) {
-      uint32_t mn = 0xFFFFFFFFu, mx = 0;
-      unsigned long long kk = 0;
-      for_each_list<INIT, false>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
-        if (INIT) {
-          // raw rows are unsorted: reduce over every element
-          const unsigned long long b = __shfl(lb, 0, kWave);
-          const uint32_t l = __shfl(len, 0, kWave);
-          for (uint32_t t = lane; t < l; t += kWave) {
-            const uint32_t x = col[b + t];
-            mn = min(mn, x);
-            mx = max(mx, x);
-          }
-          kk += len;
-        } else if (len) {
-          mn = min(mn, col[lb]);
-          mx = max(mx, col[lb + len - 1]);
-          kk += len;
-        }
-      }, [](unsigned long long, uint32_t, uint32_t) {});
-      kk = wave_sum_u64(kk);
-      if (kk == 0) continue;
-      lo = wave_min_u32(mn) & ~31u;  // word-aligned, for packed rows
-      hi = wave_max_u32(mx);
-    }
-    const uint32_t nchunks = (hi - lo) / kChunkBits + 1;
     unsigned long long krow = 0;
     uint32_t cnt = 0;
     unsigned long long pos = 0;
-    if (nchunks == 1) {
-      cnt = process_chunk<INIT>(a, r, A, u, lo, hi - lo + 1, w, lane, krow);
-      if (cnt) {
-        pos = reserve(cnt);
-        if (pos + cnt <= a.staging_cap)
-          emit_chunk(a, lo, hi - lo + 1, pos, w, lane);
-        else if (lane == 0)
-          atomicExch(&a.ctr->overflow, 1u);
+    // attempt 0: the kChunkBits ids centred on u (rows of graphs with id
+    // locality fit; a candidate outside it sends the row to attempt 1);
+    // attempt 1: the exact candidate window, in chunks when it is wider.
+    bool empty = false;
+    for (int attempt = (!INIT && a.guess && a.n > kChunkBits) ? 0 : 1; attempt < 2; ++attempt) {
+      uint32_t lo = 0, hi = a.n - 1;
+      if (attempt == 0) {
+        const uint32_t half = kChunkBits / 2;
+        lo = (u > half ? u - half : 0u) & ~31u;
+        if (lo + kChunkBits > a.n) lo = (a.n - kChunkBits) & ~31u;
+        hi = min(a.n - 1, lo + kChunkBits - 1);
+      } else if (a.n > kChunkBits) {
+        uint32_t mn = 0xFFFFFFFFu, mx = 0;
+        unsigned long long kk = 0;
+        for_each_list<INIT, false>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
+          if (INIT) {
+            // raw rows are unsorted: reduce over every element
+            const unsigned long long b = __shfl(lb, 0, kWave);
+            const uint32_t l = __shfl(len, 0, kWave);
+            for (uint32_t t = lane; t < l; t += kWave) {
+              const uint32_t x = col[b + t];
+              mn = min(mn, x);
+              mx = max(mx, x);
+            }
+            kk += len;
+          } else if (len) {
+            mn = min(mn, col[lb]);
+            mx = max(mx, col[lb + len - 1]);
+            kk += len;
+          }
+        }, [](unsigned long long, uint32_t, uint32_t) {});
+        kk = wave_sum_u64(kk);
+        if (kk == 0) {
+          empty = true;
+          break;
+        }
+        lo = wave_min_u32(mn) & ~31u;  // word-aligned, for packed rows
+        hi = wave_max_u32(mx);
       }
-    } else {
-      for (uint32_t ch = 0; ch < nchunks; ++ch) {
-        const uint32_t c0 = lo + ch * kChunkBits;
-        const uint32_t span = min(kChunkBits, hi - c0 + 1);
-        unsigned long long kch = 0;
-        cnt += process_chunk<INIT>(a, r, A, u, c0, span, w, lane, kch);
-        if (ch == 0) krow = kch;
-      }
-      if (cnt) {
-        pos = reserve(cnt);
-        if (pos + cnt <= a.staging_cap) {
-          unsigned long long p = pos;
-          for (uint32_t ch = 0; ch < nchunks; ++ch) {
-            const uint32_t c0 = lo + ch * kChunkBits;
-            const uint32_t span = min(kChunkBits, hi - c0 + 1);
-            unsigned long long kch = 0;
-            const uint32_t c = process_chunk<INIT>(a, r, A, u, c0, span, w, lane, kch);
+      const uint32_t nchunks = (hi - lo) / kChunkBits + 1;
+      // windows wider than one chunk: a counting pass, one reservation, then
+      // a writing pass
+      bool missed = false;
+      uint32_t total = 0;
+      unsigned long long p = 0;
+      for (int pass = nchunks > 1 ? 0 : 1; pass < 2; ++pass) {
+        if (pass == 1 && nchunks > 1) {
+          cnt = total;
+          if (total) p = pos = reserve(total);
+          if (total && pos + total > a.staging_cap) {
+            if (lane == 0) atomicExch(&a.ctr->overflow, 1u);
+            break;
+          }
+          if (!total) break;
+        }
+        for (uint32_t ch = 0; ch < nchunks; ++ch) {
+          const uint32_t c0 = lo + ch * kChunkBits;
+          const uint32_t span = min(kChunkBits, hi - c0 + 1);
+          unsigned long long kch = 0;
+          bool miss = false;
+          const uint32_t c = process_chunk<INIT>(a, r, A, u, c0, span, w, lane, kch, miss);
+          if (attempt == 0 && miss) {
+            missed = true;
+            break;
+          }
+          if (pass == 0) {
+            total += c;
+            if (ch == 0) krow = kch;
+            continue;
+          }
+          if (nchunks == 1) {
+            krow = kch;
+            cnt = c;
+            if (c) {
+              pos = reserve(c);
+              if (pos + c <= a.staging_cap)
+                emit_chunk(a, c0, span, pos, w, lane);
+              else if (lane == 0)
+                atomicExch(&a.ctr->overflow, 1u);
+            }
+          } else {
             emit_chunk(a, c0, span, p, w, lane);
             p += c;
           }
-        } else if (lane == 0) {
-          atomicExch(&a.ctr->overflow, 1u);
         }
+        if (missed) break;
       }
+      if (!missed) break;
+      if (lane == 0) atomicAdd(&a.ctr->misses, 1u);
+      krow = 0;
+      cnt = 0;
+      pos = 0;
     }
+    if (empty) continue;
     if (lane == 0) {
       a.seg_off[r] = pos;
       a.seg_cnt[r] = cnt;
@@ -819,54 +894,6 @@ __global__ __launch_bounds__(kBlock) void k_expand(const ExpandArgs a) {
     if (kc) atomicAdd(&a.ctr->cand, kc);
     if (orr) atomicAdd(&a.ctr->oreads, orr);
     if (ne) atomicAdd(&a.ctr->newe, ne);
-  }
-}
-
-// Packs the rows of the labels in `labels` into (word index, mask) entries.
-// COUNT pass writes cnt[r] (0 for rows of other labels); FILL pass writes the
-// entries at ptr[r]. One wave per row; ids are gathered 64 at a time into a
-// 64-word LDS window (2048 ids) that is flushed in word order.
-template <bool FILL>
-__global__ __launch_bounds__(kBlock) void k_pack(DCsr c, uint32_t n, uint32_t rows, DPacked lab, uint32_t* cnt,
-                                                 const unsigned long long* ptr, unsigned long long* ent) {
-  __shared__ uint32_t s_win[kWavesPerBlock][kWave];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wv = threadIdx.x / kWave;
-  uint32_t* win = s_win[wv];
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  const unsigned long long below = (1ull << lane) - 1;
-  for (uint32_t r = wave; r < rows; r += nwaves) {
-    if (!packed_label(lab, r / n)) {
-      if (!FILL && lane == 0) cnt[r] = 0;
-      continue;
-    }
-    unsigned long long i = c.ptr[r];
-    const unsigned long long e = c.ptr[r + 1];
-    uint32_t written = 0;
-    while (i < e) {
-      const uint32_t c0 = c.col[i] & ~31u;
-      const unsigned long long c1 = (unsigned long long)c0 + 32u * kWave;
-      win[lane] = 0;
-      wave_sync();
-      for (;;) {
-        const unsigned long long t = i + lane;
-        const uint32_t x = t < e ? c.col[t] : 0u;
-        const bool in = t < e && x < c1;
-        if (in) atomicOr(&win[(x - c0) >> 5], 1u << (x & 31));
-        const uint32_t used = uint32_t(__popcll(__ballot(in)));
-        i += used;
-        if (used < kWave) break;
-      }
-      wave_sync();
-      const uint32_t word = win[lane];
-      const unsigned long long nz = __ballot(word != 0);
-      if (FILL && word)
-        ent[ptr[r] + written + __popcll(nz & below)] = ((unsigned long long)word << 32) | ((c0 >> 5) + uint32_t(lane));
-      written += uint32_t(__popcll(nz));
-      wave_sync();
-    }
-    if (!FILL && lane == 0) cnt[r] = written;
   }
 }
 
@@ -900,12 +927,61 @@ __device__ inline uint32_t lower_rank(const uint32_t* col, unsigned long long b,
   return uint32_t(lo - b);
 }
 
-// old' = old U delta, row by row; rowptr(old') = rowptr(old) + rowptr(delta).
-__global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t rows, uint32_t* out_col) {
+// Packs the union of the sorted id lists a[ia .. ea) and b[ib .. eb) into
+// (word index, mask) entries at ent[0 ..]; returns the entry count. Ids are
+// gathered 64 at a time into a 64-word LDS window (2048 ids) that is flushed
+// in word order.
+__device__ inline uint32_t pack_union(const uint32_t* a, unsigned long long ia, unsigned long long ea,
+                                      const uint32_t* b, unsigned long long ib, unsigned long long eb,
+                                      unsigned long long* ent, uint32_t* win, int lane) {
+  const unsigned long long below = (1ull << lane) - 1;
+  uint32_t written = 0;
+  auto absorb = [&](const uint32_t* col, unsigned long long& i, unsigned long long e, uint32_t c0,
+                    unsigned long long c1) {
+    for (;;) {
+      const unsigned long long t = i + lane;
+      const uint32_t x = t < e ? col[t] : 0u;
+      const bool in = t < e && x < c1;
+      if (in) atomicOr(&win[(x - c0) >> 5], 1u << (x & 31));
+      const uint32_t used = uint32_t(__popcll(__ballot(in)));
+      i += used;
+      if (used < kWave) break;
+    }
+  };
+  while (ia < ea || ib < eb) {
+    const uint32_t xa = ia < ea ? a[ia] : 0xFFFFFFFFu;
+    const uint32_t xb = ib < eb ? b[ib] : 0xFFFFFFFFu;
+    const uint32_t c0 = min(xa, xb) & ~31u;
+    const unsigned long long c1 = (unsigned long long)c0 + 32u * kWave;
+    win[lane] = 0;
+    wave_sync();
+    absorb(a, ia, ea, c0, c1);
+    absorb(b, ib, eb, c0, c1);
+    wave_sync();
+    const uint32_t word = win[lane];
+    const unsigned long long nz = __ballot(word != 0);
+    if (word) ent[written + __popcll(nz & below)] = ((unsigned long long)word << 32) | ((c0 >> 5) + uint32_t(lane));
+    written += uint32_t(__popcll(nz));
+    wave_sync();
+  }
+  return written;
+}
+
+// old' = old U delta, row by row, for the labels whose delta is not empty
+// (rowptr(old') = rowptr(old) + rowptr(delta)); the other labels are moved in
+// bulk by k_copy_labels. Rows of packed labels are packed after the merge
+// into out_pk at the row's new offset, entry count in pk_len[r].
+__global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t rows, uint32_t n, const Counters* ctr,
+                                                  DPacked plab, const unsigned long long* old_pk, uint32_t* out_col,
+                                                  unsigned long long* out_pk, uint32_t* pk_len) {
+  __shared__ uint32_t s_win[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
+  uint32_t* win = s_win[threadIdx.x / kWave];
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
   for (uint32_t r = wave; r < rows; r += nwaves) {
+    const uint32_t l = r / n;
+    if (!label_nonempty(ctr, l)) continue;
     const unsigned long long ob = old.ptr[r], oe = old.ptr[r + 1];
     const unsigned long long db = dl.ptr[r], de = dl.ptr[r + 1];
     uint32_t* out = out_col + ob + db;
@@ -924,6 +1000,33 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
         out[t + lower_rank(old.col, ob, oe, x)] = x;
       }
     }
+    if (out_pk && packed_label(plab, l)) {
+      if (dln == 0 && old_pk) {
+        // unchanged row: its packed entries move with it
+        const uint32_t c = pk_len[r];
+        for (uint32_t t = lane; t < c; t += kWave) out_pk[ob + db + t] = old_pk[ob + t];
+      } else {
+        const uint32_t c = pack_union(old.col, ob, oe, dl.col, db, de, out_pk + ob + db, win, lane);
+        if (lane == 0) pk_len[r] = c;
+      }
+    }
+  }
+}
+
+// Labels whose delta is empty keep their rows: their id (and packed) segment
+// moves in one piece, by the label's constant delta rowptr.
+__global__ __launch_bounds__(kBlock) void k_copy_labels(DCsr old, DCsr dl, uint32_t n, const Counters* ctr, DPacked plab,
+                                                        const unsigned long long* old_pk, uint32_t* out_col,
+                                                        unsigned long long* out_pk) {
+  const uint32_t l = blockIdx.y;
+  if (label_nonempty(ctr, l)) return;
+  const unsigned long long b = old.ptr[uint64_t(l) * n], e = old.ptr[uint64_t(l + 1) * n];
+  const unsigned long long shift = dl.ptr[uint64_t(l) * n];
+  const bool pk = out_pk && old_pk && packed_label(plab, l);
+  for (unsigned long long i = b + blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < e;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    out_col[i + shift] = old.col[i];
+    if (pk) out_pk[i + shift] = old_pk[i];
   }
 }
 
@@ -993,9 +1096,10 @@ class Engine {
   // delta, frontier pair lists
   DevBuf ot_ptr, ot_col, ot2_ptr, ot2_col, dt_ptr, dt_col, dt_cnt, pr_ptr, pr_col, pr_cnt;
   // packed old rows (labels in pack_labels)
-  DevBuf pk_ptr, pk_ent, pk_cnt;
+  DevBuf pk_len, pk_ent, pk2_ent;
   uint32_t pack_labels[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool pk_ready = false;
+  bool guess_window = true;  // per round: last round's guessed windows rarely missed
   uint64_t old_nnz = 0, dl_nnz = 0, ot_nnz = 0;
   bool pairs_ready = false;
   bool has_closure = false;
@@ -1005,7 +1109,7 @@ class Engine {
     DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
                      &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active, &nonempty,
                      &tiles,  &ctr,    &ot_ptr,  &ot_col, &ot2_ptr, &ot2_col, &dt_ptr,  &dt_col, &dt_cnt,
-                     &pr_ptr, &pr_col, &pr_cnt, &pk_ptr, &pk_ent, &pk_cnt};
+                     &pr_ptr, &pr_col, &pr_cnt, &pk_len, &pk_ent, &pk2_ent};
     for (DevBuf* b : all)
       if (b->p) (void)hipFree(b->p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -1115,7 +1219,7 @@ class Engine {
     a.raw = {init ? P<unsigned long long>(raw_ptr) : nullptr, init ? P<uint32_t>(raw_col) : nullptr};
     a.nonempty = P<uint32_t>(nonempty);
     a.b2 = {pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr, pairs_ready ? P<uint32_t>(pr_col) : nullptr};
-    a.pk.ptr = (!init && pk_ready) ? P<unsigned long long>(pk_ptr) : nullptr;
+    a.pk.len = (!init && pk_ready) ? P<uint32_t>(pk_len) : nullptr;
     a.pk.ent = P<unsigned long long>(pk_ent);
     for (int i = 0; i < 8; ++i) a.pk.labels[i] = pack_labels[i];
     a.n = n;
@@ -1126,6 +1230,7 @@ class Engine {
     a.staging_cap = staging.bytes / 4;
     a.seg_off = P<unsigned long long>(seg_off);
     a.seg_cnt = P<uint32_t>(seg_cnt);
+    a.guess = guess_window ? 1 : 0;
     return a;
   }
 
@@ -1271,11 +1376,12 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   }
   ot_nnz = 0;
   pk_ready = false;
+  guess_window = true;
   bool pack_any = false;
   for (int i = 0; i < 8; ++i) pack_any |= pack_labels[i] != 0;
   if (pack_any) {
-    GG_TRY(alloc(pk_ptr, (uint64_t(R) + 1) * 8, err));
-    GG_TRY(alloc(pk_cnt, uint64_t(R) * 4, err));
+    GG_TRY(alloc(pk_len, uint64_t(R) * 4, err));
+    fill_u32(pk_len, 0, R);
   }
 
   while (dl_nnz > 0) {
@@ -1288,13 +1394,13 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       const DCsr dlc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
       const DCsr otc{P<unsigned long long>(ot_ptr), P<uint32_t>(ot_col)};
       fill_u32(pr_cnt, 0, NS);
-      k_pairs<false><<<grid_for(NS * kWave), kBlock, 0, stream>>>(g, n, dlc, otc, P<uint32_t>(pr_cnt), nullptr, nullptr);
+      k_pairs<false><<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, otc, P<uint32_t>(pr_cnt), nullptr, nullptr);
       GG_TRY(scan(pr_cnt, NS, pr_ptr, err));
       uint64_t npairs = 0;
       GG_TRY(read_u64(P<unsigned long long>(pr_ptr) + NS, npairs, err));
       GG_TRY(grow(pr_col, std::max<uint64_t>(npairs, 1) * 4, err));
       fill_u32(pr_cnt, 0, NS);
-      k_pairs<true><<<grid_for(NS * kWave), kBlock, 0, stream>>>(g, n, dlc, otc, P<uint32_t>(pr_cnt),
+      k_pairs<true><<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, otc, P<uint32_t>(pr_cnt),
                                                                 P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
       GG_HIP(hipEventRecord(o.b, stream));
       pairs_ready = true;
@@ -1311,7 +1417,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       act.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
       act.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
       act.b2ptr = pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr;
-      if (act_work) k_activate<<<grid_for(act_work), kBlock, 0, stream>>>(act);
+      if (act_work) k_activate<<<grid_for(n), kBlock, 0, stream>>>(act);
       GG_HIP(hipEventRecord(o.b, stream));
       EventPair& x = mk(ev_expand);
       GG_HIP(hipEventRecord(x.a, stream));
@@ -1325,6 +1431,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
     }
     N = h_ctr->newe;
+    guess_window = h_ctr->misses * 8u <= h_ctr->active;
     S.rounds++;
     S.probe_reads += h_ctr->pkeys + h_ctr->cand;
     S.candidates += h_ctr->cand;
@@ -1353,19 +1460,33 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     GG_TRY(grow(o2_col, std::max<uint64_t>(o2n, 1) * 4, err));
     k_add_ptr<<<grid_for(uint64_t(R) + 1), kBlock, 0, stream>>>(P<unsigned long long>(o2_ptr), P<unsigned long long>(old_ptr),
                                                                P<unsigned long long>(dl_ptr), uint64_t(R) + 1);
-    k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>({P<unsigned long long>(old_ptr), P<uint32_t>(old_col)},
-                                                      {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)}, R,
-                                                      P<uint32_t>(o2_col));
+    {
+      DPacked plab{};
+      for (int i = 0; i < 8; ++i) plab.labels[i] = pack_labels[i];
+      unsigned long long* opk = nullptr;
+      if (pack_any) {
+        GG_TRY(grow(pk2_ent, std::max<uint64_t>(o2n, 1) * 8, err));
+        opk = P<unsigned long long>(pk2_ent);
+      }
+      const DCsr oc{P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
+      const DCsr dc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
+      k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, dc, R, n, C(), plab,
+                                                        pk_ready ? P<unsigned long long>(pk_ent) : nullptr,
+                                                        P<uint32_t>(o2_col), opk, P<uint32_t>(pk_len));
+      if (old_nnz)
+        k_copy_labels<<<dim3(uint32_t(cus), uint32_t(L)), kBlock, 0, stream>>>(
+            oc, dc, n, C(), plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr, P<uint32_t>(o2_col), opk);
+    }
     if (NT && N) {
       const DCsr dlc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
       fill_u32(dt_cnt, 0, NT);
-      k_dtrans<false><<<grid_for(NT * kWave), kBlock, 0, stream>>>(g, n, dlc, P<uint32_t>(dt_cnt), nullptr, nullptr);
+      k_dtrans<false><<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, P<uint32_t>(dt_cnt), nullptr, nullptr);
       GG_TRY(scan(dt_cnt, NT, dt_ptr, err));
       uint64_t ndt = 0;
       GG_TRY(read_u64(P<unsigned long long>(dt_ptr) + NT, ndt, err));
       GG_TRY(grow(dt_col, std::max<uint64_t>(ndt, 1) * 4, err));
       fill_u32(dt_cnt, 0, NT);
-      k_dtrans<true><<<grid_for(NT * kWave), kBlock, 0, stream>>>(g, n, dlc, P<uint32_t>(dt_cnt),
+      k_dtrans<true><<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, P<uint32_t>(dt_cnt),
                                                                  P<unsigned long long>(dt_ptr), P<uint32_t>(dt_col));
       GG_TRY(grow(ot2_col, std::max<uint64_t>(ot_nnz + ndt, 1) * 4, err));
       k_add_ptr<<<grid_for(NT + 1), kBlock, 0, stream>>>(P<unsigned long long>(ot2_ptr), P<unsigned long long>(ot_ptr),
@@ -1377,21 +1498,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       std::swap(ot_col, ot2_col);
       ot_nnz += ndt;
     }
-    // packed form of old' for the next round
-    pk_ready = false;
-    if (pack_any && N) {
-      DPacked lab{};
-      for (int i = 0; i < 8; ++i) lab.labels[i] = pack_labels[i];
-      const DCsr oc{P<unsigned long long>(o2_ptr), P<uint32_t>(o2_col)};
-      k_pack<false><<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, n, R, lab, P<uint32_t>(pk_cnt), nullptr, nullptr);
-      GG_TRY(scan(pk_cnt, R, pk_ptr, err));
-      uint64_t npk = 0;
-      GG_TRY(read_u64(P<unsigned long long>(pk_ptr) + R, npk, err));
-      GG_TRY(grow(pk_ent, std::max<uint64_t>(npk, 1) * 8, err));
-      k_pack<true><<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, n, R, lab, P<uint32_t>(pk_cnt),
-                                                            P<unsigned long long>(pk_ptr), P<unsigned long long>(pk_ent));
-      pk_ready = true;
-    }
+    pk_ready = pack_any;
+    if (pack_any) std::swap(pk_ent, pk2_ent);
     GG_HIP(hipEventRecord(mg.b, stream));
     GG_HIP(hipGetLastError());
     std::swap(old_ptr, o2_ptr);
@@ -1679,7 +1787,7 @@ int engine_release(Engine* e) {
   DevBuf* all[] = {&e->old_ptr, &e->old_col, &e->dl_ptr,  &e->dl_col,   &e->nx_ptr, &e->nx_col,   &e->o2_ptr,
                    &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt,  &e->active, &e->nonempty, &e->tiles,
                    &e->ot_ptr,  &e->ot_col,  &e->ot2_ptr, &e->ot2_col,  &e->dt_ptr, &e->dt_col,   &e->dt_cnt,
-                   &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_ptr, &e->pk_ent, &e->pk_cnt};
+                   &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_len, &e->pk_ent, &e->pk2_ent};
   for (DevBuf* b : all) e->release(*b);
   e->has_closure = false;
   return GG_OK;
