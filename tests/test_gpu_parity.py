@@ -159,3 +159,10 @@ def test_gpu_unpacked_path_matches(built, monkeypatch):
     say("unpacked path alias n=20000")
     monkeypatch.setenv("GG_NO_PACK", "1")
     check_against_oracle(gen.alias(20_000, seed=12))
+
+
+def test_gpu_alias_1m_full_size(built):
+    """The benchmark workload itself (alias generator, n = 1,000,000): per-label
+    counts, checksum, round count and the P/K/O/N counters equal the oracle's."""
+    say("alias n=1000000 (benchmark workload)")
+    check_against_oracle(gen.alias(1_000_000, seed=0), full_triples=False)
