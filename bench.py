@@ -76,6 +76,11 @@ def main():
     ap.add_argument("--n", type=int, default=WORKLOAD_N, help="vertex count (default: the 1M-vertex workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="per-round engine statistics on stderr")
+    ap.add_argument("--mode", choices=("single", "batch"), default="single",
+                    help="single: the 1M-vertex alias closure (default); batch: graphs/s of the batched closure "
+                         "against the same engine one graph at a time")
+    ap.add_argument("--batch-stride", type=int, default=16,
+                    help="batch mode: close every k-th graph of the 4096-graph set")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE against known bytes) and exit")
@@ -96,6 +101,9 @@ def main():
 
     import graspan_g as gg
     from graspan_g import gen
+
+    if args.mode == "batch":
+        return bench_batch(args)
 
     t = time.time()
     g = gen.alias(args.n, seed=0)
@@ -148,6 +156,68 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on the n=100000 sample)")
         line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+def bench_batch(args):
+    """graphs/s of the batched closure (every graph of the sample in one
+    launch group, one launch sequence per round) against the same kernels run
+    one graph at a time, on a uniform sample of the 4096-graph set (vertex
+    counts log-uniform in [500, 200000], alternating dataflow / alias)."""
+    import graspan_g as gg
+    from graspan_g import gen
+
+    t = time.time()
+    sizes = gen.log_uniform_sizes(4096, 500, 200_000, seed=0)
+    idx = list(range(0, 4096, args.batch_stride))
+    graphs = [gen.alias_as_dataflow(int(sizes[i]), seed=i) if i % 2 == 0 else gen.alias(int(sizes[i]), seed=i)
+              for i in idx]
+    nv = sum(g.n for g in graphs)
+    log(f"generated {len(graphs)} graphs ({nv} vertices) in {time.time() - t:.1f}s")
+    ctx = gg.Context()
+    b = gg.Batch(ctx)
+    for g in graphs:
+        b.add_spec(g)
+    b.run()  # warmup
+    best_b = None
+    for i in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        st = b.run()
+        dt = time.perf_counter() - t0
+        best_b = dt if best_b is None else min(best_b, dt)
+        log(f"batched step {i}: {dt * 1e3:.1f} ms, {st.rounds} rounds, closure {st.total_edges}")
+    batched = [b.result(i) for i in range(len(graphs))]
+    one = gg.Context()
+    t0 = time.perf_counter()
+    single = []
+    for g in graphs:
+        one.set_spec(g)
+        one.run(0)
+        single.append((one.counts()[1], one.checksum()))
+    dt_s = time.perf_counter() - t0
+    log(f"one graph at a time: {dt_s * 1e3:.1f} ms")
+    for i, (tot, cs) in enumerate(single):
+        if (tot, cs) != (batched[i][1], batched[i][2]):
+            raise SystemExit(f"graph {idx[i]}: batched result differs from the single-graph result")
+    line = {
+        "metric": "graphs/sec (batched closure)",
+        "value": len(graphs) / best_b,
+        "unit": "graphs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": best_b * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"batched closure of graphs {idx[0]}, {idx[1]}, ... (every {args.batch_stride}th) "
+                               f"of the 4096-graph set: {len(graphs)} graphs, {nv} vertices",
+                   "one_at_a_time_graphs_per_s": len(graphs) / dt_s,
+                   "speedup_vs_one_at_a_time": dt_s / best_b,
+                   "closure_edges": int(sum(x[1] for x in batched))},
+    }
     print(json.dumps(line), flush=True)
 
 

@@ -1,0 +1,242 @@
+// Batched closure of many independent graphs (gg_batch_*; no counterpart in
+// the reference, which closes one graph per process).
+//
+// The graphs of one launch group are laid side by side: graph g's vertices
+// are shifted by v0[g], and each distinct grammar (same rule triples and label
+// count) gets its own range of combined label ids. Eps rules are replaced by
+// explicit self-loop edges on the vertices of the graphs that use them, so
+// the combined grammar has only unary and binary rules and no rule can touch
+// another graph's vertices or labels. The closure of this disjoint union is
+// the disjoint union of the per-graph closures, computed by one engine run
+// (one launch sequence per semi-naive round for the whole group).
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "formats.h"
+#include "graspan_g.h"
+
+struct gg_ctx;
+namespace gg {
+Engine* ctx_engine(gg_ctx* c, int& status, std::string*& err);
+}
+
+struct gg_batch {
+  gg_ctx* ctx = nullptr;
+  struct G {
+    std::vector<int32_t> rules;
+    int L = 0;
+    uint32_t n = 0;
+    std::vector<uint32_t> src, dst;
+    std::vector<uint8_t> lab;
+    // results
+    std::vector<uint64_t> counts;
+    uint64_t total = 0, checksum = 0;
+    std::vector<uint32_t> triples;
+    bool done = false;
+  };
+  std::vector<G> graphs;
+  int groups = 0;
+  std::string err;
+};
+
+namespace {
+
+int fail(gg_batch* b, int st, const std::string& m) {
+  b->err = m;
+  return st;
+}
+
+void add_stats(gg_stats& a, const gg_stats& b) {
+  a.rounds = std::max(a.rounds, b.rounds);
+  a.retries += b.retries;
+  a.input_unique += b.input_unique;
+  a.total_edges += b.total_edges;
+  a.fixpoint_ms += b.fixpoint_ms;
+  a.expand_ms += b.expand_ms;
+  a.compact_ms += b.compact_ms;
+  a.merge_ms += b.merge_ms;
+  a.other_ms += b.other_ms;
+  a.expand_launches += b.expand_launches;
+  a.probe_reads += b.probe_reads;
+  a.candidates += b.candidates;
+  a.old_reads += b.old_reads;
+  a.new_edges += b.new_edges;
+  a.device_bytes_peak = std::max(a.device_bytes_peak, b.device_bytes_peak);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gg_batch_create(gg_ctx* ctx, gg_batch** out) {
+  if (!ctx || !out) return GG_E_ARG;
+  auto* b = new gg_batch();
+  b->ctx = ctx;
+  *out = b;
+  return GG_OK;
+}
+
+int gg_batch_destroy(gg_batch* b) {
+  if (!b) return GG_E_ARG;
+  delete b;
+  return GG_OK;
+}
+
+int gg_batch_add(gg_batch* b, int num_rules, const int32_t* rules, int num_labels, uint32_t n, uint64_t m,
+                 const uint32_t* src, const uint32_t* dst, const uint8_t* lab, int* graph_id) {
+  if (!b) return GG_E_ARG;
+  gg::Grammar tmp;
+  int s = gg::grammar_from_rules(num_rules, rules, num_labels, tmp, b->err);
+  if (s != GG_OK) return s;
+  if (n == 0) return fail(b, GG_E_ARG, "graph has no vertices");
+  if (m && (!src || !dst || !lab)) return fail(b, GG_E_ARG, "null edge array");
+  for (uint64_t i = 0; i < m; ++i) {
+    if (src[i] >= n || dst[i] >= n) return fail(b, GG_E_ARG, "edge " + std::to_string(i) + ": vertex out of range");
+    if (lab[i] < 1 || lab[i] > num_labels) return fail(b, GG_E_ARG, "edge " + std::to_string(i) + ": bad label");
+  }
+  gg_batch::G g;
+  g.rules.assign(rules, rules + 3 * num_rules);
+  g.L = num_labels;
+  g.n = n;
+  g.src.assign(src, src + m);
+  g.dst.assign(dst, dst + m);
+  g.lab.assign(lab, lab + m);
+  b->graphs.push_back(std::move(g));
+  if (graph_id) *graph_id = int(b->graphs.size() - 1);
+  return GG_OK;
+}
+
+int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) {
+  if (!b) return GG_E_ARG;
+  int st = GG_OK;
+  std::string* cerr = nullptr;
+  gg::Engine* e = gg::ctx_engine(b->ctx, st, cerr);
+  if (!e) return fail(b, st, cerr ? *cerr : "no device");
+  gg_stats total{};
+  b->groups = 0;
+  const size_t G = b->graphs.size();
+  size_t i = 0;
+  while (i < G) {
+    // one launch group: graphs i..j-1
+    std::map<std::pair<int, std::vector<int32_t>>, uint32_t> grammar_base;
+    uint32_t labels = 0;
+    uint64_t verts = 0;
+    size_t j = i;
+    for (; j < G; ++j) {
+      const auto& g = b->graphs[j];
+      auto key = std::make_pair(g.L, g.rules);
+      const bool fresh = grammar_base.find(key) == grammar_base.end();
+      const uint32_t nl = labels + (fresh ? uint32_t(g.L) : 0u);
+      const uint64_t nv = verts + g.n;
+      const uint64_t lim = nl ? (0xFFFFFFFEull / nl) : nv;
+      const bool over = nl > uint32_t(gg::kMaxLabels) || nv > lim || (max_vertices && nv > max_vertices);
+      if (over && j > i) break;
+      if (over) return fail(b, GG_E_UNSUPPORTED, "graph " + std::to_string(j) + " alone exceeds the launch limits");
+      if (fresh) {
+        grammar_base[key] = labels;
+        labels = nl;
+      }
+      verts = nv;
+    }
+    // combined grammar
+    std::vector<int32_t> rules;
+    for (auto& kv : grammar_base) {
+      const auto& r = kv.first.second;
+      for (size_t k = 0; k + 2 < r.size(); k += 3) {
+        if (r[k + 1] == 0) continue;  // eps: explicit loops below
+        rules.push_back(r[k] + int32_t(kv.second));
+        rules.push_back(r[k + 1] + int32_t(kv.second));
+        rules.push_back(r[k + 2] ? r[k + 2] + int32_t(kv.second) : 0);
+      }
+    }
+    gg::Grammar cg;
+    int s = gg::grammar_from_rules(int(rules.size() / 3), rules.data(), int(std::max(labels, 1u)), cg, b->err);
+    if (s != GG_OK) return s;
+    // super graph
+    gg::Graph sg;
+    sg.n = uint32_t(verts);
+    std::vector<uint32_t> v0(1, 0), l0, lg;
+    for (size_t k = i; k < j; ++k) {
+      const auto& g = b->graphs[k];
+      const uint32_t base = v0.back(), lb = grammar_base[std::make_pair(g.L, g.rules)];
+      for (size_t e2 = 0; e2 < g.src.size(); ++e2) {
+        sg.src.push_back(g.src[e2] + base);
+        sg.dst.push_back(g.dst[e2] + base);
+        sg.lab.push_back(uint8_t(g.lab[e2] + lb));
+      }
+      for (size_t r = 0; r + 2 < g.rules.size(); r += 3) {
+        if (g.rules[r + 1] != 0) continue;
+        for (uint32_t v = 0; v < g.n; ++v) {
+          sg.src.push_back(base + v);
+          sg.dst.push_back(base + v);
+          sg.lab.push_back(uint8_t(g.rules[r] + int32_t(lb)));
+        }
+      }
+      l0.push_back(lb);
+      lg.push_back(uint32_t(g.L));
+      v0.push_back(base + g.n);
+    }
+    if ((s = gg::engine_set_grammar(e, cg, b->err)) != GG_OK) return s;
+    if ((s = gg::engine_set_graph(e, sg, b->err)) != GG_OK) return s;
+    gg_stats one{};
+    if ((s = gg::engine_run(e, 0, &one, b->err)) != GG_OK) return s;
+    add_stats(total, one);
+    std::vector<uint64_t> counts, sums;
+    if ((s = gg::engine_slices(e, v0, l0, lg, counts, sums, b->err)) != GG_OK) return s;
+    size_t c = 0;
+    for (size_t k = i; k < j; ++k) {
+      auto& g = b->graphs[k];
+      g.counts.assign(counts.begin() + long(c), counts.begin() + long(c + size_t(g.L)));
+      c += size_t(g.L);
+      g.total = 0;
+      for (uint64_t x : g.counts) g.total += x;
+      g.checksum = sums[k - i];
+      g.triples.clear();
+      if (keep) {
+        g.triples.resize(size_t(g.total) * 3);
+        uint64_t cnt = 0;
+        if ((s = gg::engine_export_slice(e, v0[k - i], v0[k - i + 1], l0[k - i], lg[k - i], g.triples.data(), g.total,
+                                         &cnt, b->err)) != GG_OK)
+          return s;
+      }
+      g.done = true;
+    }
+    ++b->groups;
+    i = j;
+  }
+  if (stats) *stats = total;
+  return GG_OK;
+}
+
+int gg_batch_info(const gg_batch* b, int* ng, int* ngroups) {
+  if (!b) return GG_E_ARG;
+  if (ng) *ng = int(b->graphs.size());
+  if (ngroups) *ngroups = b->groups;
+  return GG_OK;
+}
+
+int gg_batch_result(const gg_batch* b, int id, uint64_t* per_label, uint64_t* total, uint64_t* checksum) {
+  if (!b || id < 0 || size_t(id) >= b->graphs.size()) return GG_E_ARG;
+  const auto& g = b->graphs[size_t(id)];
+  if (!g.done) return GG_E_STATE;
+  if (per_label)
+    for (int l = 0; l < g.L; ++l) per_label[l] = g.counts[size_t(l)];
+  if (total) *total = g.total;
+  if (checksum) *checksum = g.checksum;
+  return GG_OK;
+}
+
+int gg_batch_export(const gg_batch* b, int id, uint32_t* buf, uint64_t cap, uint64_t* count) {
+  if (!b || id < 0 || size_t(id) >= b->graphs.size()) return GG_E_ARG;
+  const auto& g = b->graphs[size_t(id)];
+  if (!g.done || g.triples.size() != size_t(g.total) * 3) return GG_E_STATE;
+  if (count) *count = g.total;
+  if (!buf || cap < g.total) return GG_E_ARG;
+  std::copy(g.triples.begin(), g.triples.end(), buf);
+  return GG_OK;
+}
+
+}  // extern "C"

@@ -11,7 +11,6 @@
 //   old, delta : rowptr uint64[L*n + 1] + col uint32[], rows sorted/unique.
 //   staging    : uint32[] bump-allocated output of the expand kernel.
 //   seg_off/seg_cnt : per-row (offset, count) of that output.
-//   nonempty   : one bit per row, set when the delta row is not empty.
 //
 // One round (rule semantics of src/kernel.cu:403-464; end-of-step update of
 // src/kernel.cu:943-985):
@@ -228,28 +227,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* in, uint6
   }
 }
 
-// Bit r of `bits` = delta row r is not empty.
-__global__ void k_nonempty(const unsigned long long* ptr, uint64_t rows, uint32_t* bits) {
-  const int lane = threadIdx.x & (kWave - 1);
-  for (uint64_t base = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) & ~uint64_t(kWave - 1); base < rows;
-       base += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t r = base + lane;
-    const bool ne = r < rows && ptr[r + 1] > ptr[r];
-    const unsigned long long m = __ballot(ne);
-    if (lane == 0) {
-      bits[base / 32] = uint32_t(m);
-      bits[base / 32 + 1] = uint32_t(m >> 32);
-    }
-  }
-}
-
 // label_flags bit l = the delta of label l is not empty.
 __global__ void k_label_flags(const unsigned long long* ptr, uint32_t n, uint32_t L, Counters* ctr) {
   for (uint32_t l = threadIdx.x; l < L; l += blockDim.x)
     if (ptr[uint64_t(l + 1) * n] > ptr[uint64_t(l) * n]) atomicOr(&ctr->label_flags[l >> 5], 1u << (l & 31));
 }
 
-__device__ inline bool row_nonempty(const uint32_t* bits, uint32_t r) { return (bits[r >> 5] >> (r & 31)) & 1u; }
 
 // ---------------------------------------------------------------------------
 // initial delta: raw rows from input COO + eps loops
@@ -462,7 +445,6 @@ struct ExpandArgs {
   DCsr old, dl, raw;
   DPacked pk;  // packed old rows
   DCsr b2;  // rows slot*n + u: keys v with old_B(u,v) and delta_C(v) non-empty
-  const uint32_t* nonempty;
   uint32_t n;
   DevGrammar g;
   const uint32_t* active;
@@ -1046,6 +1028,48 @@ __global__ __launch_bounds__(kBlock) void k_checksum(DCsr c, uint32_t rows, uint
 }
 
 // ---------------------------------------------------------------------------
+// per-graph results of a batched closure: graph g owns vertices
+// [v0[g], v0[g+1]) and the combined labels [l0[g], l0[g] + lg[g]); in its own
+// numbering a vertex is (id - v0[g]) and a label (combined - l0[g] + 1).
+
+struct DSlices {
+  const uint32_t* v0;    // G + 1
+  const uint32_t* vg;    // graph of every vertex
+  const uint32_t* l0;    // G
+  const uint32_t* lg;    // G
+  const uint32_t* coff;  // G: offset of graph g's per-label counts
+  uint32_t G;
+};
+
+__global__ void k_slice_counts(const unsigned long long* ptr, uint32_t n, DSlices s, unsigned long long* out) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < s.G; g += gridDim.x * blockDim.x) {
+    const uint32_t a = s.v0[g], b = s.v0[g + 1];
+    for (uint32_t i = 0; i < s.lg[g]; ++i) {
+      const uint64_t base = uint64_t(s.l0[g] + i) * n;
+      out[s.coff[g] + i] = ptr[base + b] - ptr[base + a];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_slice_checksum(DCsr c, uint32_t rows, uint32_t n, DSlices s,
+                                                           unsigned long long* sums) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  for (uint32_t r = wave; r < rows; r += nwaves) {
+    const unsigned long long b = c.ptr[r], e = c.ptr[r + 1];
+    if (b == e) continue;
+    const uint32_t l = r / n, u = r - l * n;
+    const uint32_t g = s.vg[u];
+    const uint32_t base = s.v0[g], lab = l - s.l0[g] + 1;
+    unsigned long long h = 0;
+    for (unsigned long long i = b + lane; i < e; i += kWave) h += triple_hash(u - base, c.col[i] - base, lab);
+    h = wave_sum_u64(h);
+    if (lane == 0) atomicAdd(&sums[g], h);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 #define GG_HIP(expr)                                                                           \
@@ -1091,7 +1115,7 @@ class Engine {
 
   // closure state
   DevBuf old_ptr, old_col, dl_ptr, dl_col, nx_ptr, nx_col, o2_ptr, o2_col;
-  DevBuf raw_ptr, raw_col, staging, seg_off, seg_cnt, active, nonempty, tiles, ctr;
+  DevBuf raw_ptr, raw_col, staging, seg_off, seg_cnt, active, tiles, ctr;
   // transpose of old over the transposed labels (+ ping-pong), transpose of
   // delta, frontier pair lists
   DevBuf ot_ptr, ot_col, ot2_ptr, ot2_col, dt_ptr, dt_col, dt_cnt, pr_ptr, pr_col, pr_cnt;
@@ -1107,7 +1131,7 @@ class Engine {
 
   ~Engine() {
     DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
-                     &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active, &nonempty,
+                     &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active,
                      &tiles,  &ctr,    &ot_ptr,  &ot_col, &ot2_ptr, &ot2_col, &dt_ptr,  &dt_col, &dt_cnt,
                      &pr_ptr, &pr_col, &pr_cnt, &pk_len, &pk_ent, &pk2_ent};
     for (DevBuf* b : all)
@@ -1217,7 +1241,6 @@ class Engine {
     a.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
     a.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
     a.raw = {init ? P<unsigned long long>(raw_ptr) : nullptr, init ? P<uint32_t>(raw_col) : nullptr};
-    a.nonempty = P<uint32_t>(nonempty);
     a.b2 = {pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr, pairs_ready ? P<uint32_t>(pr_col) : nullptr};
     a.pk.len = (!init && pk_ready) ? P<uint32_t>(pk_len) : nullptr;
     a.pk.ent = P<unsigned long long>(pk_ent);
@@ -1265,7 +1288,6 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   GG_HIP(hipSetDevice(device));
   gg_stats S{};
   const uint32_t R = rows();
-  const uint64_t words = (uint64_t(R) + 63) / 64 * 2;
   std::vector<EventPair> ev_expand, ev_compact, ev_merge, ev_other;
   auto mk = [&](std::vector<EventPair>& v) -> EventPair& {
     v.emplace_back();
@@ -1292,7 +1314,6 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   GG_TRY(alloc(seg_off, uint64_t(R) * 8, err));
   GG_TRY(alloc(seg_cnt, uint64_t(R) * 4, err));
   GG_TRY(alloc(active, uint64_t(R) * 4, err));
-  GG_TRY(alloc(nonempty, words * 4, err));
   if (!h_ctr) GG_HIP(hipHostMalloc((void**)&h_ctr, sizeof(Counters), hipHostMallocDefault));
   if (!h_u64) GG_HIP(hipHostMalloc((void**)&h_u64, 8, hipHostMallocDefault));
 
@@ -1711,64 +1732,137 @@ int engine_degrees(Engine* e, uint64_t* deg, std::string& err) {
   return GG_OK;
 }
 
-int engine_export(Engine* e, uint32_t* buf, uint64_t cap, uint64_t* count, std::string& err) {
-  GG_TRY(need_closure(e, err));
+// Triples of vertices [v0, v1) and combined labels [l0, l0+lg), renumbered to
+// the slice (vertex - v0, label - l0 + 1), sorted by (src, dst, label).
+static int export_slice(Engine* e, uint32_t v0, uint32_t v1, uint32_t l0, uint32_t lg, uint32_t* buf, uint64_t cap,
+                        uint64_t* count, std::string& err) {
   GG_HIP(hipSetDevice(e->device));
-  const uint64_t R = e->rows();
-  const uint32_t n = e->n;
-  const int L = e->L;
-  std::vector<unsigned long long> ptr(R + 1);
-  GG_HIP(hipMemcpy(ptr.data(), e->old_ptr.p, (R + 1) * 8, hipMemcpyDeviceToHost));
-  const uint64_t total = ptr[R];
+  const uint32_t n = e->n, ns = v1 - v0;
+  // row pointers of the slice's rows, label by label
+  std::vector<unsigned long long> ptr(size_t(lg) * (ns + 1));
+  for (uint32_t l = 0; l < lg; ++l)
+    GG_HIP(hipMemcpy(ptr.data() + size_t(l) * (ns + 1),
+                     static_cast<unsigned long long*>(e->old_ptr.p) + uint64_t(l0 + l) * n + v0, (ns + 1) * 8,
+                     hipMemcpyDeviceToHost));
+  uint64_t total = 0;
+  for (uint32_t l = 0; l < lg; ++l) total += ptr[size_t(l) * (ns + 1) + ns] - ptr[size_t(l) * (ns + 1)];
   if (count) *count = total;
   if (buf == nullptr || cap < total) {
     err = "export buffer too small: need " + std::to_string(total) + " triples";
     return GG_E_ARG;
   }
-  std::vector<uint32_t> col(total);
-  if (total) GG_HIP(hipMemcpy(col.data(), e->old_col.p, total * 4, hipMemcpyDeviceToHost));
-  std::vector<uint64_t> off(size_t(n) + 1, 0);
-  for (uint32_t u = 0; u < n; ++u) {
+  // ids of each label's rows (contiguous per label)
+  std::vector<std::vector<uint32_t>> col(lg);
+  for (uint32_t l = 0; l < lg; ++l) {
+    const unsigned long long b = ptr[size_t(l) * (ns + 1)], en = ptr[size_t(l) * (ns + 1) + ns];
+    col[l].resize(en - b);
+    if (en > b)
+      GG_HIP(hipMemcpy(col[l].data(), static_cast<uint32_t*>(e->old_col.p) + b, (en - b) * 4, hipMemcpyDeviceToHost));
+  }
+  std::vector<uint64_t> off(size_t(ns) + 1, 0);
+  for (uint32_t u = 0; u < ns; ++u) {
     uint64_t s = 0;
-    for (int l = 0; l < L; ++l) s += ptr[uint64_t(l) * n + u + 1] - ptr[uint64_t(l) * n + u];
+    for (uint32_t l = 0; l < lg; ++l) s += ptr[size_t(l) * (ns + 1) + u + 1] - ptr[size_t(l) * (ns + 1) + u];
     off[u + 1] = off[u] + s;
   }
-  // per vertex: merge the L sorted label rows by (dst, label)
+  // per vertex: merge the lg sorted label rows by (dst, label)
   auto work = [&](uint32_t u0, uint32_t u1) {
-    std::vector<uint64_t> pos(L), end(L);
+    std::vector<uint64_t> pos(lg), end(lg);
     for (uint32_t u = u0; u < u1; ++u) {
-      for (int l = 0; l < L; ++l) {
-        pos[l] = ptr[uint64_t(l) * n + u];
-        end[l] = ptr[uint64_t(l) * n + u + 1];
+      for (uint32_t l = 0; l < lg; ++l) {
+        const unsigned long long lb = ptr[size_t(l) * (ns + 1)];
+        pos[l] = ptr[size_t(l) * (ns + 1) + u] - lb;
+        end[l] = ptr[size_t(l) * (ns + 1) + u + 1] - lb;
       }
       uint32_t* o = buf + 3 * off[u];
       for (;;) {
         int best = -1;
         uint32_t bd = 0;
-        for (int l = 0; l < L; ++l) {
-          if (pos[l] < end[l] && (best < 0 || col[pos[l]] < bd)) {
-            best = l;
-            bd = col[pos[l]];
+        for (uint32_t l = 0; l < lg; ++l) {
+          if (pos[l] < end[l] && (best < 0 || col[l][pos[l]] < bd)) {
+            best = int(l);
+            bd = col[l][pos[l]];
           }
         }
         if (best < 0) break;
         o[0] = u;
-        o[1] = bd;
+        o[1] = bd - v0;
         o[2] = uint32_t(best + 1);
         o += 3;
-        ++pos[best];
+        ++pos[size_t(best)];
       }
     }
   };
   const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   std::vector<std::thread> th;
-  const uint32_t per = (n + nt - 1) / nt;
+  const uint32_t per = (ns + nt - 1) / nt;
   for (unsigned t = 0; t < nt; ++t) {
-    const uint32_t a = std::min<uint64_t>(n, uint64_t(t) * per), b = std::min<uint64_t>(n, uint64_t(t + 1) * per);
+    const uint32_t a = std::min<uint64_t>(ns, uint64_t(t) * per), b = std::min<uint64_t>(ns, uint64_t(t + 1) * per);
     if (a < b) th.emplace_back(work, a, b);
   }
   for (auto& t : th) t.join();
   return GG_OK;
+}
+
+int engine_export(Engine* e, uint32_t* buf, uint64_t cap, uint64_t* count, std::string& err) {
+  GG_TRY(need_closure(e, err));
+  return export_slice(e, 0, e->n, 0, uint32_t(e->L), buf, cap, count, err);
+}
+
+int engine_slices(Engine* e, const std::vector<uint32_t>& v0, const std::vector<uint32_t>& l0,
+                  const std::vector<uint32_t>& lg, std::vector<uint64_t>& counts, std::vector<uint64_t>& sums,
+                  std::string& err) {
+  GG_TRY(need_closure(e, err));
+  GG_HIP(hipSetDevice(e->device));
+  const uint32_t G = uint32_t(l0.size());
+  std::vector<uint32_t> vg(e->n), coff(G);
+  uint32_t nc = 0;
+  for (uint32_t g = 0; g < G; ++g) {
+    for (uint32_t u = v0[g]; u < v0[g + 1]; ++u) vg[u] = g;
+    coff[g] = nc;
+    nc += lg[g];
+  }
+  DevBuf d_v0, d_vg, d_l0, d_lg, d_coff, d_cnt, d_sum;
+  struct Guard {
+    Engine* e;
+    DevBuf* b[7];
+    ~Guard() {
+      for (DevBuf* x : b) e->release(*x);
+    }
+  } guard{e, {&d_v0, &d_vg, &d_l0, &d_lg, &d_coff, &d_cnt, &d_sum}};
+  GG_TRY(e->alloc(d_v0, (G + 1) * 4, err));
+  GG_TRY(e->alloc(d_vg, uint64_t(e->n) * 4, err));
+  GG_TRY(e->alloc(d_l0, G * 4, err));
+  GG_TRY(e->alloc(d_lg, G * 4, err));
+  GG_TRY(e->alloc(d_coff, G * 4, err));
+  GG_TRY(e->alloc(d_cnt, uint64_t(nc) * 8, err));
+  GG_TRY(e->alloc(d_sum, uint64_t(G) * 8, err));
+  GG_HIP(hipMemcpyAsync(d_v0.p, v0.data(), (G + 1) * 4, hipMemcpyHostToDevice, e->stream));
+  GG_HIP(hipMemcpyAsync(d_vg.p, vg.data(), uint64_t(e->n) * 4, hipMemcpyHostToDevice, e->stream));
+  GG_HIP(hipMemcpyAsync(d_l0.p, l0.data(), G * 4, hipMemcpyHostToDevice, e->stream));
+  GG_HIP(hipMemcpyAsync(d_lg.p, lg.data(), G * 4, hipMemcpyHostToDevice, e->stream));
+  GG_HIP(hipMemcpyAsync(d_coff.p, coff.data(), G * 4, hipMemcpyHostToDevice, e->stream));
+  e->fill_u64(d_sum, 0, G);
+  const DSlices sl{static_cast<uint32_t*>(d_v0.p), static_cast<uint32_t*>(d_vg.p), static_cast<uint32_t*>(d_l0.p),
+                   static_cast<uint32_t*>(d_lg.p), static_cast<uint32_t*>(d_coff.p), G};
+  k_slice_counts<<<e->grid_for(G), kBlock, 0, e->stream>>>(static_cast<unsigned long long*>(e->old_ptr.p), e->n, sl,
+                                                           static_cast<unsigned long long*>(d_cnt.p));
+  k_slice_checksum<<<uint32_t(e->cus) * 8, kBlock, 0, e->stream>>>(
+      {static_cast<unsigned long long*>(e->old_ptr.p), static_cast<uint32_t*>(e->old_col.p)}, e->rows(), e->n, sl,
+      static_cast<unsigned long long*>(d_sum.p));
+  GG_HIP(hipGetLastError());
+  counts.assign(nc, 0);
+  sums.assign(G, 0);
+  GG_HIP(hipMemcpyAsync(counts.data(), d_cnt.p, uint64_t(nc) * 8, hipMemcpyDeviceToHost, e->stream));
+  GG_HIP(hipMemcpyAsync(sums.data(), d_sum.p, uint64_t(G) * 8, hipMemcpyDeviceToHost, e->stream));
+  GG_HIP(hipStreamSynchronize(e->stream));
+  return GG_OK;
+}
+
+int engine_export_slice(Engine* e, uint32_t v0, uint32_t v1, uint32_t l0, uint32_t lg, uint32_t* buf, uint64_t cap,
+                        uint64_t* count, std::string& err) {
+  GG_TRY(need_closure(e, err));
+  return export_slice(e, v0, v1, l0, lg, buf, cap, count, err);
 }
 
 int engine_checksum(Engine* e, uint64_t* sum, std::string& err) {
@@ -1785,7 +1879,7 @@ int engine_checksum(Engine* e, uint64_t* sum, std::string& err) {
 
 int engine_release(Engine* e) {
   DevBuf* all[] = {&e->old_ptr, &e->old_col, &e->dl_ptr,  &e->dl_col,   &e->nx_ptr, &e->nx_col,   &e->o2_ptr,
-                   &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt,  &e->active, &e->nonempty, &e->tiles,
+                   &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt,  &e->active, &e->tiles,
                    &e->ot_ptr,  &e->ot_col,  &e->ot2_ptr, &e->ot2_col,  &e->dt_ptr, &e->dt_col,   &e->dt_cnt,
                    &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_len, &e->pk_ent, &e->pk2_ent};
   for (DevBuf* b : all) e->release(*b);

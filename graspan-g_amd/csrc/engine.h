@@ -28,6 +28,14 @@ int engine_counts(Engine* e, uint64_t* per_label, uint64_t* total, std::string& 
 int engine_export(Engine* e, uint32_t* buf, uint64_t cap, uint64_t* count, std::string& err);
 int engine_degrees(Engine* e, uint64_t* deg, std::string& err);
 int engine_checksum(Engine* e, uint64_t* sum, std::string& err);
+// Per-graph results of a batched closure (graph g: vertices [v0[g], v0[g+1]),
+// combined labels [l0[g], l0[g]+lg[g])): per-label counts (concatenated over
+// graphs) and per-graph checksums in the graph's own numbering.
+int engine_slices(Engine* e, const std::vector<uint32_t>& v0, const std::vector<uint32_t>& l0,
+                  const std::vector<uint32_t>& lg, std::vector<uint64_t>& counts, std::vector<uint64_t>& sums,
+                  std::string& err);
+int engine_export_slice(Engine* e, uint32_t v0, uint32_t v1, uint32_t l0, uint32_t lg, uint32_t* buf, uint64_t cap,
+                        uint64_t* count, std::string& err);
 int engine_release(Engine* e);
 
 int device_query(int* count, int preferred, int* selected, char* name, size_t name_cap, uint64_t* total,

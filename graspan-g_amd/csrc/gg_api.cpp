@@ -50,6 +50,17 @@ int push_graph(gg_ctx* c) {
 
 }  // namespace
 
+namespace gg {
+// The engine of a context, created on first use (batch.cpp).
+Engine* ctx_engine(gg_ctx* c, int& status, std::string*& err) {
+  err = &c->err;
+  status = ensure_engine(c);
+  if (status != GG_OK) return nullptr;
+  c->graph_on_device = false;  // a batch replaces the context's device graph
+  return c->engine;
+}
+}  // namespace gg
+
 extern "C" {
 
 const char* gg_version(void) { return "graspan-g-amd 0.1 (gfx950)"; }

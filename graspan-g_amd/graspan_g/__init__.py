@@ -33,6 +33,8 @@ EXPORTED = (
     "gg_graph_load_file", "gg_graph_set_coo", "gg_graph_info", "gg_graph_edges",
     "gg_closure_run", "gg_closure_counts", "gg_closure_export", "gg_closure_degrees", "gg_closure_checksum",
     "gg_triple_hash", "gg_closure_release", "gg_device_info",
+    "gg_batch_create", "gg_batch_destroy", "gg_batch_add", "gg_batch_run", "gg_batch_info", "gg_batch_result",
+    "gg_batch_export",
 )
 
 
@@ -89,6 +91,10 @@ def lib():
             "gg_closure_checksum": (I, [P, P]), "gg_triple_hash": (U64, [U32, U32, U32]),
             "gg_closure_release": (I, [P]),
             "gg_device_info": (I, [P, P, P, ctypes.c_char_p, ctypes.c_size_t, P, P]),
+            "gg_batch_create": (I, [P, P]), "gg_batch_destroy": (I, [P]),
+            "gg_batch_add": (I, [P, I, P, I, U32, U64, P, P, P, P]),
+            "gg_batch_run": (I, [P, U64, I, P]), "gg_batch_info": (I, [P, P, P]),
+            "gg_batch_result": (I, [P, I, P, P, P]), "gg_batch_export": (I, [P, I, P, U64, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -244,6 +250,72 @@ class Context:
                                            ctypes.byref(tot), ctypes.byref(fr)), "gg_device_info")
         return {"count": cnt.value, "selected": sel.value, "name": name.value.decode(), "total": tot.value,
                 "free": fr.value}
+
+
+class Batch:
+    """Many graphs closed together (gg_batch_*). Uses a Context's device."""
+
+    def __init__(self, ctx: Context):
+        self._ctx = ctx
+        self._L = ctx._L
+        self._h = ctypes.c_void_p()
+        self._labels = []
+        ctx._check(self._L.gg_batch_create(ctx._h, ctypes.byref(self._h)), "gg_batch_create")
+
+    def close(self):
+        if self._h:
+            self._L.gg_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != GG_OK:
+            raise GGError(st, what)
+
+    def add(self, rules, num_labels, n, src, dst, lab) -> int:
+        rules = np.ascontiguousarray(rules, dtype=np.int32).reshape(-1, 3)
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        lab = np.ascontiguousarray(lab, dtype=np.uint8)
+        gid = ctypes.c_int()
+        self._check(self._L.gg_batch_add(self._h, rules.shape[0], _ptr(rules), num_labels, n, src.shape[0],
+                                         _ptr(src), _ptr(dst), _ptr(lab), ctypes.byref(gid)), "gg_batch_add")
+        self._labels.append(num_labels)
+        return gid.value
+
+    def add_spec(self, g) -> int:
+        return self.add(g.rules, g.num_labels, g.n, g.src, g.dst, g.lab)
+
+    def run(self, max_vertices: int = 0, keep_closures: bool = False) -> Stats:
+        st = Stats()
+        s = self._L.gg_batch_run(self._h, max_vertices, 1 if keep_closures else 0, ctypes.byref(st))
+        if s != GG_OK:
+            raise GGError(s, "gg_batch_run: " + self._ctx.last_error)
+        return st
+
+    def info(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.gg_batch_info(self._h, ctypes.byref(a), ctypes.byref(b)), "gg_batch_info")
+        return a.value, b.value
+
+    def result(self, gid: int):
+        per = np.zeros(self._labels[gid], dtype=np.uint64)
+        tot, cs = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self._L.gg_batch_result(self._h, gid, _ptr(per), ctypes.byref(tot), ctypes.byref(cs)),
+                    "gg_batch_result")
+        return per, tot.value, cs.value
+
+    def export(self, gid: int) -> np.ndarray:
+        _, tot, _ = self.result(gid)
+        buf = np.empty((tot, 3), dtype=np.uint32)
+        cnt = ctypes.c_uint64()
+        self._check(self._L.gg_batch_export(self._h, gid, _ptr(buf), tot, ctypes.byref(cnt)), "gg_batch_export")
+        return buf
 
 
 def closure_of(g, filter_flag: int = 0, device: int = -1) -> tuple:

@@ -156,6 +156,41 @@ uint64_t gg_triple_hash(uint32_t src, uint32_t dst, uint32_t label);
 /* Releases the device state of the last closure (the input stays). */
 int gg_closure_release(gg_ctx *ctx);
 
+/* ---- batched closure ---------------------------------------------------
+ * No counterpart in the reference, which closes one graph per process. Many
+ * independent graphs (each with its own grammar) are closed together: the
+ * graphs get disjoint vertex ranges and each distinct grammar its own label
+ * range, eps rules become explicit self-loops, and the disjoint union is
+ * closed by one launch sequence per round (closure of a disjoint union is the
+ * disjoint union of the closures). */
+
+typedef struct gg_batch gg_batch;
+
+/* The batch uses ctx's device and budget. */
+int gg_batch_create(gg_ctx *ctx, gg_batch **out);
+int gg_batch_destroy(gg_batch *batch);
+
+/* Adds one graph: rule triples with ids 1..num_labels, edges with label ids
+ * 1..num_labels, vertices 0..num_vertices-1. *graph_id receives its index. */
+int gg_batch_add(gg_batch *batch, int num_rules, const int32_t *rules, int num_labels, uint32_t num_vertices,
+                 uint64_t num_edges, const uint32_t *src, const uint32_t *dst, const uint8_t *label, int *graph_id);
+
+/* Closes every added graph. Graphs are taken in order into launches of at
+ * most max_vertices vertices in total (0 = as many as the 2^32 row limit
+ * allows); keep_closures keeps each graph's triples on the host for
+ * gg_batch_export. stats (may be NULL) sums the launches. */
+int gg_batch_run(gg_batch *batch, uint64_t max_vertices, int keep_closures, gg_stats *stats);
+
+/* Number of graphs and of launch groups the last run used. */
+int gg_batch_info(const gg_batch *batch, int *num_graphs, int *num_groups);
+
+/* Results of one graph, in its own numbering: per-label counts (num_labels
+ * entries), total, checksum (as gg_closure_checksum). Pointers may be NULL. */
+int gg_batch_result(const gg_batch *batch, int graph_id, uint64_t *per_label, uint64_t *total, uint64_t *checksum);
+
+/* Sorted (src, dst, label) triples of one graph (needs keep_closures). */
+int gg_batch_export(const gg_batch *batch, int graph_id, uint32_t *buf, uint64_t cap, uint64_t *count);
+
 /* Device description used for the report lines of src/main.cpp:34,54,57.
  * name_cap bytes are written to name. */
 int gg_device_info(gg_ctx *ctx, int *device_count, int *selected, char *name, size_t name_cap,

@@ -166,3 +166,56 @@ def test_gpu_alias_1m_full_size(built):
     counts, checksum, round count and the P/K/O/N counters equal the oracle's."""
     say("alias n=1000000 (benchmark workload)")
     check_against_oracle(gen.alias(1_000_000, seed=0), full_triples=False)
+
+
+def _mixed_set(count, lo, hi, seed):
+    sizes = gen.log_uniform_sizes(count, lo, hi, seed)
+    out = []
+    for i, nv in enumerate(sizes):
+        out.append(gen.alias_as_dataflow(int(nv), seed=seed * 7919 + i) if i % 2 == 0
+                   else gen.alias(int(nv), seed=seed * 7919 + i))
+    return out
+
+
+def test_gpu_batch_matches_oracle(built):
+    """Graphs of both grammars closed together equal their separate closures."""
+    say("batch of 24 mixed graphs")
+    graphs = _mixed_set(24, 50, 3000, seed=21)
+    c = gg.Context()
+    b = gg.Batch(c)
+    ids = [b.add_spec(g) for g in graphs]
+    b.run(keep_closures=True)
+    assert b.info() == (24, 1)
+    for gid, g in zip(ids, graphs):
+        r = oracle.closure_of(g)
+        per, tot, cs = b.result(gid)
+        assert tot == r.total and cs == r.checksum
+        np.testing.assert_array_equal(per, r.counts)
+        np.testing.assert_array_equal(b.export(gid), r.triples)
+
+
+def test_gpu_batch_groups_and_alone_equal(built):
+    """Splitting the batch into several launch groups changes nothing, and a
+    graph closed alone gives the same result as inside the batch."""
+    say("batch split into groups")
+    graphs = _mixed_set(16, 200, 5000, seed=22)
+    c = gg.Context()
+    b1 = gg.Batch(c)
+    for g in graphs:
+        b1.add_spec(g)
+    b1.run()
+    whole = [b1.result(i) for i in range(len(graphs))]
+    b2 = gg.Batch(c)
+    for g in graphs:
+        b2.add_spec(g)
+    b2.run(max_vertices=6000)
+    assert b2.info()[1] > 1
+    for i in range(len(graphs)):
+        p1, t1, c1 = whole[i]
+        p2, t2, c2 = b2.result(i)
+        assert (t1, c1) == (t2, c2)
+        np.testing.assert_array_equal(p1, p2)
+    alone = gg.Context()
+    alone.set_spec(graphs[3])
+    alone.run(0)
+    assert alone.counts()[1] == whole[3][1] and alone.checksum() == whole[3][2]
