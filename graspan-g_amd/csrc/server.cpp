@@ -1,0 +1,195 @@
+// Serving loop over the batched closure (gg_server_*; no counterpart in the
+// reference). See include/graspan_g.h for the admission policy.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "graspan_g.h"
+
+namespace {
+
+struct Request {
+  int64_t id = 0;
+  std::vector<int32_t> rules;
+  int L = 0;
+  uint32_t n = 0;
+  std::vector<uint32_t> src, dst;
+  std::vector<uint8_t> lab;
+  uint64_t est = 0;
+  gg_server_result res{};
+  std::vector<uint64_t> counts;
+  bool done = false;
+};
+
+}  // namespace
+
+struct gg_server {
+  gg_server_options opt{};
+  std::chrono::steady_clock::time_point t0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> queue;
+  std::unordered_map<int64_t, Request*> all;
+  int64_t next_id = 1;
+  uint64_t in_flight = 0;
+  bool stopping = false;
+  std::vector<std::thread> workers;
+
+  double now() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  void worker(int slot) {
+    gg_options o{opt.device, 0, 0};
+    gg_ctx* ctx = nullptr;
+    const int cs = gg_ctx_create(&o, &ctx);
+    for (;;) {
+      std::vector<Request*> take;
+      uint64_t est = 0;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+          if (!queue.empty()) {
+            // admit in arrival order while the footprint fits the budget left
+            // by the other slots; a request larger than the whole budget is
+            // admitted alone when nothing else is running
+            while (!queue.empty() && (opt.max_batch_requests == 0 || take.size() < opt.max_batch_requests)) {
+              Request* r = queue.front();
+              const bool fits = in_flight + est + r->est <= opt.budget_bytes;
+              const bool alone = take.empty() && in_flight == 0;
+              if (!fits && !alone) break;
+              take.push_back(r);
+              est += r->est;
+              queue.pop_front();
+            }
+            if (!take.empty()) {
+              in_flight += est;
+              break;
+            }
+          }
+          if (stopping && queue.empty()) {
+            if (ctx) gg_ctx_destroy(ctx);
+            return;
+          }
+          cv_work.wait(lk);
+        }
+      }
+      const double start = now();
+      int status = cs;
+      std::vector<int> ids(take.size(), -1);
+      gg_batch* b = nullptr;
+      if (status == GG_OK) status = gg_batch_create(ctx, &b);
+      for (size_t i = 0; status == GG_OK && i < take.size(); ++i) {
+        Request* r = take[i];
+        status = gg_batch_add(b, int(r->rules.size() / 3), r->rules.data(), r->L, r->n, r->src.size(), r->src.data(),
+                              r->dst.data(), r->lab.data(), &ids[i]);
+      }
+      if (status == GG_OK) status = gg_batch_run(b, 0, 0, nullptr);
+      std::vector<std::vector<uint64_t>> counts(take.size());
+      std::vector<uint64_t> totals(take.size(), 0), sums(take.size(), 0);
+      for (size_t i = 0; status == GG_OK && i < take.size(); ++i) {
+        counts[i].assign(size_t(take[i]->L), 0);
+        status = gg_batch_result(b, ids[i], counts[i].data(), &totals[i], &sums[i]);
+      }
+      if (b) gg_batch_destroy(b);
+      if (ctx) gg_closure_release(ctx);  // the slot's device memory goes back
+      const double end = now();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t i = 0; i < take.size(); ++i) {
+          Request* r = take[i];
+          r->res.status = status;
+          r->res.total_edges = totals[i];
+          r->res.checksum = sums[i];
+          r->res.start_ms = start;
+          r->res.done_ms = end;
+          r->res.batch_size = uint32_t(take.size());
+          r->res.slot = slot;
+          r->counts = std::move(counts[i]);
+          r->done = true;
+          // inputs are no longer needed
+          std::vector<uint32_t>().swap(r->src);
+          std::vector<uint32_t>().swap(r->dst);
+          std::vector<uint8_t>().swap(r->lab);
+        }
+        in_flight -= est;
+      }
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+  }
+};
+
+extern "C" {
+
+int gg_server_create(const gg_server_options* opts, gg_server** out) {
+  if (!opts || !out || opts->slots < 1 || opts->budget_bytes == 0) return GG_E_ARG;
+  auto* s = new gg_server();
+  s->opt = *opts;
+  s->t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < opts->slots; ++i) s->workers.emplace_back(&gg_server::worker, s, i);
+  *out = s;
+  return GG_OK;
+}
+
+int gg_server_submit(gg_server* s, int num_rules, const int32_t* rules, int num_labels, uint32_t n, uint64_t m,
+                     const uint32_t* src, const uint32_t* dst, const uint8_t* lab, int64_t* id) {
+  if (!s || num_rules < 0 || (num_rules && !rules) || num_labels < 1 || n == 0 || (m && (!src || !dst || !lab)))
+    return GG_E_ARG;
+  auto* r = new Request();
+  r->rules.assign(rules, rules + 3 * num_rules);
+  r->L = num_labels;
+  r->n = n;
+  r->src.assign(src, src + m);
+  r->dst.assign(dst, dst + m);
+  r->lab.assign(lab, lab + m);
+  r->est = s->opt.bytes_per_vertex * n + 64 * m;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) {
+      delete r;
+      return GG_E_STATE;
+    }
+    r->id = s->next_id++;
+    r->res.submit_ms = s->now();
+    s->all[r->id] = r;
+    s->queue.push_back(r);
+    if (id) *id = r->id;
+  }
+  s->cv_work.notify_one();
+  return GG_OK;
+}
+
+int gg_server_wait(gg_server* s, int64_t id, gg_server_result* res, uint64_t* per_label) {
+  if (!s) return GG_E_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->all.find(id);
+  if (it == s->all.end()) return GG_E_ARG;
+  Request* r = it->second;
+  s->cv_done.wait(lk, [&] { return r->done; });
+  if (res) *res = r->res;
+  if (per_label)
+    for (size_t l = 0; l < r->counts.size(); ++l) per_label[l] = r->counts[l];
+  return GG_OK;
+}
+
+double gg_server_now_ms(const gg_server* s) { return s ? s->now() : 0.0; }
+
+int gg_server_destroy(gg_server* s) {
+  if (!s) return GG_E_ARG;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+  }
+  s->cv_work.notify_all();
+  for (auto& t : s->workers) t.join();
+  for (auto& kv : s->all) delete kv.second;
+  delete s;
+  return GG_OK;
+}
+
+}  // extern "C"

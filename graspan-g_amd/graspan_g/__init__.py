@@ -35,6 +35,7 @@ EXPORTED = (
     "gg_triple_hash", "gg_closure_release", "gg_device_info",
     "gg_batch_create", "gg_batch_destroy", "gg_batch_add", "gg_batch_run", "gg_batch_info", "gg_batch_result",
     "gg_batch_export",
+    "gg_server_create", "gg_server_submit", "gg_server_wait", "gg_server_now_ms", "gg_server_destroy",
 )
 
 
@@ -95,6 +96,10 @@ def lib():
             "gg_batch_add": (I, [P, I, P, I, U32, U64, P, P, P, P]),
             "gg_batch_run": (I, [P, U64, I, P]), "gg_batch_info": (I, [P, P, P]),
             "gg_batch_result": (I, [P, I, P, P, P]), "gg_batch_export": (I, [P, I, P, U64, P]),
+            "gg_server_create": (I, [P, P]),
+            "gg_server_submit": (I, [P, I, P, I, U32, U64, P, P, P, P]),
+            "gg_server_wait": (I, [P, ctypes.c_int64, P, P]), "gg_server_now_ms": (ctypes.c_double, [P]),
+            "gg_server_destroy": (I, [P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -250,6 +255,80 @@ class Context:
                                            ctypes.byref(tot), ctypes.byref(fr)), "gg_device_info")
         return {"count": cnt.value, "selected": sel.value, "name": name.value.decode(), "total": tot.value,
                 "free": fr.value}
+
+
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("budget_bytes", ctypes.c_uint64),
+                ("bytes_per_vertex", ctypes.c_uint64), ("max_batch_requests", ctypes.c_uint32)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("total_edges", ctypes.c_uint64), ("checksum", ctypes.c_uint64),
+                ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("batch_size", ctypes.c_uint32), ("slot", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Server:
+    """Serving loop (gg_server_*): submit requests, wait for results."""
+
+    def __init__(self, device: int = -1, slots: int = 2, budget_bytes: int = 200 << 30,
+                 bytes_per_vertex: int = 16 << 10, max_batch_requests: int = 0):
+        self._L = lib()
+        self._h = ctypes.c_void_p()
+        self._labels = {}
+        o = ServerOptions(device, slots, budget_bytes, bytes_per_vertex, max_batch_requests)
+        st = self._L.gg_server_create(ctypes.byref(o), ctypes.byref(self._h))
+        if st != GG_OK:
+            raise GGError(st, "gg_server_create")
+
+    def submit(self, rules, num_labels, n, src, dst, lab) -> int:
+        rules = np.ascontiguousarray(rules, dtype=np.int32).reshape(-1, 3)
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        lab = np.ascontiguousarray(lab, dtype=np.uint8)
+        rid = ctypes.c_int64()
+        st = self._L.gg_server_submit(self._h, rules.shape[0], _ptr(rules), num_labels, n, src.shape[0], _ptr(src),
+                                      _ptr(dst), _ptr(lab), ctypes.byref(rid))
+        if st != GG_OK:
+            raise GGError(st, "gg_server_submit")
+        self._labels[rid.value] = num_labels
+        return rid.value
+
+    def submit_spec(self, g) -> int:
+        return self.submit(g.rules, g.num_labels, g.n, g.src, g.dst, g.lab)
+
+    def wait(self, rid: int):
+        res = ServerResult()
+        per = np.zeros(self._labels[rid], dtype=np.uint64)
+        st = self._L.gg_server_wait(self._h, rid, ctypes.byref(res), _ptr(per))
+        if st != GG_OK:
+            raise GGError(st, "gg_server_wait")
+        if res.status != GG_OK:
+            raise GGError(res.status, f"request {rid} failed")
+        return res, per
+
+    def now_ms(self) -> float:
+        return float(self._L.gg_server_now_ms(self._h))
+
+    def close(self):
+        if self._h:
+            self._L.gg_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

@@ -191,6 +191,51 @@ int gg_batch_result(const gg_batch *batch, int graph_id, uint64_t *per_label, ui
 /* Sorted (src, dst, label) triples of one graph (needs keep_closures). */
 int gg_batch_export(const gg_batch *batch, int graph_id, uint32_t *buf, uint64_t cap, uint64_t *count);
 
+/* ---- serving loop ---------------------------------------------------------
+ * No counterpart in the reference. A request is one graph with its grammar.
+ * Worker slots (one context and HIP stream each) take requests from a FIFO
+ * queue: whenever a slot is idle it admits queued requests in arrival order
+ * while their estimated device footprint fits the budget left by the batches
+ * other slots are running, closes them as one batch (gg_batch_*), publishes
+ * each result, frees its device memory and looks at the queue again. Requests
+ * that arrive while batches run are therefore admitted as slots and memory
+ * free up, and the batches of different slots overlap on the device. */
+
+typedef struct gg_server gg_server;
+
+typedef struct gg_server_options {
+  int device;                  /* as gg_options.device */
+  int slots;                   /* concurrent batches (>= 1) */
+  uint64_t budget_bytes;       /* device memory for all in-flight requests */
+  uint64_t bytes_per_vertex;   /* footprint estimate of a request: this times
+                                  its vertex count plus 64 bytes per input edge */
+  uint32_t max_batch_requests; /* cap on requests per batch (0 = no cap) */
+} gg_server_options;
+
+typedef struct gg_server_result {
+  int status;                  /* GG_OK or the error of the request's batch */
+  uint64_t total_edges;
+  uint64_t checksum;           /* as gg_closure_checksum, in the request's numbering */
+  double submit_ms;            /* times on the server's clock (ms since creation) */
+  double start_ms;             /* its batch started */
+  double done_ms;              /* its result was published */
+  uint32_t batch_size;         /* requests in its batch */
+  int slot;
+} gg_server_result;
+
+int gg_server_create(const gg_server_options *opts, gg_server **out);
+/* Queues one request (copied); never blocks on the GPU. Thread-safe. */
+int gg_server_submit(gg_server *srv, int num_rules, const int32_t *rules, int num_labels, uint32_t num_vertices,
+                     uint64_t num_edges, const uint32_t *src, const uint32_t *dst, const uint8_t *label,
+                     int64_t *request_id);
+/* Blocks until the request is done and copies its result; per_label (may be
+ * NULL) receives num_labels counts. */
+int gg_server_wait(gg_server *srv, int64_t request_id, gg_server_result *result, uint64_t *per_label);
+/* Milliseconds since the server was created (the clock of the results). */
+double gg_server_now_ms(const gg_server *srv);
+/* Finishes queued work and stops the workers. */
+int gg_server_destroy(gg_server *srv);
+
 /* Device description used for the report lines of src/main.cpp:34,54,57.
  * name_cap bytes are written to name. */
 int gg_device_info(gg_ctx *ctx, int *device_count, int *selected, char *name, size_t name_cap,

@@ -219,3 +219,24 @@ def test_gpu_batch_groups_and_alone_equal(built):
     alone.set_spec(graphs[3])
     alone.run(0)
     assert alone.counts()[1] == whole[3][1] and alone.checksum() == whole[3][2]
+
+
+def test_gpu_server_results_equal_alone(built):
+    """Requests served in a mix (two slots, batches formed from whatever is
+    queued) give the same closures as each request closed alone."""
+    say("server mix")
+    graphs = _mixed_set(40, 100, 20_000, seed=23)
+    with gg.Server(slots=2, budget_bytes=2 << 30, bytes_per_vertex=16 << 10, max_batch_requests=8) as srv:
+        ids = [srv.submit_spec(g) for g in graphs]
+        results = [srv.wait(i) for i in ids]
+    sizes = [r.batch_size for r, _ in results]
+    assert max(sizes) > 1                       # requests were batched together
+    assert len({r.slot for r, _ in results}) == 2  # both slots ran
+    alone = gg.Context()
+    for g, (r, per) in zip(graphs, results):
+        alone.set_spec(g)
+        alone.run(0)
+        p2, t2 = alone.counts()
+        assert (r.total_edges, r.checksum) == (t2, alone.checksum())
+        np.testing.assert_array_equal(per, p2)
+        assert r.submit_ms <= r.start_ms <= r.done_ms
