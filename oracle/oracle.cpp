@@ -44,6 +44,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
 #include <string>
 #include <vector>
 
@@ -81,6 +82,9 @@ struct Result {
   uint64_t input_unique = 0;  // |dedup input U eps loops|
   double seconds = 0.0;
   std::string err;
+  // mode R only
+  uint32_t supersteps = 0, num_parts = 0;
+  std::vector<uint32_t> part_first, part_last;
 };
 
 // Build one CSR per label from COO; rows sorted + unique.
@@ -311,6 +315,369 @@ Result* run_g(uint32_t n, int L, int R, const int32_t* rules, uint64_t m, const 
   return res;
 }
 
+// ---------------------------------------------------------------------------
+// Mode R: the reference's own schedule, restated at the level of edge sets.
+// Needed for filter != 0, whose result depends on the partitioning and the
+// sweep order.
+//
+//   partitions   initializePartitions (src/main.cpp:141-175) over the degree
+//                pre-pass of getGraphInfo (src/main.cpp:106-139): per source
+//                the number of input lines and of distinct (dst/192, label)
+//                "elements"; heap = DEVMEMUSED_MB 10000 -> 2,293,760,000
+//                words (src/kernel.cu:35, src/kernel.h:26), partition limit =
+//                heap/4 words of 8*elements + roundup32(vertices)*labels.
+//   superstep 0  precomputation (src/main.cpp:764-804): per partition, repeat
+//                spagpu_s (src/kernel.cu:466-521) + mergeAndDiff
+//                (src/kernel.cu:943-985) until delta is empty, then
+//                delta <- old, old <- {}.
+//   superstep k  run_computation (src/main.cpp:806-858): for i, for j > i:
+//                spagpu_b (src/kernel.cu:523-628) once, then spagpu
+//                (src/kernel.cu:630-780) until nothing is added; then
+//                mergeAndDiff(i). Done when every delta is empty
+//                (src/main.cpp:892-906).
+//   compute11(src, dst1, dst2, tmp): tmp_A(u) |= dstk_C(v) for every binary
+//                rule A ::= B C, u in src's partition, v in src_B(u) and v in
+//                dstk's partition (src/kernel.cu:297-362, 403-419).
+//   compute10    tmp_A(u) |= from_B(u) for unary rules (src/kernel.cu:422-442).
+//   compute00    tmp_A(u) |= {u} for eps rules (src/kernel.cu:445-464).
+// A region that does not exist (size 0 in the reference) joins nothing; the
+// existence flags below follow oldSize/deltaSize/tmpSize.
+namespace moder {
+
+using Rows = std::vector<std::vector<uint32_t>>;  // [label * n + u], sorted unique
+
+struct Part {
+  uint32_t first = 0, last = 0;
+  bool has_old = false, has_dl = false, has_tmp = false;
+};
+
+struct State {
+  uint32_t n = 0;
+  int L = 0;
+  const Grammar* g = nullptr;
+  int filter = 0;
+  Rows old, dl, tmp;
+  std::vector<Part> parts;
+  bool changed = false;
+  uint32_t supersteps = 0;
+};
+
+enum Region { kOld, kDl, kTmp };
+
+Rows& region(State& s, Region r) { return r == kOld ? s.old : (r == kDl ? s.dl : s.tmp); }
+bool exists(const Part& p, Region r) { return r == kOld ? p.has_old : (r == kDl ? p.has_dl : p.has_tmp); }
+
+// tmp_A(u) |= add (sorted unique); sets s.changed when something is new.
+void add_to(State& s, size_t row, std::vector<uint32_t>& add) {
+  if (add.empty()) return;
+  std::sort(add.begin(), add.end());
+  add.erase(std::unique(add.begin(), add.end()), add.end());
+  std::vector<uint32_t>& t = s.tmp[row];
+  std::vector<uint32_t> out;
+  out.reserve(t.size() + add.size());
+  std::set_union(t.begin(), t.end(), add.begin(), add.end(), std::back_inserter(out));
+  if (out.size() != t.size()) s.changed = true;
+  t.swap(out);
+}
+
+struct Dst {
+  Region r;
+  int part;
+};
+
+// compute11(src region of partition p, dsts..., tmp of p). All reads see the
+// regions as they were when the operator started (the result of the
+// enclosing loops does not depend on it: every loop runs to its fixpoint or
+// applies operators that read no tmp).
+void join(State& s, int p, Region src, std::initializer_list<Dst> dsts_in) {
+  const Part& P = s.parts[p];
+  if (!exists(P, src)) return;
+  const uint32_t n = s.n;
+  Rows& S = region(s, src);
+  const std::vector<Dst> dsts(dsts_in);
+  for (int A = 0; A < s.L; ++A) {
+    if (s.g->binary[A].empty()) continue;
+    // candidates of every row of partition p first (reads only), then the
+    // row-wise unions (each row written by one thread)
+    const uint32_t cnt = P.last - P.first + 1;
+    std::vector<std::vector<uint32_t>> add(cnt);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < int64_t(cnt); ++k) {
+      const uint32_t u = P.first + uint32_t(k);
+      for (auto& bc : s.g->binary[A]) {
+        for (uint32_t v : S[size_t(bc.first) * n + u]) {
+          for (const Dst& d : dsts) {
+            const Part& Q = s.parts[d.part];
+            if (!exists(Q, d.r) || v < Q.first || v > Q.last) continue;
+            const auto& lst = region(s, d.r)[size_t(bc.second) * n + v];
+            add[k].insert(add[k].end(), lst.begin(), lst.end());
+          }
+        }
+      }
+    }
+    bool ch = false;
+#pragma omp parallel for schedule(dynamic, 64) reduction(|| : ch)
+    for (int64_t k = 0; k < int64_t(cnt); ++k) {
+      if (add[k].empty()) continue;
+      std::vector<uint32_t>& a = add[k];
+      std::sort(a.begin(), a.end());
+      a.erase(std::unique(a.begin(), a.end()), a.end());
+      std::vector<uint32_t>& t = s.tmp[size_t(A) * n + P.first + uint32_t(k)];
+      std::vector<uint32_t> out;
+      out.reserve(t.size() + a.size());
+      std::set_union(t.begin(), t.end(), a.begin(), a.end(), std::back_inserter(out));
+      if (out.size() != t.size()) ch = true;
+      t.swap(out);
+    }
+    if (ch) s.changed = true;
+  }
+}
+
+// compute10 from region `from` of partition p.
+void unary(State& s, int p, Region from) {
+  const Part& P = s.parts[p];
+  if (!exists(P, from)) return;
+  const uint32_t n = s.n;
+  std::vector<std::pair<size_t, std::vector<uint32_t>>> adds;
+  for (int A = 0; A < s.L; ++A)
+    for (int B : s.g->unary[A])
+      for (uint32_t u = P.first; u <= P.last; ++u) {
+        const auto& lst = region(s, from)[size_t(B) * n + u];
+        if (!lst.empty()) adds.emplace_back(size_t(A) * n + u, lst);
+      }
+  for (auto& a : adds) add_to(s, a.first, a.second);
+}
+
+void eps(State& s, int p) {
+  const Part& P = s.parts[p];
+  for (int A : s.g->eps)
+    for (uint32_t u = P.first; u <= P.last; ++u) {
+      std::vector<uint32_t> one{u};
+      add_to(s, size_t(A) * s.n + u, one);
+    }
+}
+
+// mergeAndDiff (src/kernel.cu:943-985) for partition p.
+void merge_and_diff(State& s, int p) {
+  Part& P = s.parts[p];
+  const uint32_t n = s.n;
+  if (P.has_dl) {
+    for (int A = 0; A < s.L; ++A)
+      for (uint32_t u = P.first; u <= P.last; ++u) {
+        const size_t r = size_t(A) * n + u;
+        if (!P.has_old) {
+          s.old[r].swap(s.dl[r]);
+        } else {
+          std::vector<uint32_t> out;
+          std::set_union(s.old[r].begin(), s.old[r].end(), s.dl[r].begin(), s.dl[r].end(), std::back_inserter(out));
+          s.old[r].swap(out);
+        }
+        s.dl[r].clear();
+      }
+    P.has_old = true;
+    P.has_dl = false;
+  }
+  if (P.has_tmp) {
+    bool any = false;
+    for (int A = 0; A < s.L; ++A)
+      for (uint32_t u = P.first; u <= P.last; ++u) {
+        const size_t r = size_t(A) * n + u;
+        std::vector<uint32_t> out;
+        std::set_difference(s.tmp[r].begin(), s.tmp[r].end(), s.old[r].begin(), s.old[r].end(),
+                            std::back_inserter(out));
+        any |= !out.empty();
+        s.dl[r].swap(out);
+        s.tmp[r].clear();
+      }
+    P.has_dl = any;
+    P.has_tmp = false;
+  }
+}
+
+void spagpu_s(State& s, int p) {
+  Part& P = s.parts[p];
+  P.has_tmp = true;
+  if (!P.has_old) eps(s, p);
+  if (s.filter == 0) unary(s, p, kDl);
+  if (P.has_old) {
+    if (s.filter == 0) join(s, p, kOld, {{kDl, p}});
+    join(s, p, kDl, {{kOld, p}, {kDl, p}});
+  } else {
+    join(s, p, kDl, {{kDl, p}});
+  }
+}
+
+void spagpu_b(State& s, int i, int j) {
+  s.parts[i].has_tmp = true;
+  s.parts[j].has_tmp = true;
+  if (s.filter == 0) join(s, i, kOld, {{kDl, j}});
+  join(s, i, kDl, {{kOld, j}, {kDl, j}});
+  if (s.filter == 0) join(s, j, kOld, {{kDl, i}});
+  join(s, j, kDl, {{kOld, i}, {kDl, i}});
+}
+
+void spagpu(State& s, int i, int j) {
+  for (;;) {
+    s.changed = false;
+    for (int side = 0; side < 2; ++side) {
+      const int p = side ? j : i, q = side ? i : j;
+      if (s.filter == 0) join(s, p, kOld, {{kTmp, i}, {kTmp, j}});
+      join(s, p, kDl, {{kTmp, i}, {kTmp, j}});
+      join(s, p, kTmp, {{kOld, p}, {kDl, p}});
+      join(s, p, kTmp, {{kOld, q}, {kDl, q}});
+      if (s.filter == 0) unary(s, p, kTmp);
+      join(s, p, kTmp, {{kTmp, i}, {kTmp, j}});
+    }
+    if (!s.changed) break;
+  }
+}
+
+// initializePartitions over the getGraphInfo degree pre-pass. Returns false
+// for the degenerate splits where the reference's unsigned arithmetic breaks
+// (a first vertex holding more than half the elements, or no elements).
+bool make_partitions(State& s, uint64_t m, const uint32_t* src, const uint32_t* dst, const uint8_t* lab,
+                     std::string& err) {
+  const uint32_t n = s.n;
+  std::vector<uint64_t> edges(n, 0), elems(n, 0);
+  {
+    std::vector<std::vector<uint64_t>> keys(n);
+    for (uint64_t i = 0; i < m; ++i) {
+      edges[src[i]]++;
+      keys[src[i]].push_back((uint64_t(dst[i] / 192) << 8) | lab[i]);
+    }
+    for (uint32_t u = 0; u < n; ++u) {
+      auto& k = keys[u];
+      std::sort(k.begin(), k.end());
+      elems[u] = uint64_t(std::unique(k.begin(), k.end()) - k.begin());
+    }
+  }
+  const uint64_t heap = (10000ull / 8 * 7) * 1024 * 256;
+  const uint64_t limit = heap / 4;
+  uint64_t t1 = 0, t3 = 0;
+  auto rup32 = [](uint64_t x) { return (x + 31) / 32 * 32; };
+  for (uint32_t i = 0; i < n; ++i) {
+    t1 += elems[i];
+    if (t1 * 8 + rup32(i - t3 + 1) * uint64_t(s.L) > limit) {
+      if (i == 0 || uint64_t(i - 1) < t3) {
+        err = "degenerate partitioning";
+        return false;
+      }
+      Part p;
+      p.first = uint32_t(t3);
+      p.last = i - 1;
+      s.parts.push_back(p);
+      t1 = elems[i];
+      t3 = i;
+    }
+  }
+  Part last;
+  last.first = uint32_t(t3);
+  last.last = n - 1;
+  s.parts.push_back(last);
+  if (s.parts.size() == 1) {
+    uint64_t t4 = 0;
+    bool split = false;
+    for (uint32_t i = 0; i < n; ++i) {
+      t4 += elems[i];
+      if (t4 > t1 / 2) {
+        if (i == 0) break;
+        s.parts[0].last = i - 1;
+        Part p2;
+        p2.first = i;
+        p2.last = n - 1;
+        s.parts.push_back(p2);
+        split = true;
+        break;
+      }
+    }
+    if (!split) {
+      err = "degenerate partitioning (the reference's small-graph split needs a second vertex interval)";
+      return false;
+    }
+  }
+  return true;
+}
+
+}  // namespace moder
+
+Result* run_r(uint32_t n, int L, int R, const int32_t* rules, uint64_t m, const uint32_t* src, const uint32_t* dst,
+              const uint8_t* lab, int filter) {
+  auto* res = new Result();
+  res->n = n;
+  res->L = L;
+  Grammar g;
+  if (!parse_rules(L, R, rules, g, res->err)) return res;
+  for (uint64_t i = 0; i < m; ++i)
+    if (src[i] >= n || dst[i] >= n || lab[i] < 1 || lab[i] > L) {
+      res->err = "bad edge " + std::to_string(i);
+      return res;
+    }
+  auto t0 = std::chrono::steady_clock::now();
+  moder::State s;
+  s.n = n;
+  s.L = L;
+  s.g = &g;
+  s.filter = filter;
+  if (!moder::make_partitions(s, m, src, dst, lab, res->err)) return res;
+  const size_t rows = size_t(L) * n;
+  s.old.assign(rows, {});
+  s.dl.assign(rows, {});
+  s.tmp.assign(rows, {});
+  for (uint64_t i = 0; i < m; ++i) s.dl[size_t(lab[i] - 1) * n + src[i]].push_back(dst[i]);
+  for (auto& r : s.dl) {
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+  }
+  for (auto& p : s.parts) p.has_dl = true;  // loaded from the graph file
+  // superstep 0
+  for (size_t i = 0; i < s.parts.size(); ++i) {
+    for (;;) {
+      moder::spagpu_s(s, int(i));
+      moder::merge_and_diff(s, int(i));
+      if (!s.parts[i].has_dl) {
+        moder::Part& P = s.parts[i];
+        for (int A = 0; A < L; ++A)
+          for (uint32_t u = P.first; u <= P.last; ++u) s.dl[size_t(A) * n + u].swap(s.old[size_t(A) * n + u]);
+        P.has_dl = P.has_old;
+        P.has_old = false;
+        break;
+      }
+    }
+  }
+  // supersteps 1..
+  for (;;) {
+    ++s.supersteps;
+    for (size_t i = 0; i < s.parts.size(); ++i) {
+      for (size_t j = i + 1; j < s.parts.size(); ++j) {
+        moder::spagpu_b(s, int(i), int(j));
+        moder::spagpu(s, int(i), int(j));
+      }
+      moder::merge_and_diff(s, int(i));
+    }
+    bool done = true;
+    for (auto& p : s.parts) done &= !p.has_dl;
+    if (done) break;
+  }
+  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  // closure = every partition's old region
+  res->rel.assign(L, Csr());
+  for (int A = 0; A < L; ++A) {
+    Csr& c = res->rel[A];
+    c.ptr.assign(size_t(n) + 1, 0);
+    for (uint32_t u = 0; u < n; ++u) c.ptr[u + 1] = c.ptr[u] + s.old[size_t(A) * n + u].size();
+    c.col.resize(c.ptr[n]);
+    for (uint32_t u = 0; u < n; ++u)
+      std::copy(s.old[size_t(A) * n + u].begin(), s.old[size_t(A) * n + u].end(), c.col.begin() + long(c.ptr[u]));
+  }
+  res->supersteps = s.supersteps;
+  res->num_parts = uint32_t(s.parts.size());
+  for (auto& p : s.parts) {
+    res->part_first.push_back(p.first);
+    res->part_last.push_back(p.last);
+  }
+  return res;
+}
+
 }  // namespace
 
 extern "C" {
@@ -327,6 +694,23 @@ void* orc_run(uint32_t n, int L, int R, const int32_t* rules, uint64_t m, const 
     return r;
   }
   return run_g(n, L, R, rules, m, src, dst, lab, max_rounds);
+}
+
+// Mode R (the reference's partitioned schedule); any filter value.
+void* orc_run_r(uint32_t n, int L, int R, const int32_t* rules, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                const uint8_t* lab, int filter) {
+  return run_r(n, L, R, rules, m, src, dst, lab, filter);
+}
+
+// Mode R schedule facts: supersteps, partition count, partition bounds.
+uint32_t orc_supersteps(void* h) { return static_cast<Result*>(h)->supersteps; }
+uint32_t orc_num_parts(void* h) { return static_cast<Result*>(h)->num_parts; }
+void orc_parts(void* h, uint32_t* first, uint32_t* last) {
+  auto* r = static_cast<Result*>(h);
+  for (size_t i = 0; i < r->part_first.size(); ++i) {
+    first[i] = r->part_first[i];
+    last[i] = r->part_last[i];
+  }
 }
 
 const char* orc_error(void* h) { return static_cast<Result*>(h)->err.c_str(); }

@@ -50,6 +50,14 @@ def _load():
         lib.orc_free.argtypes = [P]
         lib.orc_checksum.restype = ctypes.c_uint64
         lib.orc_checksum.argtypes = [P]
+        lib.orc_run_r.restype = P
+        lib.orc_run_r.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, P, ctypes.c_uint64, P, P, P,
+                                  ctypes.c_int]
+        lib.orc_supersteps.restype = ctypes.c_uint32
+        lib.orc_supersteps.argtypes = [P]
+        lib.orc_num_parts.restype = ctypes.c_uint32
+        lib.orc_num_parts.argtypes = [P]
+        lib.orc_parts.argtypes = [P, P, P]
         _lib = lib
     return _lib
 
@@ -69,6 +77,8 @@ class OracleResult:
     seconds: float
     triples: Optional[np.ndarray]  # uint32 [total, 3] sorted (src, dst, label)
     checksum: int = 0
+    supersteps: int = 0          # mode R only
+    parts: Optional[list] = None  # mode R only: [(first, last), ...]
 
     @property
     def alg_bytes(self) -> int:
@@ -106,6 +116,42 @@ def closure(n: int, rules: np.ndarray, src: np.ndarray, dst: np.ndarray, lab: np
                             float(lib.orc_seconds(h)), triples, int(lib.orc_checksum(h)))
     finally:
         lib.orc_free(h)
+
+
+def closure_r(n: int, rules: np.ndarray, src: np.ndarray, dst: np.ndarray, lab: np.ndarray, num_labels: int,
+              filter_flag: int = 0, want_triples: bool = True) -> OracleResult:
+    """Mode R: the reference's partitioned schedule (any filter value)."""
+    lib = _load()
+    rules = np.ascontiguousarray(rules, dtype=np.int32).reshape(-1, 3)
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    lab = np.ascontiguousarray(lab, dtype=np.uint8)
+    h = lib.orc_run_r(n, num_labels, rules.shape[0], _ptr(rules), src.shape[0], _ptr(src), _ptr(dst), _ptr(lab),
+                      filter_flag)
+    try:
+        err = lib.orc_error(h).decode()
+        if err:
+            raise ValueError(err)
+        total = lib.orc_total(h)
+        counts = np.zeros(num_labels, dtype=np.uint64)
+        lib.orc_counts(h, _ptr(counts))
+        triples = None
+        if want_triples:
+            triples = np.empty((total, 3), dtype=np.uint32)
+            lib.orc_export(h, _ptr(triples), total)
+        npart = lib.orc_num_parts(h)
+        first = np.zeros(npart, np.uint32)
+        last = np.zeros(npart, np.uint32)
+        lib.orc_parts(h, _ptr(first), _ptr(last))
+        return OracleResult(n, num_labels, int(total), 0, counts, np.zeros((0, 4), np.uint64),
+                            float(lib.orc_seconds(h)), triples, int(lib.orc_checksum(h)),
+                            int(lib.orc_supersteps(h)), list(zip(first.tolist(), last.tolist())))
+    finally:
+        lib.orc_free(h)
+
+
+def closure_r_of(g, **kw) -> OracleResult:
+    return closure_r(g.n, g.rules, g.src, g.dst, g.lab, g.num_labels, **kw)
 
 
 def closure_of(g, **kw) -> OracleResult:
