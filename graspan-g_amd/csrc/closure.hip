@@ -40,6 +40,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <functional>
 #include <thread>
 #include <vector>
 
@@ -265,6 +266,12 @@ __global__ void k_scatter_raw(const uint32_t* src, const uint32_t* dst, const ui
   }
 }
 
+// Active list = every row 0..rows-1 (region operators).
+__global__ void k_all_active(uint32_t rows, uint32_t* active, Counters* ctr) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += gridDim.x * blockDim.x) active[i] = i;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->active = rows;
+}
+
 // Active list of the initial dedup: rows with raw entries.
 __global__ void k_activate_raw(const unsigned long long* ptr, uint32_t rows, uint32_t* active, Counters* ctr) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -441,8 +448,27 @@ struct DPacked {
 
 __device__ inline bool packed_label(const DPacked& p, uint32_t l) { return (p.labels[l >> 5] >> (l & 31)) & 1u; }
 
+// One operator of the reference's partitioned schedule (MODE 2 of the
+// expand kernel). Rows are local to partition [first, first+np): row
+// label*np + (u - first). Candidates for row (A,u) go into the seed region
+// (tmp) as new ids:
+//   kind 0 (compute11): for A ::= B C, v in key_B(u), dst[d]_C(v) for every
+//                      dst whose partition holds v;
+//   kind 1 (compute10): for A ::= B, key_B(u);
+//   kind 2 (compute00): u itself, for eps rules A;
+//   kind 3 (difference): key_A(u) (seed = old: the fresh ids are key - old).
+struct RegionOp {
+  int kind;
+  uint32_t first, np;
+  DCsr seed, key;
+  DCsr dst[2];
+  uint32_t dfirst[2], dn[2];
+  int ndst;
+};
+
 struct ExpandArgs {
   DCsr old, dl, raw;
+  RegionOp ro;
   DPacked pk;  // packed old rows
   DCsr b2;  // rows slot*n + u: keys v with old_B(u,v) and delta_C(v) non-empty
   uint32_t n;
@@ -548,13 +574,51 @@ __device__ inline void expand_words(const unsigned long long* __restrict__ ent, 
 // per batch with one list per lane (len 0 = no list on that lane). With
 // PACKED, old_C rows of packed labels go to visitw(pb, plen, len) instead
 // (plen packed words standing for len ids).
-template <bool INIT, bool PACKED, class V, class W>
+template <int MODE, bool PACKED, class V, class W>
 __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A, uint32_t u, int lane, V&& visit,
                                      W&& visitw) {
   const uint32_t n = a.n;
-  if (INIT) {
+  if (MODE == 1) {
     const unsigned long long b = a.raw.ptr[r], e = a.raw.ptr[r + 1];
     visit(a.raw.col, lane == 0 ? b : 0ull, lane == 0 ? uint32_t(e - b) : 0u);
+    return;
+  }
+  if (MODE == 2) {
+    const RegionOp& ro = a.ro;
+    const uint32_t ul = u - ro.first;
+    if (ro.kind == 3) {
+      const uint32_t rk = A * ro.np + ul;
+      const unsigned long long b = ro.key.ptr[rk], e = ro.key.ptr[rk + 1];
+      visit(ro.key.col, lane == 0 ? b : 0ull, lane == 0 ? uint32_t(e - b) : 0u);
+    } else if (ro.kind == 1) {
+      for (int i = 0; i < a.g.nu; ++i) {
+        if (a.g.u_lhs[i] != A) continue;
+        const uint32_t rk = uint32_t(a.g.u_rhs[i]) * ro.np + ul;
+        const unsigned long long b = ro.key.ptr[rk], e = ro.key.ptr[rk + 1];
+        visit(ro.key.col, lane == 0 ? b : 0ull, lane == 0 ? uint32_t(e - b) : 0u);
+      }
+    } else if (ro.kind == 0) {
+      for (int i = 0; i < a.g.nb; ++i) {
+        if (a.g.b_lhs[i] != A) continue;
+        const uint32_t B = a.g.b_rhs1[i], C = a.g.b_rhs2[i];
+        const uint32_t rk = B * ro.np + ul;
+        const unsigned long long kb = ro.key.ptr[rk], ke = ro.key.ptr[rk + 1];
+        for (unsigned long long k0 = kb; k0 < ke; k0 += kWave) {
+          const unsigned long long k = k0 + lane;
+          const uint32_t v = k < ke ? ro.key.col[k] : 0u;
+          for (int d = 0; d < ro.ndst; ++d) {
+            unsigned long long lb = 0;
+            uint32_t len = 0;
+            if (k < ke && v - ro.dfirst[d] < ro.dn[d]) {
+              const uint32_t rc = C * ro.dn[d] + (v - ro.dfirst[d]);
+              lb = ro.dst[d].ptr[rc];
+              len = uint32_t(ro.dst[d].ptr[rc + 1] - lb);
+            }
+            visit(ro.dst[d].col, lb, len);
+          }
+        }
+      }
+    }
     return;
   }
   for (int i = 0; i < a.g.nu; ++i) {
@@ -616,7 +680,7 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
 // Clears the window [c0, c0+span), seeds it with old_A(u) U delta_A(u) and
 // streams every candidate through it. Returns the number of new edges (set
 // bits of w.fresh) and stores the candidate count in kcount.
-template <bool INIT>
+template <int MODE>
 __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const ExpandArgs& a, uint32_t r, uint32_t A,
                                                                         uint32_t u, uint32_t c0, uint32_t span,
                                                                         WaveLds& w, int lane, unsigned long long& kcount,
@@ -628,7 +692,15 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
     w.fresh[i] = 0;
   }
   wave_sync();
-  if (!INIT) {
+  if (MODE == 2) {
+    const unsigned long long sb = a.ro.seed.ptr[r], se = a.ro.seed.ptr[r + 1];
+    for (unsigned long long i = sb + lane; i < se; i += kWave) {
+      const uint32_t off = a.ro.seed.col[i] - c0;
+      if (off < span) atomicOr(&w.seen[off >> 5], 1u << (off & 31));
+    }
+    wave_sync();
+  }
+  if (MODE == 0) {
     auto seed = [&](uint32_t x) {
       const uint32_t off = x - c0;
       if (off < span) atomicOr(&w.seen[off >> 5], 1u << (off & 31));
@@ -677,7 +749,7 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
     }
   };
   unsigned long long kc = 0;
-  for_each_list<INIT, true>(
+  for_each_list<MODE, true>(
       a, r, A, u, lane,
       [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
         kc += len;
@@ -687,6 +759,16 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
         kc += len;
         expand_words(a.pk.ent, pb, pl, w, lane, mark_word);
       });
+  if (MODE == 2 && a.ro.kind == 2) {
+    // eps: the row's own vertex
+    bool has_eps = false;
+    for (int i = 0; i < a.g.ne; ++i) has_eps |= a.g.e_lhs[i] == A;
+    if (has_eps) {
+      if (lane == 0) mark(u);
+      kc += lane == 0 ? 1 : 0;
+      wave_sync();
+    }
+  }
   kcount = wave_sum_u64(kc);
   miss = __ballot(miss_l) != 0;
   wave_sync();
@@ -715,7 +797,7 @@ __device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t spa
   }
 }
 
-template <bool INIT>
+template <int MODE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandWaves, kExpandWaves))) void k_expand(
     const ExpandArgs a) {
   __shared__ WaveLds s_lds[kWavesPerBlock];
@@ -766,7 +848,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
     }
     const uint32_t idx = next++;
     const uint32_t r = a.active[idx];
-    const uint32_t A = r / a.n, u = r - A * a.n;
+    const uint32_t rn = MODE == 2 ? a.ro.np : a.n;
+    const uint32_t A = r / rn, u = r - A * rn + (MODE == 2 ? a.ro.first : 0u);
     unsigned long long krow = 0;
     uint32_t cnt = 0;
     unsigned long long pos = 0;
@@ -774,7 +857,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
     // locality fit; a candidate outside it sends the row to attempt 1);
     // attempt 1: the exact candidate window, in chunks when it is wider.
     bool empty = false;
-    for (int attempt = (!INIT && a.guess && a.n > kChunkBits) ? 0 : 1; attempt < 2; ++attempt) {
+    for (int attempt = (MODE == 0 && a.guess && a.n > kChunkBits) ? 0 : 1; attempt < 2; ++attempt) {
       uint32_t lo = 0, hi = a.n - 1;
       if (attempt == 0) {
         const uint32_t half = kChunkBits / 2;
@@ -784,8 +867,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       } else if (a.n > kChunkBits) {
         uint32_t mn = 0xFFFFFFFFu, mx = 0;
         unsigned long long kk = 0;
-        for_each_list<INIT, false>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
-          if (INIT) {
+        for_each_list<MODE, false>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
+          if (MODE == 1) {
             // raw rows are unsorted: reduce over every element
             const unsigned long long b = __shfl(lb, 0, kWave);
             const uint32_t l = __shfl(len, 0, kWave);
@@ -801,6 +884,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
             kk += len;
           }
         }, [](unsigned long long, uint32_t, uint32_t) {});
+        if (MODE == 2 && a.ro.kind == 2) {
+          bool has_eps = false;
+          for (int i = 0; i < a.g.ne; ++i) has_eps |= a.g.e_lhs[i] == A;
+          if (has_eps && lane == 0) {
+            mn = min(mn, u);
+            mx = max(mx, u);
+            kk += 1;
+          }
+        }
         kk = wave_sum_u64(kk);
         if (kk == 0) {
           empty = true;
@@ -830,7 +922,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
           const uint32_t span = min(kChunkBits, hi - c0 + 1);
           unsigned long long kch = 0;
           bool miss = false;
-          const uint32_t c = process_chunk<INIT>(a, r, A, u, c0, span, w, lane, kch, miss);
+          const uint32_t c = process_chunk<MODE>(a, r, A, u, c0, span, w, lane, kch, miss);
           if (attempt == 0 && miss) {
             missed = true;
             break;
@@ -869,7 +961,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       a.seg_cnt[r] = cnt;
       kc += krow;
       ne += cnt;
-      if (!INIT && krow) orr += (a.old.ptr[r + 1] - a.old.ptr[r]) + (a.dl.ptr[r + 1] - a.dl.ptr[r]);
+      if (MODE == 0 && krow) orr += (a.old.ptr[r + 1] - a.old.ptr[r]) + (a.dl.ptr[r + 1] - a.dl.ptr[r]);
     }
   }
   if (lane == 0) {
@@ -956,6 +1048,7 @@ __device__ inline uint32_t pack_union(const uint32_t* a, unsigned long long ia, 
 __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t rows, uint32_t n, const Counters* ctr,
                                                   DPacked plab, const unsigned long long* old_pk, uint32_t* out_col,
                                                   unsigned long long* out_pk, uint32_t* pk_len) {
+  // ctr == nullptr: merge every row (no label is skipped)
   __shared__ uint32_t s_win[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   uint32_t* win = s_win[threadIdx.x / kWave];
@@ -963,7 +1056,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
   for (uint32_t r = wave; r < rows; r += nwaves) {
     const uint32_t l = r / n;
-    if (!label_nonempty(ctr, l)) continue;
+    if (ctr && !label_nonempty(ctr, l)) continue;
     const unsigned long long ob = old.ptr[r], oe = old.ptr[r + 1];
     const unsigned long long db = dl.ptr[r], de = dl.ptr[r + 1];
     uint32_t* out = out_col + ob + db;
@@ -1124,6 +1217,10 @@ class Engine {
   uint32_t pack_labels[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool pk_ready = false;
   bool guess_window = true;  // per round: last round's guessed windows rarely missed
+  uint32_t supersteps_r = 0;  // supersteps of the last partitioned-schedule run
+  // host copy of the input (the partitioned schedule builds its regions from it)
+  std::vector<uint32_t> h_src, h_dst;
+  std::vector<uint8_t> h_lab;
   uint64_t old_nnz = 0, dl_nnz = 0, ot_nnz = 0;
   bool pairs_ready = false;
   bool has_closure = false;
@@ -1262,6 +1359,7 @@ class Engine {
   uint32_t rows() const { return uint32_t(uint64_t(L) * n); }
 
   int run(int filter, gg_stats* st, std::string& err);
+  int run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& parts, gg_stats* st, std::string& err);
 };
 
 namespace {
@@ -1278,8 +1376,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     return GG_E_STATE;
   }
   if (filter != 0) {
-    err = "filter_flag != 0 (the reference's dataflow-only schedule) is not supported by this build";
-    return GG_E_UNSUPPORTED;
+    err = "the semi-naive engine computes the least fixpoint only; filter_flag != 0 runs the partitioned schedule";
+    return GG_E_ARG;
   }
   if (uint64_t(L) * n >= 0xFFFFFFFFull || uint64_t(g.nslot) * n >= 0xFFFFFFFFull) {
     err = "num_labels * num_vertices (and binary rules with a derived second label * num_vertices) must be < 2^32";
@@ -1349,7 +1447,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       fill_u32(seg_cnt, 0, R);
       k_activate_raw<<<grid_for(R), kBlock, 0, stream>>>(P<unsigned long long>(raw_ptr), R, P<uint32_t>(active), C());
       GG_HIP(hipEventRecord(x.a, stream));
-      k_expand<true><<<expand_grid(), kBlock, 0, stream>>>(expand_args(true));
+      k_expand<1><<<expand_grid(), kBlock, 0, stream>>>(expand_args(true));
       GG_HIP(hipEventRecord(x.b, stream));
       GG_HIP(hipGetLastError());
       GG_TRY(read_ctr(err));
@@ -1442,7 +1540,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_HIP(hipEventRecord(o.b, stream));
       EventPair& x = mk(ev_expand);
       GG_HIP(hipEventRecord(x.a, stream));
-      k_expand<false><<<expand_grid(), kBlock, 0, stream>>>(expand_args(false));
+      k_expand<0><<<expand_grid(), kBlock, 0, stream>>>(expand_args(false));
       GG_HIP(hipEventRecord(x.b, stream));
       GG_HIP(hipGetLastError());
       GG_TRY(read_ctr(err));
@@ -1557,6 +1655,351 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
 }
 
 // ---------------------------------------------------------------------------
+// The reference's partitioned schedule (filter != 0 needs it): superstep 0
+// closes every partition on its own (precomputation, src/main.cpp:764-804),
+// later supersteps sweep the partition pairs i < j (run_computation,
+// src/main.cpp:806-858), with the operator sequences of spagpu_s / spagpu_b /
+// spagpu (src/kernel.cu:466-780) and mergeAndDiff (src/kernel.cu:943-985).
+// Every region (old, delta, tmp of a partition) is a CSR over the
+// partition's own rows; an operator is one MODE 2 expand launch plus the
+// compaction of its new ids and their merge into tmp.
+
+int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& parts, gg_stats* st,
+                  std::string& err) {
+  if (!has_grammar || !has_graph) {
+    err = "grammar and graph must be loaded before gg_closure_run";
+    return GG_E_STATE;
+  }
+  GG_HIP(hipSetDevice(device));
+  struct Region {
+    DevBuf ptr, col;
+    uint64_t nnz = 0;
+  };
+  struct Part {
+    uint32_t first, np;
+    Region old, dl, tmp;
+    bool has_old = false, has_dl = false, has_tmp = false;
+  };
+  const int NP = int(parts.size());
+  std::vector<Part> pt(static_cast<size_t>(NP));
+  uint32_t max_np = 0;
+  for (int p = 0; p < NP; ++p) {
+    pt[p].first = parts[p].first;
+    pt[p].np = parts[p].second - parts[p].first + 1;
+    max_np = std::max(max_np, pt[p].np);
+  }
+  gg_stats S{};
+  const auto t0 = std::chrono::steady_clock::now();
+  auto cleanup = [&]() {
+    for (auto& q : pt)
+      for (Region* r : {&q.old, &q.dl, &q.tmp}) {
+        release(r->ptr);
+        release(r->col);
+      }
+  };
+  struct Cleanup {
+    std::function<void()> f;
+    ~Cleanup() { f(); }
+  } cl{cleanup};
+  const uint64_t max_rows = uint64_t(L) * max_np;
+  GG_TRY(alloc(ctr, sizeof(Counters), err));
+  GG_TRY(alloc(seg_off, max_rows * 8, err));
+  GG_TRY(alloc(seg_cnt, max_rows * 4, err));
+  GG_TRY(alloc(active, max_rows * 4, err));
+  GG_TRY(grow(staging, uint64_t(expand_grid()) * kWavesPerBlock * kStageBlock * 2 * 4, err));
+  if (!h_ctr) GG_HIP(hipHostMalloc((void**)&h_ctr, sizeof(Counters), hipHostMallocDefault));
+  if (!h_u64) GG_HIP(hipHostMalloc((void**)&h_u64, 8, hipHostMallocDefault));
+  auto rows_of = [&](const Part& q) { return uint64_t(L) * q.np; };
+  auto make_empty = [&](Region& r, uint64_t rows) -> int {
+    GG_TRY(alloc(r.ptr, (rows + 1) * 8, err));
+    fill_u64(r.ptr, 0, rows + 1);
+    r.nnz = 0;
+    return GG_OK;
+  };
+  auto dcsr = [&](Region& r) { return DCsr{P<unsigned long long>(r.ptr), P<uint32_t>(r.col)}; };
+  // initial delta of each partition: its input edges, deduplicated (host)
+  for (int p = 0; p < NP; ++p) {
+    Part& q = pt[p];
+    const uint64_t rows = rows_of(q);
+    std::vector<std::vector<uint32_t>> rr(rows);
+    for (uint64_t i = 0; i < m; ++i) {
+      const uint32_t u = h_src[i];
+      if (u < q.first || u - q.first >= q.np) continue;
+      rr[uint64_t(h_lab[i] - 1) * q.np + (u - q.first)].push_back(h_dst[i]);
+    }
+    std::vector<unsigned long long> ptr(rows + 1, 0);
+    std::vector<uint32_t> col;
+    for (uint64_t r = 0; r < rows; ++r) {
+      auto& v = rr[r];
+      std::sort(v.begin(), v.end());
+      v.erase(std::unique(v.begin(), v.end()), v.end());
+      col.insert(col.end(), v.begin(), v.end());
+      ptr[r + 1] = col.size();
+    }
+    GG_TRY(alloc(q.dl.ptr, (rows + 1) * 8, err));
+    GG_TRY(alloc(q.dl.col, std::max<size_t>(col.size(), 1) * 4, err));
+    GG_HIP(hipMemcpy(q.dl.ptr.p, ptr.data(), (rows + 1) * 8, hipMemcpyHostToDevice));
+    if (!col.empty()) GG_HIP(hipMemcpy(q.dl.col.p, col.data(), col.size() * 4, hipMemcpyHostToDevice));
+    q.dl.nnz = col.size();
+    q.has_dl = true;
+    GG_TRY(make_empty(q.old, rows));
+    GG_TRY(make_empty(q.tmp, rows));
+    S.input_unique += col.size();
+  }
+  // dst = a U b (disjoint sorted rows), into fresh buffers
+  auto merge_regions = [&](Region& a, Region& b, uint64_t rows, uint32_t np, Region& out) -> int {
+    GG_TRY(alloc(out.ptr, (rows + 1) * 8, err));
+    GG_TRY(alloc(out.col, std::max<uint64_t>(a.nnz + b.nnz, 1) * 4, err));
+    k_add_ptr<<<grid_for(rows + 1), kBlock, 0, stream>>>(P<unsigned long long>(out.ptr), P<unsigned long long>(a.ptr),
+                                                        P<unsigned long long>(b.ptr), rows + 1);
+    k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(dcsr(a), dcsr(b), uint32_t(rows), np, nullptr, DPacked{}, nullptr,
+                                                      P<uint32_t>(out.col), nullptr, nullptr);
+    GG_HIP(hipGetLastError());
+    out.nnz = a.nnz + b.nnz;
+    return GG_OK;
+  };
+  // One operator on partition p; fresh ids go to `fresh` (kind 3) or are
+  // merged into p's tmp. Returns the number of fresh ids in *nfresh.
+  struct Dst {
+    Region* r;
+    int part;
+  };
+  auto op = [&](int p, int kind, Region* key, std::vector<Dst> dsts, Region* seed, Region* fresh,
+                uint64_t* nfresh) -> int {
+    Part& q = pt[p];
+    const uint64_t rows = rows_of(q);
+    ExpandArgs a{};
+    a.n = n;
+    a.g = g;
+    a.active = P<uint32_t>(active);
+    a.ctr = C();
+    a.seg_off = P<unsigned long long>(seg_off);
+    a.seg_cnt = P<uint32_t>(seg_cnt);
+    a.ro.kind = kind;
+    a.ro.first = q.first;
+    a.ro.np = q.np;
+    a.ro.seed = dcsr(*seed);
+    a.ro.key = key ? dcsr(*key) : DCsr{nullptr, nullptr};
+    a.ro.ndst = int(dsts.size());
+    for (size_t d = 0; d < dsts.size(); ++d) {
+      a.ro.dst[d] = dcsr(*dsts[d].r);
+      a.ro.dfirst[d] = pt[dsts[d].part].first;
+      a.ro.dn[d] = pt[dsts[d].part].np;
+    }
+    for (;;) {
+      a.staging = P<uint32_t>(staging);
+      a.staging_cap = staging.bytes / 4;
+      zero_ctr();
+      fill_u32(seg_cnt, 0, rows);
+      k_all_active<<<grid_for(rows), kBlock, 0, stream>>>(uint32_t(rows), P<uint32_t>(active), C());
+      EventPair x;
+      k_expand<2><<<expand_grid(), kBlock, 0, stream>>>(a);
+      GG_HIP(hipGetLastError());
+      GG_TRY(read_ctr(err));
+      S.expand_launches++;
+      if (!h_ctr->overflow) break;
+      S.retries++;
+      GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
+    }
+    const uint64_t N = h_ctr->newe;
+    *nfresh = N;
+    Region F;
+    Region* out = fresh ? fresh : &F;
+    GG_TRY(alloc(out->ptr, (rows + 1) * 8, err));
+    GG_TRY(scan(seg_cnt, rows, out->ptr, err));
+    GG_TRY(alloc(out->col, std::max<uint64_t>(N, 1) * 4, err));
+    if (N)
+      k_gather<<<expand_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
+                                                     P<unsigned long long>(seg_off), P<uint32_t>(staging),
+                                                     P<unsigned long long>(out->ptr), P<uint32_t>(out->col));
+    out->nnz = N;
+    if (!fresh) {
+      if (N) {
+        Region M;
+        GG_TRY(merge_regions(*seed, F, rows, q.np, M));
+        std::swap(seed->ptr, M.ptr);
+        std::swap(seed->col, M.col);
+        seed->nnz = M.nnz;
+        GG_HIP(hipStreamSynchronize(stream));
+        release(M.ptr);
+        release(M.col);
+      }
+      GG_HIP(hipStreamSynchronize(stream));
+      release(F.ptr);
+      release(F.col);
+    }
+    return GG_OK;
+  };
+  bool changed = false;
+  // compute11 / compute10 / compute00 into tmp of p (skipped when the key
+  // region does not exist, as the reference's size checks do)
+  auto exists = [&](int p, int which) {
+    return which == 0 ? pt[p].has_old : which == 1 ? pt[p].has_dl : pt[p].has_tmp;
+  };
+  auto reg = [&](int p, int which) -> Region* { return which == 0 ? &pt[p].old : which == 1 ? &pt[p].dl : &pt[p].tmp; };
+  auto join = [&](int p, int key, std::vector<std::pair<int, int>> dsts) -> int {
+    if (!exists(p, key)) return GG_OK;
+    std::vector<Dst> ds;
+    for (auto& d : dsts)
+      if (exists(d.second, d.first)) ds.push_back({reg(d.second, d.first), d.second});
+    if (ds.empty()) return GG_OK;
+    uint64_t nf = 0;
+    GG_TRY(op(p, 0, reg(p, key), ds, &pt[p].tmp, nullptr, &nf));
+    changed |= nf > 0;
+    return GG_OK;
+  };
+  auto unary = [&](int p, int from) -> int {
+    if (!exists(p, from) || g.nu == 0) return GG_OK;
+    uint64_t nf = 0;
+    GG_TRY(op(p, 1, reg(p, from), {}, &pt[p].tmp, nullptr, &nf));
+    changed |= nf > 0;
+    return GG_OK;
+  };
+  auto eps = [&](int p) -> int {
+    if (g.ne == 0) return GG_OK;
+    uint64_t nf = 0;
+    GG_TRY(op(p, 2, nullptr, {}, &pt[p].tmp, nullptr, &nf));
+    changed |= nf > 0;
+    return GG_OK;
+  };
+  auto merge_and_diff = [&](int p) -> int {
+    Part& q = pt[p];
+    const uint64_t rows = rows_of(q);
+    if (q.has_dl) {
+      if (!q.has_old) {
+        std::swap(q.old, q.dl);
+      } else {
+        Region M;
+        GG_TRY(merge_regions(q.old, q.dl, rows, q.np, M));
+        GG_HIP(hipStreamSynchronize(stream));
+        release(q.old.ptr);
+        release(q.old.col);
+        q.old.ptr = M.ptr;
+        q.old.col = M.col;
+        q.old.nnz = M.nnz;
+      }
+      release(q.dl.ptr);
+      release(q.dl.col);
+      GG_TRY(make_empty(q.dl, rows));
+      q.has_old = true;
+      q.has_dl = false;
+    }
+    if (q.has_tmp) {
+      Region D;
+      uint64_t nf = 0;
+      GG_TRY(op(p, 3, &q.tmp, {}, &q.old, &D, &nf));
+      GG_HIP(hipStreamSynchronize(stream));
+      release(q.dl.ptr);
+      release(q.dl.col);
+      q.dl.ptr = D.ptr;
+      q.dl.col = D.col;
+      q.dl.nnz = nf;
+      q.has_dl = nf > 0;
+      release(q.tmp.ptr);
+      release(q.tmp.col);
+      GG_TRY(make_empty(q.tmp, rows));
+      q.has_tmp = false;
+    }
+    return GG_OK;
+  };
+  enum { kO = 0, kD = 1, kT = 2 };
+  // superstep 0
+  for (int p = 0; p < NP; ++p) {
+    for (;;) {
+      Part& q = pt[p];
+      q.has_tmp = true;
+      if (!q.has_old) GG_TRY(eps(p));
+      if (filter == 0) GG_TRY(unary(p, kD));
+      if (q.has_old) {
+        if (filter == 0) GG_TRY(join(p, kO, {{kD, p}}));
+        GG_TRY(join(p, kD, {{kO, p}, {kD, p}}));
+      } else {
+        GG_TRY(join(p, kD, {{kD, p}}));
+      }
+      GG_TRY(merge_and_diff(p));
+      S.rounds++;
+      if (!q.has_dl) {
+        std::swap(q.dl, q.old);
+        q.has_dl = q.has_old;
+        q.has_old = false;
+        break;
+      }
+    }
+  }
+  // supersteps 1..
+  uint32_t supersteps = 0;
+  for (;;) {
+    ++supersteps;
+    for (int i = 0; i < NP; ++i) {
+      for (int j = i + 1; j < NP; ++j) {
+        pt[i].has_tmp = true;
+        pt[j].has_tmp = true;
+        if (filter == 0) GG_TRY(join(i, kO, {{kD, j}}));
+        GG_TRY(join(i, kD, {{kO, j}, {kD, j}}));
+        if (filter == 0) GG_TRY(join(j, kO, {{kD, i}}));
+        GG_TRY(join(j, kD, {{kO, i}, {kD, i}}));
+        for (;;) {
+          changed = false;
+          for (int side = 0; side < 2; ++side) {
+            const int p = side ? j : i, o = side ? i : j;
+            if (filter == 0) GG_TRY(join(p, kO, {{kT, i}, {kT, j}}));
+            GG_TRY(join(p, kD, {{kT, i}, {kT, j}}));
+            GG_TRY(join(p, kT, {{kO, p}, {kD, p}}));
+            GG_TRY(join(p, kT, {{kO, o}, {kD, o}}));
+            if (filter == 0) GG_TRY(unary(p, kT));
+            GG_TRY(join(p, kT, {{kT, i}, {kT, j}}));
+          }
+          S.rounds++;
+          if (!changed) break;
+        }
+      }
+      GG_TRY(merge_and_diff(i));
+    }
+    bool done = true;
+    for (auto& q : pt) done &= !q.has_dl;
+    if (done) break;
+  }
+  supersteps_r = supersteps;
+  // the closure is every partition's old region: assemble the global CSR
+  GG_HIP(hipStreamSynchronize(stream));
+  const uint64_t R = rows();
+  std::vector<unsigned long long> gptr(R + 1, 0);
+  std::vector<std::vector<unsigned long long>> lptr(static_cast<size_t>(NP));
+  for (int p = 0; p < NP; ++p) {
+    lptr[p].resize(rows_of(pt[p]) + 1);
+    GG_HIP(hipMemcpy(lptr[p].data(), pt[p].old.ptr.p, lptr[p].size() * 8, hipMemcpyDeviceToHost));
+  }
+  uint64_t total = 0;
+  for (int l = 0; l < L; ++l)
+    for (int p = 0; p < NP; ++p)
+      for (uint32_t k = 0; k < pt[p].np; ++k) {
+        const uint64_t lr = uint64_t(l) * pt[p].np + k;
+        const uint64_t gr = uint64_t(l) * n + pt[p].first + k;
+        gptr[gr + 1] = lptr[p][lr + 1] - lptr[p][lr];
+      }
+  for (uint64_t r = 0; r < R; ++r) gptr[r + 1] += gptr[r];
+  total = gptr[R];
+  GG_TRY(alloc(old_ptr, (R + 1) * 8, err));
+  GG_TRY(alloc(old_col, std::max<uint64_t>(total, 1) * 4, err));
+  GG_HIP(hipMemcpy(old_ptr.p, gptr.data(), (R + 1) * 8, hipMemcpyHostToDevice));
+  for (int l = 0; l < L; ++l)
+    for (int p = 0; p < NP; ++p) {
+      const uint64_t lb = lptr[p][uint64_t(l) * pt[p].np], le = lptr[p][uint64_t(l + 1) * pt[p].np];
+      if (le > lb)
+        GG_HIP(hipMemcpy(static_cast<uint32_t*>(old_col.p) + gptr[uint64_t(l) * n + pt[p].first],
+                         static_cast<uint32_t*>(pt[p].old.col.p) + lb, (le - lb) * 4, hipMemcpyDeviceToDevice));
+    }
+  GG_HIP(hipStreamSynchronize(stream));
+  S.total_edges = total;
+  S.fixpoint_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  S.device_bytes_peak = peak;
+  old_nnz = total;
+  has_closure = true;
+  if (st) *st = S;
+  return GG_OK;
+}
+
+// ---------------------------------------------------------------------------
 // engine API
 
 Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std::string& err) {
@@ -1598,7 +2041,7 @@ Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std
   }
   en->cus = prop.multiProcessorCount;
   int bpc = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_expand<false>), kBlock, 0) ==
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_expand<0>), kBlock, 0) ==
           hipSuccess &&
       bpc > 0)
     en->expand_blocks_per_cu = bpc;
@@ -1682,6 +2125,9 @@ int engine_set_graph(Engine* e, const Graph& gr, std::string& err) {
     GG_HIP(hipMemcpyAsync(e->in_lab.p, gr.lab.data(), e->m, hipMemcpyHostToDevice, e->stream));
   }
   GG_HIP(hipStreamSynchronize(e->stream));
+  e->h_src = gr.src;
+  e->h_dst = gr.dst;
+  e->h_lab = gr.lab;
   e->has_graph = true;
   e->has_closure = false;
   return GG_OK;
@@ -1689,7 +2135,25 @@ int engine_set_graph(Engine* e, const Graph& gr, std::string& err) {
 
 int engine_run(Engine* e, int filter, gg_stats* stats, std::string& err) {
   e->has_closure = false;
-  return e->run(filter, stats, err);
+  const int s = e->run(filter, stats, err);
+  if (s == GG_OK && stats) {
+    stats->supersteps = 0;
+    stats->partitions = 1;
+  }
+  return s;
+}
+
+int engine_run_partitioned(Engine* e, int filter, const std::vector<std::pair<uint32_t, uint32_t>>& parts,
+                           gg_stats* stats, std::string& err) {
+  e->has_closure = false;
+  gg_stats st{};
+  const int s = e->run_r(filter, parts, &st, err);
+  if (s == GG_OK && stats) {
+    *stats = st;
+    stats->supersteps = e->supersteps_r;
+    stats->partitions = uint32_t(parts.size());
+  }
+  return s;
 }
 
 namespace {

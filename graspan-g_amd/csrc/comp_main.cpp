@@ -134,7 +134,12 @@ int main(int argc, char** argv) {
   std::printf("NUM VERTICES: %u\nINITIAL NUM EDGES: %llu\n", nv, (unsigned long long)ne);
   const uint64_t heap = (opts.device_budget ? opts.device_budget : fr) / 4;
   std::printf("HEAP SIZE: %llu (in 32-bit words)\n", (unsigned long long)heap);
-  std::printf("INITIAL NUM PARTITIONS: 1\n");
+  int nparts = 1;
+  if (filter != 0) {
+    // the reference's schedule: its own vertex intervals
+    if ((s = gg_reference_partitions(c, nullptr, nullptr, 0, &nparts)) != GG_OK) return die(c, "partitions", s);
+  }
+  std::printf("INITIAL NUM PARTITIONS: %d\n", nparts);
   std::fflush(stdout);
 
   gg_stats st{};
@@ -144,8 +149,8 @@ int main(int argc, char** argv) {
   if ((s = gg_closure_counts(c, per.data(), &total)) != GG_OK) return die(c, "counts", s);
 
   std::printf("===== SPAGPU FINISHED =====\n");
-  std::printf("NUM SUPERSTEPS: %u\n", st.rounds);
-  std::printf("FINAL NUM PARTITIONS: 1\n");
+  std::printf("NUM SUPERSTEPS: %u\n", filter != 0 ? st.supersteps : st.rounds);
+  std::printf("FINAL NUM PARTITIONS: %u\n", st.partitions);
   std::printf("FINAL NUM EDGES: %llu\n", (unsigned long long)total);
   std::printf("TOTAL SPAGPU TIME: %ldmin.\n", long((clock() - start) / (CLOCKS_PER_SEC * 60)));
   std::fflush(stdout);

@@ -1,7 +1,10 @@
 // C ABI of the closure engine (declarations and reference citations in
 // include/graspan_g.h). Host C++; the device work lives in closure.hip.
+#include <algorithm>
 #include <cstring>
 #include <new>
+#include <utility>
+#include <vector>
 #include <string>
 
 #include "engine.h"
@@ -46,6 +49,61 @@ int push_graph(gg_ctx* c) {
   s = gg::engine_set_graph(c->engine, c->graph, c->err);
   if (s == GG_OK) c->graph_on_device = true;
   return s;
+}
+
+// The reference's vertex intervals: initializePartitions (src/main.cpp:141-175)
+// over the degree pre-pass of getGraphInfo (src/main.cpp:106-139), which
+// counts per source its input lines and its distinct (dst/192, label)
+// "elements" (192-id bitmap elements, src/kernel.h:35-41). The heap is the
+// reference's fixed DEVMEMUSED_MB = 10000 (src/kernel.h:26, src/kernel.cu:35);
+// a partition may hold heap/4 words of 8*elements + roundup32(vertices)*labels.
+// One partition that fits is split at the element midpoint.
+int reference_partitions(const gg::Graph& gr, int L, std::vector<std::pair<uint32_t, uint32_t>>& parts,
+                         std::string& err) {
+  const uint32_t n = gr.n;
+  std::vector<uint64_t> elems(n, 0);
+  {
+    std::vector<std::vector<uint64_t>> keys(n);
+    for (size_t i = 0; i < gr.src.size(); ++i) keys[gr.src[i]].push_back((uint64_t(gr.dst[i] / 192) << 8) | gr.lab[i]);
+    for (uint32_t u = 0; u < n; ++u) {
+      auto& k = keys[u];
+      std::sort(k.begin(), k.end());
+      elems[u] = uint64_t(std::unique(k.begin(), k.end()) - k.begin());
+    }
+  }
+  const uint64_t heap = (10000ull / 8 * 7) * 1024 * 256;
+  const uint64_t limit = heap / 4;
+  auto rup32 = [](uint64_t x) { return (x + 31) / 32 * 32; };
+  parts.clear();
+  uint64_t t1 = 0, t3 = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    t1 += elems[i];
+    if (t1 * 8 + rup32(i - t3 + 1) * uint64_t(L) > limit) {
+      if (i == 0 || uint64_t(i - 1) < t3) {
+        err = "degenerate partitioning";
+        return GG_E_UNSUPPORTED;
+      }
+      parts.push_back({uint32_t(t3), i - 1});
+      t1 = elems[i];
+      t3 = i;
+    }
+  }
+  parts.push_back({uint32_t(t3), n - 1});
+  if (parts.size() == 1) {
+    uint64_t t4 = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      t4 += elems[i];
+      if (t4 > t1 / 2) {
+        if (i == 0) break;
+        parts[0].second = i - 1;
+        parts.push_back({i, n - 1});
+        return GG_OK;
+      }
+    }
+    err = "degenerate partitioning: the reference's small-graph split needs a second vertex interval";
+    return GG_E_UNSUPPORTED;
+  }
+  return GG_OK;
 }
 
 }  // namespace
@@ -191,9 +249,40 @@ int gg_graph_edges(const gg_ctx* c, uint32_t* src, uint32_t* dst, uint8_t* lab) 
   return GG_OK;
 }
 
+int gg_closure_run_partitioned(gg_ctx* c, int filter_flag, gg_stats* stats) {
+  if (!c) return GG_E_ARG;
+  if (!c->has_grammar || !c->has_graph) return fail(c, GG_E_STATE, "load a grammar and a graph first");
+  std::vector<std::pair<uint32_t, uint32_t>> parts;
+  int s = reference_partitions(c->graph, c->grammar.num_labels(), parts, c->err);
+  if (s != GG_OK) return s;
+  s = ensure_engine(c);
+  if (s != GG_OK) return s;
+  if (!c->graph_on_device) {
+    s = push_graph(c);
+    if (s != GG_OK) return s;
+  }
+  return gg::engine_run_partitioned(c->engine, filter_flag, parts, stats, c->err);
+}
+
+int gg_reference_partitions(const gg_ctx* c, uint32_t* first, uint32_t* last, int cap, int* count) {
+  if (!c || !count) return GG_E_ARG;
+  if (!c->has_grammar || !c->has_graph) return GG_E_STATE;
+  std::vector<std::pair<uint32_t, uint32_t>> parts;
+  std::string err;
+  const int s = reference_partitions(c->graph, c->grammar.num_labels(), parts, err);
+  if (s != GG_OK) return s;
+  *count = int(parts.size());
+  for (int i = 0; i < cap && size_t(i) < parts.size(); ++i) {
+    if (first) first[i] = parts[size_t(i)].first;
+    if (last) last[i] = parts[size_t(i)].second;
+  }
+  return GG_OK;
+}
+
 int gg_closure_run(gg_ctx* c, int filter_flag, gg_stats* stats) {
   if (!c) return GG_E_ARG;
   if (!c->has_grammar || !c->has_graph) return fail(c, GG_E_STATE, "load a grammar and a graph first");
+  if (filter_flag != 0) return gg_closure_run_partitioned(c, filter_flag, stats);
   int s = ensure_engine(c);
   if (s != GG_OK) return s;
   if (!c->graph_on_device) {

@@ -31,7 +31,7 @@ EXPORTED = (
     "gg_ctx_create", "gg_ctx_destroy", "gg_last_error", "gg_version",
     "gg_grammar_load_file", "gg_grammar_set", "gg_grammar_info", "gg_grammar_rules", "gg_label_name", "gg_label_id",
     "gg_graph_load_file", "gg_graph_set_coo", "gg_graph_info", "gg_graph_edges",
-    "gg_closure_run", "gg_closure_counts", "gg_closure_export", "gg_closure_degrees", "gg_closure_checksum",
+    "gg_closure_run", "gg_closure_run_partitioned", "gg_reference_partitions", "gg_closure_counts", "gg_closure_export", "gg_closure_degrees", "gg_closure_checksum",
     "gg_triple_hash", "gg_closure_release", "gg_device_info",
     "gg_batch_create", "gg_batch_destroy", "gg_batch_add", "gg_batch_run", "gg_batch_info", "gg_batch_result",
     "gg_batch_export",
@@ -58,6 +58,7 @@ class Stats(ctypes.Structure):
         ("expand_launches", ctypes.c_uint32),
         ("probe_reads", ctypes.c_uint64), ("candidates", ctypes.c_uint64), ("old_reads", ctypes.c_uint64),
         ("new_edges", ctypes.c_uint64), ("device_bytes_peak", ctypes.c_uint64),
+        ("supersteps", ctypes.c_uint32), ("partitions", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -87,7 +88,8 @@ def lib():
             "gg_label_name": (ctypes.c_char_p, [P, I]), "gg_label_id": (I, [P, ctypes.c_char_p]),
             "gg_graph_load_file": (I, [P, ctypes.c_char_p]), "gg_graph_set_coo": (I, [P, U32, U64, P, P, P]),
             "gg_graph_info": (I, [P, P, P]), "gg_graph_edges": (I, [P, P, P, P]),
-            "gg_closure_run": (I, [P, I, P]), "gg_closure_counts": (I, [P, P, P]),
+            "gg_closure_run": (I, [P, I, P]), "gg_closure_run_partitioned": (I, [P, I, P]),
+            "gg_closure_counts": (I, [P, P, P]), "gg_reference_partitions": (I, [P, P, P, I, P]),
             "gg_closure_export": (I, [P, P, U64, P]), "gg_closure_degrees": (I, [P, P]),
             "gg_closure_checksum": (I, [P, P]), "gg_triple_hash": (U64, [U32, U32, U32]),
             "gg_closure_release": (I, [P]),
@@ -218,6 +220,21 @@ class Context:
         st = Stats()
         self._check(self._L.gg_closure_run(self._h, filter_flag, ctypes.byref(st)), "gg_closure_run")
         return st
+
+    def run_partitioned(self, filter_flag: int = 0) -> Stats:
+        """The reference's partitioned schedule (any filter value)."""
+        st = Stats()
+        self._check(self._L.gg_closure_run_partitioned(self._h, filter_flag, ctypes.byref(st)),
+                    "gg_closure_run_partitioned")
+        return st
+
+    def reference_partitions(self):
+        cnt = ctypes.c_int()
+        first = np.zeros(64, np.uint32)
+        last = np.zeros(64, np.uint32)
+        self._check(self._L.gg_reference_partitions(self._h, _ptr(first), _ptr(last), 64, ctypes.byref(cnt)),
+                    "gg_reference_partitions")
+        return list(zip(first[:cnt.value].tolist(), last[:cnt.value].tolist()))
 
     def counts(self):
         _, L = self.grammar_info()

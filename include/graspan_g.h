@@ -69,6 +69,9 @@ typedef struct gg_stats {
   uint64_t old_reads;
   uint64_t new_edges;
   uint64_t device_bytes_peak; /* high-water mark of the context's device arena */
+  uint32_t supersteps;      /* partitioned schedule: supersteps after superstep 0
+                               (the reference's NUM SUPERSTEPS); 0 otherwise */
+  uint32_t partitions;      /* partitioned schedule: vertex intervals; 1 otherwise */
 } gg_stats;
 
 /* ---- context ------------------------------------------------------------ */
@@ -124,10 +127,28 @@ int gg_graph_edges(const gg_ctx *ctx, uint32_t *src, uint32_t *dst, uint8_t *lab
 /* ---- closure ------------------------------------------------------------ */
 
 /* Runs the fixpoint on the GPU. filter_flag follows the reference's third
- * argument (src/main.cpp:890, src/kernel.cu:490-780): 0 computes the least
- * fixpoint; a non-zero value is GG_E_UNSUPPORTED in this build. stats may be
- * NULL. The input edges stay resident on the device across runs. */
+ * argument (src/main.cpp:890, src/kernel.cu:490-780). 0 computes the least
+ * fixpoint with the device-resident semi-naive engine (its result does not
+ * depend on the schedule). A non-zero value skips the unary rules and the
+ * old x delta joins, which makes the result depend on the reference's
+ * partitions and sweep order, so it runs gg_closure_run_partitioned. stats
+ * may be NULL. The input edges stay resident on the device across runs. */
 int gg_closure_run(gg_ctx *ctx, int filter_flag, gg_stats *stats);
+
+/* Runs the reference's own schedule (initializePartitions, precomputation,
+ * run_computation, spagpu_s/spagpu_b/spagpu, mergeAndDiff;
+ * src/main.cpp:141-175, 764-906, src/kernel.cu:466-985) on the GPU, one
+ * device operator per reference kernel launch, for any filter_flag. With
+ * filter 0 it reaches the same least fixpoint as gg_closure_run, more slowly.
+ * GG_E_UNSUPPORTED when the reference's small-graph split is degenerate (a
+ * first vertex holding more than half of the elements, or no elements). */
+int gg_closure_run_partitioned(gg_ctx *ctx, int filter_flag, gg_stats *stats);
+
+/* The reference's vertex intervals for the loaded graph (host only; no GPU):
+ * up to cap (first, last) pairs, *count receives their number. Follows
+ * initializePartitions (src/main.cpp:141-175) over getGraphInfo's degree
+ * pre-pass (src/main.cpp:106-139). */
+int gg_reference_partitions(const gg_ctx *ctx, uint32_t *first, uint32_t *last, int cap, int *count);
 
 /* Per-label edge counts of the last closure (num_labels uint64, index l-1 for
  * label l) and their sum (the reference's FINAL NUM EDGES, src/main.cpp:909-917,

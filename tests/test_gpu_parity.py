@@ -121,12 +121,43 @@ def test_gpu_empty_graph_eps_only(built):
     np.testing.assert_array_equal(t, np.asarray(exp, np.uint32))
 
 
-def test_gpu_filter_flag_unsupported(built):
-    say("filter=1")
-    d = load_golden(golden_files()[0])
+def _cases_r():
+    out = []
+    for p in golden_files():
+        d = load_golden(p)
+        out.append((d["name"], d["n"], d["rules_arr"], d["src"], d["dst"], d["lab"], d["num_labels"]))
+    for name, g in [("alias700", gen.alias(700, seed=41)), ("aliasdf2000", gen.alias_as_dataflow(2000, seed=42)),
+                    ("dataflow800", gen.dataflow(800, seed=43))]:
+        out.append((name, g.n, g.rules, g.src, g.dst, g.lab, g.num_labels))
+    return out
+
+
+@pytest.mark.parametrize("flt", [1, 0])
+def test_gpu_partitioned_schedule_matches_oracle(built, flt):
+    """The reference's partitioned schedule on the GPU against the oracle's
+    transcription of it (mode R): same closure, same superstep count, same
+    partitions; with filter 0 both also equal the least fixpoint."""
+    for name, n, rules, src, dst, lab, L in _cases_r():
+        say("partitioned", name, "filter", flt)
+        ref = oracle.closure_r(n, rules, src, dst, lab, L, flt)
+        c = gg.Context()
+        c.set_grammar(rules, L)
+        c.set_graph(n, src, dst, lab)
+        assert c.reference_partitions() == ref.parts
+        st = c.run(flt) if flt else c.run_partitioned(0)
+        assert st.supersteps == ref.supersteps and st.partitions == len(ref.parts)
+        np.testing.assert_array_equal(c.export(), ref.triples)
+        if flt == 0:
+            g_ref = oracle.closure(n, rules, src, dst, lab, L)
+            np.testing.assert_array_equal(ref.triples, g_ref.triples)
+
+
+def test_gpu_partitioned_degenerate_split_rejected(built):
+    """No input edges: the reference's split has no second interval."""
+    say("degenerate split")
     c = gg.Context()
-    c.set_grammar(d["rules_arr"], d["num_labels"])
-    c.set_graph(d["n"], d["src"], d["dst"], d["lab"])
+    c.set_grammar(np.array([[1, 0, 0]], np.int32), 1)
+    c.set_graph(5, np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint8))
     with pytest.raises(gg.GGError) as ei:
         c.run(1)
     assert ei.value.status == gg.GG_E_UNSUPPORTED
@@ -147,6 +178,14 @@ def test_comp_cli_report_and_dump(built, tmp_path):
         assert want in lines, want
     got = np.fromfile(dump, dtype=np.uint32).reshape(-1, 3)
     np.testing.assert_array_equal(got, d["expected_arr"])
+    # filter 1: the reference's schedule, its partitions and superstep count
+    ref = oracle.closure_r(d["n"], d["rules_arr"], d["src"], d["dst"], d["lab"], d["num_labels"], 1)
+    p1 = subprocess.run([gg.COMP_PATH, os.path.join(GOLDEN, "sample_graph.txt"),
+                         os.path.join(GOLDEN, "sample_grammar.txt"), "1"], capture_output=True, text=True, timeout=120)
+    assert p1.returncode == 0, p1.stderr
+    for want in [f"INITIAL NUM PARTITIONS: {len(ref.parts)}", f"NUM SUPERSTEPS: {ref.supersteps}",
+                 f"FINAL NUM EDGES: {ref.total}"]:
+        assert want in p1.stdout.splitlines(), want
     # code order (grammar first) is detected and swapped
     p2 = subprocess.run([gg.COMP_PATH, os.path.join(GOLDEN, "sample_grammar.txt"),
                          os.path.join(GOLDEN, "sample_graph.txt")], capture_output=True, text=True, timeout=120)

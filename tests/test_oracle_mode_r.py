@@ -15,8 +15,6 @@ from graspan_g import gen
 @pytest.mark.parametrize("path", golden_files(), ids=lambda p: p.rsplit("/", 1)[-1])
 def test_mode_r_filter0_is_least_fixpoint(built, path):
     d = load_golden(path)
-    if d["name"] == "tiny_dups.json":
-        pytest.skip("vertex 5 alone holds the upper half of the elements: the reference's split is degenerate")
     r = oracle.closure_r(d["n"], d["rules_arr"], d["src"], d["dst"], d["lab"], d["num_labels"], 0)
     np.testing.assert_array_equal(r.triples, d["expected_arr"])
 
@@ -49,6 +47,14 @@ def test_mode_r_filter1_dataflow_derives_nothing(built):
     f1 = oracle.closure_r_of(g, filter_flag=1)
     assert f1.total == g.m  # only the input e edges: N ::= e is unary and skipped
     assert f1.counts[0] == 0
+
+
+def test_mode_r_degenerate_split_rejected(built):
+    """With no input edges the reference's small-graph split finds no second
+    interval (its loop never passes the midpoint); the oracle refuses."""
+    with pytest.raises(ValueError):
+        oracle.closure_r(5, np.array([[1, 0, 0]], np.int32), np.zeros(0, np.uint32), np.zeros(0, np.uint32),
+                         np.zeros(0, np.uint8), 1, 1)
 
 
 def test_mode_r_partitions_split_at_element_midpoint(built):
