@@ -59,6 +59,7 @@ constexpr int kUnroll = 8;              // loads in flight per lane in the candi
 constexpr uint32_t kGrab = 16;          // active rows taken per work-counter atomic
 constexpr uint32_t kXcds = 8;           // MI355X: 8 XCDs, each with its own L2
 constexpr int kExpandWaves = 4;         // k_expand occupancy target (waves per SIMD)
+constexpr uint32_t kMaxCrossParts = 1024;  // partitions of the out-of-core schedule
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
 constexpr int kMaxRules = 64;
 constexpr int kScanItems = 8;
@@ -264,6 +265,33 @@ __global__ void k_scatter_raw(const uint32_t* src, const uint32_t* dst, const ui
     uint32_t r = uint32_t(g.e_lhs[e]) * n + v;
     col[ptr[r] + atomicAdd(&fill[r], 1u)] = v;
   }
+}
+
+// Histogram of the partitions the ids of a region point into (bounds holds
+// the P+1 first vertices). Per-block LDS counters, flushed with one atomic
+// each.
+__global__ __launch_bounds__(kBlock) void k_cross(DCsr c, uint32_t rows, const uint32_t* bounds, uint32_t P,
+                                                  unsigned long long* out) {
+  __shared__ unsigned long long hist[kMaxCrossParts];
+  for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  const unsigned long long total = c.ptr[rows];
+  for (unsigned long long i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < total;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t v = c.col[i];
+    uint32_t lo = 0, hi = P;  // largest q with bounds[q] <= v
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (bounds[mid] <= v)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    atomicAdd(&hist[lo], 1ull);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < P; i += blockDim.x)
+    if (hist[i]) atomicAdd(&out[i], hist[i]);
 }
 
 // Active list = every row 0..rows-1 (region operators).
@@ -1359,7 +1387,8 @@ class Engine {
   uint32_t rows() const { return uint32_t(uint64_t(L) * n); }
 
   int run(int filter, gg_stats* st, std::string& err);
-  int run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& parts, gg_stats* st, std::string& err);
+  int run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& parts, const PartOpts& po, gg_stats* st,
+            std::string& err);
 };
 
 namespace {
@@ -1664,22 +1693,29 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
 // partition's own rows; an operator is one MODE 2 expand launch plus the
 // compaction of its new ids and their merge into tmp.
 
-int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& parts, gg_stats* st,
-                  std::string& err) {
+int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& parts, const PartOpts& po,
+                  gg_stats* st, std::string& err) {
   if (!has_grammar || !has_graph) {
     err = "grammar and graph must be loaded before gg_closure_run";
     return GG_E_STATE;
   }
   GG_HIP(hipSetDevice(device));
+  struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
   struct Region {
     DevBuf ptr, col;
     uint64_t nnz = 0;
+    HostBuf hptr, hcol;  // spilled copy (pinned)
   };
   struct Part {
-    uint32_t first, np;
-    Region old, dl, tmp;
-    bool has_old = false, has_dl = false, has_tmp = false;
+    uint32_t first = 0, np = 0;
+    Region r[3];  // old, delta, tmp
+    bool has[3] = {false, false, false};
+    bool resident = false;
   };
+  enum { kO = 0, kD = 1, kT = 2 };
   const int NP = int(parts.size());
   std::vector<Part> pt(static_cast<size_t>(NP));
   uint32_t max_np = 0;
@@ -1690,17 +1726,21 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   }
   gg_stats S{};
   const auto t0 = std::chrono::steady_clock::now();
-  auto cleanup = [&]() {
-    for (auto& q : pt)
-      for (Region* r : {&q.old, &q.dl, &q.tmp}) {
-        release(r->ptr);
-        release(r->col);
-      }
-  };
+  hipStream_t copy_stream = nullptr;
+  GG_HIP(hipStreamCreate(&copy_stream));
   struct Cleanup {
     std::function<void()> f;
     ~Cleanup() { f(); }
-  } cl{cleanup};
+  } cl{[&]() {
+    for (auto& q : pt)
+      for (auto& r : q.r) {
+        release(r.ptr);
+        release(r.col);
+        if (r.hptr.p) (void)hipHostFree(r.hptr.p);
+        if (r.hcol.p) (void)hipHostFree(r.hcol.p);
+      }
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+  }};
   const uint64_t max_rows = uint64_t(L) * max_np;
   GG_TRY(alloc(ctr, sizeof(Counters), err));
   GG_TRY(alloc(seg_off, max_rows * 8, err));
@@ -1710,61 +1750,141 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   if (!h_ctr) GG_HIP(hipHostMalloc((void**)&h_ctr, sizeof(Counters), hipHostMallocDefault));
   if (!h_u64) GG_HIP(hipHostMalloc((void**)&h_u64, 8, hipHostMallocDefault));
   auto rows_of = [&](const Part& q) { return uint64_t(L) * q.np; };
+  auto dcsr = [&](Region& r) { return DCsr{P<unsigned long long>(r.ptr), P<uint32_t>(r.col)}; };
   auto make_empty = [&](Region& r, uint64_t rows) -> int {
-    GG_TRY(alloc(r.ptr, (rows + 1) * 8, err));
+    GG_TRY(grow(r.ptr, (rows + 1) * 8, err));
     fill_u64(r.ptr, 0, rows + 1);
     r.nnz = 0;
     return GG_OK;
   };
-  auto dcsr = [&](Region& r) { return DCsr{P<unsigned long long>(r.ptr), P<uint32_t>(r.col)}; };
-  // initial delta of each partition: its input edges, deduplicated (host)
-  for (int p = 0; p < NP; ++p) {
-    Part& q = pt[p];
-    const uint64_t rows = rows_of(q);
-    std::vector<std::vector<uint32_t>> rr(rows);
-    for (uint64_t i = 0; i < m; ++i) {
-      const uint32_t u = h_src[i];
-      if (u < q.first || u - q.first >= q.np) continue;
-      rr[uint64_t(h_lab[i] - 1) * q.np + (u - q.first)].push_back(h_dst[i]);
-    }
-    std::vector<unsigned long long> ptr(rows + 1, 0);
-    std::vector<uint32_t> col;
-    for (uint64_t r = 0; r < rows; ++r) {
-      auto& v = rr[r];
-      std::sort(v.begin(), v.end());
-      v.erase(std::unique(v.begin(), v.end()), v.end());
-      col.insert(col.end(), v.begin(), v.end());
-      ptr[r + 1] = col.size();
-    }
-    GG_TRY(alloc(q.dl.ptr, (rows + 1) * 8, err));
-    GG_TRY(alloc(q.dl.col, std::max<size_t>(col.size(), 1) * 4, err));
-    GG_HIP(hipMemcpy(q.dl.ptr.p, ptr.data(), (rows + 1) * 8, hipMemcpyHostToDevice));
-    if (!col.empty()) GG_HIP(hipMemcpy(q.dl.col.p, col.data(), col.size() * 4, hipMemcpyHostToDevice));
-    q.dl.nnz = col.size();
-    q.has_dl = true;
-    GG_TRY(make_empty(q.old, rows));
-    GG_TRY(make_empty(q.tmp, rows));
-    S.input_unique += col.size();
-  }
-  // dst = a U b (disjoint sorted rows), into fresh buffers
-  auto merge_regions = [&](Region& a, Region& b, uint64_t rows, uint32_t np, Region& out) -> int {
-    GG_TRY(alloc(out.ptr, (rows + 1) * 8, err));
-    GG_TRY(alloc(out.col, std::max<uint64_t>(a.nnz + b.nnz, 1) * 4, err));
-    k_add_ptr<<<grid_for(rows + 1), kBlock, 0, stream>>>(P<unsigned long long>(out.ptr), P<unsigned long long>(a.ptr),
-                                                        P<unsigned long long>(b.ptr), rows + 1);
-    k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(dcsr(a), dcsr(b), uint32_t(rows), np, nullptr, DPacked{}, nullptr,
-                                                      P<uint32_t>(out.col), nullptr, nullptr);
-    GG_HIP(hipGetLastError());
-    out.nnz = a.nnz + b.nnz;
+  auto host_alloc = [&](HostBuf& h, size_t bytes) -> int {
+    bytes = std::max<size_t>(bytes, 256);
+    if (h.bytes >= bytes) return GG_OK;
+    if (h.p) (void)hipHostFree(h.p);
+    h.p = nullptr;
+    h.bytes = 0;
+    GG_HIP(hipHostMalloc(&h.p, bytes, hipHostMallocDefault));
+    h.bytes = bytes;
     return GG_OK;
   };
-  // One operator on partition p; fresh ids go to `fresh` (kind 3) or are
-  // merged into p's tmp. Returns the number of fresh ids in *nfresh.
+  // residency: a spilled partition's regions sit in pinned host memory; they
+  // move over the copy stream, ordered after the compute stream's work
+  hipEvent_t ev = nullptr;
+  GG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  auto spill = [&](int p) -> int {
+    Part& q = pt[p];
+    if (!po.spill || !q.resident) return GG_OK;
+    const uint64_t rows = rows_of(q);
+    GG_HIP(hipEventRecord(ev, stream));
+    GG_HIP(hipStreamWaitEvent(copy_stream, ev, 0));
+    for (auto& r : q.r) {
+      GG_TRY(host_alloc(r.hptr, (rows + 1) * 8));
+      GG_TRY(host_alloc(r.hcol, r.nnz * 4));
+      GG_HIP(hipMemcpyAsync(r.hptr.p, r.ptr.p, (rows + 1) * 8, hipMemcpyDeviceToHost, copy_stream));
+      if (r.nnz) GG_HIP(hipMemcpyAsync(r.hcol.p, r.col.p, r.nnz * 4, hipMemcpyDeviceToHost, copy_stream));
+      S.spill_bytes += (rows + 1) * 8 + r.nnz * 4;
+    }
+    GG_HIP(hipStreamSynchronize(copy_stream));
+    for (auto& r : q.r) {
+      release(r.ptr);
+      release(r.col);
+    }
+    q.resident = false;
+    return GG_OK;
+  };
+  auto fetch = [&](int p) -> int {
+    Part& q = pt[p];
+    if (q.resident) return GG_OK;
+    const uint64_t rows = rows_of(q);
+    for (auto& r : q.r) {
+      GG_TRY(alloc(r.ptr, (rows + 1) * 8, err));
+      GG_TRY(alloc(r.col, std::max<uint64_t>(r.nnz, 1) * 4, err));
+      GG_HIP(hipMemcpyAsync(r.ptr.p, r.hptr.p, (rows + 1) * 8, hipMemcpyHostToDevice, copy_stream));
+      if (r.nnz) GG_HIP(hipMemcpyAsync(r.col.p, r.hcol.p, r.nnz * 4, hipMemcpyHostToDevice, copy_stream));
+      S.spill_bytes += (rows + 1) * 8 + r.nnz * 4;
+    }
+    GG_HIP(hipEventRecord(ev, copy_stream));
+    GG_HIP(hipStreamWaitEvent(stream, ev, 0));
+    q.resident = true;
+    return GG_OK;
+  };
+  // initial delta of every partition: its input edges, deduplicated, built
+  // on the host in one pass (global CSR by row, then per-partition slices)
+  {
+    const uint64_t R = rows();
+    std::vector<uint64_t> cnt(R + 1, 0);
+    for (uint64_t i = 0; i < m; ++i) cnt[uint64_t(h_lab[i] - 1) * n + h_src[i] + 1]++;
+    for (uint64_t r = 0; r < R; ++r) cnt[r + 1] += cnt[r];
+    std::vector<uint32_t> col(m);
+    {
+      std::vector<uint64_t> fillp(cnt.begin(), cnt.end() - 1);
+      for (uint64_t i = 0; i < m; ++i) col[fillp[uint64_t(h_lab[i] - 1) * n + h_src[i]]++] = h_dst[i];
+    }
+    for (int p = 0; p < NP; ++p) {
+      Part& q = pt[p];
+      const uint64_t rows = rows_of(q);
+      Region& d = q.r[kD];
+      std::vector<unsigned long long> lptr(rows + 1, 0);
+      std::vector<uint32_t> lcol;
+      for (int l = 0; l < L; ++l)
+        for (uint32_t k = 0; k < q.np; ++k) {
+          const uint64_t gr = uint64_t(l) * n + q.first + k;
+          uint32_t* b = col.data() + cnt[gr];
+          uint32_t* e = col.data() + cnt[gr + 1];
+          std::sort(b, e);
+          e = std::unique(b, e);
+          lcol.insert(lcol.end(), b, e);
+          lptr[uint64_t(l) * q.np + k + 1] = lcol.size();
+        }
+      d.nnz = lcol.size();
+      S.input_unique += d.nnz;
+      GG_TRY(host_alloc(d.hptr, (rows + 1) * 8));
+      GG_TRY(host_alloc(d.hcol, d.nnz * 4));
+      std::memcpy(d.hptr.p, lptr.data(), (rows + 1) * 8);
+      if (d.nnz) std::memcpy(d.hcol.p, lcol.data(), d.nnz * 4);
+      for (int k : {int(kO), int(kT)}) {
+        Region& e2 = q.r[k];
+        GG_TRY(host_alloc(e2.hptr, (rows + 1) * 8));
+        std::memset(e2.hptr.p, 0, (rows + 1) * 8);
+        e2.nnz = 0;
+      }
+      q.has[kD] = true;
+    }
+  }
+  // scratch regions reused by every operator
+  Region F, M;
+  struct ScratchGuard {
+    Engine* e;
+    Region* r[2];
+    ~ScratchGuard() {
+      for (Region* x : r) {
+        e->release(x->ptr);
+        e->release(x->col);
+      }
+    }
+  } sg{this, {&F, &M}};
+  // M = a U b (disjoint sorted rows), then swap M into a
+  auto merge_into = [&](Region& a, Region& b, uint64_t rows, uint32_t np) -> int {
+    GG_TRY(grow(M.ptr, (rows + 1) * 8, err));
+    GG_TRY(grow(M.col, std::max<uint64_t>(a.nnz + b.nnz, 1) * 4, err));
+    k_add_ptr<<<grid_for(rows + 1), kBlock, 0, stream>>>(P<unsigned long long>(M.ptr), P<unsigned long long>(a.ptr),
+                                                        P<unsigned long long>(b.ptr), rows + 1);
+    k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(dcsr(a), dcsr(b), uint32_t(rows), np, nullptr, DPacked{}, nullptr,
+                                                      P<uint32_t>(M.col), nullptr, nullptr);
+    GG_HIP(hipGetLastError());
+    M.nnz = a.nnz + b.nnz;
+    std::swap(a.ptr, M.ptr);
+    std::swap(a.col, M.col);
+    std::swap(a.nnz, M.nnz);
+    return GG_OK;
+  };
   struct Dst {
     Region* r;
     int part;
   };
-  auto op = [&](int p, int kind, Region* key, std::vector<Dst> dsts, Region* seed, Region* fresh,
+  // One operator on partition p; its new ids are merged into `seed`
+  // (kinds 0-2) or become `into` (kind 3).
+  auto op = [&](int p, int kind, Region* key, const std::vector<Dst>& dsts, Region* seed, Region* into,
                 uint64_t* nfresh) -> int {
     Part& q = pt[p];
     const uint64_t rows = rows_of(q);
@@ -1792,7 +1912,6 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
       zero_ctr();
       fill_u32(seg_cnt, 0, rows);
       k_all_active<<<grid_for(rows), kBlock, 0, stream>>>(uint32_t(rows), P<uint32_t>(active), C());
-      EventPair x;
       k_expand<2><<<expand_grid(), kBlock, 0, stream>>>(a);
       GG_HIP(hipGetLastError());
       GG_TRY(read_ctr(err));
@@ -1803,114 +1922,120 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     }
     const uint64_t N = h_ctr->newe;
     *nfresh = N;
-    Region F;
-    Region* out = fresh ? fresh : &F;
-    GG_TRY(alloc(out->ptr, (rows + 1) * 8, err));
-    GG_TRY(scan(seg_cnt, rows, out->ptr, err));
-    GG_TRY(alloc(out->col, std::max<uint64_t>(N, 1) * 4, err));
+    S.candidates += h_ctr->cand;
+    if (!into && N == 0) return GG_OK;
+    GG_TRY(grow(F.ptr, (rows + 1) * 8, err));
+    GG_TRY(scan(seg_cnt, rows, F.ptr, err));
+    GG_TRY(grow(F.col, std::max<uint64_t>(N, 1) * 4, err));
     if (N)
       k_gather<<<expand_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
                                                      P<unsigned long long>(seg_off), P<uint32_t>(staging),
-                                                     P<unsigned long long>(out->ptr), P<uint32_t>(out->col));
-    out->nnz = N;
-    if (!fresh) {
-      if (N) {
-        Region M;
-        GG_TRY(merge_regions(*seed, F, rows, q.np, M));
-        std::swap(seed->ptr, M.ptr);
-        std::swap(seed->col, M.col);
-        seed->nnz = M.nnz;
-        GG_HIP(hipStreamSynchronize(stream));
-        release(M.ptr);
-        release(M.col);
-      }
-      GG_HIP(hipStreamSynchronize(stream));
-      release(F.ptr);
-      release(F.col);
+                                                     P<unsigned long long>(F.ptr), P<uint32_t>(F.col));
+    F.nnz = N;
+    if (into) {
+      std::swap(into->ptr, F.ptr);
+      std::swap(into->col, F.col);
+      into->nnz = N;
+    } else {
+      GG_TRY(merge_into(*seed, F, rows, q.np));
     }
     return GG_OK;
   };
   bool changed = false;
-  // compute11 / compute10 / compute00 into tmp of p (skipped when the key
-  // region does not exist, as the reference's size checks do)
-  auto exists = [&](int p, int which) {
-    return which == 0 ? pt[p].has_old : which == 1 ? pt[p].has_dl : pt[p].has_tmp;
-  };
-  auto reg = [&](int p, int which) -> Region* { return which == 0 ? &pt[p].old : which == 1 ? &pt[p].dl : &pt[p].tmp; };
   auto join = [&](int p, int key, std::vector<std::pair<int, int>> dsts) -> int {
-    if (!exists(p, key)) return GG_OK;
+    if (!pt[p].has[key]) return GG_OK;
     std::vector<Dst> ds;
     for (auto& d : dsts)
-      if (exists(d.second, d.first)) ds.push_back({reg(d.second, d.first), d.second});
+      if (pt[d.second].has[d.first]) ds.push_back({&pt[d.second].r[d.first], d.second});
     if (ds.empty()) return GG_OK;
     uint64_t nf = 0;
-    GG_TRY(op(p, 0, reg(p, key), ds, &pt[p].tmp, nullptr, &nf));
+    GG_TRY(op(p, 0, &pt[p].r[key], ds, &pt[p].r[kT], nullptr, &nf));
     changed |= nf > 0;
     return GG_OK;
   };
   auto unary = [&](int p, int from) -> int {
-    if (!exists(p, from) || g.nu == 0) return GG_OK;
+    if (!pt[p].has[from] || g.nu == 0) return GG_OK;
     uint64_t nf = 0;
-    GG_TRY(op(p, 1, reg(p, from), {}, &pt[p].tmp, nullptr, &nf));
+    GG_TRY(op(p, 1, &pt[p].r[from], {}, &pt[p].r[kT], nullptr, &nf));
     changed |= nf > 0;
     return GG_OK;
   };
   auto eps = [&](int p) -> int {
     if (g.ne == 0) return GG_OK;
     uint64_t nf = 0;
-    GG_TRY(op(p, 2, nullptr, {}, &pt[p].tmp, nullptr, &nf));
+    GG_TRY(op(p, 2, nullptr, {}, &pt[p].r[kT], nullptr, &nf));
     changed |= nf > 0;
     return GG_OK;
   };
   auto merge_and_diff = [&](int p) -> int {
     Part& q = pt[p];
     const uint64_t rows = rows_of(q);
-    if (q.has_dl) {
-      if (!q.has_old) {
-        std::swap(q.old, q.dl);
+    if (q.has[kD]) {
+      if (!q.has[kO]) {
+        std::swap(q.r[kO].ptr, q.r[kD].ptr);
+        std::swap(q.r[kO].col, q.r[kD].col);
+        std::swap(q.r[kO].nnz, q.r[kD].nnz);
       } else {
-        Region M;
-        GG_TRY(merge_regions(q.old, q.dl, rows, q.np, M));
-        GG_HIP(hipStreamSynchronize(stream));
-        release(q.old.ptr);
-        release(q.old.col);
-        q.old.ptr = M.ptr;
-        q.old.col = M.col;
-        q.old.nnz = M.nnz;
+        GG_TRY(merge_into(q.r[kO], q.r[kD], rows, q.np));
       }
-      release(q.dl.ptr);
-      release(q.dl.col);
-      GG_TRY(make_empty(q.dl, rows));
-      q.has_old = true;
-      q.has_dl = false;
+      GG_TRY(make_empty(q.r[kD], rows));
+      q.has[kO] = true;
+      q.has[kD] = false;
     }
-    if (q.has_tmp) {
-      Region D;
+    if (q.has[kT]) {
       uint64_t nf = 0;
-      GG_TRY(op(p, 3, &q.tmp, {}, &q.old, &D, &nf));
-      GG_HIP(hipStreamSynchronize(stream));
-      release(q.dl.ptr);
-      release(q.dl.col);
-      q.dl.ptr = D.ptr;
-      q.dl.col = D.col;
-      q.dl.nnz = nf;
-      q.has_dl = nf > 0;
-      release(q.tmp.ptr);
-      release(q.tmp.col);
-      GG_TRY(make_empty(q.tmp, rows));
-      q.has_tmp = false;
+      GG_TRY(op(p, 3, &q.r[kT], {}, &q.r[kO], &q.r[kD], &nf));
+      q.has[kD] = nf > 0;
+      GG_TRY(make_empty(q.r[kT], rows));
+      q.has[kT] = false;
     }
     return GG_OK;
   };
-  enum { kO = 0, kD = 1, kT = 2 };
-  // superstep 0
+  // cross-partition edge counts of partition p (all three regions), the
+  // pair weights of the DDM schedule
+  std::vector<std::vector<uint64_t>> ddm(size_t(NP), std::vector<uint64_t>(size_t(NP), 0));
+  DevBuf d_bounds, d_hist;
+  struct BGuard {
+    Engine* e;
+    DevBuf* b[2];
+    ~BGuard() {
+      for (DevBuf* x : b) e->release(*x);
+    }
+  } bg{this, {&d_bounds, &d_hist}};
+  if (po.ddm) {
+    std::vector<uint32_t> bounds(size_t(NP) + 1);
+    for (int p = 0; p < NP; ++p) bounds[p] = pt[p].first;
+    bounds[NP] = n;
+    GG_TRY(alloc(d_bounds, bounds.size() * 4, err));
+    GG_TRY(alloc(d_hist, size_t(NP) * 8, err));
+    GG_HIP(hipMemcpy(d_bounds.p, bounds.data(), bounds.size() * 4, hipMemcpyHostToDevice));
+  }
+  std::vector<unsigned long long> hist(static_cast<size_t>(NP));
+  auto cross = [&](int p) -> int {
+    if (!po.ddm) return GG_OK;
+    Part& q = pt[p];
+    fill_u64(d_hist, 0, uint64_t(NP));
+    for (auto& r : q.r)
+      if (r.nnz)
+        k_cross<<<uint32_t(cus) * 4, kBlock, 0, stream>>>(dcsr(r), uint32_t(rows_of(q)), P<uint32_t>(d_bounds),
+                                                         uint32_t(NP), P<unsigned long long>(d_hist));
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipMemcpyAsync(hist.data(), d_hist.p, size_t(NP) * 8, hipMemcpyDeviceToHost, stream));
+    GG_HIP(hipStreamSynchronize(stream));
+    for (int j = 0; j < NP; ++j) ddm[p][j] = j == p ? 0 : hist[j];
+    return GG_OK;
+  };
+  if (!po.spill)
+    for (int p = 0; p < NP; ++p) GG_TRY(fetch(p));
+  // superstep 0 (precomputation)
   for (int p = 0; p < NP; ++p) {
+    GG_TRY(fetch(p));
     for (;;) {
       Part& q = pt[p];
-      q.has_tmp = true;
-      if (!q.has_old) GG_TRY(eps(p));
+      q.has[kT] = true;
+      if (!q.has[kO]) GG_TRY(eps(p));
       if (filter == 0) GG_TRY(unary(p, kD));
-      if (q.has_old) {
+      if (q.has[kO]) {
         if (filter == 0) GG_TRY(join(p, kO, {{kD, p}}));
         GG_TRY(join(p, kD, {{kO, p}, {kD, p}}));
       } else {
@@ -1918,22 +2043,36 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
       }
       GG_TRY(merge_and_diff(p));
       S.rounds++;
-      if (!q.has_dl) {
-        std::swap(q.dl, q.old);
-        q.has_dl = q.has_old;
-        q.has_old = false;
+      if (!q.has[kD]) {
+        std::swap(q.r[kD].ptr, q.r[kO].ptr);
+        std::swap(q.r[kD].col, q.r[kO].col);
+        std::swap(q.r[kD].nnz, q.r[kO].nnz);
+        q.has[kD] = q.has[kO];
+        q.has[kO] = false;
         break;
       }
     }
+    GG_TRY(cross(p));
+    GG_TRY(spill(p));
   }
-  // supersteps 1..
+  // supersteps 1.. (run_computation); with the DDM schedule a pair runs only
+  // when edges cross between the two partitions and one of them has new
+  // edges (delta or tmp) to push
   uint32_t supersteps = 0;
+  auto dirty = [&](int p) { return pt[p].has[kD] || pt[p].r[kT].nnz > 0; };
   for (;;) {
     ++supersteps;
     for (int i = 0; i < NP; ++i) {
+      GG_TRY(fetch(i));
       for (int j = i + 1; j < NP; ++j) {
-        pt[i].has_tmp = true;
-        pt[j].has_tmp = true;
+        if (po.ddm && (ddm[i][j] + ddm[j][i] == 0 || !(dirty(i) || dirty(j)))) {
+          S.pairs_skipped++;
+          continue;
+        }
+        GG_TRY(fetch(j));
+        S.pairs_run++;
+        pt[i].has[kT] = true;
+        pt[j].has[kT] = true;
         if (filter == 0) GG_TRY(join(i, kO, {{kD, j}}));
         GG_TRY(join(i, kD, {{kO, j}, {kD, j}}));
         if (filter == 0) GG_TRY(join(j, kO, {{kD, i}}));
@@ -1952,24 +2091,33 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
           S.rounds++;
           if (!changed) break;
         }
+        GG_TRY(cross(i));
+        GG_TRY(cross(j));
+        GG_TRY(spill(j));
       }
       GG_TRY(merge_and_diff(i));
+      GG_TRY(cross(i));
+      GG_TRY(spill(i));
     }
     bool done = true;
-    for (auto& q : pt) done &= !q.has_dl;
+    for (auto& q : pt) done &= !q.has[kD];
     if (done) break;
   }
   supersteps_r = supersteps;
+  (void)hipEventDestroy(ev);
   // the closure is every partition's old region: assemble the global CSR
   GG_HIP(hipStreamSynchronize(stream));
   const uint64_t R = rows();
   std::vector<unsigned long long> gptr(R + 1, 0);
   std::vector<std::vector<unsigned long long>> lptr(static_cast<size_t>(NP));
   for (int p = 0; p < NP; ++p) {
-    lptr[p].resize(rows_of(pt[p]) + 1);
-    GG_HIP(hipMemcpy(lptr[p].data(), pt[p].old.ptr.p, lptr[p].size() * 8, hipMemcpyDeviceToHost));
+    Part& q = pt[p];
+    lptr[p].resize(rows_of(q) + 1);
+    if (q.resident)
+      GG_HIP(hipMemcpy(lptr[p].data(), q.r[kO].ptr.p, lptr[p].size() * 8, hipMemcpyDeviceToHost));
+    else
+      std::memcpy(lptr[p].data(), q.r[kO].hptr.p, lptr[p].size() * 8);
   }
-  uint64_t total = 0;
   for (int l = 0; l < L; ++l)
     for (int p = 0; p < NP; ++p)
       for (uint32_t k = 0; k < pt[p].np; ++k) {
@@ -1978,16 +2126,25 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
         gptr[gr + 1] = lptr[p][lr + 1] - lptr[p][lr];
       }
   for (uint64_t r = 0; r < R; ++r) gptr[r + 1] += gptr[r];
-  total = gptr[R];
+  const uint64_t total = gptr[R];
+  // the per-round scratch is not needed any more
+  release(F.ptr);
+  release(F.col);
+  release(M.ptr);
+  release(M.col);
   GG_TRY(alloc(old_ptr, (R + 1) * 8, err));
   GG_TRY(alloc(old_col, std::max<uint64_t>(total, 1) * 4, err));
   GG_HIP(hipMemcpy(old_ptr.p, gptr.data(), (R + 1) * 8, hipMemcpyHostToDevice));
   for (int l = 0; l < L; ++l)
     for (int p = 0; p < NP; ++p) {
-      const uint64_t lb = lptr[p][uint64_t(l) * pt[p].np], le = lptr[p][uint64_t(l + 1) * pt[p].np];
-      if (le > lb)
-        GG_HIP(hipMemcpy(static_cast<uint32_t*>(old_col.p) + gptr[uint64_t(l) * n + pt[p].first],
-                         static_cast<uint32_t*>(pt[p].old.col.p) + lb, (le - lb) * 4, hipMemcpyDeviceToDevice));
+      Part& q = pt[p];
+      const uint64_t lb = lptr[p][uint64_t(l) * q.np], le = lptr[p][uint64_t(l + 1) * q.np];
+      if (le <= lb) continue;
+      uint32_t* dst = static_cast<uint32_t*>(old_col.p) + gptr[uint64_t(l) * n + q.first];
+      if (q.resident)
+        GG_HIP(hipMemcpy(dst, static_cast<uint32_t*>(q.r[kO].col.p) + lb, (le - lb) * 4, hipMemcpyDeviceToDevice));
+      else
+        GG_HIP(hipMemcpy(dst, static_cast<uint32_t*>(q.r[kO].hcol.p) + lb, (le - lb) * 4, hipMemcpyHostToDevice));
     }
   GG_HIP(hipStreamSynchronize(stream));
   S.total_edges = total;
@@ -2144,10 +2301,14 @@ int engine_run(Engine* e, int filter, gg_stats* stats, std::string& err) {
 }
 
 int engine_run_partitioned(Engine* e, int filter, const std::vector<std::pair<uint32_t, uint32_t>>& parts,
-                           gg_stats* stats, std::string& err) {
+                           const PartOpts& po, gg_stats* stats, std::string& err) {
   e->has_closure = false;
+  if (parts.size() > kMaxCrossParts) {
+    err = "at most " + std::to_string(kMaxCrossParts) + " partitions";
+    return GG_E_UNSUPPORTED;
+  }
   gg_stats st{};
-  const int s = e->run_r(filter, parts, &st, err);
+  const int s = e->run_r(filter, parts, po, &st, err);
   if (s == GG_OK && stats) {
     *stats = st;
     stats->supersteps = e->supersteps_r;

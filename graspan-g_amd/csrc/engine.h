@@ -24,9 +24,16 @@ int engine_set_grammar(Engine* e, const Grammar& g, std::string& err);
 int engine_set_graph(Engine* e, const Graph& g, std::string& err);
 
 int engine_run(Engine* e, int filter, gg_stats* stats, std::string& err);
+// Options of the partitioned schedule: spill = partitions not in the current
+// session live in pinned host memory; ddm = run a partition pair only when
+// edges cross between the two and one of them has new edges.
+struct PartOpts {
+  bool spill = false;
+  bool ddm = false;
+};
 // The reference's partitioned schedule over the vertex intervals `parts`.
 int engine_run_partitioned(Engine* e, int filter, const std::vector<std::pair<uint32_t, uint32_t>>& parts,
-                           gg_stats* stats, std::string& err);
+                           const PartOpts& po, gg_stats* stats, std::string& err);
 int engine_counts(Engine* e, uint64_t* per_label, uint64_t* total, std::string& err);
 int engine_export(Engine* e, uint32_t* buf, uint64_t cap, uint64_t* count, std::string& err);
 int engine_degrees(Engine* e, uint64_t* deg, std::string& err);

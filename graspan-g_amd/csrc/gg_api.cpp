@@ -261,7 +261,30 @@ int gg_closure_run_partitioned(gg_ctx* c, int filter_flag, gg_stats* stats) {
     s = push_graph(c);
     if (s != GG_OK) return s;
   }
-  return gg::engine_run_partitioned(c->engine, filter_flag, parts, stats, c->err);
+  return gg::engine_run_partitioned(c->engine, filter_flag, parts, gg::PartOpts{}, stats, c->err);
+}
+
+int gg_closure_run_out_of_core(gg_ctx* c, int num_partitions, gg_stats* stats) {
+  if (!c) return GG_E_ARG;
+  if (!c->has_grammar || !c->has_graph) return fail(c, GG_E_STATE, "load a grammar and a graph first");
+  const uint32_t n = c->graph.n;
+  if (num_partitions < 1 || uint32_t(num_partitions) > n) return fail(c, GG_E_ARG, "bad partition count");
+  std::vector<std::pair<uint32_t, uint32_t>> parts;
+  for (int p = 0; p < num_partitions; ++p) {
+    const uint32_t a = uint32_t(uint64_t(n) * uint64_t(p) / uint64_t(num_partitions));
+    const uint32_t b = uint32_t(uint64_t(n) * uint64_t(p + 1) / uint64_t(num_partitions));
+    if (b > a) parts.push_back({a, b - 1});
+  }
+  int s = ensure_engine(c);
+  if (s != GG_OK) return s;
+  if (!c->graph_on_device) {
+    s = push_graph(c);
+    if (s != GG_OK) return s;
+  }
+  gg::PartOpts po;
+  po.spill = true;
+  po.ddm = true;
+  return gg::engine_run_partitioned(c->engine, 0, parts, po, stats, c->err);
 }
 
 int gg_reference_partitions(const gg_ctx* c, uint32_t* first, uint32_t* last, int cap, int* count) {

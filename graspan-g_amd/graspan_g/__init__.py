@@ -31,7 +31,7 @@ EXPORTED = (
     "gg_ctx_create", "gg_ctx_destroy", "gg_last_error", "gg_version",
     "gg_grammar_load_file", "gg_grammar_set", "gg_grammar_info", "gg_grammar_rules", "gg_label_name", "gg_label_id",
     "gg_graph_load_file", "gg_graph_set_coo", "gg_graph_info", "gg_graph_edges",
-    "gg_closure_run", "gg_closure_run_partitioned", "gg_reference_partitions", "gg_closure_counts", "gg_closure_export", "gg_closure_degrees", "gg_closure_checksum",
+    "gg_closure_run", "gg_closure_run_partitioned", "gg_closure_run_out_of_core", "gg_reference_partitions", "gg_closure_counts", "gg_closure_export", "gg_closure_degrees", "gg_closure_checksum",
     "gg_triple_hash", "gg_closure_release", "gg_device_info",
     "gg_batch_create", "gg_batch_destroy", "gg_batch_add", "gg_batch_run", "gg_batch_info", "gg_batch_result",
     "gg_batch_export",
@@ -59,6 +59,7 @@ class Stats(ctypes.Structure):
         ("probe_reads", ctypes.c_uint64), ("candidates", ctypes.c_uint64), ("old_reads", ctypes.c_uint64),
         ("new_edges", ctypes.c_uint64), ("device_bytes_peak", ctypes.c_uint64),
         ("supersteps", ctypes.c_uint32), ("partitions", ctypes.c_uint32),
+        ("pairs_run", ctypes.c_uint32), ("pairs_skipped", ctypes.c_uint32), ("spill_bytes", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -89,6 +90,7 @@ def lib():
             "gg_graph_load_file": (I, [P, ctypes.c_char_p]), "gg_graph_set_coo": (I, [P, U32, U64, P, P, P]),
             "gg_graph_info": (I, [P, P, P]), "gg_graph_edges": (I, [P, P, P, P]),
             "gg_closure_run": (I, [P, I, P]), "gg_closure_run_partitioned": (I, [P, I, P]),
+            "gg_closure_run_out_of_core": (I, [P, I, P]),
             "gg_closure_counts": (I, [P, P, P]), "gg_reference_partitions": (I, [P, P, P, I, P]),
             "gg_closure_export": (I, [P, P, U64, P]), "gg_closure_degrees": (I, [P, P]),
             "gg_closure_checksum": (I, [P, P]), "gg_triple_hash": (U64, [U32, U32, U32]),
@@ -226,6 +228,13 @@ class Context:
         st = Stats()
         self._check(self._L.gg_closure_run_partitioned(self._h, filter_flag, ctypes.byref(st)),
                     "gg_closure_run_partitioned")
+        return st
+
+    def run_out_of_core(self, num_partitions: int) -> Stats:
+        """Least fixpoint with partitions spilled to host DRAM (see the header)."""
+        st = Stats()
+        self._check(self._L.gg_closure_run_out_of_core(self._h, num_partitions, ctypes.byref(st)),
+                    "gg_closure_run_out_of_core")
         return st
 
     def reference_partitions(self):

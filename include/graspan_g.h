@@ -72,6 +72,9 @@ typedef struct gg_stats {
   uint32_t supersteps;      /* partitioned schedule: supersteps after superstep 0
                                (the reference's NUM SUPERSTEPS); 0 otherwise */
   uint32_t partitions;      /* partitioned schedule: vertex intervals; 1 otherwise */
+  uint32_t pairs_run;       /* out-of-core: partition pairs closed */
+  uint32_t pairs_skipped;   /* out-of-core: pairs with no crossing edges or nothing new */
+  uint64_t spill_bytes;     /* out-of-core: bytes moved between host DRAM and HBM */
 } gg_stats;
 
 /* ---- context ------------------------------------------------------------ */
@@ -143,6 +146,20 @@ int gg_closure_run(gg_ctx *ctx, int filter_flag, gg_stats *stats);
  * GG_E_UNSUPPORTED when the reference's small-graph split is degenerate (a
  * first vertex holding more than half of the elements, or no elements). */
 int gg_closure_run_partitioned(gg_ctx *ctx, int filter_flag, gg_stats *stats);
+
+/* Out-of-core least fixpoint (filter 0) for graphs whose closure does not fit
+ * the device budget (gg_options.device_budget). The vertices are cut into
+ * num_partitions equal contiguous intervals. The reference's partition-pair
+ * schedule runs (superstep 0 per partition, then pairs i < j; the reference
+ * keeps the partitions it is not using in Partition<k> files,
+ * src/main.cpp:335-631, 764-858): only the partitions of the current session
+ * are in HBM, the others are spilled to pinned host DRAM with hipMemcpyAsync
+ * on a copy stream. Pairs are chosen by their cross-partition edge counts,
+ * recomputed on the device whenever a partition changes: a pair runs only
+ * when edges cross between the two partitions and one of them has new edges
+ * (the weights of the reference's DDM, src/ddm.cpp:27-35, which the reference
+ * never calls). */
+int gg_closure_run_out_of_core(gg_ctx *ctx, int num_partitions, gg_stats *stats);
 
 /* The reference's vertex intervals for the loaded graph (host only; no GPU):
  * up to cap (first, last) pairs, *count receives their number. Follows

@@ -279,3 +279,25 @@ def test_gpu_server_results_equal_alone(built):
         assert (r.total_edges, r.checksum) == (t2, alone.checksum())
         np.testing.assert_array_equal(per, p2)
         assert r.submit_ms <= r.start_ms <= r.done_ms
+
+
+@pytest.mark.parametrize("parts", [1, 4, 16])
+def test_gpu_out_of_core_matches_oracle(built, parts):
+    """Partitions spilled to host DRAM, pairs chosen by crossing edges: the
+    least fixpoint, equal to the oracle's."""
+    cases = [gen.dataflow_blocks(30_000, seed=51)]
+    if parts <= 4:
+        cases.append(gen.alias(1_000, seed=52))
+    for g in cases:
+        say("out-of-core", g.name, "partitions", parts)
+        ref = oracle.closure_of(g, want_triples=False)
+        c = gg.Context(device_budget=2 << 30)
+        c.set_spec(g)
+        st = c.run_out_of_core(parts)
+        per, tot = c.counts()
+        assert tot == ref.total and c.checksum() == ref.checksum
+        np.testing.assert_array_equal(per, ref.counts)
+        assert st.partitions == parts and st.spill_bytes > 0
+        if parts == 16 and g.name.startswith("dataflow"):
+            # blocks only straddle neighbouring partitions
+            assert st.pairs_skipped > st.pairs_run > 0
