@@ -328,7 +328,7 @@ def bench_ooc(args):
     t = time.time()
     g = gen.dataflow_blocks(args.ooc_n, seed=0)
     log(f"generated blocks n={g.n} m={g.m} in {time.time() - t:.1f}s")
-    ctx = gg.Context(device_budget=int(args.ooc_budget_gb * (1 << 30)))
+    ctx = gg.Context(device_budget=int(args.ooc_budget_gb * (1 << 30)), verbose=True)  # progress on stderr
     ctx.set_spec(g)
     t0 = time.perf_counter()
     st = ctx.run_out_of_core(args.ooc_partitions)

@@ -705,23 +705,67 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
   }
 }
 
+// First position in sorted ids col[b, e) whose id is >= x.
+__device__ inline unsigned long long lower_pos(const uint32_t* col, unsigned long long b, unsigned long long e,
+                                               unsigned long long x) {
+  while (b < e) {
+    const unsigned long long mid = (b + e) >> 1;
+    if (col[mid] < x)
+      b = mid + 1;
+    else
+      e = mid;
+  }
+  return b;
+}
+
+// Same over packed entries (word index in the low 32 bits, ascending).
+__device__ inline unsigned long long lower_pos_word(const unsigned long long* ent, unsigned long long b,
+                                                    unsigned long long e, uint32_t wi) {
+  while (b < e) {
+    const unsigned long long mid = (b + e) >> 1;
+    if (uint32_t(ent[mid]) < wi)
+      b = mid + 1;
+    else
+      e = mid;
+  }
+  return b;
+}
+
 // Clears the window [c0, c0+span), seeds it with old_A(u) U delta_A(u) and
 // streams every candidate through it. Returns the number of new edges (set
 // bits of w.fresh) and stores the candidate count in kcount.
+//
+// ranged (rows whose ids span several windows; sorted lists only): every
+// list and seed row is cut to [c0, c0+span) by binary search, so a window
+// costs its own ids plus a search per list, and `next` receives the
+// smallest candidate id at or past the window's end (0xFFFFFFFF if none):
+// windows with no candidate are skipped.
 template <int MODE>
 __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const ExpandArgs& a, uint32_t r, uint32_t A,
                                                                         uint32_t u, uint32_t c0, uint32_t span,
-                                                                        WaveLds& w, int lane, unsigned long long& kcount,
-                                                                        bool& miss) {
+                                                                        bool ranged, WaveLds& w, int lane,
+                                                                        unsigned long long& kcount, bool& miss,
+                                                                        uint32_t& next) {
   bool miss_l = false;
   const uint32_t words = (span + 31) >> 5;
+  const unsigned long long c1 = (unsigned long long)c0 + span;
+  const uint32_t w0 = c0 >> 5;
+  uint32_t nxt = 0xFFFFFFFFu;
   for (uint32_t i = lane; i < words; i += kWave) {
     w.seen[i] = 0;
     w.fresh[i] = 0;
   }
   wave_sync();
+  // [b, e) of a sorted id row, cut to the window when ranged
+  auto cut = [&](const uint32_t* col, unsigned long long& b, unsigned long long& e) {
+    if (ranged && b < e) {
+      b = lower_pos(col, b, e, c0);
+      e = lower_pos(col, b, e, c1);
+    }
+  };
   if (MODE == 2) {
-    const unsigned long long sb = a.ro.seed.ptr[r], se = a.ro.seed.ptr[r + 1];
+    unsigned long long sb = a.ro.seed.ptr[r], se = a.ro.seed.ptr[r + 1];
+    cut(a.ro.seed.col, sb, se);
     for (unsigned long long i = sb + lane; i < se; i += kWave) {
       const uint32_t off = a.ro.seed.col[i] - c0;
       if (off < span) atomicOr(&w.seen[off >> 5], 1u << (off & 31));
@@ -735,18 +779,23 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
     };
     if (a.pk.len && packed_label(a.pk, A)) {
       // seed old_A(u) word by word (c0 is a multiple of 32)
-      const uint32_t w0 = c0 >> 5;
-      const unsigned long long pb = a.old.ptr[r], pe = pb + a.pk.len[r];
+      unsigned long long pb = a.old.ptr[r], pe = pb + a.pk.len[r];
+      if (ranged && pb < pe) {
+        pb = lower_pos_word(a.pk.ent, pb, pe, w0);
+        pe = lower_pos_word(a.pk.ent, pb, pe, w0 + words);
+      }
       for (unsigned long long i = pb + lane; i < pe; i += kWave) {
         const unsigned long long e = a.pk.ent[i];
         const uint32_t wi = uint32_t(e) - w0;
         if (wi < words) atomicOr(&w.seen[wi], uint32_t(e >> 32));
       }
     } else {
-      const unsigned long long ob = a.old.ptr[r], oe = a.old.ptr[r + 1];
+      unsigned long long ob = a.old.ptr[r], oe = a.old.ptr[r + 1];
+      cut(a.old.col, ob, oe);
       for (unsigned long long i = ob + lane; i < oe; i += kWave) seed(a.old.col[i]);
     }
-    const unsigned long long db = a.dl.ptr[r], de = a.dl.ptr[r + 1];
+    unsigned long long db = a.dl.ptr[r], de = a.dl.ptr[r + 1];
+    cut(a.dl.col, db, de);
     for (unsigned long long i = db + lane; i < de; i += kWave) seed(a.dl.col[i]);
     wave_sync();
   }
@@ -763,11 +812,12 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
       miss_l = true;
     }
   };
-  const uint32_t w0 = c0 >> 5;
+  unsigned long long kc = 0;
   auto mark_word = [&](unsigned long long e) {
     const uint32_t wi = uint32_t(e) - w0;
     if (wi < words) {
       const uint32_t bits = uint32_t(e >> 32);
+      if (ranged) kc += __popc(bits);
       if (bits & ~vseen[wi]) {
         const uint32_t nb = bits & ~atomicOr(&w.seen[wi], bits);
         if (nb) atomicOr(&w.fresh[wi], nb);
@@ -776,29 +826,43 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
       miss_l = true;
     }
   };
-  unsigned long long kc = 0;
   for_each_list<MODE, true>(
       a, r, A, u, lane,
       [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
-        kc += len;
-        expand_lists(col, lb, len, w, lane, mark);
+        unsigned long long b = lb, e = lb + len;
+        cut(col, b, e);
+        if (ranged && e < lb + len) nxt = min(nxt, col[e]);
+        kc += e - b;
+        expand_lists(col, b, uint32_t(e - b), w, lane, mark);
       },
       [&](unsigned long long pb, uint32_t pl, uint32_t len) {
-        kc += len;
-        expand_words(a.pk.ent, pb, pl, w, lane, mark_word);
+        unsigned long long b = pb, e = pb + pl;
+        if (ranged && b < e) {
+          b = lower_pos_word(a.pk.ent, b, e, w0);
+          e = lower_pos_word(a.pk.ent, b, e, w0 + words);
+          if (e < pb + pl) nxt = min(nxt, uint32_t(a.pk.ent[e]) << 5);
+        } else {
+          kc += len;
+        }
+        expand_words(a.pk.ent, b, uint32_t(e - b), w, lane, mark_word);
       });
   if (MODE == 2 && a.ro.kind == 2) {
     // eps: the row's own vertex
     bool has_eps = false;
     for (int i = 0; i < a.g.ne; ++i) has_eps |= a.g.e_lhs[i] == A;
     if (has_eps) {
-      if (lane == 0) mark(u);
-      kc += lane == 0 ? 1 : 0;
+      if (!ranged || (u >= c0 && u < c1)) {
+        if (lane == 0) mark(u);
+        kc += lane == 0 ? 1 : 0;
+      } else if (u >= c1) {
+        nxt = min(nxt, u);
+      }
       wave_sync();
     }
   }
   kcount = wave_sum_u64(kc);
   miss = __ballot(miss_l) != 0;
+  next = wave_min_u32(nxt);
   wave_sync();
   uint32_t c = 0;
   for (uint32_t i = lane; i < words; i += kWave) c += __popc(w.fresh[i]);
@@ -931,7 +995,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       }
       const uint32_t nchunks = (hi - lo) / kChunkBits + 1;
       // windows wider than one chunk: a counting pass, one reservation, then
-      // a writing pass
+      // a writing pass; with sorted lists (every mode but the raw rows) each
+      // pass visits only the windows that hold candidates
+      const bool ranged = MODE != 1 && nchunks > 1;
       bool missed = false;
       uint32_t total = 0;
       unsigned long long p = 0;
@@ -945,19 +1011,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
           }
           if (!total) break;
         }
-        for (uint32_t ch = 0; ch < nchunks; ++ch) {
-          const uint32_t c0 = lo + ch * kChunkBits;
+        uint32_t c0 = lo;
+        for (uint32_t ch = 0;; ++ch) {
           const uint32_t span = min(kChunkBits, hi - c0 + 1);
           unsigned long long kch = 0;
           bool miss = false;
-          const uint32_t c = process_chunk<MODE>(a, r, A, u, c0, span, w, lane, kch, miss);
+          uint32_t nxt = 0xFFFFFFFFu;
+          const uint32_t c = process_chunk<MODE>(a, r, A, u, c0, span, ranged, w, lane, kch, miss, nxt);
           if (attempt == 0 && miss) {
             missed = true;
             break;
           }
+          // the next window: past the last one, or (ranged) at the next
+          // candidate; c0 stays a multiple of 32 and grows every step
+          bool last;
+          uint32_t c0n = 0;
+          if (ranged) {
+            last = nxt == 0xFFFFFFFFu || nxt > hi;
+            c0n = nxt & ~31u;
+          } else {
+            last = ch + 1 >= nchunks;
+            c0n = c0 + kChunkBits;
+          }
           if (pass == 0) {
             total += c;
-            if (ch == 0) krow = kch;
+            if (ranged)
+              krow += kch;
+            else if (ch == 0)
+              krow = kch;
+            if (last) break;
+            c0 = c0n;
             continue;
           }
           if (nchunks == 1) {
@@ -974,6 +1057,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
             emit_chunk(a, c0, span, p, w, lane);
             p += c;
           }
+          if (last) break;
+          c0 = c0n;
         }
         if (missed) break;
       }
@@ -1223,6 +1308,9 @@ class Engine {
   int verbose = 0;
   hipStream_t stream = nullptr;
   size_t used = 0, peak = 0;
+  std::function<bool()> on_pressure;  // frees device memory, if it can, when an allocation does not fit
+  double alloc_ms = 0;                // host time spent in hipMalloc / hipFree
+  uint64_t alloc_calls = 0;
 
   DevGrammar g{};
   int L = 0;
@@ -1268,6 +1356,7 @@ class Engine {
 
   int alloc(DevBuf& b, size_t bytes, std::string& err) {
     if (b.p && b.bytes >= bytes) return GG_OK;
+    const auto a0 = std::chrono::steady_clock::now();
     if (b.p) {
       (void)hipFree(b.p);
       used -= b.bytes;
@@ -1275,14 +1364,26 @@ class Engine {
       b.bytes = 0;
     }
     bytes = std::max<size_t>(bytes, 256);
-    if (budget && used + bytes > budget) {
-      err = "device budget exceeded: need " + std::to_string(used + bytes) + " bytes, budget " + std::to_string(budget);
-      return GG_E_NOMEM;
-    }
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) {
+    struct Clock {
+      double& ms;
+      std::chrono::steady_clock::time_point t;
+      ~Clock() { ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count(); }
+    } clk{alloc_ms, a0};
+    ++alloc_calls;
+    // under memory pressure the out-of-core schedule may move a partition to
+    // host DRAM (on_pressure returns true when it freed something)
+    for (;;) {
+      if (budget && used + bytes > budget) {
+        if (on_pressure && on_pressure()) continue;
+        err = "device budget exceeded: need " + std::to_string(used + bytes) + " bytes, budget " +
+              std::to_string(budget);
+        return GG_E_NOMEM;
+      }
+      hipError_t e = hipMalloc(&b.p, bytes);
+      if (e == hipSuccess) break;
       (void)hipGetLastError();
       b.p = nullptr;
+      if (on_pressure && on_pressure()) continue;
       err = "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e);
       return GG_E_NOMEM;
     }
@@ -1296,6 +1397,7 @@ class Engine {
   int grow(DevBuf& b, size_t bytes, std::string& err) {
     if (b.p && b.bytes >= bytes) return GG_OK;
     size_t want = std::max(bytes, b.bytes + b.bytes / 2);
+    if (budget && used - b.bytes + want > budget) want = bytes;  // no slack under the budget's edge
     int s = alloc(b, want, err);
     if (s == GG_E_NOMEM && want > bytes) s = alloc(b, bytes, err);
     return s;
@@ -1303,7 +1405,9 @@ class Engine {
 
   void release(DevBuf& b) {
     if (b.p) {
+      const auto a0 = std::chrono::steady_clock::now();
       (void)hipFree(b.p);
+      alloc_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a0).count();
       used -= b.bytes;
       b.p = nullptr;
       b.bytes = 0;
@@ -1714,6 +1818,7 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     Region r[3];  // old, delta, tmp
     bool has[3] = {false, false, false};
     bool resident = false;
+    bool evicted = false;  // has been spilled at least once
   };
   enum { kO = 0, kD = 1, kT = 2 };
   const int NP = int(parts.size());
@@ -1757,31 +1862,51 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     r.nnz = 0;
     return GG_OK;
   };
+  // pinned host buffers grow with 25% slack: pinning is slow, and a spilled
+  // region usually comes back a little larger
   auto host_alloc = [&](HostBuf& h, size_t bytes) -> int {
     bytes = std::max<size_t>(bytes, 256);
     if (h.bytes >= bytes) return GG_OK;
     if (h.p) (void)hipHostFree(h.p);
     h.p = nullptr;
     h.bytes = 0;
+    if (bytes > (1u << 20)) bytes += bytes / 4;
     GG_HIP(hipHostMalloc(&h.p, bytes, hipHostMallocDefault));
     h.bytes = bytes;
     return GG_OK;
   };
-  // residency: a spilled partition's regions sit in pinned host memory; they
-  // move over the copy stream, ordered after the compute stream's work
+  // Residency. A partition is resident (its three regions in HBM) or spilled
+  // (in pinned host DRAM). Partitions come in when an operator needs them and
+  // leave only under pressure: when an allocation would pass the device
+  // budget (or hipMalloc fails), the least recently used resident partition
+  // that the current step does not hold is copied out over the copy stream.
+  // GG_OOC_MAX_RESIDENT (a test hook) also caps the number of resident
+  // partitions, so that small graphs go through the spill path.
   hipEvent_t ev = nullptr;
   GG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  struct EventGuard {
+    hipEvent_t& e;
+    ~EventGuard() { (void)hipEventDestroy(e); }
+  } eg{ev};
+  std::vector<uint64_t> last_use(static_cast<size_t>(NP), 0);
+  uint64_t tick = 0;
+  int hold[2] = {-1, -1};
+  int max_resident = 0;
+  if (const char* mr = std::getenv("GG_OOC_MAX_RESIDENT")) max_resident = std::max(2, std::atoi(mr));
+  double copy_ms = 0;
   auto spill = [&](int p) -> int {
     Part& q = pt[p];
     if (!po.spill || !q.resident) return GG_OK;
+    const auto c0 = std::chrono::steady_clock::now();
     const uint64_t rows = rows_of(q);
     GG_HIP(hipEventRecord(ev, stream));
     GG_HIP(hipStreamWaitEvent(copy_stream, ev, 0));
     for (auto& r : q.r) {
+      if (!r.nnz) continue;  // an empty region comes back as zeros
       GG_TRY(host_alloc(r.hptr, (rows + 1) * 8));
       GG_TRY(host_alloc(r.hcol, r.nnz * 4));
       GG_HIP(hipMemcpyAsync(r.hptr.p, r.ptr.p, (rows + 1) * 8, hipMemcpyDeviceToHost, copy_stream));
-      if (r.nnz) GG_HIP(hipMemcpyAsync(r.hcol.p, r.col.p, r.nnz * 4, hipMemcpyDeviceToHost, copy_stream));
+      GG_HIP(hipMemcpyAsync(r.hcol.p, r.col.p, r.nnz * 4, hipMemcpyDeviceToHost, copy_stream));
       S.spill_bytes += (rows + 1) * 8 + r.nnz * 4;
     }
     GG_HIP(hipStreamSynchronize(copy_stream));
@@ -1790,23 +1915,56 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
       release(r.col);
     }
     q.resident = false;
+    q.evicted = true;
+    copy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
     return GG_OK;
   };
+  auto evict_one = [&]() -> bool {
+    int best = -1;
+    for (int p = 0; p < NP; ++p)
+      if (pt[p].resident && p != hold[0] && p != hold[1] && (best < 0 || last_use[p] < last_use[best])) best = p;
+    return best >= 0 && spill(best) == GG_OK;
+  };
+  struct PressureGuard {
+    Engine* e;
+    ~PressureGuard() { e->on_pressure = nullptr; }
+  } pg{this};
+  if (po.spill) on_pressure = evict_one;
   auto fetch = [&](int p) -> int {
     Part& q = pt[p];
+    last_use[p] = ++tick;
     if (q.resident) return GG_OK;
+    if (po.spill && max_resident) {
+      int res = 0;
+      for (auto& x : pt) res += x.resident;
+      while (res >= max_resident && evict_one()) --res;
+    }
+    const auto c0 = std::chrono::steady_clock::now();
     const uint64_t rows = rows_of(q);
     for (auto& r : q.r) {
       GG_TRY(alloc(r.ptr, (rows + 1) * 8, err));
       GG_TRY(alloc(r.col, std::max<uint64_t>(r.nnz, 1) * 4, err));
+      if (!r.nnz) {
+        fill_u64(r.ptr, 0, rows + 1);
+        continue;
+      }
       GG_HIP(hipMemcpyAsync(r.ptr.p, r.hptr.p, (rows + 1) * 8, hipMemcpyHostToDevice, copy_stream));
-      if (r.nnz) GG_HIP(hipMemcpyAsync(r.col.p, r.hcol.p, r.nnz * 4, hipMemcpyHostToDevice, copy_stream));
-      S.spill_bytes += (rows + 1) * 8 + r.nnz * 4;
+      GG_HIP(hipMemcpyAsync(r.col.p, r.hcol.p, r.nnz * 4, hipMemcpyHostToDevice, copy_stream));
+      if (q.evicted) S.spill_bytes += (rows + 1) * 8 + r.nnz * 4;  // the first upload is the input
     }
     GG_HIP(hipEventRecord(ev, copy_stream));
     GG_HIP(hipStreamWaitEvent(stream, ev, 0));
     q.resident = true;
+    copy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
     return GG_OK;
+  };
+  auto elapsed_s = [&]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  auto resident_count = [&]() {
+    int res = 0;
+    for (auto& x : pt) res += x.resident;
+    return res;
   };
   // initial delta of every partition: its input edges, deduplicated, built
   // on the host in one pass (global CSR by row, then per-partition slices)
@@ -2027,8 +2185,12 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   };
   if (!po.spill)
     for (int p = 0; p < NP; ++p) GG_TRY(fetch(p));
+  if (verbose)
+    std::fprintf(stderr, "[gg] partitioned: %d partitions, input on the host after %.1f s\n", NP, elapsed_s());
   // superstep 0 (precomputation)
   for (int p = 0; p < NP; ++p) {
+    hold[0] = p;
+    hold[1] = -1;
     GG_TRY(fetch(p));
     for (;;) {
       Part& q = pt[p];
@@ -2053,7 +2215,13 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
       }
     }
     GG_TRY(cross(p));
-    GG_TRY(spill(p));
+    if (verbose)
+      std::fprintf(stderr,
+                   "[gg] superstep 0: %d/%d partitions, %u rounds, %u launches, %u retries, closure %llu, "
+                   "%d resident, %.2f GB used, %.2f GB moved, %llu allocs (%.1f s), %.1f s\n",
+                   p + 1, NP, S.rounds, S.expand_launches, S.retries,
+                   (unsigned long long)(pt[p].r[kD].nnz + pt[p].r[kO].nnz), resident_count(), used / 1e9,
+                   S.spill_bytes / 1e9, (unsigned long long)alloc_calls, alloc_ms / 1e3, elapsed_s());
   }
   // supersteps 1.. (run_computation); with the DDM schedule a pair runs only
   // when edges cross between the two partitions and one of them has new
@@ -2063,12 +2231,14 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   for (;;) {
     ++supersteps;
     for (int i = 0; i < NP; ++i) {
-      GG_TRY(fetch(i));
       for (int j = i + 1; j < NP; ++j) {
         if (po.ddm && (ddm[i][j] + ddm[j][i] == 0 || !(dirty(i) || dirty(j)))) {
           S.pairs_skipped++;
           continue;
         }
+        hold[0] = i;
+        hold[1] = j;
+        GG_TRY(fetch(i));
         GG_TRY(fetch(j));
         S.pairs_run++;
         pt[i].has[kT] = true;
@@ -2093,18 +2263,26 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
         }
         GG_TRY(cross(i));
         GG_TRY(cross(j));
-        GG_TRY(spill(j));
       }
-      GG_TRY(merge_and_diff(i));
-      GG_TRY(cross(i));
-      GG_TRY(spill(i));
+      // a partition with neither delta nor tmp has nothing to merge
+      if (pt[i].has[kD] || pt[i].has[kT]) {
+        hold[0] = i;
+        hold[1] = -1;
+        GG_TRY(fetch(i));
+        GG_TRY(merge_and_diff(i));
+        GG_TRY(cross(i));
+      }
     }
     bool done = true;
     for (auto& q : pt) done &= !q.has[kD];
+    if (verbose)
+      std::fprintf(stderr, "[gg] superstep %u: pairs run %u skipped %u, %d resident, %.2f GB moved (%.1f s), %.1f s\n",
+                   supersteps, S.pairs_run, S.pairs_skipped, resident_count(), S.spill_bytes / 1e9, copy_ms / 1e3,
+                   elapsed_s());
     if (done) break;
   }
+  hold[0] = hold[1] = -1;
   supersteps_r = supersteps;
-  (void)hipEventDestroy(ev);
   // the closure is every partition's old region: assemble the global CSR
   GG_HIP(hipStreamSynchronize(stream));
   const uint64_t R = rows();
@@ -2112,7 +2290,8 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   std::vector<std::vector<unsigned long long>> lptr(static_cast<size_t>(NP));
   for (int p = 0; p < NP; ++p) {
     Part& q = pt[p];
-    lptr[p].resize(rows_of(q) + 1);
+    lptr[p].assign(rows_of(q) + 1, 0);
+    if (q.r[kO].nnz == 0) continue;
     if (q.resident)
       GG_HIP(hipMemcpy(lptr[p].data(), q.r[kO].ptr.p, lptr[p].size() * 8, hipMemcpyDeviceToHost));
     else
@@ -2128,10 +2307,7 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   for (uint64_t r = 0; r < R; ++r) gptr[r + 1] += gptr[r];
   const uint64_t total = gptr[R];
   // the per-round scratch is not needed any more
-  release(F.ptr);
-  release(F.col);
-  release(M.ptr);
-  release(M.col);
+  for (DevBuf* b : {&F.ptr, &F.col, &M.ptr, &M.col, &staging, &seg_off, &seg_cnt, &active}) release(*b);
   GG_TRY(alloc(old_ptr, (R + 1) * 8, err));
   GG_TRY(alloc(old_col, std::max<uint64_t>(total, 1) * 4, err));
   GG_HIP(hipMemcpy(old_ptr.p, gptr.data(), (R + 1) * 8, hipMemcpyHostToDevice));

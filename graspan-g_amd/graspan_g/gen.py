@@ -84,12 +84,30 @@ def _dedup_sorted(src, dst, lab):
     """Drop duplicate (src,dst,lab) triples; order by (src, lab, dst) so the file
     is grouped by ascending src as the reference reader requires
     (src/main.cpp:125-129, 192-204)."""
-    key = (src.astype(np.uint64) << np.uint64(40)) | (lab.astype(np.uint64) << np.uint64(32)) | dst.astype(np.uint64)
-    key = np.unique(key)
-    s = (key >> np.uint64(40)).astype(np.uint32)
-    l = ((key >> np.uint64(32)) & np.uint64(0xFF)).astype(np.uint8)
-    d = (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
-    return s, d, l
+    src = np.asarray(src, dtype=np.uint32)
+    dst = np.asarray(dst, dtype=np.uint32)
+    lab = np.asarray(lab, dtype=np.uint8)
+    if src.size == 0 or int(src.max()) < (1 << 24):
+        # (src, lab, dst) packed into 24 + 8 + 32 bits
+        key = (src.astype(np.uint64) << np.uint64(40)) | (lab.astype(np.uint64) << np.uint64(32)) | dst.astype(np.uint64)
+        key = np.unique(key)
+        s = (key >> np.uint64(40)).astype(np.uint32)
+        l = ((key >> np.uint64(32)) & np.uint64(0xFF)).astype(np.uint8)
+        d = (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+        return s, d, l
+    if np.all(lab == lab[0]):
+        # one label: (src, dst) in 32 + 32 bits
+        key = np.unique((src.astype(np.uint64) << np.uint64(32)) | dst.astype(np.uint64))
+        return ((key >> np.uint64(32)).astype(np.uint32), (key & np.uint64(0xFFFFFFFF)).astype(np.uint32),
+                np.full(key.size, lab[0], dtype=np.uint8))
+    # wider ids: (lab, dst) in 40 bits as the minor key, src the major key
+    minor = (lab.astype(np.uint64) << np.uint64(32)) | dst.astype(np.uint64)
+    order = np.lexsort((minor, src))
+    s, minor = src[order], minor[order]
+    keep = np.ones(s.size, dtype=bool)
+    keep[1:] = (s[1:] != s[:-1]) | (minor[1:] != minor[:-1])
+    s, minor = s[keep], minor[keep]
+    return s, (minor & np.uint64(0xFFFFFFFF)).astype(np.uint32), (minor >> np.uint64(32)).astype(np.uint8)
 
 
 def dataflow_edges(n: int, seed: int = 0, window: int = 50, p_local: float = 0.8,

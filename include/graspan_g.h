@@ -74,7 +74,8 @@ typedef struct gg_stats {
   uint32_t partitions;      /* partitioned schedule: vertex intervals; 1 otherwise */
   uint32_t pairs_run;       /* out-of-core: partition pairs closed */
   uint32_t pairs_skipped;   /* out-of-core: pairs with no crossing edges or nothing new */
-  uint64_t spill_bytes;     /* out-of-core: bytes moved between host DRAM and HBM */
+  uint64_t spill_bytes;     /* out-of-core: bytes of evicted partitions moved between HBM and host DRAM
+                               (out and back in; the input's first upload is not counted) */
 } gg_stats;
 
 /* ---- context ------------------------------------------------------------ */

@@ -171,3 +171,19 @@ def test_triple_hash_definition():
         return x
     for t in [(0, 0, 1), (1, 2, 3), (4294967295, 123456, 255)]:
         assert gg.triple_hash(*t) == ref(*t)
+
+
+@pytest.mark.parametrize("labels", [1, 3])
+def test_generator_dedup_wide_vertex_ids(labels):
+    """Vertex ids of 2^24 and above keep their value through the
+    generators' dedup-and-sort (every id width, one label or several)."""
+    rng = np.random.default_rng(7)
+    src = rng.integers(0, 1 << 32, 50_000, dtype=np.uint64).astype(np.uint32)
+    dst = rng.integers(0, 1 << 32, 50_000, dtype=np.uint64).astype(np.uint32)
+    lab = rng.integers(1, labels + 1, 50_000).astype(np.uint8)
+    src, dst, lab = (np.concatenate([x, x[:3000]]) for x in (src, dst, lab))
+    s, d, l = gen._dedup_sorted(src, dst, lab)
+    want = sorted(set(zip(src.tolist(), lab.tolist(), dst.tolist())))
+    assert list(zip(s.tolist(), l.tolist(), d.tolist())) == want
+    g = gen.dataflow_blocks(17_000_000, seed=3)
+    assert int(g.src.max()) >= (1 << 24) and int(g.dst.max()) < g.n

@@ -81,7 +81,7 @@ def test_gpu_alias_50k_wide_window(built):
 
 
 def test_gpu_multichunk_rows(built):
-    """Rows whose candidate window is wider than one LDS chunk (32768 ids)."""
+    """Rows whose candidate window is wider than one LDS chunk (16384 ids)."""
     say("wide rows n=200000")
     rng = np.random.default_rng(4)
     n = 200_000
@@ -91,6 +91,32 @@ def test_gpu_multichunk_rows(built):
     lab = np.full(src.shape[0], 2, np.uint8)
     g = gen.GraphSpec(n, gen.DATAFLOW_RULES, src, dst, lab, "wide")
     check_against_oracle(g)
+
+
+def _paired_blocks(n, seed):
+    """Block dataflow graph plus far edges u -> u + n/2 from 2% of the
+    vertices of the lower half: a lower block reaches the blocks half the id
+    range above it, so its rows hold two clusters of ids many LDS windows
+    apart."""
+    g = gen.dataflow_blocks(n, seed=seed)
+    rng = np.random.default_rng(seed + 1000)
+    far = np.flatnonzero(rng.random(n // 2) < 0.02).astype(np.uint32)
+    src = np.concatenate([g.src, far])
+    dst = np.concatenate([g.dst, far + np.uint32(n // 2)])
+    s, d, l = gen._dedup_sorted(src, dst, np.full(src.shape[0], g.lab[0], np.uint8))
+    return gen.GraphSpec(n, gen.DATAFLOW_RULES, s, d, l, f"paired-blocks-n{n}-s{seed}")
+
+
+def test_gpu_sparse_wide_rows(built):
+    """Rows whose ids sit in a few clusters far apart: the windows between
+    the clusters are skipped, in core and in the partitioned schedule."""
+    g = _paired_blocks(120_000, seed=61)
+    say("sparse wide rows", g.name)
+    c, st, ref = check_against_oracle(g, full_triples=False)
+    o = gg.Context()
+    o.set_spec(g)
+    o.run_out_of_core(4)
+    assert o.counts()[1] == ref.total and o.checksum() == ref.checksum
 
 
 def test_gpu_staging_retry(built, monkeypatch):
@@ -282,9 +308,11 @@ def test_gpu_server_results_equal_alone(built):
 
 
 @pytest.mark.parametrize("parts", [1, 4, 16])
-def test_gpu_out_of_core_matches_oracle(built, parts):
-    """Partitions spilled to host DRAM, pairs chosen by crossing edges: the
-    least fixpoint, equal to the oracle's."""
+def test_gpu_out_of_core_matches_oracle(built, parts, monkeypatch):
+    """Partitions spilled to host DRAM (at most two resident, through the
+    GG_OOC_MAX_RESIDENT hook), pairs chosen by crossing edges: the least
+    fixpoint, equal to the oracle's."""
+    monkeypatch.setenv("GG_OOC_MAX_RESIDENT", "2")
     cases = [gen.dataflow_blocks(30_000, seed=51)]
     if parts <= 4:
         cases.append(gen.alias(1_000, seed=52))
@@ -297,7 +325,31 @@ def test_gpu_out_of_core_matches_oracle(built, parts):
         per, tot = c.counts()
         assert tot == ref.total and c.checksum() == ref.checksum
         np.testing.assert_array_equal(per, ref.counts)
-        assert st.partitions == parts and st.spill_bytes > 0
+        assert st.partitions == parts
+        assert (st.spill_bytes > 0) == (parts > 2)
         if parts == 16 and g.name.startswith("dataflow"):
             # blocks only straddle neighbouring partitions
             assert st.pairs_skipped > st.pairs_run > 0
+
+
+def test_gpu_out_of_core_budget_pressure(built):
+    """With a device budget below the unconstrained peak, partitions leave
+    HBM only when an allocation would not fit; the closure is unchanged."""
+    g = gen.dataflow_blocks(400_000, seed=53)
+    say("out-of-core budget", g.name)
+    free = gg.Context()
+    free.set_spec(g)
+    s0 = free.run_out_of_core(16)
+    ref = (free.counts()[1], free.checksum())
+    assert s0.spill_bytes == 0
+    free.release()
+    budget = int(s0.device_bytes_peak * 0.75)
+    c = gg.Context(device_budget=budget)
+    c.set_spec(g)
+    st = c.run_out_of_core(16)
+    assert (c.counts()[1], c.checksum()) == ref
+    assert st.spill_bytes > 0 and st.device_bytes_peak <= budget
+    inc = gg.Context()
+    inc.set_spec(g)
+    inc.run(0)
+    assert (inc.counts()[1], inc.checksum()) == ref
