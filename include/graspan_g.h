@@ -153,9 +153,12 @@ int gg_closure_run_partitioned(gg_ctx *ctx, int filter_flag, gg_stats *stats);
  * num_partitions equal contiguous intervals. The reference's partition-pair
  * schedule runs (superstep 0 per partition, then pairs i < j; the reference
  * keeps the partitions it is not using in Partition<k> files,
- * src/main.cpp:335-631, 764-858): only the partitions of the current session
- * are in HBM, the others are spilled to pinned host DRAM with hipMemcpyAsync
- * on a copy stream. Pairs are chosen by their cross-partition edge counts,
+ * src/main.cpp:335-631, 764-858). Partitions are loaded into HBM when the
+ * schedule needs them and stay there. When an allocation would pass the
+ * budget (or hipMalloc fails), the least recently used partition that the
+ * current step does not hold is spilled to pinned host DRAM with
+ * hipMemcpyAsync on a copy stream. The finished closure is assembled in HBM
+ * and must fit the budget. Pairs are chosen by their cross-partition edge counts,
  * recomputed on the device whenever a partition changes: a pair runs only
  * when edges cross between the two partitions and one of them has new edges
  * (the weights of the reference's DDM, src/ddm.cpp:27-35, which the reference
