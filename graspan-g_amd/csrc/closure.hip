@@ -58,9 +58,10 @@ constexpr uint32_t kChunkWords = kChunkBits / 32;
 constexpr int kUnroll = 8;              // loads in flight per lane in the candidate stream
 constexpr uint32_t kGrab = 16;          // active rows taken per work-counter atomic
 constexpr uint32_t kXcds = 8;           // MI355X: 8 XCDs, each with its own L2
-constexpr int kExpandWaves = 4;         // k_expand occupancy target (waves per SIMD)
+constexpr int kExpandWaves = 5;         // k_expand occupancy target (waves per SIMD)
 constexpr uint32_t kMaxCrossParts = 1024;  // partitions of the out-of-core schedule
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
+constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
 constexpr int kMaxRules = 64;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -96,6 +97,7 @@ struct Counters {
   unsigned int misses;         // rows whose guessed window missed a candidate
   unsigned int label_flags[8]; // bit l: delta of label l is not empty
   unsigned int pad;
+  unsigned long long xs[4];    // built with GG_EXPAND_STATS: packed words, list ids, keys, rows of k_expand
 };
 
 struct DCsr {
@@ -112,6 +114,16 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Values the whole wave holds alike, moved to scalar registers (the first
+// active lane's copy), which frees vector registers for occupancy.
+// (The builtin works on int: each half goes back through uint32_t before it
+// is widened, so no sign bit spreads into the high word.)
+__device__ inline uint32_t uni_u32(uint32_t x) { return uint32_t(__builtin_amdgcn_readfirstlane(int(x))); }
+__device__ inline unsigned long long uni_u64(unsigned long long x) {
+  const uint32_t lo = uni_u32(uint32_t(x)), hi = uni_u32(uint32_t(x >> 32));
+  return (unsigned long long)lo | ((unsigned long long)hi << 32);
+}
+
 __device__ inline uint32_t wave_incl_scan(uint32_t x, int lane) {
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) {
@@ -121,22 +133,23 @@ __device__ inline uint32_t wave_incl_scan(uint32_t x, int lane) {
   return x;
 }
 
+// Butterfly reductions: every lane ends with the result, so it is uniform.
 __device__ inline unsigned long long wave_sum_u64(unsigned long long x) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
-  return x;
+  return uni_u64(x);
 }
 
 __device__ inline uint32_t wave_min_u32(uint32_t x) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) x = min(x, (uint32_t)__shfl_xor(x, d, kWave));
-  return x;
+  return uni_u32(x);
 }
 
 __device__ inline uint32_t wave_max_u32(uint32_t x) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) x = max(x, (uint32_t)__shfl_xor(x, d, kWave));
-  return x;
+  return uni_u32(x);
 }
 
 __device__ inline bool label_nonempty(const Counters* c, uint32_t l) { return (c->label_flags[l >> 5] >> (l & 31)) & 1u; }
@@ -507,7 +520,7 @@ struct ExpandArgs {
   unsigned long long staging_cap;
   unsigned long long* seg_off;
   uint32_t* seg_cnt;
-  int guess;  // try a window centred on the row's vertex first
+  uint32_t guess;  // ids of a window centred on the row's vertex tried first (0: none)
 };
 
 struct WaveLds {
@@ -531,7 +544,7 @@ __device__ inline void expand_lists(const uint32_t* __restrict__ col, unsigned l
     const unsigned long long y = __shfl_up(incl, d, kWave);
     if (lane >= d) incl += y;
   }
-  const unsigned long long total = __shfl(incl, kWave - 1, kWave);
+  const unsigned long long total = uni_u64(__shfl(incl, kWave - 1, kWave));
   if (total == 0) return;
   wave_sync();
   w.pref[lane] = incl;
@@ -570,7 +583,7 @@ __device__ inline void expand_words(const unsigned long long* __restrict__ ent, 
     const unsigned long long y = __shfl_up(incl, d, kWave);
     if (lane >= d) incl += y;
   }
-  const unsigned long long total = __shfl(incl, kWave - 1, kWave);
+  const unsigned long long total = uni_u64(__shfl(incl, kWave - 1, kWave));
   if (total == 0) return;
   wave_sync();
   w.pref[lane] = incl;
@@ -649,60 +662,97 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
     }
     return;
   }
-  for (int i = 0; i < a.g.nu; ++i) {
-    if (a.g.u_lhs[i] != A) continue;
-    const uint32_t rb = uint32_t(a.g.u_rhs[i]) * n + u;
-    const unsigned long long b = a.dl.ptr[rb], e = a.dl.ptr[rb + 1];
-    visit(a.dl.col, lane == 0 ? b : 0ull, lane == 0 ? uint32_t(e - b) : 0u);
-  }
-  for (int i = 0; i < a.g.nb; ++i) {
-    if (a.g.b_lhs[i] != A) continue;
-    const uint32_t B = a.g.b_rhs1[i], C = a.g.b_rhs2[i];
-    // delta_B(u) x (old_C U delta_C)
+  // MODE 0. The row's lists come from segments: a unary rule's Δ_B(u); per
+  // binary rule, the keys Δ_B(u) (lists old_C(v) and Δ_C(v)) and the keys of
+  // the frontier pair list (lists Δ_C(v)). Lane j describes segment j. The
+  // bounds of every segment are loaded together, and the keys of all keyed
+  // segments are flattened into batches of 64, so a row costs one round trip
+  // per stage (bounds, keys, key rows, lists) whatever its number of rules.
+  // a segment descriptor: kind (bits 0-1), packed old_C rows (bit 2), C (bits 8-15)
+  enum { kSegU = 0, kSegD = 1, kSegP = 2 };
+  uint32_t s_desc = 0, s_row = 0;
+  uint32_t ns = 0;  // segments in the current group (uniform)
+  auto run_group = [&]() {
+    unsigned long long b = 0, e = 0;
+    const uint32_t kind_l = s_desc & 3u;
+    if (uint32_t(lane) < ns) {
+      const unsigned long long* ptr = kind_l == kSegP ? a.b2.ptr : a.dl.ptr;
+      b = ptr[s_row];
+      e = ptr[s_row + 1];
+    }
     {
-      const bool packed = PACKED && a.pk.len && packed_label(a.pk, C);
-      const uint32_t rb = B * n + u;
-      const unsigned long long kb = a.dl.ptr[rb], ke = a.dl.ptr[rb + 1];
-      for (unsigned long long k0 = kb; k0 < ke; k0 += kWave) {
-        const unsigned long long k = k0 + lane;
-        unsigned long long ob = 0, db = 0, pb = 0;
-        uint32_t ol = 0, dlen = 0, pl = 0;
-        if (k < ke) {
-          const uint32_t rc = C * n + a.dl.col[k];
+      const bool ul = uint32_t(lane) < ns && kind_l == kSegU;
+      visit(a.dl.col, ul ? b : 0ull, ul ? uint32_t(e - b) : 0u);
+    }
+    const unsigned long long klen = (uint32_t(lane) < ns && kind_l != kSegU) ? e - b : 0ull;
+    unsigned long long incl = klen;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const unsigned long long y = __shfl_up(incl, d, kWave);
+      if (lane >= d) incl += y;
+    }
+    const unsigned long long total = uni_u64(__shfl(incl, kWave - 1, kWave));
+#ifdef GG_EXPAND_STATS
+    if (PACKED && lane == 0 && total) atomicAdd(&a.ctr->xs[2], total);
+#endif
+    for (unsigned long long t0 = 0; t0 < total; t0 += kWave) {
+      const unsigned long long t = t0 + lane;
+      // segment of key t: the first j whose inclusive end is past t
+      int j = 0;
+      unsigned long long ex = 0;
+      for (uint32_t q = 0; q + 1 < ns; ++q) {
+        const unsigned long long iq = __shfl(incl, int(q), kWave);
+        if (iq <= t) {
+          j = int(q) + 1;
+          ex = iq;
+        }
+      }
+      const uint32_t desc = __shfl(s_desc, j, kWave);
+      const uint32_t kind = desc & 3u, C = desc >> 8;
+      const bool pk = (desc & 4u) != 0;
+      const unsigned long long bj = __shfl(b, j, kWave);
+      unsigned long long ob = 0, db = 0, pb = 0;
+      uint32_t ol = 0, dlen = 0, pl = 0;
+      bool is_pk = false;
+      if (t < total) {
+        const unsigned long long k = bj + (t - ex);
+        const uint32_t v = kind == kSegP ? a.b2.col[k] : a.dl.col[k];
+        const uint32_t rc = C * n + v;
+        db = a.dl.ptr[rc];
+        dlen = uint32_t(a.dl.ptr[rc + 1] - db);
+        if (kind == kSegD) {
           ob = a.old.ptr[rc];
           ol = uint32_t(a.old.ptr[rc + 1] - ob);
-          db = a.dl.ptr[rc];
-          dlen = uint32_t(a.dl.ptr[rc + 1] - db);
-          if (packed) {
+          if (PACKED && pk) {
+            is_pk = true;
             pb = ob;
             pl = a.pk.len[rc];
           }
         }
-        if (packed)
-          visitw(pb, pl, ol);
-        else
-          visit(a.old.col, ob, ol);
-        visit(a.dl.col, db, dlen);
       }
+      if (PACKED) visitw(is_pk ? pb : 0ull, is_pk ? pl : 0u, is_pk ? ol : 0u);
+      visit(a.old.col, is_pk ? 0ull : ob, is_pk ? 0u : ol);
+      visit(a.dl.col, db, dlen);
     }
-    // old_B(u) x delta_C, keys from the frontier pair list
+    ns = 0;
+  };
+  auto add = [&](uint32_t kind, uint32_t label, uint32_t row, bool pk) {
+    if (uint32_t(lane) == ns) {
+      s_desc = kind | (pk ? 4u : 0u) | (label << 8);
+      s_row = row;
+    }
+    if (++ns == uint32_t(kWave)) run_group();
+  };
+  for (int i = 0; i < a.g.nu; ++i)
+    if (a.g.u_lhs[i] == A) add(kSegU, 0, uint32_t(a.g.u_rhs[i]) * n + u, false);
+  for (int i = 0; i < a.g.nb; ++i) {
+    if (a.g.b_lhs[i] != A) continue;
+    const uint32_t B = a.g.b_rhs1[i], C = a.g.b_rhs2[i];
+    add(kSegD, C, B * n + u, a.pk.len && packed_label(a.pk, C));
     const uint32_t slot = a.g.b_slot[i];
-    if (slot != 0xFFu && a.b2.ptr) {
-      const uint32_t rb = slot * n + u;
-      const unsigned long long kb = a.b2.ptr[rb], ke = a.b2.ptr[rb + 1];
-      for (unsigned long long k0 = kb; k0 < ke; k0 += kWave) {
-        const unsigned long long k = k0 + lane;
-        unsigned long long db = 0;
-        uint32_t dlen = 0;
-        if (k < ke) {
-          const uint32_t rc = C * n + a.b2.col[k];
-          db = a.dl.ptr[rc];
-          dlen = uint32_t(a.dl.ptr[rc + 1] - db);
-        }
-        visit(a.dl.col, db, dlen);
-      }
-    }
+    if (slot != 0xFFu && a.b2.ptr) add(kSegP, C, slot * n + u, false);
   }
+  if (ns) run_group();
 }
 
 // First position in sorted ids col[b, e) whose id is >= x.
@@ -799,29 +849,25 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
     for (unsigned long long i = db + lane; i < de; i += kWave) seed(a.dl.col[i]);
     wave_sync();
   }
-  volatile uint32_t* vseen = w.seen;
+  // candidates go to `fresh` with plain (no-return) LDS ORs; the ids already
+  // known (`seen`) are taken out once, word by word, at the end
   auto mark = [&](uint32_t x) {
     const uint32_t off = x - c0;
-    if (off < span) {
-      const uint32_t wi = off >> 5, m = 1u << (off & 31);
-      if (!(vseen[wi] & m)) {
-        const uint32_t prev = atomicOr(&w.seen[wi], m);
-        if (!(prev & m)) atomicOr(&w.fresh[wi], m);
-      }
-    } else {
+    if (off < span)
+      __hip_atomic_fetch_or(&w.fresh[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
       miss_l = true;
-    }
   };
   unsigned long long kc = 0;
+#ifdef GG_EXPAND_STATS
+  unsigned long long st_words = 0, st_ids = 0;
+#endif
   auto mark_word = [&](unsigned long long e) {
     const uint32_t wi = uint32_t(e) - w0;
     if (wi < words) {
       const uint32_t bits = uint32_t(e >> 32);
       if (ranged) kc += __popc(bits);
-      if (bits & ~vseen[wi]) {
-        const uint32_t nb = bits & ~atomicOr(&w.seen[wi], bits);
-        if (nb) atomicOr(&w.fresh[wi], nb);
-      }
+      __hip_atomic_fetch_or(&w.fresh[wi], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     } else {
       miss_l = true;
     }
@@ -833,6 +879,9 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
         cut(col, b, e);
         if (ranged && e < lb + len) nxt = min(nxt, col[e]);
         kc += e - b;
+#ifdef GG_EXPAND_STATS
+        st_ids += e - b;
+#endif
         expand_lists(col, b, uint32_t(e - b), w, lane, mark);
       },
       [&](unsigned long long pb, uint32_t pl, uint32_t len) {
@@ -844,8 +893,20 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
         } else {
           kc += len;
         }
+#ifdef GG_EXPAND_STATS
+        st_words += e - b;
+#endif
         expand_words(a.pk.ent, b, uint32_t(e - b), w, lane, mark_word);
       });
+#ifdef GG_EXPAND_STATS
+  {
+    const unsigned long long sw = wave_sum_u64(st_words), si = wave_sum_u64(st_ids);
+    if (lane == 0) {
+      atomicAdd(&a.ctr->xs[0], sw);
+      atomicAdd(&a.ctr->xs[1], si);
+    }
+  }
+#endif
   if (MODE == 2 && a.ro.kind == 2) {
     // eps: the row's own vertex
     bool has_eps = false;
@@ -865,7 +926,12 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
   next = wave_min_u32(nxt);
   wave_sync();
   uint32_t c = 0;
-  for (uint32_t i = lane; i < words; i += kWave) c += __popc(w.fresh[i]);
+  for (uint32_t i = lane; i < words; i += kWave) {
+    const uint32_t f = w.fresh[i] & ~w.seen[i];
+    w.fresh[i] = f;
+    c += __popc(f);
+  }
+  wave_sync();
   return uint32_t(wave_sum_u64(c));
 }
 
@@ -906,7 +972,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       const unsigned long long grab = max<unsigned long long>(cnt, kStageBlock);
       unsigned long long b = 0;
       if (lane == 0) b = atomicAdd(&a.ctr->cursor, grab);
-      b = __shfl(b, 0, kWave);
+      b = uni_u64(b);  // lane 0's value: every lane is active here
       blk_pos = b;
       blk_end = b + grab;
     }
@@ -926,7 +992,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
         const uint32_t se = uint32_t((unsigned long long)nactive * (seg + 1) / kXcds);
         uint32_t b = 0;
         if (lane == 0) b = atomicAdd(&a.ctr->work_xcd[seg], kGrab);
-        b = sb + __shfl(b, 0, kWave);
+        b = sb + uni_u32(b);
         if (b < se) {
           next = b;
           end = min(b + kGrab, se);
@@ -939,23 +1005,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       if (!got) break;
     }
     const uint32_t idx = next++;
-    const uint32_t r = a.active[idx];
+    const uint32_t r = uni_u32(a.active[idx]);
+#ifdef GG_EXPAND_STATS
+    if (lane == 0) atomicAdd(&a.ctr->xs[3], 1ull);
+#endif
     const uint32_t rn = MODE == 2 ? a.ro.np : a.n;
     const uint32_t A = r / rn, u = r - A * rn + (MODE == 2 ? a.ro.first : 0u);
     unsigned long long krow = 0;
     uint32_t cnt = 0;
     unsigned long long pos = 0;
-    // attempt 0: the kChunkBits ids centred on u (rows of graphs with id
+    // attempt 0: the a.guess ids centred on u (rows of graphs with id
     // locality fit; a candidate outside it sends the row to attempt 1);
     // attempt 1: the exact candidate window, in chunks when it is wider.
     bool empty = false;
-    for (int attempt = (MODE == 0 && a.guess && a.n > kChunkBits) ? 0 : 1; attempt < 2; ++attempt) {
+    for (int attempt = (MODE == 0 && a.guess && a.n > a.guess) ? 0 : 1; attempt < 2; ++attempt) {
       uint32_t lo = 0, hi = a.n - 1;
       if (attempt == 0) {
-        const uint32_t half = kChunkBits / 2;
+        const uint32_t half = a.guess / 2;
         lo = (u > half ? u - half : 0u) & ~31u;
-        if (lo + kChunkBits > a.n) lo = (a.n - kChunkBits) & ~31u;
-        hi = min(a.n - 1, lo + kChunkBits - 1);
+        if (lo + a.guess > a.n) lo = (a.n - a.guess) & ~31u;
+        hi = min(a.n - 1, lo + a.guess - 1);
       } else if (a.n > kChunkBits) {
         uint32_t mn = 0xFFFFFFFFu, mx = 0;
         unsigned long long kk = 0;
@@ -1332,7 +1401,10 @@ class Engine {
   DevBuf pk_len, pk_ent, pk2_ent;
   uint32_t pack_labels[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool pk_ready = false;
-  bool guess_window = true;  // per round: last round's guessed windows rarely missed
+  // Guessed window (ids) for the next round: starts small; a round where
+  // more than 1/8 of the rows missed moves it up a size (4096, 16384, off),
+  // and after a round without guessing it is tried again at 16384.
+  uint32_t guess_bits = kGuessMin;
   uint32_t supersteps_r = 0;  // supersteps of the last partitioned-schedule run
   // host copy of the input (the partitioned schedule builds its regions from it)
   std::vector<uint32_t> h_src, h_dst;
@@ -1482,7 +1554,7 @@ class Engine {
     a.staging_cap = staging.bytes / 4;
     a.seg_off = P<unsigned long long>(seg_off);
     a.seg_cnt = P<uint32_t>(seg_cnt);
-    a.guess = guess_window ? 1 : 0;
+    a.guess = guess_bits;
     return a;
   }
 
@@ -1628,7 +1700,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   }
   ot_nnz = 0;
   pk_ready = false;
-  guess_window = true;
+  guess_bits = kGuessMin;
   bool pack_any = false;
   for (int i = 0; i < 8; ++i) pack_any |= pack_labels[i] != 0;
   if (pack_any) {
@@ -1683,7 +1755,11 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
     }
     N = h_ctr->newe;
-    guess_window = h_ctr->misses * 8u <= h_ctr->active;
+    const uint32_t used_guess = guess_bits;
+    if (guess_bits == 0)
+      guess_bits = kChunkBits;
+    else if (uint64_t(h_ctr->misses) * 8u > h_ctr->active)
+      guess_bits = guess_bits < kChunkBits ? kChunkBits : 0u;
     S.rounds++;
     S.probe_reads += h_ctr->pkeys + h_ctr->cand;
     S.candidates += h_ctr->cand;
@@ -1692,8 +1768,11 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     if (verbose) {
       float xms = 0;
       (void)hipEventElapsedTime(&xms, ev_expand.back().a, ev_expand.back().b);
-      std::fprintf(stderr, "[gg] round %u: active %u cand %llu new %llu expand %.2f ms\n", S.rounds, h_ctr->active,
-                   (unsigned long long)h_ctr->cand, (unsigned long long)N, xms);
+      std::fprintf(stderr,
+                   "[gg] round %u: active %u cand %llu new %llu guess %u misses %u expand %.2f ms"
+                   " (words %llu ids %llu keys %llu rows %llu)\n",
+                   S.rounds, h_ctr->active, (unsigned long long)h_ctr->cand, (unsigned long long)N, used_guess,
+                   h_ctr->misses, xms, h_ctr->xs[0], h_ctr->xs[1], h_ctr->xs[2], h_ctr->xs[3]);
     }
     // delta' from staging
     EventPair& c = mk(ev_compact);
