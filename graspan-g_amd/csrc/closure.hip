@@ -58,7 +58,7 @@ constexpr uint32_t kChunkWords = kChunkBits / 32;
 constexpr int kUnroll = 8;              // loads in flight per lane in the candidate stream
 constexpr uint32_t kGrab = 16;          // active rows taken per work-counter atomic
 constexpr uint32_t kXcds = 8;           // MI355X: 8 XCDs, each with its own L2
-constexpr int kExpandWaves = 5;         // k_expand occupancy target (waves per SIMD)
+constexpr int kExpandWaves = 6;         // k_expand occupancy target (waves per SIMD)
 constexpr uint32_t kMaxCrossParts = 1024;  // partitions of the out-of-core schedule
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
 constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
@@ -124,32 +124,58 @@ __device__ inline unsigned long long uni_u64(unsigned long long x) {
   return (unsigned long long)lo | ((unsigned long long)hi << 32);
 }
 
-__device__ inline uint32_t wave_incl_scan(uint32_t x, int lane) {
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    uint32_t y = __shfl_up(x, d, kWave);
-    if (lane >= d) x += y;
-  }
+// Wave64 scans and reductions on DPP row operations (GCN/CDNA: row_shr
+// within 16-lane rows, then row_bcast:15 and row_bcast:31 across rows):
+// register moves, no LDS round trip. A lane with no DPP source keeps the
+// identity `id`.
+template <class Op>
+__device__ inline uint32_t dpp_scan_u32(uint32_t x, uint32_t id, Op op) {
+  x = op(x, uint32_t(__builtin_amdgcn_update_dpp(int(id), int(x), 0x111, 0xf, 0xf, false)));  // row_shr:1
+  x = op(x, uint32_t(__builtin_amdgcn_update_dpp(int(id), int(x), 0x112, 0xf, 0xf, false)));  // row_shr:2
+  x = op(x, uint32_t(__builtin_amdgcn_update_dpp(int(id), int(x), 0x114, 0xf, 0xf, false)));  // row_shr:4
+  x = op(x, uint32_t(__builtin_amdgcn_update_dpp(int(id), int(x), 0x118, 0xf, 0xf, false)));  // row_shr:8
+  x = op(x, uint32_t(__builtin_amdgcn_update_dpp(int(id), int(x), 0x142, 0xa, 0xf, false)));  // row_bcast:15
+  x = op(x, uint32_t(__builtin_amdgcn_update_dpp(int(id), int(x), 0x143, 0xc, 0xf, false)));  // row_bcast:31
   return x;
 }
 
-// Butterfly reductions: every lane ends with the result, so it is uniform.
-__device__ inline unsigned long long wave_sum_u64(unsigned long long x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
-  return uni_u64(x);
+template <int CTRL, int ROW_MASK>
+__device__ inline unsigned long long dpp_u64(unsigned long long x) {
+  const uint32_t lo = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(x)), CTRL, ROW_MASK, 0xf, false));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(x >> 32)), CTRL, ROW_MASK, 0xf, false));
+  return (unsigned long long)lo | ((unsigned long long)hi << 32);
 }
 
+// Inclusive prefix sum over the wave (lane i gets x_0 + ... + x_i).
+__device__ inline unsigned long long wave_incl_scan_u64(unsigned long long x) {
+  x += dpp_u64<0x111, 0xf>(x);
+  x += dpp_u64<0x112, 0xf>(x);
+  x += dpp_u64<0x114, 0xf>(x);
+  x += dpp_u64<0x118, 0xf>(x);
+  x += dpp_u64<0x142, 0xa>(x);
+  x += dpp_u64<0x143, 0xc>(x);
+  return x;
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t x, int) {
+  return dpp_scan_u32(x, 0u, [](uint32_t p, uint32_t q) { return p + q; });
+}
+
+__device__ inline uint32_t lane63_u32(uint32_t x) { return uint32_t(__builtin_amdgcn_readlane(int(x), 63)); }
+__device__ inline unsigned long long lane63_u64(unsigned long long x) {
+  return (unsigned long long)lane63_u32(uint32_t(x)) | ((unsigned long long)lane63_u32(uint32_t(x >> 32)) << 32);
+}
+
+// Reductions: the last lane of the scan holds the result; it is read into a
+// scalar register, so the result is uniform.
+__device__ inline unsigned long long wave_sum_u64(unsigned long long x) { return lane63_u64(wave_incl_scan_u64(x)); }
+
 __device__ inline uint32_t wave_min_u32(uint32_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x = min(x, (uint32_t)__shfl_xor(x, d, kWave));
-  return uni_u32(x);
+  return lane63_u32(dpp_scan_u32(x, 0xFFFFFFFFu, [](uint32_t p, uint32_t q) { return p < q ? p : q; }));
 }
 
 __device__ inline uint32_t wave_max_u32(uint32_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x = max(x, (uint32_t)__shfl_xor(x, d, kWave));
-  return uni_u32(x);
+  return lane63_u32(dpp_scan_u32(x, 0u, [](uint32_t p, uint32_t q) { return p > q ? p : q; }));
 }
 
 __device__ inline bool label_nonempty(const Counters* c, uint32_t l) { return (c->label_flags[l >> 5] >> (l & 31)) & 1u; }
@@ -178,12 +204,7 @@ __global__ void k_add_ptr(unsigned long long* out, const unsigned long long* a, 
 __device__ inline unsigned long long block_excl_scan_u64(unsigned long long x, unsigned long long* s_tmp,
                                                          unsigned long long& total) {
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  unsigned long long incl = x;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    unsigned long long y = __shfl_up(incl, d, kWave);
-    if (lane >= d) incl += y;
-  }
+  const unsigned long long incl = wave_incl_scan_u64(x);
   if (lane == kWave - 1) s_tmp[w] = incl;
   __syncthreads();
   unsigned long long off = 0, tot = 0;
@@ -324,7 +345,7 @@ __global__ void k_activate_raw(const unsigned long long* ptr, uint32_t rows, uin
     if (!m) continue;
     uint32_t pos = 0;
     if (lane == 0) pos = atomicAdd(&ctr->active, uint32_t(__popcll(m)));
-    pos = __shfl(pos, 0, kWave);
+    pos = uni_u32(pos);  // lane 0: the branch is uniform
     if (act) active[pos + __popcll(m & ((1ull << lane) - 1))] = r;
   }
 }
@@ -388,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
       if (m) {
         uint32_t pos = 0;
         if (lane == 0) pos = atomicAdd(&a.ctr->active, uint32_t(__popcll(m)));
-        pos = __shfl(pos, 0, kWave);
+        pos = uni_u32(pos);  // lane 0: the branch is uniform
         if (act) a.active[pos + __popcll(m & ((1ull << lane) - 1))] = A * a.n + u;
       }
     }
@@ -511,6 +532,7 @@ struct ExpandArgs {
   DCsr old, dl, raw;
   RegionOp ro;
   DPacked pk;  // packed old rows
+  DPacked dpk;  // packed delta rows (same labels; entries at the row's delta offset)
   DCsr b2;  // rows slot*n + u: keys v with old_B(u,v) and delta_C(v) non-empty
   uint32_t n;
   DevGrammar g;
@@ -538,13 +560,8 @@ struct WaveLds {
 template <class F>
 __device__ inline void expand_lists(const uint32_t* __restrict__ col, unsigned long long lb, uint32_t len, WaveLds& w,
                                     int lane, F&& elem) {
-  unsigned long long incl = len;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const unsigned long long y = __shfl_up(incl, d, kWave);
-    if (lane >= d) incl += y;
-  }
-  const unsigned long long total = uni_u64(__shfl(incl, kWave - 1, kWave));
+  const unsigned long long incl = wave_incl_scan_u64(len);
+  const unsigned long long total = lane63_u64(incl);
   if (total == 0) return;
   wave_sync();
   w.pref[lane] = incl;
@@ -577,13 +594,8 @@ __device__ inline void expand_lists(const uint32_t* __restrict__ col, unsigned l
 template <class F>
 __device__ inline void expand_words(const unsigned long long* __restrict__ ent, unsigned long long pb, uint32_t plen,
                                     WaveLds& w, int lane, F&& elem) {
-  unsigned long long incl = plen;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const unsigned long long y = __shfl_up(incl, d, kWave);
-    if (lane >= d) incl += y;
-  }
-  const unsigned long long total = uni_u64(__shfl(incl, kWave - 1, kWave));
+  const unsigned long long incl = wave_incl_scan_u64(plen);
+  const unsigned long long total = lane63_u64(incl);
   if (total == 0) return;
   wave_sync();
   w.pref[lane] = incl;
@@ -613,8 +625,8 @@ __device__ inline void expand_words(const unsigned long long* __restrict__ ent, 
 
 // Visits every candidate list of row (A,u): calls visit(col, lb, len) once
 // per batch with one list per lane (len 0 = no list on that lane). With
-// PACKED, old_C rows of packed labels go to visitw(pb, plen, len) instead
-// (plen packed words standing for len ids).
+// PACKED, rows of packed labels (old and delta) go to visitw(ent, pb, plen,
+// len) instead: plen packed words at ent[pb ..] standing for len ids.
 template <int MODE, bool PACKED, class V, class W>
 __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A, uint32_t u, int lane, V&& visit,
                                      W&& visitw) {
@@ -682,16 +694,13 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
     }
     {
       const bool ul = uint32_t(lane) < ns && kind_l == kSegU;
-      visit(a.dl.col, ul ? b : 0ull, ul ? uint32_t(e - b) : 0u);
+      const bool upk = PACKED && ul && (s_desc & 4u) && a.dpk.len && e > b;
+      if (PACKED) visitw(a.dpk.ent, upk ? b : 0ull, upk ? a.dpk.len[s_row] : 0u, upk ? uint32_t(e - b) : 0u);
+      visit(a.dl.col, ul && !upk ? b : 0ull, ul && !upk ? uint32_t(e - b) : 0u);
     }
     const unsigned long long klen = (uint32_t(lane) < ns && kind_l != kSegU) ? e - b : 0ull;
-    unsigned long long incl = klen;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const unsigned long long y = __shfl_up(incl, d, kWave);
-      if (lane >= d) incl += y;
-    }
-    const unsigned long long total = uni_u64(__shfl(incl, kWave - 1, kWave));
+    const unsigned long long incl = wave_incl_scan_u64(klen);
+    const unsigned long long total = lane63_u64(incl);
 #ifdef GG_EXPAND_STATS
     if (PACKED && lane == 0 && total) atomicAdd(&a.ctr->xs[2], total);
 #endif
@@ -701,7 +710,9 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
       int j = 0;
       unsigned long long ex = 0;
       for (uint32_t q = 0; q + 1 < ns; ++q) {
-        const unsigned long long iq = __shfl(incl, int(q), kWave);
+        const unsigned long long iq =
+            (unsigned long long)uint32_t(__builtin_amdgcn_readlane(int(uint32_t(incl)), int(q))) |
+            ((unsigned long long)uint32_t(__builtin_amdgcn_readlane(int(uint32_t(incl >> 32)), int(q))) << 32);
         if (iq <= t) {
           j = int(q) + 1;
           ex = iq;
@@ -711,28 +722,34 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
       const uint32_t kind = desc & 3u, C = desc >> 8;
       const bool pk = (desc & 4u) != 0;
       const unsigned long long bj = __shfl(b, j, kWave);
-      unsigned long long ob = 0, db = 0, pb = 0;
-      uint32_t ol = 0, dlen = 0, pl = 0;
-      bool is_pk = false;
+      unsigned long long ob = 0, db = 0;
+      uint32_t ol = 0, dlen = 0, pl = 0, dpl = 0;
+      bool is_pk = false, d_pk = false;
       if (t < total) {
         const unsigned long long k = bj + (t - ex);
         const uint32_t v = kind == kSegP ? a.b2.col[k] : a.dl.col[k];
         const uint32_t rc = C * n + v;
         db = a.dl.ptr[rc];
         dlen = uint32_t(a.dl.ptr[rc + 1] - db);
+        if (PACKED && pk && a.dpk.len && dlen) {
+          d_pk = true;
+          dpl = a.dpk.len[rc];
+        }
         if (kind == kSegD) {
           ob = a.old.ptr[rc];
           ol = uint32_t(a.old.ptr[rc + 1] - ob);
-          if (PACKED && pk) {
+          if (PACKED && pk && a.pk.len) {
             is_pk = true;
-            pb = ob;
             pl = a.pk.len[rc];
           }
         }
       }
-      if (PACKED) visitw(is_pk ? pb : 0ull, is_pk ? pl : 0u, is_pk ? ol : 0u);
+      if (PACKED) {
+        visitw(a.pk.ent, is_pk ? ob : 0ull, is_pk ? pl : 0u, is_pk ? ol : 0u);
+        visitw(a.dpk.ent, d_pk ? db : 0ull, d_pk ? dpl : 0u, d_pk ? dlen : 0u);
+      }
       visit(a.old.col, is_pk ? 0ull : ob, is_pk ? 0u : ol);
-      visit(a.dl.col, db, dlen);
+      visit(a.dl.col, d_pk ? 0ull : db, d_pk ? 0u : dlen);
     }
     ns = 0;
   };
@@ -744,11 +761,12 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
     if (++ns == uint32_t(kWave)) run_group();
   };
   for (int i = 0; i < a.g.nu; ++i)
-    if (a.g.u_lhs[i] == A) add(kSegU, 0, uint32_t(a.g.u_rhs[i]) * n + u, false);
+    if (a.g.u_lhs[i] == A)
+      add(kSegU, 0, uint32_t(a.g.u_rhs[i]) * n + u, packed_label(a.pk, uint32_t(a.g.u_rhs[i])));
   for (int i = 0; i < a.g.nb; ++i) {
     if (a.g.b_lhs[i] != A) continue;
     const uint32_t B = a.g.b_rhs1[i], C = a.g.b_rhs2[i];
-    add(kSegD, C, B * n + u, a.pk.len && packed_label(a.pk, C));
+    add(kSegD, C, B * n + u, packed_label(a.pk, C));
     const uint32_t slot = a.g.b_slot[i];
     if (slot != 0xFFu && a.b2.ptr) add(kSegP, C, slot * n + u, false);
   }
@@ -884,19 +902,19 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
 #endif
         expand_lists(col, b, uint32_t(e - b), w, lane, mark);
       },
-      [&](unsigned long long pb, uint32_t pl, uint32_t len) {
+      [&](const unsigned long long* ent, unsigned long long pb, uint32_t pl, uint32_t len) {
         unsigned long long b = pb, e = pb + pl;
         if (ranged && b < e) {
-          b = lower_pos_word(a.pk.ent, b, e, w0);
-          e = lower_pos_word(a.pk.ent, b, e, w0 + words);
-          if (e < pb + pl) nxt = min(nxt, uint32_t(a.pk.ent[e]) << 5);
+          b = lower_pos_word(ent, b, e, w0);
+          e = lower_pos_word(ent, b, e, w0 + words);
+          if (e < pb + pl) nxt = min(nxt, uint32_t(ent[e]) << 5);
         } else {
           kc += len;
         }
 #ifdef GG_EXPAND_STATS
         st_words += e - b;
 #endif
-        expand_words(a.pk.ent, b, uint32_t(e - b), w, lane, mark_word);
+        expand_words(ent, b, uint32_t(e - b), w, lane, mark_word);
       });
 #ifdef GG_EXPAND_STATS
   {
@@ -951,7 +969,7 @@ __device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t spa
       bits &= bits - 1;
       a.staging[p++] = c0 + wi * 32 + uint32_t(b);
     }
-    pos += __shfl(incl, kWave - 1, kWave);
+    pos += lane63_u32(incl);
   }
 }
 
@@ -1031,8 +1049,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
         for_each_list<MODE, false>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
           if (MODE == 1) {
             // raw rows are unsorted: reduce over every element
-            const unsigned long long b = __shfl(lb, 0, kWave);
-            const uint32_t l = __shfl(len, 0, kWave);
+            const unsigned long long b = (unsigned long long)uint32_t(__builtin_amdgcn_readlane(int(uint32_t(lb)), 0)) |
+                                         ((unsigned long long)uint32_t(__builtin_amdgcn_readlane(int(uint32_t(lb >> 32)), 0)) << 32);
+            const uint32_t l = uint32_t(__builtin_amdgcn_readlane(int(len), 0));
             for (uint32_t t = lane; t < l; t += kWave) {
               const uint32_t x = col[b + t];
               mn = min(mn, x);
@@ -1044,7 +1063,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
             mx = max(mx, col[lb + len - 1]);
             kk += len;
           }
-        }, [](unsigned long long, uint32_t, uint32_t) {});
+        }, [](const unsigned long long*, unsigned long long, uint32_t, uint32_t) {});
         if (MODE == 2 && a.ro.kind == 2) {
           bool has_eps = false;
           for (int i = 0; i < a.g.ne; ++i) has_eps |= a.g.e_lhs[i] == A;
@@ -1153,11 +1172,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
   }
 }
 
+__device__ inline uint32_t pack_union(const uint32_t* a, unsigned long long ia, unsigned long long ea,
+                                      const uint32_t* b, unsigned long long ib, unsigned long long eb,
+                                      unsigned long long* ent, uint32_t* win, int lane);
+
 // Copies each active row's staged output into the delta' CSR.
 __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* active, const Counters* ctr, const uint32_t* seg_cnt,
                                                    const unsigned long long* seg_off, const uint32_t* staging,
-                                                   const unsigned long long* out_ptr, uint32_t* out_col) {
+                                                   const unsigned long long* out_ptr, uint32_t* out_col, uint32_t n,
+                                                   DPacked plab, unsigned long long* out_pk, uint32_t* out_pklen) {
+  // rows of packed labels are also packed, at the row's own offset (out_pk
+  // null: no packing)
+  __shared__ uint32_t s_win[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
+  uint32_t* win = s_win[threadIdx.x / kWave];
   const uint32_t nactive = ctr->active;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
@@ -1167,6 +1195,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* active, const
     if (!c) continue;
     const unsigned long long s = seg_off[r], d = out_ptr[r];
     for (uint32_t t = lane; t < c; t += kWave) out_col[d + t] = staging[s + t];
+    if (out_pk && packed_label(plab, r / n)) {
+      const uint32_t pc = pack_union(staging, s, s + c, staging, s, s, out_pk + d, win, lane);
+      if (lane == 0) out_pklen[r] = pc;
+    }
   }
 }
 
@@ -1399,6 +1431,9 @@ class Engine {
   DevBuf ot_ptr, ot_col, ot2_ptr, ot2_col, dt_ptr, dt_col, dt_cnt, pr_ptr, pr_col, pr_cnt;
   // packed old rows (labels in pack_labels)
   DevBuf pk_len, pk_ent, pk2_ent;
+  // packed delta rows (current and next round), entries at the row's delta offset
+  DevBuf dpk_len, dpk_ent, npk_len, npk_ent;
+  bool dpk_ready = false;
   uint32_t pack_labels[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool pk_ready = false;
   // Guessed window (ids) for the next round: starts small; a round where
@@ -1418,7 +1453,7 @@ class Engine {
     DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
                      &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active,
                      &tiles,  &ctr,    &ot_ptr,  &ot_col, &ot2_ptr, &ot2_col, &dt_ptr,  &dt_col, &dt_cnt,
-                     &pr_ptr, &pr_col, &pr_cnt, &pk_len, &pk_ent, &pk2_ent};
+                     &pr_ptr, &pr_col, &pr_cnt, &pk_len, &pk_ent, &pk2_ent, &dpk_len, &dpk_ent, &npk_len, &npk_ent};
     for (DevBuf* b : all)
       if (b->p) (void)hipFree(b->p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -1537,6 +1572,18 @@ class Engine {
     return GG_OK;
   }
 
+  DPacked plabels() const {
+    DPacked p{};
+    for (int i = 0; i < 8; ++i) p.labels[i] = pack_labels[i];
+    return p;
+  }
+  // delta rows of packed labels are packed too (GG_NO_PACK turns both off)
+  bool pack_delta() const {
+    bool any = false;
+    for (int i = 0; i < 8; ++i) any |= pack_labels[i] != 0;
+    return any;
+  }
+
   ExpandArgs expand_args(bool init) {
     ExpandArgs a{};
     a.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
@@ -1546,6 +1593,9 @@ class Engine {
     a.pk.len = (!init && pk_ready) ? P<uint32_t>(pk_len) : nullptr;
     a.pk.ent = P<unsigned long long>(pk_ent);
     for (int i = 0; i < 8; ++i) a.pk.labels[i] = pack_labels[i];
+    a.dpk.len = (!init && dpk_ready) ? P<uint32_t>(dpk_len) : nullptr;
+    a.dpk.ent = P<unsigned long long>(dpk_ent);
+    for (int i = 0; i < 8; ++i) a.dpk.labels[i] = pack_labels[i];
     a.n = n;
     a.g = g;
     a.active = P<uint32_t>(active);
@@ -1666,9 +1716,17 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     GG_HIP(hipEventRecord(c.a, stream));
     GG_TRY(scan(seg_cnt, R, dl_ptr, err));
     GG_TRY(grow(dl_col, std::max<uint64_t>(N0, 1) * 4, err));
+    dpk_ready = false;
+    if (pack_delta()) {
+      GG_TRY(grow(dpk_len, uint64_t(R) * 4, err));
+      GG_TRY(grow(dpk_ent, std::max<uint64_t>(N0, 1) * 8, err));
+      dpk_ready = true;
+    }
     k_gather<<<expand_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
                                                    P<unsigned long long>(seg_off), P<uint32_t>(staging),
-                                                   P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col));
+                                                   P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col), n, plabels(),
+                                                   dpk_ready ? P<unsigned long long>(dpk_ent) : nullptr,
+                                                   P<uint32_t>(dpk_len));
     GG_HIP(hipEventRecord(c.b, stream));
     release(raw_col);
     release(raw_ptr);
@@ -1779,10 +1837,17 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     GG_HIP(hipEventRecord(c.a, stream));
     GG_TRY(scan(seg_cnt, R, nx_ptr, err));
     GG_TRY(grow(nx_col, std::max<uint64_t>(N, 1) * 4, err));
+    const bool npk = pack_delta();
+    if (npk) {
+      GG_TRY(grow(npk_len, uint64_t(R) * 4, err));
+      GG_TRY(grow(npk_ent, std::max<uint64_t>(N, 1) * 8, err));
+    }
     if (N)
       k_gather<<<expand_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
                                                      P<unsigned long long>(seg_off), P<uint32_t>(staging),
-                                                     P<unsigned long long>(nx_ptr), P<uint32_t>(nx_col));
+                                                     P<unsigned long long>(nx_ptr), P<uint32_t>(nx_col), n, plabels(),
+                                                     npk ? P<unsigned long long>(npk_ent) : nullptr,
+                                                     P<uint32_t>(npk_len));
     GG_HIP(hipEventRecord(c.b, stream));
     // old' = old U delta, and its transpose old'^T = old^T ++ delta^T
     EventPair& mg = mk(ev_merge);
@@ -1837,6 +1902,9 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     std::swap(old_col, o2_col);
     std::swap(dl_ptr, nx_ptr);
     std::swap(dl_col, nx_col);
+    std::swap(dpk_len, npk_len);
+    std::swap(dpk_ent, npk_ent);
+    dpk_ready = npk;
     old_nnz = o2n;
     dl_nnz = N;
   }
@@ -1861,6 +1929,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   release(nx_col);
   release(o2_col);
   release(staging);
+  for (DevBuf* b : {&dpk_len, &dpk_ent, &npk_len, &npk_ent}) release(*b);
+  dpk_ready = false;
   has_closure = true;
   if (st) *st = S;
   return GG_OK;
@@ -2167,7 +2237,8 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     if (N)
       k_gather<<<expand_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
                                                      P<unsigned long long>(seg_off), P<uint32_t>(staging),
-                                                     P<unsigned long long>(F.ptr), P<uint32_t>(F.col));
+                                                     P<unsigned long long>(F.ptr), P<uint32_t>(F.col), n, DPacked{},
+                                                     nullptr, nullptr);
     F.nnz = N;
     if (into) {
       std::swap(into->ptr, F.ptr);
@@ -2761,7 +2832,8 @@ int engine_release(Engine* e) {
   DevBuf* all[] = {&e->old_ptr, &e->old_col, &e->dl_ptr,  &e->dl_col,   &e->nx_ptr, &e->nx_col,   &e->o2_ptr,
                    &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt,  &e->active, &e->tiles,
                    &e->ot_ptr,  &e->ot_col,  &e->ot2_ptr, &e->ot2_col,  &e->dt_ptr, &e->dt_col,   &e->dt_cnt,
-                   &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_len, &e->pk_ent, &e->pk2_ent};
+                   &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_len, &e->pk_ent, &e->pk2_ent,
+                   &e->dpk_len, &e->dpk_ent, &e->npk_len, &e->npk_ent};
   for (DevBuf* b : all) e->release(*b);
   e->has_closure = false;
   return GG_OK;
