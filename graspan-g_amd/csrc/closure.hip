@@ -370,12 +370,20 @@ struct ActArgs {
 // the rows they join against in one XCD's L2 when the expand kernel splits
 // the list by XCD).
 __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
+  // A block takes 256 consecutive vertices. Each wave records one ballot per
+  // lhs label; the block then reserves its rows with a single atomic (one per
+  // block, not one per wave and label, which serialised on the counter) and
+  // every wave writes its rows label by label.
+  __shared__ unsigned long long s_m[kWavesPerBlock][256];
+  __shared__ uint32_t s_cnt[kWavesPerBlock];
+  __shared__ uint32_t s_base;
   const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
   unsigned long long pk = 0;
-  for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < a.n;
-       base += gridDim.x * blockDim.x) {
-    const uint32_t u = base + lane;
+  for (uint32_t bbase = blockIdx.x * blockDim.x; bbase < a.n; bbase += gridDim.x * blockDim.x) {
+    const uint32_t u = bbase + uint32_t(wv) * kWave + uint32_t(lane);
     const bool valid = u < a.n;
+    uint32_t wcnt = 0;
     for (uint32_t li = 0; li < a.nlhs; ++li) {
       const uint32_t A = a.lhs[li];
       bool any = false;
@@ -383,9 +391,8 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
         if (a.g.u_lhs[i] == A) any |= label_nonempty(a.ctr, a.g.u_rhs[i]);
       for (int i = 0; i < a.g.nb; ++i)
         if (a.g.b_lhs[i] == A) any |= label_nonempty(a.ctr, a.g.b_rhs1[i]) || label_nonempty(a.ctr, a.g.b_rhs2[i]);
-      if (!any) continue;
       bool act = false;
-      if (valid) {
+      if (any && valid) {
         for (int i = 0; i < a.g.nu; ++i) {
           if (a.g.u_lhs[i] != A) continue;
           const uint32_t rb = uint32_t(a.g.u_rhs[i]) * a.n + u;
@@ -406,13 +413,25 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
         }
       }
       const unsigned long long m = __ballot(act);
-      if (m) {
-        uint32_t pos = 0;
-        if (lane == 0) pos = atomicAdd(&a.ctr->active, uint32_t(__popcll(m)));
-        pos = uni_u32(pos);  // lane 0: the branch is uniform
-        if (act) a.active[pos + __popcll(m & ((1ull << lane) - 1))] = A * a.n + u;
-      }
+      if (lane == 0) s_m[wv][li] = m;
+      wcnt += uint32_t(__popcll(m));
     }
+    if (lane == 0) s_cnt[wv] = wcnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+      for (int w = 0; w < kWavesPerBlock; ++w) tot += s_cnt[w];
+      s_base = tot ? atomicAdd(&a.ctr->active, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos = s_base;
+    for (int w = 0; w < wv; ++w) pos += s_cnt[w];
+    for (uint32_t li = 0; li < a.nlhs; ++li) {
+      const unsigned long long m = s_m[wv][li];
+      if ((m >> lane) & 1ull) a.active[pos + __popcll(m & ((1ull << lane) - 1))] = uint32_t(a.lhs[li]) * a.n + u;
+      pos += uint32_t(__popcll(m));
+    }
+    __syncthreads();  // s_m, s_cnt and s_base are rewritten by the next batch
   }
   pk = wave_sum_u64(pk);
   if (lane == 0 && pk) atomicAdd(&a.ctr->pkeys, pk);
@@ -421,33 +440,58 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
 // ---------------------------------------------------------------------------
 // frontier pair lists for old_B x delta_C and the transpose of old_B
 
+// Calls f(k, pos) for every element position pos of the lanes' lists
+// [lb, lb+len) (k = the lane that owns the list): all lists of the wave are
+// flattened into one stream, 4 positions per lane in flight, so the wave does
+// not walk its lists one after another. pref/base: 64-entry LDS arrays.
+template <class F>
+__device__ inline void flat_positions(unsigned long long lb, uint32_t len, unsigned long long* pref,
+                                      unsigned long long* base, int lane, F&& f) {
+  const unsigned long long incl = wave_incl_scan_u64(len);
+  const unsigned long long total = lane63_u64(incl);
+  if (total == 0) return;
+  wave_sync();
+  pref[lane] = incl;
+  base[lane] = lb;
+  wave_sync();
+  for (unsigned long long t = lane; t < total; t += kWave) {
+    int k = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+      if (pref[k + step - 1] <= t) k += step;
+    f(k, base[k] + (t - (k ? pref[k - 1] : 0ull)));
+  }
+  wave_sync();
+}
+
 // For every slot j and frontier vertex v (delta_C(v) non-empty), every u in
 // old_B^T(v) gets key v in pair list (j, u). COUNT pass: cnt[j*n+u]++; fill
-// pass: col[ptr[j*n+u] + fill[j*n+u]++] = v.
+// pass: col[ptr[j*n+u] + fill[j*n+u]++] = v. A wave takes 64 vertices and
+// streams the transposed rows of its frontier ones together.
 template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr dl, DCsr ot, uint32_t* cnt,
                                                   const unsigned long long* ptr, uint32_t* col) {
+  __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_base[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (int j = 0; j < g.nslot; ++j) {
     const uint32_t cb = uint32_t(g.slot_c[j]) * n, tb = uint32_t(g.slot_t[j]) * n;
     for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < n; base += stride) {
-      // each lane tests one vertex; the wave then walks the frontier ones
       const uint32_t v0 = base + lane;
-      const bool fr = v0 < n && dl.ptr[cb + v0 + 1] > dl.ptr[cb + v0];
-      unsigned long long m = __ballot(fr);
-      while (m) {
-        const uint32_t v = base + uint32_t(__builtin_ctzll(m));
-        m &= m - 1;
-        const unsigned long long b = ot.ptr[tb + v], e = ot.ptr[tb + v + 1];
-        for (unsigned long long k = b + lane; k < e; k += kWave) {
-          const uint32_t idx = uint32_t(j) * n + ot.col[k];
-          if (FILL)
-            col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = v;
-          else
-            atomicAdd(&cnt[idx], 1u);
-        }
+      unsigned long long b = 0;
+      uint32_t len = 0;
+      if (v0 < n && dl.ptr[cb + v0 + 1] > dl.ptr[cb + v0]) {
+        b = ot.ptr[tb + v0];
+        len = uint32_t(ot.ptr[tb + v0 + 1] - b);
       }
+      flat_positions(b, len, s_pref[wv], s_base[wv], lane, [&](int k, unsigned long long pos) {
+        const uint32_t idx = uint32_t(j) * n + ot.col[pos];
+        if (FILL)
+          col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = base + uint32_t(k);
+        else
+          atomicAdd(&cnt[idx], 1u);
+      });
     }
   }
 }
@@ -457,26 +501,27 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
 template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_dtrans(const DevGrammar g, uint32_t n, DCsr dl, uint32_t* cnt,
                                                    const unsigned long long* ptr, uint32_t* col) {
+  __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_base[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (int t = 0; t < g.nt; ++t) {
     const uint32_t lb = uint32_t(g.t_label[t]) * n;
     for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < n; base += stride) {
       const uint32_t u0 = base + lane;
-      const bool ne = u0 < n && dl.ptr[lb + u0 + 1] > dl.ptr[lb + u0];
-      unsigned long long m = __ballot(ne);
-      while (m) {
-        const uint32_t u = base + uint32_t(__builtin_ctzll(m));
-        m &= m - 1;
-        const unsigned long long b = dl.ptr[lb + u], e = dl.ptr[lb + u + 1];
-        for (unsigned long long k = b + lane; k < e; k += kWave) {
-          const uint32_t idx = uint32_t(t) * n + dl.col[k];
-          if (FILL)
-            col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = u;
-          else
-            atomicAdd(&cnt[idx], 1u);
-        }
+      unsigned long long b = 0;
+      uint32_t len = 0;
+      if (u0 < n) {
+        b = dl.ptr[lb + u0];
+        len = uint32_t(dl.ptr[lb + u0 + 1] - b);
       }
+      flat_positions(b, len, s_pref[wv], s_base[wv], lane, [&](int k, unsigned long long pos) {
+        const uint32_t idx = uint32_t(t) * n + dl.col[pos];
+        if (FILL)
+          col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = base + uint32_t(k);
+        else
+          atomicAdd(&cnt[idx], 1u);
+      });
     }
   }
 }
@@ -1255,48 +1300,111 @@ __device__ inline uint32_t pack_union(const uint32_t* a, unsigned long long ia, 
   return written;
 }
 
-// old' = old U delta, row by row, for the labels whose delta is not empty
+// Copies the lanes' segments src[sb .. sb+len) to dst[db ..] (one segment
+// per lane, all flattened into one coalesced element stream, as in
+// expand_lists). pref/sbase/dbase: 64-entry LDS arrays of the wave.
+template <class T>
+__device__ inline void flat_copy(const T* __restrict__ src, unsigned long long sb, T* __restrict__ dst,
+                                 unsigned long long db, uint32_t len, unsigned long long* pref,
+                                 unsigned long long* sbase, unsigned long long* dbase, int lane) {
+  const unsigned long long incl = wave_incl_scan_u64(len);
+  const unsigned long long total = lane63_u64(incl);
+  if (total == 0) return;
+  wave_sync();
+  pref[lane] = incl;
+  sbase[lane] = sb;
+  dbase[lane] = db;
+  wave_sync();
+  constexpr int kU = 4;
+  for (unsigned long long t0 = 0; t0 < total; t0 += kWave * kU) {
+    T x[kU];
+    unsigned long long d[kU];
+#pragma unroll
+    for (int q = 0; q < kU; ++q) {
+      const unsigned long long t = t0 + q * kWave + lane;
+      d[q] = ~0ull;
+      if (t < total) {
+        int k = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+          if (pref[k + step - 1] <= t) k += step;
+        const unsigned long long off = t - (k ? pref[k - 1] : 0ull);
+        x[q] = src[sbase[k] + off];
+        d[q] = dbase[k] + off;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kU; ++q)
+      if (d[q] != ~0ull) dst[d[q]] = x[q];
+  }
+  wave_sync();
+}
+
+// old' = old U delta for the labels whose delta is not empty
 // (rowptr(old') = rowptr(old) + rowptr(delta)); the other labels are moved in
-// bulk by k_copy_labels. Rows of packed labels are packed after the merge
-// into out_pk at the row's new offset, entry count in pk_len[r].
+// bulk by k_copy_labels. A wave takes 64 consecutive rows. Rows without delta
+// only shift (by their delta rowptr): their ids and packed entries move in
+// one flattened copy. Rows with delta are merged one at a time (rank of each
+// id in the other list) and, for packed labels, packed into out_pk at the
+// row's new offset, entry count in pk_len[r].
 __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t rows, uint32_t n, const Counters* ctr,
                                                   DPacked plab, const unsigned long long* old_pk, uint32_t* out_col,
                                                   unsigned long long* out_pk, uint32_t* pk_len) {
   // ctr == nullptr: merge every row (no label is skipped)
   __shared__ uint32_t s_win[kWavesPerBlock][kWave];
+  __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_sb[kWavesPerBlock][kWave],
+      s_db[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
-  uint32_t* win = s_win[threadIdx.x / kWave];
+  const int wi = threadIdx.x / kWave;
+  uint32_t* win = s_win[wi];
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  for (uint32_t r = wave; r < rows; r += nwaves) {
+  const uint32_t nblk = (rows + kWave - 1) / kWave;
+  for (uint32_t blk = wave; blk < nblk; blk += nwaves) {
+    const uint32_t r = blk * kWave + lane;
     const uint32_t l = r / n;
-    if (ctr && !label_nonempty(ctr, l)) continue;
-    const unsigned long long ob = old.ptr[r], oe = old.ptr[r + 1];
-    const unsigned long long db = dl.ptr[r], de = dl.ptr[r + 1];
-    uint32_t* out = out_col + ob + db;
-    const uint32_t ol = uint32_t(oe - ob), dln = uint32_t(de - db);
-    if (dln == 0) {
-      for (uint32_t t = lane; t < ol; t += kWave) out[t] = old.col[ob + t];
-    } else if (ol == 0) {
-      for (uint32_t t = lane; t < dln; t += kWave) out[t] = dl.col[db + t];
-    } else {
-      for (uint32_t t = lane; t < ol; t += kWave) {
-        const uint32_t x = old.col[ob + t];
-        out[t + lower_rank(dl.col, db, de, x)] = x;
-      }
-      for (uint32_t t = lane; t < dln; t += kWave) {
-        const uint32_t x = dl.col[db + t];
-        out[t + lower_rank(old.col, ob, oe, x)] = x;
-      }
+    const bool live = r < rows && (!ctr || label_nonempty(ctr, l));
+    unsigned long long ob = 0, oe = 0, db = 0, de = 0;
+    if (live) {
+      ob = old.ptr[r];
+      oe = old.ptr[r + 1];
+      db = dl.ptr[r];
+      de = dl.ptr[r + 1];
     }
-    if (out_pk && packed_label(plab, l)) {
-      if (dln == 0 && old_pk) {
-        // unchanged row: its packed entries move with it
-        const uint32_t c = pk_len[r];
-        for (uint32_t t = lane; t < c; t += kWave) out_pk[ob + db + t] = old_pk[ob + t];
+    const bool same = live && de == db;  // no delta: the row only shifts
+    flat_copy(old.col, ob, out_col, ob + db, same ? uint32_t(oe - ob) : 0u, s_pref[wi], s_sb[wi], s_db[wi], lane);
+    if (out_pk && old_pk) {
+      const bool pkl = same && oe > ob && packed_label(plab, l);
+      flat_copy(old_pk, ob, out_pk, ob + db, pkl ? pk_len[r] : 0u, s_pref[wi], s_sb[wi], s_db[wi], lane);
+    }
+    // rows with delta (and, before old is packed, the packed rows to build)
+    unsigned long long m = __ballot(live && (!same || (out_pk && !old_pk && oe > ob && packed_label(plab, l))));
+    while (m) {
+      const int k = __ffsll(m) - 1;
+      m &= m - 1;
+      const uint32_t rr = blk * kWave + uint32_t(k);
+      const uint32_t lr = rr / n;
+      const unsigned long long rob = old.ptr[rr], roe = old.ptr[rr + 1];
+      const unsigned long long rdb = dl.ptr[rr], rde = dl.ptr[rr + 1];
+      uint32_t* out = out_col + rob + rdb;
+      const uint32_t ol = uint32_t(roe - rob), dln = uint32_t(rde - rdb);
+      if (dln == 0) {
+        for (uint32_t t = lane; t < ol; t += kWave) out[t] = old.col[rob + t];
+      } else if (ol == 0) {
+        for (uint32_t t = lane; t < dln; t += kWave) out[t] = dl.col[rdb + t];
       } else {
-        const uint32_t c = pack_union(old.col, ob, oe, dl.col, db, de, out_pk + ob + db, win, lane);
-        if (lane == 0) pk_len[r] = c;
+        for (uint32_t t = lane; t < ol; t += kWave) {
+          const uint32_t x = old.col[rob + t];
+          out[t + lower_rank(dl.col, rdb, rde, x)] = x;
+        }
+        for (uint32_t t = lane; t < dln; t += kWave) {
+          const uint32_t x = dl.col[rdb + t];
+          out[t + lower_rank(old.col, rob, roe, x)] = x;
+        }
+      }
+      if (out_pk && packed_label(plab, lr)) {
+        const uint32_t c = pack_union(old.col, rob, roe, dl.col, rdb, rde, out_pk + rob + rdb, win, lane);
+        if (lane == 0) pk_len[rr] = c;
       }
     }
   }
@@ -1691,7 +1799,9 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     fill_u64(dl_ptr, 0, uint64_t(R) + 1);
     // GG_STAGING_MIN_WORDS sets a smaller first staging area (tests use it to
     // exercise the grow-and-retry path).
-    uint64_t stage_words = std::max<uint64_t>(raw_n, 1 << 20);
+    // (room for the raw rows plus every wave's reservation block, so the
+    // first expand does not overflow)
+    uint64_t stage_words = raw_n + uint64_t(expand_grid()) * kWavesPerBlock * kStageBlock * 2;
     if (const char* sw = std::getenv("GG_STAGING_MIN_WORDS")) stage_words = std::max<uint64_t>(1, std::strtoull(sw, nullptr, 10));
     release(staging);
     GG_TRY(alloc(staging, stage_words * 4, err));
