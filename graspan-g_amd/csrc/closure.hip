@@ -62,6 +62,7 @@ constexpr int kExpandWaves = 6;         // k_expand occupancy target (waves per 
 constexpr uint32_t kMaxCrossParts = 1024;  // partitions of the out-of-core schedule
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
 constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
+constexpr uint32_t kMergeBits = 8192;    // k_merge: rows spanning fewer ids are merged in an LDS bitmap
 constexpr int kMaxRules = 64;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -1352,11 +1353,13 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
                                                   unsigned long long* out_pk, uint32_t* pk_len) {
   // ctr == nullptr: merge every row (no label is skipped)
   __shared__ uint32_t s_win[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_bits[kWavesPerBlock][kMergeBits / 32];
   __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_sb[kWavesPerBlock][kWave],
       s_db[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wi = threadIdx.x / kWave;
   uint32_t* win = s_win[wi];
+  uint32_t* bits = s_bits[wi];
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
   const uint32_t nblk = (rows + kWave - 1) / kWave;
@@ -1388,6 +1391,62 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
       const unsigned long long rdb = dl.ptr[rr], rde = dl.ptr[rr + 1];
       uint32_t* out = out_col + rob + rdb;
       const uint32_t ol = uint32_t(roe - rob), dln = uint32_t(rde - rdb);
+      const bool pkr = out_pk && packed_label(plab, lr);
+      // old and delta are disjoint and sorted: when their ids fit one LDS
+      // window, OR both into a bitmap and write it out in order; its non-zero
+      // words are the row's packed entries
+      {
+        uint32_t lo = 0xFFFFFFFFu, hi = 0;
+        if (ol) {
+          lo = old.col[rob];
+          hi = old.col[roe - 1];
+        }
+        if (dln) {
+          lo = min(lo, dl.col[rdb]);
+          hi = max(hi, dl.col[rde - 1]);
+        }
+        lo &= ~31u;
+        if (hi - lo < kMergeBits) {
+          const uint32_t words = ((hi - lo) >> 5) + 1;
+          for (uint32_t i = lane; i < words; i += kWave) bits[i] = 0;
+          wave_sync();
+          for (uint32_t t = lane; t < ol; t += kWave) {
+            const uint32_t off = old.col[rob + t] - lo;
+            __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+          for (uint32_t t = lane; t < dln; t += kWave) {
+            const uint32_t off = dl.col[rdb + t] - lo;
+            __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+          wave_sync();
+          const unsigned long long below = (1ull << lane) - 1;
+          unsigned long long o = 0;
+          uint32_t pe = 0;
+          for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
+            const uint32_t wd = w0 + lane < words ? bits[w0 + lane] : 0u;
+            const uint32_t pc = __popc(wd);
+            const uint32_t incl = wave_incl_scan(pc, lane);
+            uint32_t x = wd;
+            unsigned long long q = o + (incl - pc);
+            while (x) {
+              const int b = __ffs(x) - 1;
+              x &= x - 1;
+              out[q++] = lo + (w0 + uint32_t(lane)) * 32 + uint32_t(b);
+            }
+            o += lane63_u32(incl);
+            if (pkr) {
+              const unsigned long long nz = __ballot(wd != 0);
+              if (wd)
+                out_pk[rob + rdb + pe + __popcll(nz & below)] =
+                    ((unsigned long long)wd << 32) | ((lo >> 5) + w0 + uint32_t(lane));
+              pe += uint32_t(__popcll(nz));
+            }
+          }
+          if (pkr && lane == 0) pk_len[rr] = pe;
+          wave_sync();
+          continue;
+        }
+      }
       if (dln == 0) {
         for (uint32_t t = lane; t < ol; t += kWave) out[t] = old.col[rob + t];
       } else if (ol == 0) {
@@ -1402,7 +1461,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
           out[t + lower_rank(old.col, rob, roe, x)] = x;
         }
       }
-      if (out_pk && packed_label(plab, lr)) {
+      if (pkr) {
         const uint32_t c = pack_union(old.col, rob, roe, dl.col, rdb, rde, out_pk + rob + rdb, win, lane);
         if (lane == 0) pk_len[rr] = c;
       }
