@@ -72,15 +72,13 @@ struct DevGrammar {
   uint8_t u_lhs[kMaxRules], u_rhs[kMaxRules];
   uint8_t b_lhs[kMaxRules], b_rhs1[kMaxRules], b_rhs2[kMaxRules];
   uint8_t e_lhs[kMaxRules];
-  // old_B x delta_C is driven from the delta_C frontier through the transpose
-  // of old_B. b_slot[i] = pair-list slot of binary rule i (0xFF when C is
-  // never derived, so delta_C is empty after the first round);
-  // slot_c/slot_t = C label and transposed-label index of each slot;
-  // t_label = label of each transposed index.
-  int nslot = 0, nt = 0;
+  // old_B x delta_C goes through per-round pair lists (the keys v of old_B(u)
+  // whose delta_C(v) is not empty). b_slot[i] = pair-list slot of binary rule
+  // i (0xFF when C is never derived, so delta_C is empty after the first
+  // round); slot_c/slot_b = C and B label of each slot.
+  int nslot = 0;
   uint8_t b_slot[kMaxRules];
-  uint8_t slot_c[kMaxRules], slot_t[kMaxRules];
-  uint8_t t_label[kMaxRules];
+  uint8_t slot_c[kMaxRules], slot_b[kMaxRules];
 };
 
 // Round counters, one device copy per context (zeroed by a fill kernel).
@@ -465,79 +463,62 @@ __device__ inline void flat_positions(unsigned long long lb, uint32_t len, unsig
   wave_sync();
 }
 
-// For every slot j and frontier vertex v (delta_C(v) non-empty), every u in
-// old_B^T(v) gets key v in pair list (j, u). COUNT pass: cnt[j*n+u]++; fill
-// pass: col[ptr[j*n+u] + fill[j*n+u]++] = v. A wave takes 64 vertices and
-// streams the transposed rows of its frontier ones together.
+// Frontier bitmaps: bit v of slot j is set when delta_C(v) is not empty
+// (C = slot_c[j]); words of 64 vertices, `words` per slot.
+__global__ __launch_bounds__(kBlock) void k_frontier(const DevGrammar g, uint32_t n, DCsr dl, uint32_t words,
+                                                     unsigned long long* bits) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (int j = 0; j < g.nslot; ++j) {
+    const uint32_t cb = uint32_t(g.slot_c[j]) * n;
+    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < n; base += stride) {
+      const uint32_t v = base + lane;
+      const unsigned long long m = __ballot(v < n && dl.ptr[cb + v + 1] > dl.ptr[cb + v]);
+      if (lane == 0) bits[uint64_t(j) * words + base / kWave] = m;
+    }
+  }
+}
+
+// Pair lists for old_B x delta_C: row (j, u) lists the v in old_B(u) whose
+// delta_C(v) is not empty (B = slot_b[j], C = slot_c[j]). A wave takes 64
+// rows and streams their old_B rows together, testing each v against the
+// slot's frontier bitmap; per-row counts live in LDS. COUNT pass: cnt[j*n+u]
+// = |pairs|; fill pass: the pairs, at ptr[j*n+u], in no particular order.
 template <bool FILL>
-__global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr dl, DCsr ot, uint32_t* cnt,
+__global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr old,
+                                                  const unsigned long long* bits, uint32_t words, uint32_t* cnt,
                                                   const unsigned long long* ptr, uint32_t* col) {
-  __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_base[kWavesPerBlock][kWave];
+  __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_base[kWavesPerBlock][kWave],
+      s_out[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_cnt[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (int j = 0; j < g.nslot; ++j) {
-    const uint32_t cb = uint32_t(g.slot_c[j]) * n, tb = uint32_t(g.slot_t[j]) * n;
+    const uint32_t bb = uint32_t(g.slot_b[j]) * n;
+    const unsigned long long* fb = bits + uint64_t(j) * words;
     for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < n; base += stride) {
-      const uint32_t v0 = base + lane;
+      const uint32_t u = base + lane;
       unsigned long long b = 0;
       uint32_t len = 0;
-      if (v0 < n && dl.ptr[cb + v0 + 1] > dl.ptr[cb + v0]) {
-        b = ot.ptr[tb + v0];
-        len = uint32_t(ot.ptr[tb + v0 + 1] - b);
+      if (u < n) {
+        b = old.ptr[bb + u];
+        len = uint32_t(old.ptr[bb + u + 1] - b);
+        if (FILL) s_out[wv][lane] = ptr[uint64_t(j) * n + u];
       }
+      s_cnt[wv][lane] = 0;
+      wave_sync();
       flat_positions(b, len, s_pref[wv], s_base[wv], lane, [&](int k, unsigned long long pos) {
-        const uint32_t idx = uint32_t(j) * n + ot.col[pos];
-        if (FILL)
-          col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = base + uint32_t(k);
-        else
-          atomicAdd(&cnt[idx], 1u);
+        const uint32_t v = old.col[pos];
+        if ((fb[v / kWave] >> (v % kWave)) & 1ull) {
+          const uint32_t at = atomicAdd(&s_cnt[wv][k], 1u);
+          if (FILL) col[s_out[wv][k] + at] = v;
+        }
       });
+      wave_sync();
+      if (!FILL && u < n) cnt[uint64_t(j) * n + u] = s_cnt[wv][lane];
+      wave_sync();
     }
-  }
-}
-
-// Transpose of delta restricted to the transposed labels: row t*n + v lists
-// every u with delta_{t_label[t]}(u, v).
-template <bool FILL>
-__global__ __launch_bounds__(kBlock) void k_dtrans(const DevGrammar g, uint32_t n, DCsr dl, uint32_t* cnt,
-                                                   const unsigned long long* ptr, uint32_t* col) {
-  __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_base[kWavesPerBlock][kWave];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wv = threadIdx.x / kWave;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (int t = 0; t < g.nt; ++t) {
-    const uint32_t lb = uint32_t(g.t_label[t]) * n;
-    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < n; base += stride) {
-      const uint32_t u0 = base + lane;
-      unsigned long long b = 0;
-      uint32_t len = 0;
-      if (u0 < n) {
-        b = dl.ptr[lb + u0];
-        len = uint32_t(dl.ptr[lb + u0 + 1] - b);
-      }
-      flat_positions(b, len, s_pref[wv], s_base[wv], lane, [&](int k, unsigned long long pos) {
-        const uint32_t idx = uint32_t(t) * n + dl.col[pos];
-        if (FILL)
-          col[ptr[idx] + atomicAdd(&cnt[idx], 1u)] = base + uint32_t(k);
-        else
-          atomicAdd(&cnt[idx], 1u);
-      });
-    }
-  }
-}
-
-// out row r = a row r followed by b row r; rowptr(out) = rowptr(a) + rowptr(b).
-__global__ __launch_bounds__(kBlock) void k_concat(DCsr a, DCsr b, uint32_t rows, uint32_t* out_col) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  for (uint32_t r = wave; r < rows; r += nwaves) {
-    const unsigned long long ab = a.ptr[r], ae = a.ptr[r + 1], bb = b.ptr[r], be = b.ptr[r + 1];
-    uint32_t* out = out_col + ab + bb;
-    const uint32_t al = uint32_t(ae - ab), bl = uint32_t(be - bb);
-    for (uint32_t t = lane; t < al; t += kWave) out[t] = a.col[ab + t];
-    for (uint32_t t = lane; t < bl; t += kWave) out[al + t] = b.col[bb + t];
   }
 }
 
@@ -1593,9 +1574,8 @@ class Engine {
   // closure state
   DevBuf old_ptr, old_col, dl_ptr, dl_col, nx_ptr, nx_col, o2_ptr, o2_col;
   DevBuf raw_ptr, raw_col, staging, seg_off, seg_cnt, active, tiles, ctr;
-  // transpose of old over the transposed labels (+ ping-pong), transpose of
-  // delta, frontier pair lists
-  DevBuf ot_ptr, ot_col, ot2_ptr, ot2_col, dt_ptr, dt_col, dt_cnt, pr_ptr, pr_col, pr_cnt;
+  // frontier bitmaps and pair lists for old_B x delta_C
+  DevBuf fr_bits, pr_ptr, pr_col, pr_cnt;
   // packed old rows (labels in pack_labels)
   DevBuf pk_len, pk_ent, pk2_ent;
   // packed delta rows (current and next round), entries at the row's delta offset
@@ -1611,7 +1591,7 @@ class Engine {
   // host copy of the input (the partitioned schedule builds its regions from it)
   std::vector<uint32_t> h_src, h_dst;
   std::vector<uint8_t> h_lab;
-  uint64_t old_nnz = 0, dl_nnz = 0, ot_nnz = 0;
+  uint64_t old_nnz = 0, dl_nnz = 0;
   bool pairs_ready = false;
   bool has_closure = false;
   Counters* h_ctr = nullptr;  // pinned
@@ -1619,7 +1599,7 @@ class Engine {
   ~Engine() {
     DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
                      &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active,
-                     &tiles,  &ctr,    &ot_ptr,  &ot_col, &ot2_ptr, &ot2_col, &dt_ptr,  &dt_col, &dt_cnt,
+                     &tiles,  &ctr,    &fr_bits,
                      &pr_ptr, &pr_col, &pr_cnt, &pk_len, &pk_ent, &pk2_ent, &dpk_len, &dpk_ent, &npk_len, &npk_ent};
     for (DevBuf* b : all)
       if (b->p) (void)hipFree(b->p);
@@ -1913,19 +1893,13 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   act.active = P<uint32_t>(active);
   act.ctr = C();
   const uint64_t act_work = uint64_t(act.nlhs) * n;
-  const uint64_t NT = uint64_t(g.nt) * n, NS = uint64_t(g.nslot) * n;
-  if (NT) {
-    GG_TRY(alloc(ot_ptr, (NT + 1) * 8, err));
-    GG_TRY(alloc(ot2_ptr, (NT + 1) * 8, err));
-    GG_TRY(alloc(dt_ptr, (NT + 1) * 8, err));
-    GG_TRY(alloc(dt_cnt, NT * 4, err));
-    fill_u64(ot_ptr, 0, NT + 1);
-  }
+  const uint64_t NS = uint64_t(g.nslot) * n;
+  const uint32_t fr_words = (n + kWave - 1) / kWave;
   if (NS) {
     GG_TRY(alloc(pr_ptr, (NS + 1) * 8, err));
     GG_TRY(alloc(pr_cnt, NS * 4, err));
+    GG_TRY(alloc(fr_bits, uint64_t(g.nslot) * fr_words * 8, err));
   }
-  ot_nnz = 0;
   pk_ready = false;
   guess_bits = kGuessMin;
   bool pack_any = false;
@@ -1939,20 +1913,20 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     uint64_t N = 0;
     // frontier pair lists for old_B x delta_C (old is empty in the first round)
     pairs_ready = false;
-    if (NS && ot_nnz > 0) {
+    if (NS && old_nnz > 0) {
       EventPair& o = mk(ev_other);
       GG_HIP(hipEventRecord(o.a, stream));
       const DCsr dlc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
-      const DCsr otc{P<unsigned long long>(ot_ptr), P<uint32_t>(ot_col)};
-      fill_u32(pr_cnt, 0, NS);
-      k_pairs<false><<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, otc, P<uint32_t>(pr_cnt), nullptr, nullptr);
+      const DCsr oc{P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
+      k_frontier<<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, fr_words, P<unsigned long long>(fr_bits));
+      k_pairs<false><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, P<unsigned long long>(fr_bits), fr_words,
+                                                         P<uint32_t>(pr_cnt), nullptr, nullptr);
       GG_TRY(scan(pr_cnt, NS, pr_ptr, err));
       uint64_t npairs = 0;
       GG_TRY(read_u64(P<unsigned long long>(pr_ptr) + NS, npairs, err));
       GG_TRY(grow(pr_col, std::max<uint64_t>(npairs, 1) * 4, err));
-      fill_u32(pr_cnt, 0, NS);
-      k_pairs<true><<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, otc, P<uint32_t>(pr_cnt),
-                                                                P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
+      k_pairs<true><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, P<unsigned long long>(fr_bits), fr_words, nullptr,
+                                                        P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
       GG_HIP(hipEventRecord(o.b, stream));
       pairs_ready = true;
     }
@@ -2041,27 +2015,6 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       if (old_nnz)
         k_copy_labels<<<dim3(uint32_t(cus), uint32_t(L)), kBlock, 0, stream>>>(
             oc, dc, n, C(), plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr, P<uint32_t>(o2_col), opk);
-    }
-    if (NT && N) {
-      const DCsr dlc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
-      fill_u32(dt_cnt, 0, NT);
-      k_dtrans<false><<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, P<uint32_t>(dt_cnt), nullptr, nullptr);
-      GG_TRY(scan(dt_cnt, NT, dt_ptr, err));
-      uint64_t ndt = 0;
-      GG_TRY(read_u64(P<unsigned long long>(dt_ptr) + NT, ndt, err));
-      GG_TRY(grow(dt_col, std::max<uint64_t>(ndt, 1) * 4, err));
-      fill_u32(dt_cnt, 0, NT);
-      k_dtrans<true><<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, P<uint32_t>(dt_cnt),
-                                                                 P<unsigned long long>(dt_ptr), P<uint32_t>(dt_col));
-      GG_TRY(grow(ot2_col, std::max<uint64_t>(ot_nnz + ndt, 1) * 4, err));
-      k_add_ptr<<<grid_for(NT + 1), kBlock, 0, stream>>>(P<unsigned long long>(ot2_ptr), P<unsigned long long>(ot_ptr),
-                                                        P<unsigned long long>(dt_ptr), NT + 1);
-      k_concat<<<uint32_t(cus) * 8, kBlock, 0, stream>>>({P<unsigned long long>(ot_ptr), P<uint32_t>(ot_col)},
-                                                         {P<unsigned long long>(dt_ptr), P<uint32_t>(dt_col)},
-                                                         uint32_t(NT), P<uint32_t>(ot2_col));
-      std::swap(ot_ptr, ot2_ptr);
-      std::swap(ot_col, ot2_col);
-      ot_nnz += ndt;
     }
     pk_ready = pack_any;
     if (pack_any) std::swap(pk_ent, pk2_ent);
@@ -2733,17 +2686,12 @@ int engine_set_grammar(Engine* e, const Grammar& gr, std::string& err) {
     }
   }
   // pair-list slots: binary rules whose C is derived by a unary/binary rule
-  std::vector<int> tslot(size_t(gr.num_labels()) + 1, -1);
   for (int i = 0; i < g.nb; ++i) {
     g.b_slot[i] = 0xFF;
     const int C = g.b_rhs2[i] + 1, B = g.b_rhs1[i] + 1;
     if (!is_lhs[C]) continue;
-    if (tslot[B] < 0) {
-      tslot[B] = g.nt;
-      g.t_label[g.nt++] = uint8_t(B - 1);
-    }
     g.slot_c[g.nslot] = uint8_t(C - 1);
-    g.slot_t[g.nslot] = uint8_t(tslot[B]);
+    g.slot_b[g.nslot] = uint8_t(B - 1);
     g.b_slot[i] = uint8_t(g.nslot++);
   }
   for (int i = 0; i < 8; ++i) e->pack_labels[i] = 0;
@@ -3000,7 +2948,7 @@ int engine_checksum(Engine* e, uint64_t* sum, std::string& err) {
 int engine_release(Engine* e) {
   DevBuf* all[] = {&e->old_ptr, &e->old_col, &e->dl_ptr,  &e->dl_col,   &e->nx_ptr, &e->nx_col,   &e->o2_ptr,
                    &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt,  &e->active, &e->tiles,
-                   &e->ot_ptr,  &e->ot_col,  &e->ot2_ptr, &e->ot2_col,  &e->dt_ptr, &e->dt_col,   &e->dt_cnt,
+                   &e->fr_bits,
                    &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_len, &e->pk_ent, &e->pk2_ent,
                    &e->dpk_len, &e->dpk_ent, &e->npk_len, &e->npk_ent};
   for (DevBuf* b : all) e->release(*b);
