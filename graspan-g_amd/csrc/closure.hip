@@ -463,6 +463,18 @@ __device__ inline void flat_positions(unsigned long long lb, uint32_t len, unsig
   wave_sync();
 }
 
+// Packed form of a sorted row: one entry per non-empty 32-id word,
+// low 32 bits = word index (id >> 5), high 32 bits = the word's bit mask.
+// The packed entries of row r start at the row's own CSR offset (the entry
+// count never exceeds the id count) and len[r] says how many there are.
+struct DPacked {
+  const uint32_t* len;            // per row; null when absent
+  const unsigned long long* ent;
+  uint32_t labels[8];             // bit l: rows of label l are packed
+};
+
+__device__ inline bool packed_label(const DPacked& p, uint32_t l) { return (p.labels[l >> 5] >> (l & 31)) & 1u; }
+
 // Frontier bitmaps: bit v of slot j is set when delta_C(v) is not empty
 // (C = slot_c[j]); words of 64 vertices, `words` per slot.
 __global__ __launch_bounds__(kBlock) void k_frontier(const DevGrammar g, uint32_t n, DCsr dl, uint32_t words,
@@ -485,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier(const DevGrammar g, uint32_
 // slot's frontier bitmap; per-row counts live in LDS. COUNT pass: cnt[j*n+u]
 // = |pairs|; fill pass: the pairs, at ptr[j*n+u], in no particular order.
 template <bool FILL>
-__global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr old,
+__global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr old, DPacked pk,
                                                   const unsigned long long* bits, uint32_t words, uint32_t* cnt,
                                                   const unsigned long long* ptr, uint32_t* col) {
   __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_base[kWavesPerBlock][kWave],
@@ -497,24 +509,43 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
   for (int j = 0; j < g.nslot; ++j) {
     const uint32_t bb = uint32_t(g.slot_b[j]) * n;
     const unsigned long long* fb = bits + uint64_t(j) * words;
+    // packed old_B rows: 32 ids per entry, tested against the bitmap at once
+    const bool packed = pk.len && packed_label(pk, g.slot_b[j]);
     for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < n; base += stride) {
       const uint32_t u = base + lane;
       unsigned long long b = 0;
       uint32_t len = 0;
       if (u < n) {
         b = old.ptr[bb + u];
-        len = uint32_t(old.ptr[bb + u + 1] - b);
+        len = packed ? pk.len[bb + u] : uint32_t(old.ptr[bb + u + 1] - b);
         if (FILL) s_out[wv][lane] = ptr[uint64_t(j) * n + u];
       }
       s_cnt[wv][lane] = 0;
       wave_sync();
-      flat_positions(b, len, s_pref[wv], s_base[wv], lane, [&](int k, unsigned long long pos) {
-        const uint32_t v = old.col[pos];
-        if ((fb[v / kWave] >> (v % kWave)) & 1ull) {
-          const uint32_t at = atomicAdd(&s_cnt[wv][k], 1u);
-          if (FILL) col[s_out[wv][k] + at] = v;
-        }
-      });
+      if (packed) {
+        flat_positions(b, len, s_pref[wv], s_base[wv], lane, [&](int k, unsigned long long pos) {
+          const unsigned long long e = pk.ent[pos];
+          const uint32_t w = uint32_t(e);
+          uint32_t hit = uint32_t(e >> 32) & uint32_t(fb[w >> 1] >> ((w & 1u) * 32u));
+          if (hit) {
+            uint32_t at = atomicAdd(&s_cnt[wv][k], uint32_t(__popc(hit)));
+            if (FILL)
+              while (hit) {
+                const int bit = __ffs(hit) - 1;
+                hit &= hit - 1;
+                col[s_out[wv][k] + at++] = w * 32u + uint32_t(bit);
+              }
+          }
+        });
+      } else {
+        flat_positions(b, len, s_pref[wv], s_base[wv], lane, [&](int k, unsigned long long pos) {
+          const uint32_t v = old.col[pos];
+          if ((fb[v / kWave] >> (v % kWave)) & 1ull) {
+            const uint32_t at = atomicAdd(&s_cnt[wv][k], 1u);
+            if (FILL) col[s_out[wv][k] + at] = v;
+          }
+        });
+      }
       wave_sync();
       if (!FILL && u < n) cnt[uint64_t(j) * n + u] = s_cnt[wv][lane];
       wave_sync();
@@ -525,17 +556,6 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
 // ---------------------------------------------------------------------------
 // expand: fused join + dedup + difference, one wave per active row
 
-// Packed form of a sorted row: one entry per non-empty 32-id word,
-// low 32 bits = word index (id >> 5), high 32 bits = the word's bit mask.
-// The packed entries of row r start at the row's own CSR offset (the entry
-// count never exceeds the id count) and len[r] says how many there are.
-struct DPacked {
-  const uint32_t* len;            // per row; null when absent
-  const unsigned long long* ent;
-  uint32_t labels[8];             // bit l: rows of label l are packed
-};
-
-__device__ inline bool packed_label(const DPacked& p, uint32_t l) { return (p.labels[l >> 5] >> (l & 31)) & 1u; }
 
 // One operator of the reference's partitioned schedule (MODE 2 of the
 // expand kernel). Rows are local to partition [first, first+np): row
@@ -1919,14 +1939,17 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       const DCsr dlc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
       const DCsr oc{P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
       k_frontier<<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, fr_words, P<unsigned long long>(fr_bits));
-      k_pairs<false><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, P<unsigned long long>(fr_bits), fr_words,
+      DPacked opk = plabels();
+      opk.len = pk_ready ? P<uint32_t>(pk_len) : nullptr;
+      opk.ent = P<unsigned long long>(pk_ent);
+      k_pairs<false><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, opk, P<unsigned long long>(fr_bits), fr_words,
                                                          P<uint32_t>(pr_cnt), nullptr, nullptr);
       GG_TRY(scan(pr_cnt, NS, pr_ptr, err));
       uint64_t npairs = 0;
       GG_TRY(read_u64(P<unsigned long long>(pr_ptr) + NS, npairs, err));
       GG_TRY(grow(pr_col, std::max<uint64_t>(npairs, 1) * 4, err));
-      k_pairs<true><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, P<unsigned long long>(fr_bits), fr_words, nullptr,
-                                                        P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
+      k_pairs<true><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, opk, P<unsigned long long>(fr_bits), fr_words,
+                                                        nullptr, P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
       GG_HIP(hipEventRecord(o.b, stream));
       pairs_ready = true;
     }
