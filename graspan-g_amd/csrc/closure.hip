@@ -96,7 +96,7 @@ struct Counters {
   unsigned int misses;         // rows whose guessed window missed a candidate
   unsigned int label_flags[8]; // bit l: delta of label l is not empty
   unsigned int pad;
-  unsigned long long xs[4];    // built with GG_EXPAND_STATS: packed words, list ids, keys, rows of k_expand
+  unsigned long long xs[4];    // built with GG_EXPAND_STATS: k_expand cycles in seeding, key lists, list streams, whole rows
 };
 
 struct DCsr {
@@ -597,7 +597,18 @@ struct WaveLds {
   uint32_t fresh[kChunkWords];
   unsigned long long pref[kWave];
   unsigned long long base[kWave];
+#ifdef GG_EXPAND_STATS
+  unsigned long long xs[4];
+#endif
 };
+#ifdef GG_EXPAND_STATS
+#define GG_CLK(v) const unsigned long long v = __builtin_amdgcn_s_memtime()
+#define GG_ACC(w, i, t0) \
+  if ((threadIdx.x & (kWave - 1)) == 0) (w).xs[i] += __builtin_amdgcn_s_memtime() - (t0)
+#else
+#define GG_CLK(v)
+#define GG_ACC(w, i, t0)
+#endif
 
 // Calls elem(x) for every element of the lists held by the lanes (lane i
 // holds col[lb .. lb+len)). All lists of the batch are flattened into one
@@ -748,9 +759,6 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
     const unsigned long long klen = (uint32_t(lane) < ns && kind_l != kSegU) ? e - b : 0ull;
     const unsigned long long incl = wave_incl_scan_u64(klen);
     const unsigned long long total = lane63_u64(incl);
-#ifdef GG_EXPAND_STATS
-    if (PACKED && lane == 0 && total) atomicAdd(&a.ctr->xs[2], total);
-#endif
     for (unsigned long long t0 = 0; t0 < total; t0 += kWave) {
       const unsigned long long t = t0 + lane;
       // segment of key t: the first j whose inclusive end is past t
@@ -862,6 +870,7 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
                                                                         unsigned long long& kcount, bool& miss,
                                                                         uint32_t& next) {
   bool miss_l = false;
+  GG_CLK(t_seed);
   const uint32_t words = (span + 31) >> 5;
   const unsigned long long c1 = (unsigned long long)c0 + span;
   const uint32_t w0 = c0 >> 5;
@@ -924,9 +933,6 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
       miss_l = true;
   };
   unsigned long long kc = 0;
-#ifdef GG_EXPAND_STATS
-  unsigned long long st_words = 0, st_ids = 0;
-#endif
   auto mark_word = [&](unsigned long long e) {
     const uint32_t wi = uint32_t(e) - w0;
     if (wi < words) {
@@ -937,6 +943,8 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
       miss_l = true;
     }
   };
+  GG_ACC(w, 0, t_seed);
+  GG_CLK(t_keys);
   for_each_list<MODE, true>(
       a, r, A, u, lane,
       [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
@@ -944,10 +952,9 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
         cut(col, b, e);
         if (ranged && e < lb + len) nxt = min(nxt, col[e]);
         kc += e - b;
-#ifdef GG_EXPAND_STATS
-        st_ids += e - b;
-#endif
+        GG_CLK(t_s);
         expand_lists(col, b, uint32_t(e - b), w, lane, mark);
+        GG_ACC(w, 2, t_s);
       },
       [&](const unsigned long long* ent, unsigned long long pb, uint32_t pl, uint32_t len) {
         unsigned long long b = pb, e = pb + pl;
@@ -958,20 +965,11 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
         } else {
           kc += len;
         }
-#ifdef GG_EXPAND_STATS
-        st_words += e - b;
-#endif
+        GG_CLK(t_s);
         expand_words(ent, b, uint32_t(e - b), w, lane, mark_word);
+        GG_ACC(w, 2, t_s);
       });
-#ifdef GG_EXPAND_STATS
-  {
-    const unsigned long long sw = wave_sum_u64(st_words), si = wave_sum_u64(st_ids);
-    if (lane == 0) {
-      atomicAdd(&a.ctr->xs[0], sw);
-      atomicAdd(&a.ctr->xs[1], si);
-    }
-  }
-#endif
+  GG_ACC(w, 1, t_keys);
   if (MODE == 2 && a.ro.kind == 2) {
     // eps: the row's own vertex
     bool has_eps = false;
@@ -1026,6 +1024,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
   __shared__ WaveLds s_lds[kWavesPerBlock];
   const int lane = threadIdx.x & (kWave - 1);
   WaveLds& w = s_lds[threadIdx.x / kWave];
+#ifdef GG_EXPAND_STATS
+  if (lane < 4) w.xs[lane] = 0;
+#endif
   const uint32_t nactive = a.ctr->active;
   unsigned long long kc = 0, orr = 0, ne = 0;
   // rows are taken kGrab at a time and staging space kStageBlock words at a
@@ -1071,9 +1072,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
     }
     const uint32_t idx = next++;
     const uint32_t r = uni_u32(a.active[idx]);
-#ifdef GG_EXPAND_STATS
-    if (lane == 0) atomicAdd(&a.ctr->xs[3], 1ull);
-#endif
+    GG_CLK(t_row);
     const uint32_t rn = MODE == 2 ? a.ro.np : a.n;
     const uint32_t A = r / rn, u = r - A * rn + (MODE == 2 ? a.ro.first : 0u);
     unsigned long long krow = 0;
@@ -1203,6 +1202,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       cnt = 0;
       pos = 0;
     }
+    GG_ACC(w, 3, t_row);
     if (empty) continue;
     if (lane == 0) {
       a.seg_off[r] = pos;
@@ -1216,6 +1216,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
     if (kc) atomicAdd(&a.ctr->cand, kc);
     if (orr) atomicAdd(&a.ctr->oreads, orr);
     if (ne) atomicAdd(&a.ctr->newe, ne);
+#ifdef GG_EXPAND_STATS
+    for (int i = 0; i < 4; ++i) atomicAdd(&a.ctr->xs[i], w.xs[i]);
+#endif
   }
 }
 
