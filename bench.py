@@ -377,7 +377,7 @@ def bench_dataflow(args):
     from graspan_g import gen
 
     g = gen.dataflow(10_000, seed=0)
-    ctx = gg.Context()
+    ctx = gg.Context(verbose=args.verbose)
     ctx.set_spec(g)
     for _ in range(args.warmup):
         ctx.run(0)
@@ -389,7 +389,9 @@ def bench_dataflow(args):
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
             "config": {"workload": f"dataflow closure n={g.n}, {g.m} e edges, filter 0",
-                       "closure_edges": int(stats[-1].total_edges), "rounds": stats[-1].rounds}}
+                       "closure_edges": int(stats[-1].total_edges), "rounds": stats[-1].rounds},
+            "breakdown_ms": {k: getattr(stats[-1], k) for k in
+                             ("fixpoint_ms", "expand_ms", "compact_ms", "merge_ms", "other_ms")}}
     print(json.dumps(line), flush=True)
 
 

@@ -63,6 +63,7 @@ constexpr uint32_t kMaxCrossParts = 1024;  // partitions of the out-of-core sche
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
 constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
 constexpr uint32_t kMergeBits = 8192;    // k_merge: rows spanning fewer ids are merged in an LDS bitmap
+constexpr uint32_t kShortRow = 1024;     // k_merge: unchanged rows up to this length move in the block copy
 constexpr int kMaxRules = 64;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -1345,25 +1346,36 @@ __device__ inline void flat_copy(const T* __restrict__ src, unsigned long long s
   wave_sync();
 }
 
+// Copies n elements with the whole wave, 4 loads in flight per lane.
+template <class T>
+__device__ inline void wave_copy(const T* __restrict__ src, T* __restrict__ dst, uint32_t n, int lane) {
+  uint32_t t = uint32_t(lane);
+  for (; t + 3 * kWave < n; t += 4 * kWave) {
+    const T a0 = src[t], a1 = src[t + kWave], a2 = src[t + 2 * kWave], a3 = src[t + 3 * kWave];
+    dst[t] = a0;
+    dst[t + kWave] = a1;
+    dst[t + 2 * kWave] = a2;
+    dst[t + 3 * kWave] = a3;
+  }
+  for (; t < n; t += kWave) dst[t] = src[t];
+}
+
 // old' = old U delta for the labels whose delta is not empty
 // (rowptr(old') = rowptr(old) + rowptr(delta)); the other labels are moved in
-// bulk by k_copy_labels. A wave takes 64 consecutive rows. Rows without delta
-// only shift (by their delta rowptr): their ids and packed entries move in
-// one flattened copy. Rows with delta are merged one at a time (rank of each
-// id in the other list) and, for packed labels, packed into out_pk at the
-// row's new offset, entry count in pk_len[r].
+// bulk by k_copy_labels. A wave takes 64 consecutive rows. Short rows without
+// delta only shift (by their delta rowptr): their ids and packed entries move
+// in one flattened copy. Long unchanged rows, rows with delta and (before old
+// is packed) the packed rows to build are appended to mlist for
+// k_merge_rows, one wave per row.
 __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t rows, uint32_t n, const Counters* ctr,
                                                   DPacked plab, const unsigned long long* old_pk, uint32_t* out_col,
-                                                  unsigned long long* out_pk, uint32_t* pk_len) {
+                                                  unsigned long long* out_pk, uint32_t* pk_len, uint32_t* mlist,
+                                                  uint32_t* mcnt) {
   // ctr == nullptr: merge every row (no label is skipped)
-  __shared__ uint32_t s_win[kWavesPerBlock][kWave];
-  __shared__ uint32_t s_bits[kWavesPerBlock][kMergeBits / 32];
   __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_sb[kWavesPerBlock][kWave],
       s_db[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wi = threadIdx.x / kWave;
-  uint32_t* win = s_win[wi];
-  uint32_t* bits = s_bits[wi];
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
   const uint32_t nblk = (rows + kWave - 1) / kWave;
@@ -1378,97 +1390,122 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
       db = dl.ptr[r];
       de = dl.ptr[r + 1];
     }
-    const bool same = live && de == db;  // no delta: the row only shifts
+    const bool same = live && de == db && oe - ob <= kShortRow;  // short, no delta: the row only shifts
     flat_copy(old.col, ob, out_col, ob + db, same ? uint32_t(oe - ob) : 0u, s_pref[wi], s_sb[wi], s_db[wi], lane);
     if (out_pk && old_pk) {
       const bool pkl = same && oe > ob && packed_label(plab, l);
       flat_copy(old_pk, ob, out_pk, ob + db, pkl ? pk_len[r] : 0u, s_pref[wi], s_sb[wi], s_db[wi], lane);
     }
-    // rows with delta (and, before old is packed, the packed rows to build)
-    unsigned long long m = __ballot(live && (!same || (out_pk && !old_pk && oe > ob && packed_label(plab, l))));
-    while (m) {
-      const int k = __ffsll(m) - 1;
-      m &= m - 1;
-      const uint32_t rr = blk * kWave + uint32_t(k);
-      const uint32_t lr = rr / n;
-      const unsigned long long rob = old.ptr[rr], roe = old.ptr[rr + 1];
-      const unsigned long long rdb = dl.ptr[rr], rde = dl.ptr[rr + 1];
-      uint32_t* out = out_col + rob + rdb;
-      const uint32_t ol = uint32_t(roe - rob), dln = uint32_t(rde - rdb);
-      const bool pkr = out_pk && packed_label(plab, lr);
-      // old and delta are disjoint and sorted: when their ids fit one LDS
-      // window, OR both into a bitmap and write it out in order; its non-zero
-      // words are the row's packed entries
-      {
-        uint32_t lo = 0xFFFFFFFFu, hi = 0;
-        if (ol) {
-          lo = old.col[rob];
-          hi = old.col[roe - 1];
+    const bool more = live && (!same || (out_pk && !old_pk && oe > ob && packed_label(plab, l)));
+    const unsigned long long m = __ballot(more);
+    if (m) {
+      uint32_t pos = 0;
+      if (lane == 0) pos = atomicAdd(mcnt, uint32_t(__popcll(m)));
+      pos = uni_u32(pos);  // lane 0: the branch is uniform
+      if (more) mlist[pos + __popcll(m & ((1ull << lane) - 1))] = r;
+    }
+  }
+}
+
+// The rows k_merge listed, one wave per row. Old and delta are disjoint and
+// sorted: when their ids fit one LDS window they are ORed into a bitmap that
+// is written out in order, its non-zero words being the row's packed
+// entries; wider rows place each id by its rank in the other list and are
+// packed by pack_union. Packed rows record their entry count in pk_len[r].
+__global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32_t n, DPacked plab,
+                                                       const unsigned long long* old_pk, uint32_t* out_col,
+                                                       unsigned long long* out_pk, uint32_t* pk_len,
+                                                       const uint32_t* mlist, const uint32_t* mcnt) {
+  __shared__ uint32_t s_win[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_bits[kWavesPerBlock][kMergeBits / 32];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wi = threadIdx.x / kWave;
+  uint32_t* win = s_win[wi];
+  uint32_t* bits = s_bits[wi];
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  const uint32_t count = *mcnt;
+  for (uint32_t i = wave; i < count; i += nwaves) {
+    const uint32_t rr = uni_u32(mlist[i]);
+    const uint32_t lr = rr / n;
+    const unsigned long long rob = old.ptr[rr], roe = old.ptr[rr + 1];
+    const unsigned long long rdb = dl.ptr[rr], rde = dl.ptr[rr + 1];
+    uint32_t* out = out_col + rob + rdb;
+    const uint32_t ol = uint32_t(roe - rob), dln = uint32_t(rde - rdb);
+    const bool pkr = out_pk && packed_label(plab, lr);
+    if (dln == 0 && (!pkr || old_pk)) {
+      // a long unchanged row: ids and packed entries move as they are
+      wave_copy(old.col + rob, out, ol, lane);
+      if (pkr) wave_copy(old_pk + rob, out_pk + rob + rdb, pk_len[rr], lane);
+      continue;
+    }
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+    if (ol) {
+      lo = old.col[rob];
+      hi = old.col[roe - 1];
+    }
+    if (dln) {
+      lo = min(lo, dl.col[rdb]);
+      hi = max(hi, dl.col[rde - 1]);
+    }
+    lo &= ~31u;
+    if (hi - lo < kMergeBits) {
+      const uint32_t words = ((hi - lo) >> 5) + 1;
+      for (uint32_t t = lane; t < words; t += kWave) bits[t] = 0;
+      wave_sync();
+      for (uint32_t t = lane; t < ol; t += kWave) {
+        const uint32_t off = old.col[rob + t] - lo;
+        __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      for (uint32_t t = lane; t < dln; t += kWave) {
+        const uint32_t off = dl.col[rdb + t] - lo;
+        __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      wave_sync();
+      const unsigned long long below = (1ull << lane) - 1;
+      unsigned long long o = 0;
+      uint32_t pe = 0;
+      for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
+        const uint32_t wd = w0 + lane < words ? bits[w0 + lane] : 0u;
+        const uint32_t pc = __popc(wd);
+        const uint32_t incl = wave_incl_scan(pc, lane);
+        uint32_t x = wd;
+        unsigned long long q = o + (incl - pc);
+        while (x) {
+          const int b = __ffs(x) - 1;
+          x &= x - 1;
+          out[q++] = lo + (w0 + uint32_t(lane)) * 32 + uint32_t(b);
         }
-        if (dln) {
-          lo = min(lo, dl.col[rdb]);
-          hi = max(hi, dl.col[rde - 1]);
-        }
-        lo &= ~31u;
-        if (hi - lo < kMergeBits) {
-          const uint32_t words = ((hi - lo) >> 5) + 1;
-          for (uint32_t i = lane; i < words; i += kWave) bits[i] = 0;
-          wave_sync();
-          for (uint32_t t = lane; t < ol; t += kWave) {
-            const uint32_t off = old.col[rob + t] - lo;
-            __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          }
-          for (uint32_t t = lane; t < dln; t += kWave) {
-            const uint32_t off = dl.col[rdb + t] - lo;
-            __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          }
-          wave_sync();
-          const unsigned long long below = (1ull << lane) - 1;
-          unsigned long long o = 0;
-          uint32_t pe = 0;
-          for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
-            const uint32_t wd = w0 + lane < words ? bits[w0 + lane] : 0u;
-            const uint32_t pc = __popc(wd);
-            const uint32_t incl = wave_incl_scan(pc, lane);
-            uint32_t x = wd;
-            unsigned long long q = o + (incl - pc);
-            while (x) {
-              const int b = __ffs(x) - 1;
-              x &= x - 1;
-              out[q++] = lo + (w0 + uint32_t(lane)) * 32 + uint32_t(b);
-            }
-            o += lane63_u32(incl);
-            if (pkr) {
-              const unsigned long long nz = __ballot(wd != 0);
-              if (wd)
-                out_pk[rob + rdb + pe + __popcll(nz & below)] =
-                    ((unsigned long long)wd << 32) | ((lo >> 5) + w0 + uint32_t(lane));
-              pe += uint32_t(__popcll(nz));
-            }
-          }
-          if (pkr && lane == 0) pk_len[rr] = pe;
-          wave_sync();
-          continue;
+        o += lane63_u32(incl);
+        if (pkr) {
+          const unsigned long long nz = __ballot(wd != 0);
+          if (wd)
+            out_pk[rob + rdb + pe + __popcll(nz & below)] =
+                ((unsigned long long)wd << 32) | ((lo >> 5) + w0 + uint32_t(lane));
+          pe += uint32_t(__popcll(nz));
         }
       }
-      if (dln == 0) {
-        for (uint32_t t = lane; t < ol; t += kWave) out[t] = old.col[rob + t];
-      } else if (ol == 0) {
-        for (uint32_t t = lane; t < dln; t += kWave) out[t] = dl.col[rdb + t];
-      } else {
-        for (uint32_t t = lane; t < ol; t += kWave) {
-          const uint32_t x = old.col[rob + t];
-          out[t + lower_rank(dl.col, rdb, rde, x)] = x;
-        }
-        for (uint32_t t = lane; t < dln; t += kWave) {
-          const uint32_t x = dl.col[rdb + t];
-          out[t + lower_rank(old.col, rob, roe, x)] = x;
-        }
+      if (pkr && lane == 0) pk_len[rr] = pe;
+      wave_sync();
+      continue;
+    }
+    if (dln == 0) {
+      for (uint32_t t = lane; t < ol; t += kWave) out[t] = old.col[rob + t];
+    } else if (ol == 0) {
+      for (uint32_t t = lane; t < dln; t += kWave) out[t] = dl.col[rdb + t];
+    } else {
+      for (uint32_t t = lane; t < ol; t += kWave) {
+        const uint32_t x = old.col[rob + t];
+        out[t + lower_rank(dl.col, rdb, rde, x)] = x;
       }
-      if (pkr) {
-        const uint32_t c = pack_union(old.col, rob, roe, dl.col, rdb, rde, out_pk + rob + rdb, win, lane);
-        if (lane == 0) pk_len[rr] = c;
+      for (uint32_t t = lane; t < dln; t += kWave) {
+        const uint32_t x = dl.col[rdb + t];
+        out[t + lower_rank(old.col, rob, roe, x)] = x;
       }
+    }
+    if (pkr) {
+      const uint32_t c = pack_union(old.col, rob, roe, dl.col, rdb, rde, out_pk + rob + rdb, win, lane);
+      if (lane == 0) pk_len[rr] = c;
     }
   }
 }
@@ -1623,7 +1660,8 @@ class Engine {
     DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
                      &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active,
                      &tiles,  &ctr,    &fr_bits,
-                     &pr_ptr, &pr_col, &pr_cnt, &pk_len, &pk_ent, &pk2_ent, &dpk_len, &dpk_ent, &npk_len, &npk_ent};
+                     &pr_ptr, &pr_col, &pr_cnt, &pk_len, &pk_ent, &pk2_ent, &dpk_len, &dpk_ent, &npk_len, &npk_ent,
+                     &mlist,  &mcnt};
     for (DevBuf* b : all)
       if (b->p) (void)hipFree(b->p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -1752,6 +1790,23 @@ class Engine {
     bool any = false;
     for (int i = 0; i < 8; ++i) any |= pack_labels[i] != 0;
     return any;
+  }
+
+  // out = old U delta row by row (k_merge, then k_merge_rows on the rows
+  // with delta); ctr == nullptr merges every label
+  DevBuf mlist, mcnt;
+  int merge_rows(DCsr oc, DCsr dc, uint32_t rows, uint32_t nn, const Counters* c, DPacked plab,
+                 const unsigned long long* old_pk, uint32_t* out_col, unsigned long long* out_pk, uint32_t* pklen,
+                 std::string& err) {
+    GG_TRY(grow(mlist, uint64_t(rows) * 4, err));
+    GG_TRY(alloc(mcnt, 4, err));
+    fill_u32(mcnt, 0u, 1);
+    k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, dc, rows, nn, c, plab, old_pk, out_col, out_pk, pklen,
+                                                      P<uint32_t>(mlist), P<uint32_t>(mcnt));
+    k_merge_rows<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, dc, nn, plab, old_pk, out_col, out_pk, pklen,
+                                                           P<uint32_t>(mlist), P<uint32_t>(mcnt));
+    GG_HIP(hipGetLastError());
+    return GG_OK;
   }
 
   ExpandArgs expand_args(bool init) {
@@ -1969,10 +2024,16 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       act.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
       act.b2ptr = pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr;
       if (act_work) k_activate<<<grid_for(n), kBlock, 0, stream>>>(act);
+      // a round with few active rows launches only the blocks that can get
+      // some (kGrab rows per grab, 4 waves per block), not a full grid of
+      // waves that would only contend on the work counters
+      GG_TRY(read_ctr(err));
+      const uint32_t want = uint32_t((uint64_t(h_ctr->active) + kGrab * kWavesPerBlock - 1) / (kGrab * kWavesPerBlock));
+      const uint32_t grid = std::max(1u, std::min(expand_grid(), want));
       GG_HIP(hipEventRecord(o.b, stream));
       EventPair& x = mk(ev_expand);
       GG_HIP(hipEventRecord(x.a, stream));
-      k_expand<0><<<expand_grid(), kBlock, 0, stream>>>(expand_args(false));
+      k_expand<0><<<grid, kBlock, 0, stream>>>(expand_args(false));
       GG_HIP(hipEventRecord(x.b, stream));
       GG_HIP(hipGetLastError());
       GG_TRY(read_ctr(err));
@@ -2035,9 +2096,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       }
       const DCsr oc{P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
       const DCsr dc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
-      k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, dc, R, n, C(), plab,
-                                                        pk_ready ? P<unsigned long long>(pk_ent) : nullptr,
-                                                        P<uint32_t>(o2_col), opk, P<uint32_t>(pk_len));
+      GG_TRY(merge_rows(oc, dc, R, n, C(), plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr,
+                        P<uint32_t>(o2_col), opk, P<uint32_t>(pk_len), err));
       if (old_nnz)
         k_copy_labels<<<dim3(uint32_t(cus), uint32_t(L)), kBlock, 0, stream>>>(
             oc, dc, n, C(), plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr, P<uint32_t>(o2_col), opk);
@@ -2324,8 +2384,8 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     GG_TRY(grow(M.col, std::max<uint64_t>(a.nnz + b.nnz, 1) * 4, err));
     k_add_ptr<<<grid_for(rows + 1), kBlock, 0, stream>>>(P<unsigned long long>(M.ptr), P<unsigned long long>(a.ptr),
                                                         P<unsigned long long>(b.ptr), rows + 1);
-    k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(dcsr(a), dcsr(b), uint32_t(rows), np, nullptr, DPacked{}, nullptr,
-                                                      P<uint32_t>(M.col), nullptr, nullptr);
+    GG_TRY(merge_rows(dcsr(a), dcsr(b), uint32_t(rows), np, nullptr, DPacked{}, nullptr, P<uint32_t>(M.col), nullptr,
+                      nullptr, err));
     GG_HIP(hipGetLastError());
     M.nnz = a.nnz + b.nnz;
     std::swap(a.ptr, M.ptr);
@@ -2976,7 +3036,7 @@ int engine_release(Engine* e) {
                    &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt,  &e->active, &e->tiles,
                    &e->fr_bits,
                    &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_len, &e->pk_ent, &e->pk2_ent,
-                   &e->dpk_len, &e->dpk_ent, &e->npk_len, &e->npk_ent};
+                   &e->dpk_len, &e->dpk_ent, &e->npk_len, &e->npk_ent, &e->mlist, &e->mcnt};
   for (DevBuf* b : all) e->release(*b);
   e->has_closure = false;
   return GG_OK;
