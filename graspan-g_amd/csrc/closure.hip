@@ -41,6 +41,7 @@
 #include <numeric>
 #include <string>
 #include <functional>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -1618,7 +1619,7 @@ class Engine {
   hipStream_t stream = nullptr;
   size_t used = 0, peak = 0;
   std::function<bool()> on_pressure;  // frees device memory, if it can, when an allocation does not fit
-  double alloc_ms = 0;                // host time spent in hipMalloc / hipFree
+  double alloc_ms = 0;                // host time spent allocating and freeing device memory
   uint64_t alloc_calls = 0;
 
   DevGrammar g{};
@@ -1662,33 +1663,75 @@ class Engine {
                      &tiles,  &ctr,    &fr_bits,
                      &pr_ptr, &pr_col, &pr_cnt, &pk_len, &pk_ent, &pk2_ent, &dpk_len, &dpk_ent, &npk_len, &npk_ent,
                      &mlist,  &mcnt};
+    if (stream) (void)hipStreamSynchronize(stream);
     for (DevBuf* b : all)
       if (b->p) (void)hipFree(b->p);
+    for (auto& kv : cache) (void)hipFree(kv.second);
     if (h_ctr) (void)hipHostFree(h_ctr);
     if (h_u64) (void)hipHostFree(h_u64);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  // Device memory. Buffers that are released or outgrown go to a per-engine
+  // block cache and are handed out again for requests up to half their size:
+  // the round loop's ping-pong buffers grow every round and then trade the
+  // same blocks, instead of returning multi-GB blocks to the driver and
+  // allocating fresh ones each round (large fresh allocations were seen to
+  // cost seconds per closure once memory had been freed before). The cache is
+  // trimmed (hipFree) when an allocation would not fit the budget or the
+  // device, and when a context releases its closure.
+  std::multimap<size_t, void*> cache;  // size -> block
+  size_t cached = 0;
+
+  void trim_cache() {
+    if (cache.empty()) return;
+    (void)hipStreamSynchronize(stream);
+    for (auto& kv : cache) (void)hipFree(kv.second);
+    cache.clear();
+    cached = 0;
+  }
+
+  void put_block(DevBuf& b) {
+    if (!b.p) return;
+    cache.emplace(b.bytes, b.p);
+    cached += b.bytes;
+    used -= b.bytes;
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+
   int alloc(DevBuf& b, size_t bytes, std::string& err) {
     if (b.p && b.bytes >= bytes) return GG_OK;
     const auto a0 = std::chrono::steady_clock::now();
-    if (b.p) {
-      (void)hipFree(b.p);
-      used -= b.bytes;
-      b.p = nullptr;
-      b.bytes = 0;
-    }
-    bytes = std::max<size_t>(bytes, 256);
     struct Clock {
       double& ms;
       std::chrono::steady_clock::time_point t;
       ~Clock() { ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count(); }
     } clk{alloc_ms, a0};
+    put_block(b);
+    bytes = std::max<size_t>(bytes, 256);
+    if (bytes >= (2u << 20)) bytes = (bytes + (2u << 20) - 1) & ~size_t((2u << 20) - 1);
+    // a cached block of at least this size and at most twice it
+    auto it = cache.lower_bound(bytes);
+    if (it != cache.end() && it->first <= 2 * bytes) {
+      b.p = it->second;
+      b.bytes = it->first;
+      cached -= it->first;
+      cache.erase(it);
+      used += b.bytes;
+      peak = std::max(peak, used);
+      return GG_OK;
+    }
     ++alloc_calls;
-    // under memory pressure the out-of-core schedule may move a partition to
-    // host DRAM (on_pressure returns true when it freed something)
+    // under memory pressure: first the cache goes back, then the out-of-core
+    // schedule may move a partition to host DRAM (on_pressure returns true
+    // when it freed something)
     for (;;) {
-      if (budget && used + bytes > budget) {
+      if (budget && used + cached + bytes > budget) {
+        if (cached) {
+          trim_cache();
+          continue;
+        }
         if (on_pressure && on_pressure()) continue;
         err = "device budget exceeded: need " + std::to_string(used + bytes) + " bytes, budget " +
               std::to_string(budget);
@@ -1698,6 +1741,10 @@ class Engine {
       if (e == hipSuccess) break;
       (void)hipGetLastError();
       b.p = nullptr;
+      if (cached) {
+        trim_cache();
+        continue;
+      }
       if (on_pressure && on_pressure()) continue;
       err = "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e);
       return GG_E_NOMEM;
@@ -1718,16 +1765,9 @@ class Engine {
     return s;
   }
 
-  void release(DevBuf& b) {
-    if (b.p) {
-      const auto a0 = std::chrono::steady_clock::now();
-      (void)hipFree(b.p);
-      alloc_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a0).count();
-      used -= b.bytes;
-      b.p = nullptr;
-      b.bytes = 0;
-    }
-  }
+  // The buffer's block goes to the cache (its memory stays with the engine
+  // until the cache is trimmed).
+  void release(DevBuf& b) { put_block(b); }
 
   uint32_t grid_for(uint64_t work, uint32_t per_block = kBlock) const {
     uint64_t g = (work + per_block - 1) / per_block;
@@ -1884,6 +1924,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     }
   } guard{{&ev_expand, &ev_compact, &ev_merge, &ev_other}};
 
+  peak = used;  // device_bytes_peak is per run
   GG_TRY(alloc(ctr, sizeof(Counters), err));
   GG_TRY(alloc(old_ptr, (uint64_t(R) + 1) * 8, err));
   GG_TRY(alloc(dl_ptr, (uint64_t(R) + 1) * 8, err));
@@ -2204,6 +2245,7 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
   }};
   const uint64_t max_rows = uint64_t(L) * max_np;
+  peak = used;  // device_bytes_peak is per run
   GG_TRY(alloc(ctr, sizeof(Counters), err));
   GG_TRY(alloc(seg_off, max_rows * 8, err));
   GG_TRY(alloc(seg_cnt, max_rows * 4, err));
@@ -2301,10 +2343,14 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     for (auto& r : q.r) {
       GG_TRY(alloc(r.ptr, (rows + 1) * 8, err));
       GG_TRY(alloc(r.col, std::max<uint64_t>(r.nnz, 1) * 4, err));
-      if (!r.nnz) {
-        fill_u64(r.ptr, 0, rows + 1);
-        continue;
-      }
+      if (!r.nnz) fill_u64(r.ptr, 0, rows + 1);
+    }
+    // the buffers are allocated in the compute stream's order: the copy
+    // stream starts after that point
+    GG_HIP(hipEventRecord(ev, stream));
+    GG_HIP(hipStreamWaitEvent(copy_stream, ev, 0));
+    for (auto& r : q.r) {
+      if (!r.nnz) continue;
       GG_HIP(hipMemcpyAsync(r.ptr.p, r.hptr.p, (rows + 1) * 8, hipMemcpyHostToDevice, copy_stream));
       GG_HIP(hipMemcpyAsync(r.col.p, r.hcol.p, r.nnz * 4, hipMemcpyHostToDevice, copy_stream));
       if (q.evicted) S.spill_bytes += (rows + 1) * 8 + r.nnz * 4;  // the first upload is the input
@@ -3038,6 +3084,7 @@ int engine_release(Engine* e) {
                    &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_len, &e->pk_ent, &e->pk2_ent,
                    &e->dpk_len, &e->dpk_ent, &e->npk_len, &e->npk_ent, &e->mlist, &e->mcnt};
   for (DevBuf* b : all) e->release(*b);
+  e->trim_cache();  // the memory goes back to the device (other contexts may need it)
   e->has_closure = false;
   return GG_OK;
 }

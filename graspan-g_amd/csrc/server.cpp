@@ -1,8 +1,11 @@
 // Serving loop over the batched closure (gg_server_*; no counterpart in the
 // reference). See include/graspan_g.h for the admission policy.
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -43,6 +46,12 @@ struct gg_server {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 
+  // GG_SERVER_TRACE=1: one line per batch on stderr (sizes and phase times)
+  const bool trace = [] {
+    const char* t = std::getenv("GG_SERVER_TRACE");
+    return t && t[0] == '1';
+  }();
+
   void worker(int slot) {
     gg_options o{opt.device, 0, 0};
     gg_ctx* ctx = nullptr;
@@ -79,30 +88,68 @@ struct gg_server {
         }
       }
       const double start = now();
-      int status = cs;
-      std::vector<int> ids(take.size(), -1);
-      gg_batch* b = nullptr;
-      if (status == GG_OK) status = gg_batch_create(ctx, &b);
-      for (size_t i = 0; status == GG_OK && i < take.size(); ++i) {
-        Request* r = take[i];
-        status = gg_batch_add(b, int(r->rules.size() / 3), r->rules.data(), r->L, r->n, r->src.size(), r->src.data(),
-                              r->dst.data(), r->lab.data(), &ids[i]);
-      }
-      if (status == GG_OK) status = gg_batch_run(b, 0, 0, nullptr);
+      // one batch on the slot's context; a batch that runs out of device
+      // memory is split in halves and retried (the footprint estimate is
+      // only an estimate), down to single requests
       std::vector<std::vector<uint64_t>> counts(take.size());
       std::vector<uint64_t> totals(take.size(), 0), sums(take.size(), 0);
-      for (size_t i = 0; status == GG_OK && i < take.size(); ++i) {
-        counts[i].assign(size_t(take[i]->L), 0);
-        status = gg_batch_result(b, ids[i], counts[i].data(), &totals[i], &sums[i]);
-      }
-      if (b) gg_batch_destroy(b);
-      if (ctx) gg_closure_release(ctx);  // the slot's device memory goes back
+      std::vector<int> rstatus(take.size(), cs);
+      double t_add = 0, t_run = 0;
+      gg_stats bst{};
+      std::function<void(size_t, size_t)> exec = [&](size_t lo, size_t hi) {
+        int status = cs;
+        gg_batch* b = nullptr;
+        std::vector<int> ids(hi - lo, -1);
+        const double a0 = now();
+        if (status == GG_OK) status = gg_batch_create(ctx, &b);
+        for (size_t i = lo; status == GG_OK && i < hi; ++i) {
+          Request* r = take[i];
+          status = gg_batch_add(b, int(r->rules.size() / 3), r->rules.data(), r->L, r->n, r->src.size(),
+                                r->src.data(), r->dst.data(), r->lab.data(), &ids[i - lo]);
+        }
+        const double a1 = now();
+        if (status == GG_OK) status = gg_batch_run(b, 0, 0, &bst);
+        t_add += a1 - a0;
+        t_run += now() - a1;
+        for (size_t i = lo; status == GG_OK && i < hi; ++i) {
+          counts[i].assign(size_t(take[i]->L), 0);
+          status = gg_batch_result(b, ids[i - lo], counts[i].data(), &totals[i], &sums[i]);
+        }
+        if (b) gg_batch_destroy(b);
+        // A slot keeps its device memory for its next batch (freeing and
+        // re-allocating tens of GB per batch costs seconds) while the batch
+        // stayed within the slot's share of the budget; otherwise, or after
+        // an error, the memory goes back for the other slots.
+        const uint64_t share = opt.budget_bytes / uint64_t(opt.slots);
+        if (ctx && (status != GG_OK || bst.device_bytes_peak > share)) gg_closure_release(ctx);
+        if (status == GG_E_NOMEM && hi - lo > 1) {
+          const size_t mid = lo + (hi - lo) / 2;
+          exec(lo, mid);
+          exec(mid, hi);
+          return;
+        }
+        for (size_t i = lo; i < hi; ++i) rstatus[i] = status;
+      };
+      exec(0, take.size());
+      const double t_res = now();
       const double end = now();
+      if (trace) {
+        uint64_t nv = 0, edges = 0;
+        for (size_t i = 0; i < take.size(); ++i) {
+          nv += take[i]->n;
+          edges += totals[i];
+        }
+        std::fprintf(stderr,
+                     "[gg-server] slot %d: %zu requests, %llu vertices, %llu closure edges: add %.0f ms, run %.0f ms "
+                     "(last fixpoint %.0f, %u rounds), total %.0f ms\n",
+                     slot, take.size(), (unsigned long long)nv, (unsigned long long)edges, t_add, t_run,
+                     bst.fixpoint_ms, bst.rounds, t_res - start);
+      }
       {
         std::lock_guard<std::mutex> lk(mu);
         for (size_t i = 0; i < take.size(); ++i) {
           Request* r = take[i];
-          r->res.status = status;
+          r->res.status = rstatus[i];
           r->res.total_edges = totals[i];
           r->res.checksum = sums[i];
           r->res.start_ms = start;

@@ -301,7 +301,7 @@ class Server:
     """Serving loop (gg_server_*): submit requests, wait for results."""
 
     def __init__(self, device: int = -1, slots: int = 2, budget_bytes: int = 200 << 30,
-                 bytes_per_vertex: int = 16 << 10, max_batch_requests: int = 0):
+                 bytes_per_vertex: int = 32 << 10, max_batch_requests: int = 0):
         self._L = lib()
         self._h = ctypes.c_void_p()
         self._labels = {}

@@ -250,7 +250,10 @@ typedef struct gg_server_options {
   int slots;                   /* concurrent batches (>= 1) */
   uint64_t budget_bytes;       /* device memory for all in-flight requests */
   uint64_t bytes_per_vertex;   /* footprint estimate of a request: this times
-                                  its vertex count plus 64 bytes per input edge */
+                                  its vertex count plus 64 bytes per input edge
+                                  (the engine needs ~24 bytes per closure edge);
+                                  a batch that still runs out of device memory
+                                  is split in halves and retried */
   uint32_t max_batch_requests; /* cap on requests per batch (0 = no cap) */
 } gg_server_options;
 
