@@ -1392,7 +1392,19 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
       de = dl.ptr[r + 1];
     }
     const bool same = live && de == db && oe - ob <= kShortRow;  // short, no delta: the row only shifts
-    flat_copy(old.col, ob, out_col, ob + db, same ? uint32_t(oe - ob) : 0u, s_pref[wi], s_sb[wi], s_db[wi], lane);
+    // consecutive such rows are one contiguous range with one shift (no
+    // delta between them): copied run by run, no per-id row search
+    unsigned long long runs = __ballot(same);
+    while (runs) {
+      const int k0 = __ffsll(runs) - 1;
+      const unsigned long long from = runs >> k0;
+      const int len = (~from == 0ull) ? kWave - k0 : __ffsll(~from) - 1;
+      runs &= ~(len == kWave ? ~0ull : (((1ull << len) - 1) << k0));
+      const unsigned long long sb = uni_u64((unsigned long long)__shfl((long long)ob, k0, kWave));
+      const unsigned long long se = uni_u64((unsigned long long)__shfl((long long)oe, k0 + len - 1, kWave));
+      const unsigned long long sh = uni_u64((unsigned long long)__shfl((long long)db, k0, kWave));
+      if (se > sb) wave_copy(old.col + sb, out_col + sb + sh, uint32_t(se - sb), lane);
+    }
     if (out_pk && old_pk) {
       const bool pkl = same && oe > ob && packed_label(plab, l);
       flat_copy(old_pk, ob, out_pk, ob + db, pkl ? pk_len[r] : 0u, s_pref[wi], s_sb[wi], s_db[wi], lane);
