@@ -2289,8 +2289,9 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   // Residency. A partition is resident (its three regions in HBM) or spilled
   // (in pinned host DRAM). Partitions come in when an operator needs them and
   // leave only under pressure: when an allocation would pass the device
-  // budget (or hipMalloc fails), the least recently used resident partition
-  // that the current step does not hold is copied out over the copy stream.
+  // budget (or hipMalloc fails), the resident partition that the current
+  // step does not hold and whose next use in the sweep is farthest is copied
+  // out over the copy stream.
   // GG_OOC_MAX_RESIDENT (a test hook) also caps the number of resident
   // partitions, so that small graphs go through the spill path.
   hipEvent_t ev = nullptr;
@@ -2299,8 +2300,6 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     hipEvent_t& e;
     ~EventGuard() { (void)hipEventDestroy(e); }
   } eg{ev};
-  std::vector<uint64_t> last_use(static_cast<size_t>(NP), 0);
-  uint64_t tick = 0;
   int hold[2] = {-1, -1};
   int max_resident = 0;
   if (const char* mr = std::getenv("GG_OOC_MAX_RESIDENT")) max_resident = std::max(2, std::atoi(mr));
@@ -2330,10 +2329,23 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     copy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
     return GG_OK;
   };
+  // The sweeps visit the partitions in index order, so the next use of
+  // partition q is known: at q if the sweep has not passed it yet, else in
+  // the next sweep. The victim is the partition used farthest ahead
+  // (Belady's choice), not the least recently used one, which for a cyclic
+  // sweep is the one needed next.
+  int sweep_pos = 0;
   auto evict_one = [&]() -> bool {
     int best = -1;
-    for (int p = 0; p < NP; ++p)
-      if (pt[p].resident && p != hold[0] && p != hold[1] && (best < 0 || last_use[p] < last_use[best])) best = p;
+    long best_next = -1;
+    for (int p = 0; p < NP; ++p) {
+      if (!pt[p].resident || p == hold[0] || p == hold[1]) continue;
+      const long next_use = p >= sweep_pos ? long(p) : long(NP) + p;
+      if (next_use > best_next) {
+        best_next = next_use;
+        best = p;
+      }
+    }
     return best >= 0 && spill(best) == GG_OK;
   };
   struct PressureGuard {
@@ -2343,7 +2355,6 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   if (po.spill) on_pressure = evict_one;
   auto fetch = [&](int p) -> int {
     Part& q = pt[p];
-    last_use[p] = ++tick;
     if (q.resident) return GG_OK;
     if (po.spill && max_resident) {
       int res = 0;
@@ -2607,6 +2618,7 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   for (int p = 0; p < NP; ++p) {
     hold[0] = p;
     hold[1] = -1;
+    sweep_pos = p;
     GG_TRY(fetch(p));
     for (;;) {
       Part& q = pt[p];
@@ -2647,6 +2659,7 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   for (;;) {
     ++supersteps;
     for (int i = 0; i < NP; ++i) {
+      sweep_pos = i;
       for (int j = i + 1; j < NP; ++j) {
         if (po.ddm && (ddm[i][j] + ddm[j][i] == 0 || !(dirty(i) || dirty(j)))) {
           S.pairs_skipped++;

@@ -155,8 +155,8 @@ int gg_closure_run_partitioned(gg_ctx *ctx, int filter_flag, gg_stats *stats);
  * keeps the partitions it is not using in Partition<k> files,
  * src/main.cpp:335-631, 764-858). Partitions are loaded into HBM when the
  * schedule needs them and stay there. When an allocation would pass the
- * budget (or hipMalloc fails), the least recently used partition that the
- * current step does not hold is spilled to pinned host DRAM with
+ * budget (or hipMalloc fails), the partition that the current step does not
+ * hold and whose next use in the sweep is farthest is spilled to pinned host DRAM with
  * hipMemcpyAsync on a copy stream. The finished closure is assembled in HBM
  * and must fit the budget. Pairs are chosen by their cross-partition edge counts,
  * recomputed on the device whenever a partition changes: a pair runs only
