@@ -597,8 +597,8 @@ struct ExpandArgs {
 struct WaveLds {
   uint32_t seen[kChunkWords];
   uint32_t fresh[kChunkWords];
-  unsigned long long pref[kWave];
-  unsigned long long base[kWave];
+  unsigned long long lbase[kWave];   // expand_stream: base address per non-empty list
+  unsigned long long tmask[kUnroll];  // expand_stream: list starts inside the tile
 #ifdef GG_EXPAND_STATS
   unsigned long long xs[4];
 #endif
@@ -613,74 +613,64 @@ struct WaveLds {
 #endif
 
 // Calls elem(x) for every element of the lists held by the lanes (lane i
-// holds col[lb .. lb+len)). All lists of the batch are flattened into one
-// element stream: a wave prefix sum of the lengths goes to LDS and each
-// element index is mapped back to its list by a 6-step binary search. Every
-// lane keeps kUnroll loads in flight before it consumes them.
-template <class F>
-__device__ inline void expand_lists(const uint32_t* __restrict__ col, unsigned long long lb, uint32_t len, WaveLds& w,
-                                    int lane, F&& elem) {
+// holds src[lb .. lb+len)). All lists of the batch are flattened into one
+// element stream, cut into tiles of kWave*kUnroll elements; lane l takes
+// elements t0 + q*kWave + l of a tile, so neighbouring lanes read
+// neighbouring ids. An element's list is the last non-empty list that starts
+// at or before it. Per tile, every list starting inside the tile sets its
+// start's bit in an LDS bitmap (one 64-bit word per q); the list of element
+// t0 + q*kWave + l is then found by counting, with mbcnt, the bitmap bits of
+// word q up to bit l, plus the lists that started before. That rank indexes a
+// table of the non-empty lists' base addresses. Every lane keeps kUnroll
+// loads in flight before it consumes them. All lanes of the wave take part.
+template <class T, class F>
+__device__ inline void expand_stream(const T* __restrict__ src, unsigned long long lb, uint32_t len, WaveLds& w,
+                                     int lane, F&& elem) {
+  constexpr uint32_t kTile = kWave * kUnroll;
   const unsigned long long incl = wave_incl_scan_u64(len);
   const unsigned long long total = lane63_u64(incl);
   if (total == 0) return;
+  const unsigned long long start = incl - len;
+  const unsigned long long nonempty = __ballot(len != 0);
   wave_sync();
-  w.pref[lane] = incl;
-  w.base[lane] = lb;
+  // element t of the stream lies at byte address lbase[rank of its list] + t*sizeof(T)
+  if (len) {
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(nonempty >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(nonempty), 0u));
+    w.lbase[rank] = reinterpret_cast<unsigned long long>(src) + (lb - start) * sizeof(T);
+  }
+  if (lane < kUnroll) w.tmask[lane] = 0;
   wave_sync();
-  for (unsigned long long t0 = 0; t0 < total; t0 += kWave * kUnroll) {
-    uint32_t x[kUnroll];
+  for (unsigned long long t0 = 0; t0 < total; t0 += kTile) {
+    const unsigned long long pos = start - t0;
+    if (len && pos < kTile)
+      __hip_atomic_fetch_or(&w.tmask[pos / kWave], 1ull << (pos % kWave), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // non-empty lists that start before the tile, less one: the rank of the list
+    // holding element t0 - 1
+    uint32_t c = uint32_t(__popcll(__ballot(len != 0 && start < t0))) - 1u;
+    wave_sync();
+    const unsigned long long rem = total - t0;
+    const uint32_t lim = rem < kTile ? uint32_t(rem) : kTile;
+    const unsigned long long tb = (t0 + uint32_t(lane)) * sizeof(T);
+    T x[kUnroll];
 #pragma unroll
     for (int q = 0; q < kUnroll; ++q) {
-      const unsigned long long t = t0 + q * kWave + lane;
+      const unsigned long long m = uni_u64(w.tmask[q]);
+      const unsigned long long up = m >> 1;  // mbcnt counts bits below the lane: bits 1..l of m, plus bit 0
+      const uint32_t rk =
+          __builtin_amdgcn_mbcnt_hi(uint32_t(up >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(up), c + uint32_t(m & 1)));
+      c += uint32_t(__popcll(m));
       x[q] = 0;
-      if (t < total) {
-        int k = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-          if (w.pref[k + step - 1] <= t) k += step;
-        const unsigned long long prev = k ? w.pref[k - 1] : 0ull;
-        x[q] = col[w.base[k] + (t - prev)];
-      }
+      if (lane < int(lim) - q * kWave)  // lane against a scalar bound: no per-q lane constants
+        x[q] = *reinterpret_cast<const __attribute__((address_space(1))) T*>(w.lbase[rk] + tb +
+                                                                              uint32_t(q * kWave) * sizeof(T));
     }
+    wave_sync();
+    if (lane < kUnroll) w.tmask[lane] = 0;
 #pragma unroll
     for (int q = 0; q < kUnroll; ++q)
-      if (t0 + q * kWave + lane < total) elem(x[q]);
+      if (lane < int(lim) - q * kWave) elem(x[q]);
+    wave_sync();
   }
-  wave_sync();
-}
-
-// expand_lists for packed lists: calls elem(entry) for every packed word of
-// the lists held by the lanes (lane i holds ent[pb .. pb+plen)).
-template <class F>
-__device__ inline void expand_words(const unsigned long long* __restrict__ ent, unsigned long long pb, uint32_t plen,
-                                    WaveLds& w, int lane, F&& elem) {
-  const unsigned long long incl = wave_incl_scan_u64(plen);
-  const unsigned long long total = lane63_u64(incl);
-  if (total == 0) return;
-  wave_sync();
-  w.pref[lane] = incl;
-  w.base[lane] = pb;
-  wave_sync();
-  for (unsigned long long t0 = 0; t0 < total; t0 += kWave * kUnroll) {
-    unsigned long long x[kUnroll];
-#pragma unroll
-    for (int q = 0; q < kUnroll; ++q) {
-      const unsigned long long t = t0 + q * kWave + lane;
-      x[q] = 0;
-      if (t < total) {
-        int k = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-          if (w.pref[k + step - 1] <= t) k += step;
-        const unsigned long long prev = k ? w.pref[k - 1] : 0ull;
-        x[q] = ent[w.base[k] + (t - prev)];
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < kUnroll; ++q)
-      if (t0 + q * kWave + lane < total) elem(x[q]);
-  }
-  wave_sync();
 }
 
 // Visits every candidate list of row (A,u): calls visit(col, lb, len) once
@@ -955,7 +945,7 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
         if (ranged && e < lb + len) nxt = min(nxt, col[e]);
         kc += e - b;
         GG_CLK(t_s);
-        expand_lists(col, b, uint32_t(e - b), w, lane, mark);
+        expand_stream(col, b, uint32_t(e - b), w, lane, mark);
         GG_ACC(w, 2, t_s);
       },
       [&](const unsigned long long* ent, unsigned long long pb, uint32_t pl, uint32_t len) {
@@ -968,7 +958,7 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
           kc += len;
         }
         GG_CLK(t_s);
-        expand_words(ent, b, uint32_t(e - b), w, lane, mark_word);
+        expand_stream(ent, b, uint32_t(e - b), w, lane, mark_word);
         GG_ACC(w, 2, t_s);
       });
   GG_ACC(w, 1, t_keys);
