@@ -441,28 +441,71 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
 // ---------------------------------------------------------------------------
 // frontier pair lists for old_B x delta_C and the transpose of old_B
 
+// Flattened list streams. A wave's lanes each hold one list (lane k: len
+// elements starting at `start` = the exclusive prefix of the lengths); all
+// lists form one element stream, cut into tiles of U*kWave elements, and lane
+// l takes elements t0 + q*kWave + l of a tile, so neighbouring lanes read
+// neighbouring elements. An element belongs to the last non-empty list that
+// starts at or before it. Per tile every list starting inside the tile sets
+// its start's bit in an LDS bitmap `tmask` (one 64-bit word per q); the list
+// of element t0 + q*kWave + l is then its rank among the non-empty lists:
+// the bits of word q up to bit l (mbcnt) plus the lists that started before.
+// The caller keeps per-rank tables (written with stream_rank) and calls
+// stream_tile once per tile with every lane of the wave active;
+// f(q, rank) runs on every lane, in order of q.
+__device__ inline uint32_t stream_rank(uint32_t len) {
+  const unsigned long long nonempty = __ballot(len != 0);
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(nonempty >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(nonempty), 0u));
+}
+
+template <int U, class F>
+__device__ inline void stream_tile(unsigned long long start, uint32_t len, unsigned long long t0,
+                                   unsigned long long* tmask, int lane, F&& f) {
+  const unsigned long long pos = start - t0;
+  if (len && pos < uint64_t(U) * kWave)
+    __hip_atomic_fetch_or(&tmask[pos / kWave], 1ull << (pos % kWave), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  // non-empty lists that start before the tile, less one: the rank of the
+  // list holding element t0 - 1
+  uint32_t c = uint32_t(__popcll(__ballot(len != 0 && start < t0))) - 1u;
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const unsigned long long m = uni_u64(tmask[q]);
+    const unsigned long long up = m >> 1;  // mbcnt counts bits below the lane: bits 1..l of m, plus bit 0
+    f(q, __builtin_amdgcn_mbcnt_hi(uint32_t(up >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(up), c + uint32_t(m & 1))));
+    c += uint32_t(__popcll(m));
+  }
+  wave_sync();
+  if (lane < U) tmask[lane] = 0;
+  wave_sync();
+}
+
 // Calls f(k, pos) for every element position pos of the lanes' lists
-// [lb, lb+len) (k = the lane that owns the list): all lists of the wave are
-// flattened into one stream, 4 positions per lane in flight, so the wave does
-// not walk its lists one after another. pref/base: 64-entry LDS arrays.
+// [lb, lb+len) (k = the lane that owns the list), as one flattened stream (see
+// stream_tile), one element per lane in flight. tab: 64-entry LDS table of
+// the wave; tmask: one zeroed LDS word of the wave.
 template <class F>
-__device__ inline void flat_positions(unsigned long long lb, uint32_t len, unsigned long long* pref,
-                                      unsigned long long* base, int lane, F&& f) {
+__device__ inline void flat_positions(unsigned long long lb, uint32_t len, unsigned long long* tab,
+                                      unsigned long long* tmask, int lane, F&& f) {
   const unsigned long long incl = wave_incl_scan_u64(len);
   const unsigned long long total = lane63_u64(incl);
   if (total == 0) return;
+  const unsigned long long start = incl - len;
+  const uint32_t rank = stream_rank(len);
   wave_sync();
-  pref[lane] = incl;
-  base[lane] = lb;
+  // owner lane in the top 6 bits; position of element t: low bits + t
+  if (len) tab[rank] = ((lb - start) & ((1ull << 58) - 1)) | (uint64_t(lane) << 58);
+  if (lane == 0) tmask[0] = 0;
   wave_sync();
-  for (unsigned long long t = lane; t < total; t += kWave) {
-    int k = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1)
-      if (pref[k + step - 1] <= t) k += step;
-    f(k, base[k] + (t - (k ? pref[k - 1] : 0ull)));
+  for (unsigned long long t0 = 0; t0 < total; t0 += kWave) {
+    const unsigned long long rem = total - t0;
+    stream_tile<1>(start, len, t0, tmask, lane, [&](int, uint32_t rk) {
+      if (uint64_t(lane) < rem) {
+        const unsigned long long e = tab[rk];
+        f(int(e >> 58), ((e + t0 + uint32_t(lane)) & ((1ull << 58) - 1)));
+      }
+    });
   }
-  wave_sync();
 }
 
 // Packed form of a sorted row: one entry per non-empty 32-id word,
@@ -502,7 +545,7 @@ template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr old, DPacked pk,
                                                   const unsigned long long* bits, uint32_t words, uint32_t* cnt,
                                                   const unsigned long long* ptr, uint32_t* col) {
-  __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_base[kWavesPerBlock][kWave],
+  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][1],
       s_out[kWavesPerBlock][kWave];
   __shared__ uint32_t s_cnt[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
@@ -525,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
       s_cnt[wv][lane] = 0;
       wave_sync();
       if (packed) {
-        flat_positions(b, len, s_pref[wv], s_base[wv], lane, [&](int k, unsigned long long pos) {
+        flat_positions(b, len, s_tab[wv], s_tmask[wv], lane, [&](int k, unsigned long long pos) {
           const unsigned long long e = pk.ent[pos];
           const uint32_t w = uint32_t(e);
           uint32_t hit = uint32_t(e >> 32) & uint32_t(fb[w >> 1] >> ((w & 1u) * 32u));
@@ -540,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
           }
         });
       } else {
-        flat_positions(b, len, s_pref[wv], s_base[wv], lane, [&](int k, unsigned long long pos) {
+        flat_positions(b, len, s_tab[wv], s_tmask[wv], lane, [&](int k, unsigned long long pos) {
           const uint32_t v = old.col[pos];
           if ((fb[v / kWave] >> (v % kWave)) & 1ull) {
             const uint32_t at = atomicAdd(&s_cnt[wv][k], 1u);
@@ -613,16 +656,9 @@ struct WaveLds {
 #endif
 
 // Calls elem(x) for every element of the lists held by the lanes (lane i
-// holds src[lb .. lb+len)). All lists of the batch are flattened into one
-// element stream, cut into tiles of kWave*kUnroll elements; lane l takes
-// elements t0 + q*kWave + l of a tile, so neighbouring lanes read
-// neighbouring ids. An element's list is the last non-empty list that starts
-// at or before it. Per tile, every list starting inside the tile sets its
-// start's bit in an LDS bitmap (one 64-bit word per q); the list of element
-// t0 + q*kWave + l is then found by counting, with mbcnt, the bitmap bits of
-// word q up to bit l, plus the lists that started before. That rank indexes a
-// table of the non-empty lists' base addresses. Every lane keeps kUnroll
-// loads in flight before it consumes them. All lanes of the wave take part.
+// holds src[lb .. lb+len)), as one flattened stream (see stream_tile) with
+// kUnroll loads in flight per lane before they are consumed. The per-rank
+// table holds each non-empty list's base address. All lanes take part.
 template <class T, class F>
 __device__ inline void expand_stream(const T* __restrict__ src, unsigned long long lb, uint32_t len, WaveLds& w,
                                      int lane, F&& elem) {
@@ -631,44 +667,26 @@ __device__ inline void expand_stream(const T* __restrict__ src, unsigned long lo
   const unsigned long long total = lane63_u64(incl);
   if (total == 0) return;
   const unsigned long long start = incl - len;
-  const unsigned long long nonempty = __ballot(len != 0);
+  const uint32_t rank = stream_rank(len);
   wave_sync();
   // element t of the stream lies at byte address lbase[rank of its list] + t*sizeof(T)
-  if (len) {
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(nonempty >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(nonempty), 0u));
-    w.lbase[rank] = reinterpret_cast<unsigned long long>(src) + (lb - start) * sizeof(T);
-  }
+  if (len) w.lbase[rank] = reinterpret_cast<unsigned long long>(src) + (lb - start) * sizeof(T);
   if (lane < kUnroll) w.tmask[lane] = 0;
   wave_sync();
   for (unsigned long long t0 = 0; t0 < total; t0 += kTile) {
-    const unsigned long long pos = start - t0;
-    if (len && pos < kTile)
-      __hip_atomic_fetch_or(&w.tmask[pos / kWave], 1ull << (pos % kWave), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    // non-empty lists that start before the tile, less one: the rank of the list
-    // holding element t0 - 1
-    uint32_t c = uint32_t(__popcll(__ballot(len != 0 && start < t0))) - 1u;
-    wave_sync();
     const unsigned long long rem = total - t0;
-    const uint32_t lim = rem < kTile ? uint32_t(rem) : kTile;
+    const int lim = rem < kTile ? int(rem) : int(kTile);
     const unsigned long long tb = (t0 + uint32_t(lane)) * sizeof(T);
     T x[kUnroll];
-#pragma unroll
-    for (int q = 0; q < kUnroll; ++q) {
-      const unsigned long long m = uni_u64(w.tmask[q]);
-      const unsigned long long up = m >> 1;  // mbcnt counts bits below the lane: bits 1..l of m, plus bit 0
-      const uint32_t rk =
-          __builtin_amdgcn_mbcnt_hi(uint32_t(up >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(up), c + uint32_t(m & 1)));
-      c += uint32_t(__popcll(m));
+    stream_tile<kUnroll>(start, len, t0, w.tmask, lane, [&](int q, uint32_t rk) {
       x[q] = 0;
-      if (lane < int(lim) - q * kWave)  // lane against a scalar bound: no per-q lane constants
+      if (lane < lim - q * kWave)  // lane against a scalar bound: no per-q lane constants
         x[q] = *reinterpret_cast<const __attribute__((address_space(1))) T*>(w.lbase[rk] + tb +
                                                                               uint32_t(q * kWave) * sizeof(T));
-    }
-    wave_sync();
-    if (lane < kUnroll) w.tmask[lane] = 0;
+    });
 #pragma unroll
     for (int q = 0; q < kUnroll; ++q)
-      if (lane < int(lim) - q * kWave) elem(x[q]);
+      if (lane < lim - q * kWave) elem(x[q]);
     wave_sync();
   }
 }
@@ -1298,38 +1316,39 @@ __device__ inline uint32_t pack_union(const uint32_t* a, unsigned long long ia, 
 }
 
 // Copies the lanes' segments src[sb .. sb+len) to dst[db ..] (one segment
-// per lane, all flattened into one coalesced element stream, as in
-// expand_lists). pref/sbase/dbase: 64-entry LDS arrays of the wave.
+// per lane, all flattened into one coalesced element stream, see
+// stream_tile). sbase/dbase: 64-entry LDS tables of the wave; tmask: 4
+// LDS words of the wave.
 template <class T>
 __device__ inline void flat_copy(const T* __restrict__ src, unsigned long long sb, T* __restrict__ dst,
-                                 unsigned long long db, uint32_t len, unsigned long long* pref,
+                                 unsigned long long db, uint32_t len, unsigned long long* tmask,
                                  unsigned long long* sbase, unsigned long long* dbase, int lane) {
+  constexpr int kU = 4;
   const unsigned long long incl = wave_incl_scan_u64(len);
   const unsigned long long total = lane63_u64(incl);
   if (total == 0) return;
+  const unsigned long long start = incl - len;
+  const uint32_t rank = stream_rank(len);
   wave_sync();
-  pref[lane] = incl;
-  sbase[lane] = sb;
-  dbase[lane] = db;
+  if (len) {  // element t of segment k: src[sbase[k] + t] -> dst[dbase[k] + t]
+    sbase[rank] = sb - start;
+    dbase[rank] = db - start;
+  }
+  if (lane < kU) tmask[lane] = 0;
   wave_sync();
-  constexpr int kU = 4;
   for (unsigned long long t0 = 0; t0 < total; t0 += kWave * kU) {
+    const unsigned long long rem = total - t0;
+    const int lim = rem < kWave * kU ? int(rem) : kWave * kU;
+    const unsigned long long t = t0 + uint32_t(lane);
     T x[kU];
     unsigned long long d[kU];
-#pragma unroll
-    for (int q = 0; q < kU; ++q) {
-      const unsigned long long t = t0 + q * kWave + lane;
+    stream_tile<kU>(start, len, t0, tmask, lane, [&](int q, uint32_t rk) {
       d[q] = ~0ull;
-      if (t < total) {
-        int k = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-          if (pref[k + step - 1] <= t) k += step;
-        const unsigned long long off = t - (k ? pref[k - 1] : 0ull);
-        x[q] = src[sbase[k] + off];
-        d[q] = dbase[k] + off;
+      if (lane < lim - q * kWave) {
+        x[q] = src[sbase[rk] + t + uint32_t(q * kWave)];
+        d[q] = dbase[rk] + t + uint32_t(q * kWave);
       }
-    }
+    });
 #pragma unroll
     for (int q = 0; q < kU; ++q)
       if (d[q] != ~0ull) dst[d[q]] = x[q];
@@ -1363,8 +1382,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
                                                   unsigned long long* out_pk, uint32_t* pk_len, uint32_t* mlist,
                                                   uint32_t* mcnt) {
   // ctr == nullptr: merge every row (no label is skipped)
-  __shared__ unsigned long long s_pref[kWavesPerBlock][kWave], s_sb[kWavesPerBlock][kWave],
-      s_db[kWavesPerBlock][kWave];
+  __shared__ unsigned long long s_tmask[kWavesPerBlock][4], s_sb[kWavesPerBlock][kWave], s_db[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wi = threadIdx.x / kWave;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -1397,7 +1415,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
     }
     if (out_pk && old_pk) {
       const bool pkl = same && oe > ob && packed_label(plab, l);
-      flat_copy(old_pk, ob, out_pk, ob + db, pkl ? pk_len[r] : 0u, s_pref[wi], s_sb[wi], s_db[wi], lane);
+      flat_copy(old_pk, ob, out_pk, ob + db, pkl ? pk_len[r] : 0u, s_tmask[wi], s_sb[wi], s_db[wi], lane);
     }
     const bool more = live && (!same || (out_pk && !old_pk && oe > ob && packed_label(plab, l)));
     const unsigned long long m = __ballot(more);
