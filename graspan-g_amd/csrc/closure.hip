@@ -1356,18 +1356,29 @@ __device__ inline void flat_copy(const T* __restrict__ src, unsigned long long s
   wave_sync();
 }
 
-// Copies n elements with the whole wave, 4 loads in flight per lane.
+// Copies n elements with the whole wave as 16-byte vectors (only the
+// element's own alignment is assumed: global dwordx4 accesses need 4 bytes),
+// 4 vectors in flight per lane, so a wave keeps 16 KB moving; the last
+// n mod (16/sizeof(T)) elements go one by one.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef u32x4 u32x4_a4 __attribute__((aligned(4)));
 template <class T>
 __device__ inline void wave_copy(const T* __restrict__ src, T* __restrict__ dst, uint32_t n, int lane) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte elements");
+  constexpr uint32_t E = 16 / sizeof(T);
+  const uint32_t nv = n / E;
+  const u32x4_a4* __restrict__ s = reinterpret_cast<const u32x4_a4*>(src);
+  u32x4_a4* __restrict__ d = reinterpret_cast<u32x4_a4*>(dst);
   uint32_t t = uint32_t(lane);
-  for (; t + 3 * kWave < n; t += 4 * kWave) {
-    const T a0 = src[t], a1 = src[t + kWave], a2 = src[t + 2 * kWave], a3 = src[t + 3 * kWave];
-    dst[t] = a0;
-    dst[t + kWave] = a1;
-    dst[t + 2 * kWave] = a2;
-    dst[t + 3 * kWave] = a3;
+  for (; t + 3 * kWave < nv; t += 4 * kWave) {
+    const u32x4 a0 = s[t], a1 = s[t + kWave], a2 = s[t + 2 * kWave], a3 = s[t + 3 * kWave];
+    d[t] = a0;
+    d[t + kWave] = a1;
+    d[t + 2 * kWave] = a2;
+    d[t + 3 * kWave] = a3;
   }
-  for (; t < n; t += kWave) dst[t] = src[t];
+  for (; t < nv; t += kWave) d[t] = s[t];
+  for (uint32_t i = nv * E + uint32_t(lane); i < n; i += kWave) dst[i] = src[i];
 }
 
 // old' = old U delta for the labels whose delta is not empty
@@ -1484,20 +1495,36 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
       }
       wave_sync();
       const unsigned long long below = (1ull << lane) - 1;
+      // dense rows (4+ ids per word on average): two words per wave step,
+      // one bit per lane, so each step stores its ids contiguously; sparse
+      // rows: one word per lane, its bits stored one by one
+      const bool dense = uint64_t(ol) + dln >= 4ull * words;
       unsigned long long o = 0;
       uint32_t pe = 0;
+      if (dense) {
+        for (uint32_t j = 0; j < words; j += 2) {
+          const uint32_t wj = j + uint32_t(lane >> 5);
+          const uint32_t wd = wj < words ? bits[wj] : 0u;
+          const bool set = (wd >> (lane & 31)) & 1u;
+          const unsigned long long m = __ballot(set);
+          if (set) out[o + __popcll(m & below)] = lo + wj * 32 + uint32_t(lane & 31);
+          o += uint32_t(__popcll(m));
+        }
+      }
       for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
         const uint32_t wd = w0 + lane < words ? bits[w0 + lane] : 0u;
-        const uint32_t pc = __popc(wd);
-        const uint32_t incl = wave_incl_scan(pc, lane);
-        uint32_t x = wd;
-        unsigned long long q = o + (incl - pc);
-        while (x) {
-          const int b = __ffs(x) - 1;
-          x &= x - 1;
-          out[q++] = lo + (w0 + uint32_t(lane)) * 32 + uint32_t(b);
+        if (!dense) {
+          const uint32_t pc = __popc(wd);
+          const uint32_t incl = wave_incl_scan(pc, lane);
+          uint32_t x = wd;
+          unsigned long long q = o + (incl - pc);
+          while (x) {
+            const int b = __ffs(x) - 1;
+            x &= x - 1;
+            out[q++] = lo + (w0 + uint32_t(lane)) * 32 + uint32_t(b);
+          }
+          o += lane63_u32(incl);
         }
-        o += lane63_u32(incl);
         if (pkr) {
           const unsigned long long nz = __ballot(wd != 0);
           if (wd)
