@@ -60,6 +60,7 @@ constexpr int kUnroll = 8;              // loads in flight per lane in the candi
 constexpr uint32_t kGrab = 16;          // active rows taken per work-counter atomic
 constexpr uint32_t kXcds = 8;           // MI355X: 8 XCDs, each with its own L2
 constexpr int kExpandWaves = 6;         // k_expand occupancy target (waves per SIMD)
+constexpr uint64_t kDeferRatio = 128;   // pending edges kept apart while under 1/128 of old
 constexpr uint32_t kMaxCrossParts = 1024;  // partitions of the out-of-core schedule
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
 constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
@@ -357,6 +358,7 @@ __global__ void k_activate_raw(const unsigned long long* ptr, uint32_t rows, uin
 
 struct ActArgs {
   DCsr old, dl;
+  const unsigned long long* pend_ptr;  // may be null (no pending edges)
   const unsigned long long* b2ptr;  // may be null (no pair lists this round)
   uint32_t n;
   uint32_t nlhs;
@@ -405,7 +407,10 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
           const unsigned long long kl = a.dl.ptr[rb + 1] - a.dl.ptr[rb];
           pk += kl;
           act |= kl > 0;
-          if (label_nonempty(a.ctr, a.g.b_rhs2[i])) pk += a.old.ptr[rb + 1] - a.old.ptr[rb];
+          if (label_nonempty(a.ctr, a.g.b_rhs2[i])) {
+            pk += a.old.ptr[rb + 1] - a.old.ptr[rb];
+            if (a.pend_ptr) pk += a.pend_ptr[rb + 1] - a.pend_ptr[rb];
+          }
           const uint32_t slot = a.g.b_slot[i];
           if (slot != 0xFFu && a.b2ptr) {
             const uint32_t rs = slot * a.n + u;
@@ -542,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier(const DevGrammar g, uint32_
 // slot's frontier bitmap; per-row counts live in LDS. COUNT pass: cnt[j*n+u]
 // = |pairs|; fill pass: the pairs, at ptr[j*n+u], in no particular order.
 template <bool FILL>
-__global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr old, DPacked pk,
+__global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr old, DCsr pend, DPacked pk,
                                                   const unsigned long long* bits, uint32_t words, uint32_t* cnt,
                                                   const unsigned long long* ptr, uint32_t* col) {
   __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][1],
@@ -582,14 +587,26 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
               }
           }
         });
-      } else {
-        flat_positions(b, len, s_tab[wv], s_tmask[wv], lane, [&](int k, unsigned long long pos) {
-          const uint32_t v = old.col[pos];
+      }
+      // id rows: old_B(u) when it is not packed, and the pending edges of B(u)
+      auto ids = [&](const uint32_t* src, unsigned long long ib, uint32_t il) {
+        flat_positions(ib, il, s_tab[wv], s_tmask[wv], lane, [&](int k, unsigned long long pos) {
+          const uint32_t v = src[pos];
           if ((fb[v / kWave] >> (v % kWave)) & 1ull) {
             const uint32_t at = atomicAdd(&s_cnt[wv][k], 1u);
             if (FILL) col[s_out[wv][k] + at] = v;
           }
         });
+      };
+      if (!packed) ids(old.col, b, len);
+      if (pend.ptr) {
+        unsigned long long qb = 0;
+        uint32_t ql = 0;
+        if (u < n) {
+          qb = pend.ptr[bb + u];
+          ql = uint32_t(pend.ptr[bb + u + 1] - qb);
+        }
+        ids(pend.col, qb, ql);
       }
       wave_sync();
       if (!FILL && u < n) cnt[uint64_t(j) * n + u] = s_cnt[wv][lane];
@@ -622,6 +639,7 @@ struct RegionOp {
 
 struct ExpandArgs {
   DCsr old, dl, raw;
+  DCsr pend;  // edges found since old was last rebuilt (ptr null: none); old ∪ pend is the closure so far
   RegionOp ro;
   DPacked pk;  // packed old rows
   DPacked dpk;  // packed delta rows (same labels; entries at the row's delta offset)
@@ -695,7 +713,7 @@ __device__ inline void expand_stream(const T* __restrict__ src, unsigned long lo
 // per batch with one list per lane (len 0 = no list on that lane). With
 // PACKED, rows of packed labels (old and delta) go to visitw(ent, pb, plen,
 // len) instead: plen packed words at ent[pb ..] standing for len ids.
-template <int MODE, bool PACKED, class V, class W>
+template <int MODE, bool PACKED, bool PEND, class V, class W>
 __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A, uint32_t u, int lane, V&& visit,
                                      W&& visitw) {
   const uint32_t n = a.n;
@@ -787,8 +805,8 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
       const uint32_t kind = desc & 3u, C = desc >> 8;
       const bool pk = (desc & 4u) != 0;
       const unsigned long long bj = __shfl(b, j, kWave);
-      unsigned long long ob = 0, db = 0;
-      uint32_t ol = 0, dlen = 0, pl = 0, dpl = 0;
+      unsigned long long ob = 0, db = 0, qb = 0;
+      uint32_t ol = 0, dlen = 0, pl = 0, dpl = 0, ql = 0;
       bool is_pk = false, d_pk = false;
       if (t < total) {
         const unsigned long long k = bj + (t - ex);
@@ -807,6 +825,10 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
             is_pk = true;
             pl = a.pk.len[rc];
           }
+          if (PEND && a.pend.ptr) {
+            qb = a.pend.ptr[rc];
+            ql = uint32_t(a.pend.ptr[rc + 1] - qb);
+          }
         }
       }
       if (PACKED) {
@@ -815,6 +837,7 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
       }
       visit(a.old.col, is_pk ? 0ull : ob, is_pk ? 0u : ol);
       visit(a.dl.col, d_pk ? 0ull : db, d_pk ? 0u : dlen);
+      if (PEND && a.pend.ptr) visit(a.pend.col, qb, ql);
     }
     ns = 0;
   };
@@ -873,7 +896,7 @@ __device__ inline unsigned long long lower_pos_word(const unsigned long long* en
 // costs its own ids plus a search per list, and `next` receives the
 // smallest candidate id at or past the window's end (0xFFFFFFFF if none):
 // windows with no candidate are skipped.
-template <int MODE>
+template <int MODE, bool PEND>
 __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const ExpandArgs& a, uint32_t r, uint32_t A,
                                                                         uint32_t u, uint32_t c0, uint32_t span,
                                                                         bool ranged, WaveLds& w, int lane,
@@ -931,6 +954,11 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
     unsigned long long db = a.dl.ptr[r], de = a.dl.ptr[r + 1];
     cut(a.dl.col, db, de);
     for (unsigned long long i = db + lane; i < de; i += kWave) seed(a.dl.col[i]);
+    if (PEND && a.pend.ptr) {
+      unsigned long long qb = a.pend.ptr[r], qe = a.pend.ptr[r + 1];
+      cut(a.pend.col, qb, qe);
+      for (unsigned long long i = qb + lane; i < qe; i += kWave) seed(a.pend.col[i]);
+    }
     wave_sync();
   }
   // candidates go to `fresh` with plain (no-return) LDS ORs; the ids already
@@ -955,7 +983,7 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
   };
   GG_ACC(w, 0, t_seed);
   GG_CLK(t_keys);
-  for_each_list<MODE, true>(
+  for_each_list<MODE, true, PEND>(
       a, r, A, u, lane,
       [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
         unsigned long long b = lb, e = lb + len;
@@ -1028,7 +1056,9 @@ __device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t spa
   }
 }
 
-template <int MODE>
+// PEND: the round reads pending edges beside old (a separate instance, so
+// rounds without them carry none of that code)
+template <int MODE, bool PEND = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandWaves, kExpandWaves))) void k_expand(
     const ExpandArgs a) {
   __shared__ WaveLds s_lds[kWavesPerBlock];
@@ -1102,7 +1132,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       } else if (a.n > kChunkBits) {
         uint32_t mn = 0xFFFFFFFFu, mx = 0;
         unsigned long long kk = 0;
-        for_each_list<MODE, false>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
+        for_each_list<MODE, false, PEND>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
           if (MODE == 1) {
             // raw rows are unsorted: reduce over every element
             const unsigned long long b = (unsigned long long)uint32_t(__builtin_amdgcn_readlane(int(uint32_t(lb)), 0)) |
@@ -1161,7 +1191,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
           unsigned long long kch = 0;
           bool miss = false;
           uint32_t nxt = 0xFFFFFFFFu;
-          const uint32_t c = process_chunk<MODE>(a, r, A, u, c0, span, ranged, w, lane, kch, miss, nxt);
+          const uint32_t c = process_chunk<MODE, PEND>(a, r, A, u, c0, span, ranged, w, lane, kch, miss, nxt);
           if (attempt == 0 && miss) {
             missed = true;
             break;
@@ -1219,7 +1249,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       a.seg_cnt[r] = cnt;
       kc += krow;
       ne += cnt;
-      if (MODE == 0 && krow) orr += (a.old.ptr[r + 1] - a.old.ptr[r]) + (a.dl.ptr[r + 1] - a.dl.ptr[r]);
+      if (MODE == 0 && krow) {
+        orr += (a.old.ptr[r + 1] - a.old.ptr[r]) + (a.dl.ptr[r + 1] - a.dl.ptr[r]);
+        if (PEND && a.pend.ptr) orr += a.pend.ptr[r + 1] - a.pend.ptr[r];
+      }
     }
   }
   if (lane == 0) {
@@ -1681,6 +1714,13 @@ class Engine {
 
   // closure state
   DevBuf old_ptr, old_col, dl_ptr, dl_col, nx_ptr, nx_col, o2_ptr, o2_col;
+  // pending edges: the rounds' deltas since old was last rebuilt (unpacked
+  // CSR, same row numbering); old ∪ pend is the closure so far
+  DevBuf pd_ptr, pd_col, p2_ptr, p2_col;
+  uint64_t pend_nnz = 0;
+  // a round's delta goes to pend while (pend + delta) * defer_ratio <= old
+  // (0: never); GG_DEFER_RATIO overrides the default
+  uint64_t defer_ratio = kDeferRatio;
   DevBuf raw_ptr, raw_col, staging, seg_off, seg_cnt, active, tiles, ctr;
   // frontier bitmaps and pair lists for old_B x delta_C
   DevBuf fr_bits, pr_ptr, pr_col, pr_cnt;
@@ -1709,7 +1749,7 @@ class Engine {
                      &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active,
                      &tiles,  &ctr,    &fr_bits,
                      &pr_ptr, &pr_col, &pr_cnt, &pk_len, &pk_ent, &pk2_ent, &dpk_len, &dpk_ent, &npk_len, &npk_ent,
-                     &mlist,  &mcnt};
+                     &mlist,  &mcnt,   &pd_ptr, &pd_col, &p2_ptr, &p2_col};
     if (stream) (void)hipStreamSynchronize(stream);
     for (DevBuf* b : all)
       if (b->p) (void)hipFree(b->p);
@@ -1882,6 +1922,38 @@ class Engine {
   // out = old U delta row by row (k_merge, then k_merge_rows on the rows
   // with delta); ctr == nullptr merges every label
   DevBuf mlist, mcnt;
+  // old' = old ∪ x (x disjoint from old, xn edges), packed rows rebuilt for
+  // the packed labels. flags: the label flags of x (labels without x edges
+  // move in one piece), or null to merge every row.
+  int fold_into_old(DCsr xc, uint64_t xn, const Counters* flags, std::string& err) {
+    const uint32_t R = rows();
+    bool pack_any = false;
+    for (int i = 0; i < 8; ++i) pack_any |= pack_labels[i] != 0;
+    const uint64_t o2n = old_nnz + xn;
+    GG_TRY(grow(o2_col, std::max<uint64_t>(o2n, 1) * 4, err));
+    k_add_ptr<<<grid_for(uint64_t(R) + 1), kBlock, 0, stream>>>(P<unsigned long long>(o2_ptr), P<unsigned long long>(old_ptr),
+                                                               xc.ptr, uint64_t(R) + 1);
+    DPacked plab{};
+    for (int i = 0; i < 8; ++i) plab.labels[i] = pack_labels[i];
+    unsigned long long* opk = nullptr;
+    if (pack_any) {
+      GG_TRY(grow(pk2_ent, std::max<uint64_t>(o2n, 1) * 8, err));
+      opk = P<unsigned long long>(pk2_ent);
+    }
+    const DCsr oc{P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
+    GG_TRY(merge_rows(oc, xc, R, n, flags, plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr,
+                      P<uint32_t>(o2_col), opk, P<uint32_t>(pk_len), err));
+    if (old_nnz && flags)
+      k_copy_labels<<<dim3(uint32_t(cus), uint32_t(L)), kBlock, 0, stream>>>(
+          oc, xc, n, flags, plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr, P<uint32_t>(o2_col), opk);
+    pk_ready = pack_any;
+    if (pack_any) std::swap(pk_ent, pk2_ent);
+    std::swap(old_ptr, o2_ptr);
+    std::swap(old_col, o2_col);
+    old_nnz = o2n;
+    return GG_OK;
+  }
+
   int merge_rows(DCsr oc, DCsr dc, uint32_t rows, uint32_t nn, const Counters* c, DPacked plab,
                  const unsigned long long* old_pk, uint32_t* out_col, unsigned long long* out_pk, uint32_t* pklen,
                  std::string& err) {
@@ -1896,12 +1968,16 @@ class Engine {
     return GG_OK;
   }
 
+  DCsr pend_csr() {
+    return {pend_nnz ? P<unsigned long long>(pd_ptr) : nullptr, pend_nnz ? P<uint32_t>(pd_col) : nullptr};
+  }
   ExpandArgs expand_args(bool init) {
     ExpandArgs a{};
     a.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
     a.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
     a.raw = {init ? P<unsigned long long>(raw_ptr) : nullptr, init ? P<uint32_t>(raw_col) : nullptr};
     a.b2 = {pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr, pairs_ready ? P<uint32_t>(pr_col) : nullptr};
+    a.pend = pend_csr();
     a.pk.len = (!init && pk_ready) ? P<uint32_t>(pk_len) : nullptr;
     a.pk.ent = P<unsigned long long>(pk_ent);
     for (int i = 0; i < 8; ++i) a.pk.labels[i] = pack_labels[i];
@@ -1946,6 +2022,9 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     err = "the semi-naive engine computes the least fixpoint only; filter_flag != 0 runs the partitioned schedule";
     return GG_E_ARG;
   }
+  pend_nnz = 0;
+  defer_ratio = kDeferRatio;
+  if (const char* dr = std::getenv("GG_DEFER_RATIO")) defer_ratio = std::strtoull(dr, nullptr, 10);
   if (uint64_t(L) * n >= 0xFFFFFFFFull || uint64_t(g.nslot) * n >= 0xFFFFFFFFull) {
     err = "num_labels * num_vertices (and binary rules with a derived second label * num_vertices) must be < 2^32";
     return GG_E_UNSUPPORTED;
@@ -2088,13 +2167,13 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       DPacked opk = plabels();
       opk.len = pk_ready ? P<uint32_t>(pk_len) : nullptr;
       opk.ent = P<unsigned long long>(pk_ent);
-      k_pairs<false><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, opk, P<unsigned long long>(fr_bits), fr_words,
+      k_pairs<false><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, pend_csr(), opk, P<unsigned long long>(fr_bits), fr_words,
                                                          P<uint32_t>(pr_cnt), nullptr, nullptr);
       GG_TRY(scan(pr_cnt, NS, pr_ptr, err));
       uint64_t npairs = 0;
       GG_TRY(read_u64(P<unsigned long long>(pr_ptr) + NS, npairs, err));
       GG_TRY(grow(pr_col, std::max<uint64_t>(npairs, 1) * 4, err));
-      k_pairs<true><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, opk, P<unsigned long long>(fr_bits), fr_words,
+      k_pairs<true><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, pend_csr(), opk, P<unsigned long long>(fr_bits), fr_words,
                                                         nullptr, P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
       GG_HIP(hipEventRecord(o.b, stream));
       pairs_ready = true;
@@ -2111,6 +2190,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       act.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
       act.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
       act.b2ptr = pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr;
+      act.pend_ptr = pend_csr().ptr;
       if (act_work) k_activate<<<grid_for(n), kBlock, 0, stream>>>(act);
       // a round with few active rows launches only the blocks that can get
       // some (kGrab rows per grab, 4 waves per block), not a full grid of
@@ -2121,7 +2201,10 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_HIP(hipEventRecord(o.b, stream));
       EventPair& x = mk(ev_expand);
       GG_HIP(hipEventRecord(x.a, stream));
-      k_expand<0><<<grid, kBlock, 0, stream>>>(expand_args(false));
+      if (pend_nnz)
+        k_expand<0, true><<<grid, kBlock, 0, stream>>>(expand_args(false));
+      else
+        k_expand<0><<<grid, kBlock, 0, stream>>>(expand_args(false));
       GG_HIP(hipEventRecord(x.b, stream));
       GG_HIP(hipGetLastError());
       GG_TRY(read_ctr(err));
@@ -2167,43 +2250,61 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
                                                      npk ? P<unsigned long long>(npk_ent) : nullptr,
                                                      P<uint32_t>(npk_len));
     GG_HIP(hipEventRecord(c.b, stream));
-    // old' = old U delta, and its transpose old'^T = old^T ++ delta^T
+    // old' = old ∪ delta. While the deltas are small next to old they go to
+    // the pending CSR instead (pend' = pend ∪ delta), so a round that finds
+    // few edges does not rewrite all of old; the joins read old ∪ pend.
     EventPair& mg = mk(ev_merge);
     GG_HIP(hipEventRecord(mg.a, stream));
-    const uint64_t o2n = old_nnz + dl_nnz;
-    GG_TRY(grow(o2_col, std::max<uint64_t>(o2n, 1) * 4, err));
-    k_add_ptr<<<grid_for(uint64_t(R) + 1), kBlock, 0, stream>>>(P<unsigned long long>(o2_ptr), P<unsigned long long>(old_ptr),
-                                                               P<unsigned long long>(dl_ptr), uint64_t(R) + 1);
-    {
-      DPacked plab{};
-      for (int i = 0; i < 8; ++i) plab.labels[i] = pack_labels[i];
-      unsigned long long* opk = nullptr;
-      if (pack_any) {
-        GG_TRY(grow(pk2_ent, std::max<uint64_t>(o2n, 1) * 8, err));
-        opk = P<unsigned long long>(pk2_ent);
+    const bool defer = defer_ratio && old_nnz && (pend_nnz + dl_nnz) * defer_ratio <= old_nnz;
+    if (defer || pend_nnz) {
+      if (!pend_nnz) {
+        GG_TRY(alloc(pd_ptr, (uint64_t(R) + 1) * 8, err));
+        fill_u64(pd_ptr, 0ull, uint64_t(R) + 1);
       }
-      const DCsr oc{P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
+      GG_TRY(alloc(p2_ptr, (uint64_t(R) + 1) * 8, err));
+      const uint64_t p2n = pend_nnz + dl_nnz;
+      GG_TRY(grow(p2_col, std::max<uint64_t>(p2n, 1) * 4, err));
+      k_add_ptr<<<grid_for(uint64_t(R) + 1), kBlock, 0, stream>>>(
+          P<unsigned long long>(p2_ptr), P<unsigned long long>(pd_ptr), P<unsigned long long>(dl_ptr), uint64_t(R) + 1);
+      const DCsr pc{P<unsigned long long>(pd_ptr), P<uint32_t>(pd_col)};
       const DCsr dc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
-      GG_TRY(merge_rows(oc, dc, R, n, C(), plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr,
-                        P<uint32_t>(o2_col), opk, P<uint32_t>(pk_len), err));
-      if (old_nnz)
-        k_copy_labels<<<dim3(uint32_t(cus), uint32_t(L)), kBlock, 0, stream>>>(
-            oc, dc, n, C(), plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr, P<uint32_t>(o2_col), opk);
+      GG_TRY(merge_rows(pc, dc, R, n, C(), DPacked{}, nullptr, P<uint32_t>(p2_col), nullptr, nullptr, err));
+      if (pend_nnz)
+        k_copy_labels<<<dim3(uint32_t(cus), uint32_t(L)), kBlock, 0, stream>>>(pc, dc, n, C(), DPacked{}, nullptr,
+                                                                              P<uint32_t>(p2_col), nullptr);
+      std::swap(pd_ptr, p2_ptr);
+      std::swap(pd_col, p2_col);
+      pend_nnz = p2n;
     }
-    pk_ready = pack_any;
-    if (pack_any) std::swap(pk_ent, pk2_ent);
+    if (!defer) {
+      // pend (which now holds delta) or delta itself goes into old
+      if (pend_nnz) {
+        GG_TRY(fold_into_old({P<unsigned long long>(pd_ptr), P<uint32_t>(pd_col)}, pend_nnz, nullptr, err));
+        pend_nnz = 0;
+      } else {
+        GG_TRY(fold_into_old({P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)}, dl_nnz, C(), err));
+      }
+    }
     GG_HIP(hipEventRecord(mg.b, stream));
     GG_HIP(hipGetLastError());
-    std::swap(old_ptr, o2_ptr);
-    std::swap(old_col, o2_col);
     std::swap(dl_ptr, nx_ptr);
     std::swap(dl_col, nx_col);
     std::swap(dpk_len, npk_len);
     std::swap(dpk_ent, npk_ent);
     dpk_ready = npk;
-    old_nnz = o2n;
     dl_nnz = N;
   }
+  if (pend_nnz) {  // the closure is old ∪ pend: rebuild old once
+    EventPair& mg = mk(ev_merge);
+    GG_HIP(hipEventRecord(mg.a, stream));
+    GG_TRY(fold_into_old({P<unsigned long long>(pd_ptr), P<uint32_t>(pd_col)}, pend_nnz, nullptr, err));
+    pend_nnz = 0;
+    GG_HIP(hipEventRecord(mg.b, stream));
+  }
+  release(pd_ptr);
+  release(pd_col);
+  release(p2_ptr);
+  release(p2_col);
   GG_HIP(hipStreamSynchronize(stream));
   const auto t1 = std::chrono::steady_clock::now();
   S.fixpoint_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -3142,8 +3243,10 @@ int engine_release(Engine* e) {
                    &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt,  &e->active, &e->tiles,
                    &e->fr_bits,
                    &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_len, &e->pk_ent, &e->pk2_ent,
-                   &e->dpk_len, &e->dpk_ent, &e->npk_len, &e->npk_ent, &e->mlist, &e->mcnt};
+                   &e->dpk_len, &e->dpk_ent, &e->npk_len, &e->npk_ent, &e->mlist, &e->mcnt,
+                   &e->pd_ptr,  &e->pd_col,  &e->p2_ptr,  &e->p2_col};
   for (DevBuf* b : all) e->release(*b);
+  e->pend_nnz = 0;
   e->trim_cache();  // the memory goes back to the device (other contexts may need it)
   e->has_closure = false;
   return GG_OK;

@@ -226,6 +226,22 @@ def test_gpu_unpacked_path_matches(built, monkeypatch):
     check_against_oracle(gen.alias(20_000, seed=12))
 
 
+@pytest.mark.parametrize("ratio", ["1", "0"])
+def test_gpu_deferred_merge_matches_oracle(built, monkeypatch, ratio):
+    """Rounds' deltas kept in the pending CSR and read beside old (ratio 1:
+    whenever pending + delta <= old, so most rounds defer) or merged into old
+    every round (ratio 0). Both equal the oracle, P/K/O/N counters included."""
+    monkeypatch.setenv("GG_DEFER_RATIO", ratio)
+    for name, g in [("alias n=20000", gen.alias(20_000, seed=21)),
+                    ("dataflow n=3000", gen.dataflow(3_000, seed=22)),
+                    ("alias-as-dataflow n=30000", gen.alias_as_dataflow(30_000, seed=23))]:
+        say("deferred merge ratio", ratio, name)
+        check_against_oracle(g)
+    say("deferred merge ratio", ratio, "unpacked alias n=20000")
+    monkeypatch.setenv("GG_NO_PACK", "1")
+    check_against_oracle(gen.alias(20_000, seed=24))
+
+
 def test_gpu_alias_1m_full_size(built):
     """The benchmark workload itself (alias generator, n = 1,000,000): per-label
     counts, checksum, round count and the P/K/O/N counters equal the oracle's."""
