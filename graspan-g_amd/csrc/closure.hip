@@ -84,23 +84,45 @@ struct DevGrammar {
   uint8_t slot_c[kMaxRules], slot_b[kMaxRules];
 };
 
-// Round counters, one device copy per context (zeroed by a fill kernel).
-struct Counters {
-  unsigned long long cursor;   // staging bump pointer (new edges + unused tails of per-wave blocks)
+// Statistics counters are added to by every block; updates of one address
+// from many blocks serialise, so blocks add to one of kSpread slots (by block
+// index), each on its own 128-byte line, and the host sums the slots.
+constexpr int kSpread = 64;
+struct alignas(128) CtrSlot {
   unsigned long long cand;     // K: candidates streamed
   unsigned long long pkeys;    // key elements scanned (binary rules)
   unsigned long long oreads;   // O: old/delta row elements read for the difference
   unsigned long long newe;     // N: new edges
   unsigned long long checksum; // closure checksum accumulator
+  unsigned long long xs[4];    // built with GG_EXPAND_STATS: k_expand cycles in seeding, key lists, list streams, whole rows
+  unsigned int misses;         // rows whose guessed window missed a candidate
+  unsigned int pad;
+};
+struct alignas(128) WorkIdx {
+  unsigned int v;  // one XCD's work index into the active list, on a line of its own
+};
+
+// Round counters, one device copy per context (zeroed by a fill kernel).
+// The fields from cand to xs are the sums of the slots, filled in on the host
+// when the counters are read back.
+struct Counters {
+  unsigned long long cursor;   // staging bump pointer (new edges + unused tails of per-wave blocks)
+  unsigned long long cand;
+  unsigned long long pkeys;
+  unsigned long long oreads;
+  unsigned long long newe;
+  unsigned long long checksum;
   unsigned int active;         // rows in the active list
   unsigned int work;           // work-stealing index
-  unsigned int work_xcd[8];    // per-XCD work indices into the active list
   unsigned int overflow;       // staging too small
-  unsigned int misses;         // rows whose guessed window missed a candidate
+  unsigned int misses;
   unsigned int label_flags[8]; // bit l: delta of label l is not empty
-  unsigned int pad;
-  unsigned long long xs[4];    // built with GG_EXPAND_STATS: k_expand cycles in seeding, key lists, list streams, whole rows
+  unsigned long long xs[4];
+  WorkIdx work_xcd[8];         // per-XCD work indices into the active list
+  CtrSlot slot[kSpread];
 };
+
+__device__ inline CtrSlot& ctr_slot(Counters* c) { return c->slot[blockIdx.x % kSpread]; }
 
 struct DCsr {
   const unsigned long long* ptr;
@@ -337,19 +359,35 @@ __global__ void k_all_active(uint32_t rows, uint32_t* active, Counters* ctr) {
 }
 
 // Active list of the initial dedup: rows with raw entries.
-__global__ void k_activate_raw(const unsigned long long* ptr, uint32_t rows, uint32_t* active, Counters* ctr) {
+__global__ __launch_bounds__(kBlock) void k_activate_raw(const unsigned long long* ptr, uint32_t rows,
+                                                         uint32_t* active, Counters* ctr) {
+  // active rows gather per wave and take their place in the list with one
+  // counter update per kBuf rows (updates of one address serialise)
+  constexpr uint32_t kBuf = 512;
+  __shared__ uint32_t s_buf[kWavesPerBlock][kBuf];
   const int lane = threadIdx.x & (kWave - 1);
+  uint32_t* buf = s_buf[threadIdx.x / kWave];
+  uint32_t nbuf = 0;  // uniform
+  auto flush = [&]() {
+    wave_sync();
+    uint32_t pos = 0;
+    if (lane == 0) pos = atomicAdd(&ctr->active, nbuf);
+    pos = uni_u32(pos);  // lane 0: every lane is active here
+    for (uint32_t i = uint32_t(lane); i < nbuf; i += kWave) active[pos + i] = buf[i];
+    nbuf = 0;
+    wave_sync();
+  };
   for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < rows;
        base += gridDim.x * blockDim.x) {
     uint32_t r = base + lane;
     bool act = r < rows && ptr[r + 1] > ptr[r];
     unsigned long long m = __ballot(act);
     if (!m) continue;
-    uint32_t pos = 0;
-    if (lane == 0) pos = atomicAdd(&ctr->active, uint32_t(__popcll(m)));
-    pos = uni_u32(pos);  // lane 0: the branch is uniform
-    if (act) active[pos + __popcll(m & ((1ull << lane) - 1))] = r;
+    if (nbuf + kWave > kBuf) flush();
+    if (act) buf[nbuf + __popcll(m & ((1ull << lane) - 1))] = r;
+    nbuf += uint32_t(__popcll(m));
   }
+  if (nbuf) flush();
 }
 
 // ---------------------------------------------------------------------------
@@ -439,8 +477,17 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
     }
     __syncthreads();  // s_m, s_cnt and s_base are rewritten by the next batch
   }
+  // one counter update per block, not per wave: updates of one address from
+  // every block serialise
   pk = wave_sum_u64(pk);
-  if (lane == 0 && pk) atomicAdd(&a.ctr->pkeys, pk);
+  __shared__ unsigned long long s_pk[kWavesPerBlock];
+  if (lane == 0) s_pk[wv] = pk;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) t += s_pk[w];
+    if (t) atomicAdd(&ctr_slot(a.ctr).pkeys, t);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1097,7 +1144,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
         const uint32_t sb = uint32_t((unsigned long long)nactive * seg / kXcds);
         const uint32_t se = uint32_t((unsigned long long)nactive * (seg + 1) / kXcds);
         uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(&a.ctr->work_xcd[seg], kGrab);
+        if (lane == 0) b = atomicAdd(&a.ctr->work_xcd[seg].v, kGrab);
         b = sb + uni_u32(b);
         if (b < se) {
           next = b;
@@ -1237,7 +1284,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
         if (missed) break;
       }
       if (!missed) break;
-      if (lane == 0) atomicAdd(&a.ctr->misses, 1u);
+      if (lane == 0) atomicAdd(&ctr_slot(a.ctr).misses, 1u);
       krow = 0;
       cnt = 0;
       pos = 0;
@@ -1256,11 +1303,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
     }
   }
   if (lane == 0) {
-    if (kc) atomicAdd(&a.ctr->cand, kc);
-    if (orr) atomicAdd(&a.ctr->oreads, orr);
-    if (ne) atomicAdd(&a.ctr->newe, ne);
+    CtrSlot& cs = ctr_slot(a.ctr);
+    if (kc) atomicAdd(&cs.cand, kc);
+    if (orr) atomicAdd(&cs.oreads, orr);
+    if (ne) atomicAdd(&cs.newe, ne);
 #ifdef GG_EXPAND_STATS
-    for (int i = 0; i < 4; ++i) atomicAdd(&a.ctr->xs[i], w.xs[i]);
+    for (int i = 0; i < 4; ++i) atomicAdd(&cs.xs[i], w.xs[i]);
 #endif
   }
 }
@@ -1427,11 +1475,25 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
                                                   uint32_t* mcnt) {
   // ctr == nullptr: merge every row (no label is skipped)
   __shared__ unsigned long long s_tmask[kWavesPerBlock][4], s_sb[kWavesPerBlock][kWave], s_db[kWavesPerBlock][kWave];
+  // listed rows gather per wave and take their place in mlist with one
+  // counter update per kListBuf rows (updates of one address serialise)
+  constexpr uint32_t kListBuf = 512;
+  __shared__ uint32_t s_list[kWavesPerBlock][kListBuf];
   const int lane = threadIdx.x & (kWave - 1);
   const int wi = threadIdx.x / kWave;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
   const uint32_t nblk = (rows + kWave - 1) / kWave;
+  uint32_t nlist = 0;  // rows in s_list (uniform)
+  auto flush = [&]() {
+    wave_sync();
+    uint32_t pos = 0;
+    if (lane == 0) pos = atomicAdd(mcnt, nlist);
+    pos = uni_u32(pos);  // lane 0: every lane is active here
+    for (uint32_t i = uint32_t(lane); i < nlist; i += kWave) mlist[pos + i] = s_list[wi][i];
+    nlist = 0;
+    wave_sync();
+  };
   for (uint32_t blk = wave; blk < nblk; blk += nwaves) {
     const uint32_t r = blk * kWave + lane;
     const uint32_t l = r / n;
@@ -1464,12 +1526,12 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
     const bool more = live && (!same || (out_pk && !old_pk && oe > ob && packed_label(plab, l)));
     const unsigned long long m = __ballot(more);
     if (m) {
-      uint32_t pos = 0;
-      if (lane == 0) pos = atomicAdd(mcnt, uint32_t(__popcll(m)));
-      pos = uni_u32(pos);  // lane 0: the branch is uniform
-      if (more) mlist[pos + __popcll(m & ((1ull << lane) - 1))] = r;
+      if (nlist + kWave > kListBuf) flush();
+      if (more) s_list[wi][nlist + __popcll(m & ((1ull << lane) - 1))] = r;
+      nlist += uint32_t(__popcll(m));
     }
   }
+  if (nlist) flush();
 }
 
 // The rows k_merge listed, one wave per row. Old and delta are disjoint and
@@ -1620,7 +1682,7 @@ __global__ __launch_bounds__(kBlock) void k_checksum(DCsr c, uint32_t rows, uint
     for (unsigned long long i = b + lane; i < e; i += kWave) s += triple_hash(u, c.col[i], l + 1);
   }
   s = wave_sum_u64(s);
-  if (lane == 0 && s) atomicAdd(&ctr->checksum, s);
+  if (lane == 0 && s) atomicAdd(&ctr_slot(ctr).checksum, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1904,6 +1966,19 @@ class Engine {
   int read_ctr(std::string& err) {
     GG_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, stream));
     GG_HIP(hipStreamSynchronize(stream));
+    Counters& c = *h_ctr;
+    c.cand = c.pkeys = c.oreads = c.newe = c.checksum = 0;
+    c.misses = 0;
+    for (auto& x : c.xs) x = 0;
+    for (const CtrSlot& q : c.slot) {
+      c.cand += q.cand;
+      c.pkeys += q.pkeys;
+      c.oreads += q.oreads;
+      c.newe += q.newe;
+      c.checksum += q.checksum;
+      c.misses += q.misses;
+      for (int i = 0; i < 4; ++i) c.xs[i] += q.xs[i];
+    }
     return GG_OK;
   }
 
