@@ -738,9 +738,21 @@ __device__ inline void expand_stream(const T* __restrict__ src, unsigned long lo
   if (len) w.lbase[rank] = reinterpret_cast<unsigned long long>(src) + (lb - start) * sizeof(T);
   if (lane < kUnroll) w.tmask[lane] = 0;
   wave_sync();
-  for (unsigned long long t0 = 0; t0 < total; t0 += kTile) {
-    const unsigned long long rem = total - t0;
-    const int lim = rem < kTile ? int(rem) : int(kTile);
+  // full tiles first, with no bounds test, then the partial last tile
+  const unsigned long long full = total / kTile * kTile;
+  for (unsigned long long t0 = 0; t0 < full; t0 += kTile) {
+    const unsigned long long tb = (t0 + uint32_t(lane)) * sizeof(T);
+    T x[kUnroll];
+    stream_tile<kUnroll>(start, len, t0, w.tmask, lane, [&](int q, uint32_t rk) {
+      x[q] = *reinterpret_cast<const __attribute__((address_space(1))) T*>(w.lbase[rk] + tb +
+                                                                            uint32_t(q * kWave) * sizeof(T));
+    });
+#pragma unroll
+    for (int q = 0; q < kUnroll; ++q) elem(x[q]);
+  }
+  if (full < total) {
+    const unsigned long long t0 = full;
+    const int lim = int(total - full);
     const unsigned long long tb = (t0 + uint32_t(lane)) * sizeof(T);
     T x[kUnroll];
     stream_tile<kUnroll>(start, len, t0, w.tmask, lane, [&](int q, uint32_t rk) {
@@ -752,8 +764,8 @@ __device__ inline void expand_stream(const T* __restrict__ src, unsigned long lo
 #pragma unroll
     for (int q = 0; q < kUnroll; ++q)
       if (lane < lim - q * kWave) elem(x[q]);
-    wave_sync();
   }
+  wave_sync();
 }
 
 // Visits every candidate list of row (A,u): calls visit(col, lb, len) once
