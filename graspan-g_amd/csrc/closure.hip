@@ -510,9 +510,11 @@ __device__ inline uint32_t stream_rank(uint32_t len) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(nonempty >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(nonempty), 0u));
 }
 
+// nq (uniform): the q slices of the tile that hold elements; the rest are
+// skipped.
 template <int U, class F>
 __device__ inline void stream_tile(unsigned long long start, uint32_t len, unsigned long long t0,
-                                   unsigned long long* tmask, int lane, F&& f) {
+                                   unsigned long long* tmask, int lane, F&& f, int nq = U) {
   const unsigned long long pos = start - t0;
   if (len && pos < uint64_t(U) * kWave)
     __hip_atomic_fetch_or(&tmask[pos / kWave], 1ull << (pos % kWave), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -522,10 +524,12 @@ __device__ inline void stream_tile(unsigned long long start, uint32_t len, unsig
   wave_sync();
 #pragma unroll
   for (int q = 0; q < U; ++q) {
-    const unsigned long long m = uni_u64(tmask[q]);
-    const unsigned long long up = m >> 1;  // mbcnt counts bits below the lane: bits 1..l of m, plus bit 0
-    f(q, __builtin_amdgcn_mbcnt_hi(uint32_t(up >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(up), c + uint32_t(m & 1))));
-    c += uint32_t(__popcll(m));
+    if (q < nq) {
+      const unsigned long long m = uni_u64(tmask[q]);
+      const unsigned long long up = m >> 1;  // mbcnt counts bits below the lane: bits 1..l of m, plus bit 0
+      f(q, __builtin_amdgcn_mbcnt_hi(uint32_t(up >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(up), c + uint32_t(m & 1))));
+      c += uint32_t(__popcll(m));
+    }
   }
   wave_sync();
   if (lane < U) tmask[lane] = 0;
@@ -753,17 +757,21 @@ __device__ inline void expand_stream(const T* __restrict__ src, unsigned long lo
   if (full < total) {
     const unsigned long long t0 = full;
     const int lim = int(total - full);
+    const int nq = (lim + kWave - 1) / kWave;
     const unsigned long long tb = (t0 + uint32_t(lane)) * sizeof(T);
     T x[kUnroll];
-    stream_tile<kUnroll>(start, len, t0, w.tmask, lane, [&](int q, uint32_t rk) {
-      x[q] = 0;
-      if (lane < lim - q * kWave)  // lane against a scalar bound: no per-q lane constants
-        x[q] = *reinterpret_cast<const __attribute__((address_space(1))) T*>(w.lbase[rk] + tb +
-                                                                              uint32_t(q * kWave) * sizeof(T));
-    });
+    stream_tile<kUnroll>(
+        start, len, t0, w.tmask, lane,
+        [&](int q, uint32_t rk) {
+          x[q] = 0;
+          if (lane < lim - q * kWave)  // lane against a scalar bound: no per-q lane constants
+            x[q] = *reinterpret_cast<const __attribute__((address_space(1))) T*>(w.lbase[rk] + tb +
+                                                                                  uint32_t(q * kWave) * sizeof(T));
+        },
+        nq);
 #pragma unroll
     for (int q = 0; q < kUnroll; ++q)
-      if (lane < lim - q * kWave) elem(x[q]);
+      if (q < nq && lane < lim - q * kWave) elem(x[q]);
   }
   wave_sync();
 }
@@ -1435,16 +1443,20 @@ __device__ inline void flat_copy(const T* __restrict__ src, unsigned long long s
     const unsigned long long t = t0 + uint32_t(lane);
     T x[kU];
     unsigned long long d[kU];
-    stream_tile<kU>(start, len, t0, tmask, lane, [&](int q, uint32_t rk) {
-      d[q] = ~0ull;
-      if (lane < lim - q * kWave) {
-        x[q] = src[sbase[rk] + t + uint32_t(q * kWave)];
-        d[q] = dbase[rk] + t + uint32_t(q * kWave);
-      }
-    });
+    const int nq = (lim + kWave - 1) / kWave;
+    stream_tile<kU>(
+        start, len, t0, tmask, lane,
+        [&](int q, uint32_t rk) {
+          d[q] = ~0ull;
+          if (lane < lim - q * kWave) {
+            x[q] = src[sbase[rk] + t + uint32_t(q * kWave)];
+            d[q] = dbase[rk] + t + uint32_t(q * kWave);
+          }
+        },
+        nq);
 #pragma unroll
     for (int q = 0; q < kU; ++q)
-      if (d[q] != ~0ull) dst[d[q]] = x[q];
+      if (q < nq && d[q] != ~0ull) dst[d[q]] = x[q];
   }
   wave_sync();
 }
