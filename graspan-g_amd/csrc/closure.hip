@@ -57,7 +57,7 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kChunkBits = 16384;  // LDS window per wave: 2 KiB seen + 2 KiB new
 constexpr uint32_t kChunkWords = kChunkBits / 32;
 constexpr int kUnroll = 8;              // loads in flight per lane in the candidate stream
-constexpr uint32_t kGrab = 16;          // active rows taken per work-counter atomic
+constexpr uint32_t kGrab = 4;           // active rows taken per work-counter atomic
 constexpr uint32_t kXcds = 8;           // MI355X: 8 XCDs, each with its own L2
 constexpr int kExpandWaves = 6;         // k_expand occupancy target (waves per SIMD)
 constexpr uint64_t kDeferRatio = 128;   // pending edges kept apart while under 1/128 of old
