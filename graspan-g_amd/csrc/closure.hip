@@ -186,6 +186,11 @@ __device__ inline uint32_t wave_incl_scan(uint32_t x, int) {
 }
 
 __device__ inline uint32_t lane63_u32(uint32_t x) { return uint32_t(__builtin_amdgcn_readlane(int(x), 63)); }
+// lane k's value (k uniform), in scalar registers
+__device__ inline unsigned long long lane_u64(unsigned long long x, uint32_t k) {
+  return (unsigned long long)uint32_t(__builtin_amdgcn_readlane(int(uint32_t(x)), int(k))) |
+         ((unsigned long long)uint32_t(__builtin_amdgcn_readlane(int(uint32_t(x >> 32)), int(k))) << 32);
+}
 __device__ inline unsigned long long lane63_u64(unsigned long long x) {
   return (unsigned long long)lane63_u32(uint32_t(x)) | ((unsigned long long)lane63_u32(uint32_t(x >> 32)) << 32);
 }
@@ -1576,103 +1581,121 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
   const uint32_t count = *mcnt;
-  for (uint32_t i = wave; i < count; i += nwaves) {
-    const uint32_t rr = uni_u32(mlist[i]);
-    const uint32_t lr = rr / n;
-    const unsigned long long rob = old.ptr[rr], roe = old.ptr[rr + 1];
-    const unsigned long long rdb = dl.ptr[rr], rde = dl.ptr[rr + 1];
-    uint32_t* out = out_col + rob + rdb;
-    const uint32_t ol = uint32_t(roe - rob), dln = uint32_t(rde - rdb);
-    const bool pkr = out_pk && packed_label(plab, lr);
-    if (dln == 0 && (!pkr || old_pk)) {
-      // a long unchanged row: ids and packed entries move as they are
-      wave_copy(old.col + rob, out, ol, lane);
-      if (pkr) wave_copy(old_pk + rob, out_pk + rob + rdb, pk_len[rr], lane);
-      continue;
-    }
-    uint32_t lo = 0xFFFFFFFFu, hi = 0;
-    if (ol) {
-      lo = old.col[rob];
-      hi = old.col[roe - 1];
-    }
-    if (dln) {
-      lo = min(lo, dl.col[rdb]);
-      hi = max(hi, dl.col[rde - 1]);
-    }
-    lo &= ~31u;
-    if (hi - lo < kMergeBits) {
-      const uint32_t words = ((hi - lo) >> 5) + 1;
-      for (uint32_t t = lane; t < words; t += kWave) bits[t] = 0;
-      wave_sync();
-      for (uint32_t t = lane; t < ol; t += kWave) {
-        const uint32_t off = old.col[rob + t] - lo;
-        __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  // rows go to the waves in batches (up to 64; fewer when the list is short,
+  // so that every wave gets some). A batch's row bounds and first and last
+  // ids are loaded at once, one row per lane, and the row loop reads them
+  // with readlane instead of fetching them row after row.
+  const uint32_t per = max(1u, min(uint32_t(kWave), count / nwaves));
+  for (uint32_t i0 = wave * per; i0 < count; i0 += nwaves * per) {
+    const uint32_t nb = min(per, count - i0);
+    uint32_t rr_l = 0, lo_l = 0xFFFFFFFFu, hi_l = 0;
+    unsigned long long rob_l = 0, roe_l = 0, rdb_l = 0, rde_l = 0;
+    if (uint32_t(lane) < nb) {
+      rr_l = mlist[i0 + lane];
+      rob_l = old.ptr[rr_l];
+      roe_l = old.ptr[rr_l + 1];
+      rdb_l = dl.ptr[rr_l];
+      rde_l = dl.ptr[rr_l + 1];
+      if (roe_l > rob_l) {
+        lo_l = old.col[rob_l];
+        hi_l = old.col[roe_l - 1];
       }
-      for (uint32_t t = lane; t < dln; t += kWave) {
-        const uint32_t off = dl.col[rdb + t] - lo;
-        __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (rde_l > rdb_l) {
+        lo_l = min(lo_l, dl.col[rdb_l]);
+        hi_l = max(hi_l, dl.col[rde_l - 1]);
       }
-      wave_sync();
-      const unsigned long long below = (1ull << lane) - 1;
-      // dense rows (4+ ids per word on average): two words per wave step,
-      // one bit per lane, so each step stores its ids contiguously; sparse
-      // rows: one word per lane, its bits stored one by one
-      const bool dense = uint64_t(ol) + dln >= 4ull * words;
-      unsigned long long o = 0;
-      uint32_t pe = 0;
-      if (dense) {
-        for (uint32_t j = 0; j < words; j += 2) {
-          const uint32_t wj = j + uint32_t(lane >> 5);
-          const uint32_t wd = wj < words ? bits[wj] : 0u;
-          const bool set = (wd >> (lane & 31)) & 1u;
-          const unsigned long long m = __ballot(set);
-          if (set) out[o + __popcll(m & below)] = lo + wj * 32 + uint32_t(lane & 31);
-          o += uint32_t(__popcll(m));
+    }
+    for (uint32_t k = 0; k < nb; ++k) {
+      const uint32_t rr = uint32_t(__builtin_amdgcn_readlane(int(rr_l), int(k)));
+      const uint32_t lr = rr / n;
+      const unsigned long long rob = lane_u64(rob_l, k), roe = lane_u64(roe_l, k);
+      const unsigned long long rdb = lane_u64(rdb_l, k), rde = lane_u64(rde_l, k);
+      uint32_t* out = out_col + rob + rdb;
+      const uint32_t ol = uint32_t(roe - rob), dln = uint32_t(rde - rdb);
+      const bool pkr = out_pk && packed_label(plab, lr);
+      if (dln == 0 && (!pkr || old_pk)) {
+        // a long unchanged row: ids and packed entries move as they are
+        wave_copy(old.col + rob, out, ol, lane);
+        if (pkr) wave_copy(old_pk + rob, out_pk + rob + rdb, pk_len[rr], lane);
+        continue;
+      }
+      uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(lo_l), int(k)));
+      const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(hi_l), int(k)));
+      lo &= ~31u;
+      if (hi - lo < kMergeBits) {
+        const uint32_t words = ((hi - lo) >> 5) + 1;
+        for (uint32_t t = lane; t < words; t += kWave) bits[t] = 0;
+        wave_sync();
+        for (uint32_t t = lane; t < ol; t += kWave) {
+          const uint32_t off = old.col[rob + t] - lo;
+          __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-      }
-      for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
-        const uint32_t wd = w0 + lane < words ? bits[w0 + lane] : 0u;
-        if (!dense) {
-          const uint32_t pc = __popc(wd);
-          const uint32_t incl = wave_incl_scan(pc, lane);
-          uint32_t x = wd;
-          unsigned long long q = o + (incl - pc);
-          while (x) {
-            const int b = __ffs(x) - 1;
-            x &= x - 1;
-            out[q++] = lo + (w0 + uint32_t(lane)) * 32 + uint32_t(b);
+        for (uint32_t t = lane; t < dln; t += kWave) {
+          const uint32_t off = dl.col[rdb + t] - lo;
+          __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        wave_sync();
+        const unsigned long long below = (1ull << lane) - 1;
+        // dense rows (4+ ids per word on average): two words per wave step,
+        // one bit per lane, so each step stores its ids contiguously; sparse
+        // rows: one word per lane, its bits stored one by one
+        const bool dense = uint64_t(ol) + dln >= 4ull * words;
+        unsigned long long o = 0;
+        uint32_t pe = 0;
+        if (dense) {
+          for (uint32_t j = 0; j < words; j += 2) {
+            const uint32_t wj = j + uint32_t(lane >> 5);
+            const uint32_t wd = wj < words ? bits[wj] : 0u;
+            const bool set = (wd >> (lane & 31)) & 1u;
+            const unsigned long long m = __ballot(set);
+            if (set) out[o + __popcll(m & below)] = lo + wj * 32 + uint32_t(lane & 31);
+            o += uint32_t(__popcll(m));
           }
-          o += lane63_u32(incl);
         }
-        if (pkr) {
-          const unsigned long long nz = __ballot(wd != 0);
-          if (wd)
-            out_pk[rob + rdb + pe + __popcll(nz & below)] =
-                ((unsigned long long)wd << 32) | ((lo >> 5) + w0 + uint32_t(lane));
-          pe += uint32_t(__popcll(nz));
+        for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
+          const uint32_t wd = w0 + lane < words ? bits[w0 + lane] : 0u;
+          if (!dense) {
+            const uint32_t pc = __popc(wd);
+            const uint32_t incl = wave_incl_scan(pc, lane);
+            uint32_t x = wd;
+            unsigned long long q = o + (incl - pc);
+            while (x) {
+              const int b = __ffs(x) - 1;
+              x &= x - 1;
+              out[q++] = lo + (w0 + uint32_t(lane)) * 32 + uint32_t(b);
+            }
+            o += lane63_u32(incl);
+          }
+          if (pkr) {
+            const unsigned long long nz = __ballot(wd != 0);
+            if (wd)
+              out_pk[rob + rdb + pe + __popcll(nz & below)] =
+                  ((unsigned long long)wd << 32) | ((lo >> 5) + w0 + uint32_t(lane));
+            pe += uint32_t(__popcll(nz));
+          }
+        }
+        if (pkr && lane == 0) pk_len[rr] = pe;
+        wave_sync();
+        continue;
+      }
+      if (dln == 0) {
+        for (uint32_t t = lane; t < ol; t += kWave) out[t] = old.col[rob + t];
+      } else if (ol == 0) {
+        for (uint32_t t = lane; t < dln; t += kWave) out[t] = dl.col[rdb + t];
+      } else {
+        for (uint32_t t = lane; t < ol; t += kWave) {
+          const uint32_t x = old.col[rob + t];
+          out[t + lower_rank(dl.col, rdb, rde, x)] = x;
+        }
+        for (uint32_t t = lane; t < dln; t += kWave) {
+          const uint32_t x = dl.col[rdb + t];
+          out[t + lower_rank(old.col, rob, roe, x)] = x;
         }
       }
-      if (pkr && lane == 0) pk_len[rr] = pe;
-      wave_sync();
-      continue;
-    }
-    if (dln == 0) {
-      for (uint32_t t = lane; t < ol; t += kWave) out[t] = old.col[rob + t];
-    } else if (ol == 0) {
-      for (uint32_t t = lane; t < dln; t += kWave) out[t] = dl.col[rdb + t];
-    } else {
-      for (uint32_t t = lane; t < ol; t += kWave) {
-        const uint32_t x = old.col[rob + t];
-        out[t + lower_rank(dl.col, rdb, rde, x)] = x;
+      if (pkr) {
+        const uint32_t c = pack_union(old.col, rob, roe, dl.col, rdb, rde, out_pk + rob + rdb, win, lane);
+        if (lane == 0) pk_len[rr] = c;
       }
-      for (uint32_t t = lane; t < dln; t += kWave) {
-        const uint32_t x = dl.col[rdb + t];
-        out[t + lower_rank(old.col, rob, roe, x)] = x;
-      }
-    }
-    if (pkr) {
-      const uint32_t c = pack_union(old.col, rob, roe, dl.col, rdb, rde, out_pk + rob + rdb, win, lane);
-      if (lane == 0) pk_len[rr] = c;
     }
   }
 }
