@@ -1341,6 +1341,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
 __device__ inline uint32_t pack_union(const uint32_t* a, unsigned long long ia, unsigned long long ea,
                                       const uint32_t* b, unsigned long long ib, unsigned long long eb,
                                       unsigned long long* ent, uint32_t* win, int lane);
+template <class T>
+__device__ inline void flat_copy(const T* __restrict__ src, unsigned long long sb, T* __restrict__ dst,
+                                 unsigned long long db, uint32_t len, unsigned long long* tmask,
+                                 unsigned long long* sbase, unsigned long long* dbase, int lane);
 
 // Copies each active row's staged output into the delta' CSR.
 __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* active, const Counters* ctr, const uint32_t* seg_cnt,
@@ -1350,18 +1354,38 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* active, const
   // rows of packed labels are also packed, at the row's own offset (out_pk
   // null: no packing)
   __shared__ uint32_t s_win[kWavesPerBlock][kWave];
+  __shared__ unsigned long long s_tmask[kWavesPerBlock][4], s_sb[kWavesPerBlock][kWave], s_db[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
-  uint32_t* win = s_win[threadIdx.x / kWave];
+  const int wi = threadIdx.x / kWave;
+  uint32_t* win = s_win[wi];
   const uint32_t nactive = ctr->active;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  for (uint32_t i = wave; i < nactive; i += nwaves) {
-    const uint32_t r = active[i];
-    const uint32_t c = seg_cnt[r];
-    if (!c) continue;
-    const unsigned long long s = seg_off[r], d = out_ptr[r];
-    for (uint32_t t = lane; t < c; t += kWave) out_col[d + t] = staging[s + t];
-    if (out_pk && packed_label(plab, r / n)) {
+  // rows go to the waves in batches (up to 64; fewer when the list is short);
+  // a batch's row descriptors are loaded at once, one row per lane, and its
+  // ids move in one flattened copy
+  const uint32_t per = max(1u, min(uint32_t(kWave), nactive / nwaves));
+  for (uint32_t i0 = wave * per; i0 < nactive; i0 += nwaves * per) {
+    const uint32_t nb = min(per, nactive - i0);
+    uint32_t r_l = 0, c_l = 0;
+    unsigned long long s_l = 0, d_l = 0;
+    if (uint32_t(lane) < nb) {
+      r_l = active[i0 + lane];
+      c_l = seg_cnt[r_l];
+      if (c_l) {
+        s_l = seg_off[r_l];
+        d_l = out_ptr[r_l];
+      }
+    }
+    flat_copy(staging, s_l, out_col, d_l, c_l, s_tmask[wi], s_sb[wi], s_db[wi], lane);
+    if (!out_pk) continue;
+    unsigned long long todo = __ballot(c_l != 0 && packed_label(plab, r_l / n));
+    while (todo) {
+      const int k = __ffsll(todo) - 1;
+      todo &= todo - 1;
+      const uint32_t r = uint32_t(__builtin_amdgcn_readlane(int(r_l), k));
+      const uint32_t c = uint32_t(__builtin_amdgcn_readlane(int(c_l), k));
+      const unsigned long long s = lane_u64(s_l, uint32_t(k)), d = lane_u64(d_l, uint32_t(k));
       const uint32_t pc = pack_union(staging, s, s + c, staging, s, s, out_pk + d, win, lane);
       if (lane == 0) out_pklen[r] = pc;
     }
