@@ -543,8 +543,9 @@ __device__ inline void stream_tile(unsigned long long start, uint32_t len, unsig
 
 // Calls f(k, pos) for every element position pos of the lanes' lists
 // [lb, lb+len) (k = the lane that owns the list), as one flattened stream (see
-// stream_tile), one element per lane in flight. tab: 64-entry LDS table of
-// the wave; tmask: one zeroed LDS word of the wave.
+// stream_tile), kPosUnroll elements per lane per tile. tab: 64-entry LDS
+// table of the wave; tmask: kPosUnroll LDS words of the wave.
+constexpr int kPosUnroll = 4;
 template <class F>
 __device__ inline void flat_positions(unsigned long long lb, uint32_t len, unsigned long long* tab,
                                       unsigned long long* tmask, int lane, F&& f) {
@@ -556,16 +557,21 @@ __device__ inline void flat_positions(unsigned long long lb, uint32_t len, unsig
   wave_sync();
   // owner lane in the top 6 bits; position of element t: low bits + t
   if (len) tab[rank] = ((lb - start) & ((1ull << 58) - 1)) | (uint64_t(lane) << 58);
-  if (lane == 0) tmask[0] = 0;
+  if (lane < kPosUnroll) tmask[lane] = 0;
   wave_sync();
-  for (unsigned long long t0 = 0; t0 < total; t0 += kWave) {
+  for (unsigned long long t0 = 0; t0 < total; t0 += kPosUnroll * kWave) {
     const unsigned long long rem = total - t0;
-    stream_tile<1>(start, len, t0, tmask, lane, [&](int, uint32_t rk) {
-      if (uint64_t(lane) < rem) {
-        const unsigned long long e = tab[rk];
-        f(int(e >> 58), ((e + t0 + uint32_t(lane)) & ((1ull << 58) - 1)));
-      }
-    });
+    const int lim = rem < kPosUnroll * kWave ? int(rem) : kPosUnroll * kWave;
+    const int nq = (lim + kWave - 1) / kWave;
+    stream_tile<kPosUnroll>(
+        start, len, t0, tmask, lane,
+        [&](int q, uint32_t rk) {
+          if (lane < lim - q * kWave) {
+            const unsigned long long e = tab[rk];
+            f(int(e >> 58), ((e + t0 + uint32_t(q * kWave + lane)) & ((1ull << 58) - 1)));
+          }
+        },
+        nq);
   }
 }
 
@@ -606,7 +612,7 @@ template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr old, DCsr pend, DPacked pk,
                                                   const unsigned long long* bits, uint32_t words, uint32_t* cnt,
                                                   const unsigned long long* ptr, uint32_t* col) {
-  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][1],
+  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][kPosUnroll],
       s_out[kWavesPerBlock][kWave];
   __shared__ uint32_t s_cnt[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
