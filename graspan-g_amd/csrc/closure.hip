@@ -1618,10 +1618,12 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
   const uint32_t per = max(1u, min(uint32_t(kWave), count / nwaves));
   for (uint32_t i0 = wave * per; i0 < count; i0 += nwaves * per) {
     const uint32_t nb = min(per, count - i0);
-    uint32_t rr_l = 0, lo_l = 0xFFFFFFFFu, hi_l = 0;
+    uint32_t rr_l = 0, lo_l = 0xFFFFFFFFu, hi_l = 0, pl_l = 0;
     unsigned long long rob_l = 0, roe_l = 0, rdb_l = 0, rde_l = 0;
     if (uint32_t(lane) < nb) {
       rr_l = mlist[i0 + lane];
+      // packed entries of the old row (packed labels): the merge reads those
+      if (old_pk && out_pk && packed_label(plab, rr_l / n)) pl_l = pk_len[rr_l];
       rob_l = old.ptr[rr_l];
       roe_l = old.ptr[rr_l + 1];
       rdb_l = dl.ptr[rr_l];
@@ -1646,7 +1648,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
       if (dln == 0 && (!pkr || old_pk)) {
         // a long unchanged row: ids and packed entries move as they are
         wave_copy(old.col + rob, out, ol, lane);
-        if (pkr) wave_copy(old_pk + rob, out_pk + rob + rdb, pk_len[rr], lane);
+        if (pkr) wave_copy(old_pk + rob, out_pk + rob + rdb, uint32_t(__builtin_amdgcn_readlane(int(pl_l), int(k))), lane);
         continue;
       }
       uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(lo_l), int(k)));
@@ -1656,9 +1658,19 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
         const uint32_t words = ((hi - lo) >> 5) + 1;
         for (uint32_t t = lane; t < words; t += kWave) bits[t] = 0;
         wave_sync();
-        for (uint32_t t = lane; t < ol; t += kWave) {
-          const uint32_t off = old.col[rob + t] - lo;
-          __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (pkr && old_pk) {
+          // the old row word by word: 32 ids per entry
+          const uint32_t opl = uint32_t(__builtin_amdgcn_readlane(int(pl_l), int(k)));
+          for (uint32_t t = lane; t < opl; t += kWave) {
+            const unsigned long long e = old_pk[rob + t];
+            __hip_atomic_fetch_or(&bits[uint32_t(e) - (lo >> 5)], uint32_t(e >> 32), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        } else {
+          for (uint32_t t = lane; t < ol; t += kWave) {
+            const uint32_t off = old.col[rob + t] - lo;
+            __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
         }
         for (uint32_t t = lane; t < dln; t += kWave) {
           const uint32_t off = dl.col[rdb + t] - lo;
