@@ -20,17 +20,19 @@
 //   delta'(A,u) = cand(A,u) - old_A(u) - delta_A(u);  old' = old U delta.
 //   Kernels: k_activate (which rows can receive candidates) -> k_expand (join
 //   + dedup + difference, fused, output sorted) -> one host read of the round
-//   counters -> k_scan_* + k_gather (compact delta') -> k_merge (old U delta).
+//   counters -> k_scan_* + k_gather (compact delta') -> k_merge (old U delta,
+//   or pending U delta while delta is small next to old: the joins then read
+//   old U pending).
 //
 // k_expand is the hot kernel. One wave64 owns one (A,u) row: it seeds an LDS
-// bitmap over the row's candidate window with old_A(u) U delta_A(u), streams
-// every candidate list through it with LDS atomicOr (a candidate whose bit
-// was clear is new and also sets a bit in a second "new" bitmap), then scans
-// the new bitmap to emit the new edges already in ascending order.
-// Candidates never go to HBM. Lists of >= 64 elements are read by the whole
-// wave (coalesced); shorter ones are packed across lanes with a wave prefix
-// sum and an LDS binary search. Windows wider than kChunkBits are processed
-// in chunks (count pass, then write pass).
+// bitmap (`seen`) over the row's candidate window with old_A(u) U delta_A(u),
+// ORs every candidate into a second bitmap (`fresh`) with no-return LDS
+// atomics, and writes fresh & ~seen out: the new edges, already in
+// ascending order. Candidates never go to HBM. The candidate lists of a
+// batch are flattened into one coalesced element stream; an element's list
+// is found by an mbcnt rank over a per-tile bitmap of list starts
+// (stream_tile). Windows wider than kChunkBits are processed in chunks
+// (count pass, then write pass).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
