@@ -118,15 +118,25 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
   gg_stats total{};
   b->groups = 0;
   const size_t G = b->graphs.size();
+  // Graphs of one grammar go into the same launch groups: a group's rows are
+  // its vertices times all its labels, and every round passes over them all,
+  // so mixing grammars would give each graph the other grammars' empty rows.
+  std::vector<size_t> ord(G);
+  for (size_t k = 0; k < G; ++k) ord[k] = k;
+  std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+    const auto& gx = b->graphs[x];
+    const auto& gy = b->graphs[y];
+    return std::make_pair(gx.L, gx.rules) < std::make_pair(gy.L, gy.rules);
+  });
   size_t i = 0;
   while (i < G) {
-    // one launch group: graphs i..j-1
+    // one launch group: graphs ord[i..j-1]
     std::map<std::pair<int, std::vector<int32_t>>, uint32_t> grammar_base;
     uint32_t labels = 0;
     uint64_t verts = 0;
     size_t j = i;
     for (; j < G; ++j) {
-      const auto& g = b->graphs[j];
+      const auto& g = b->graphs[ord[j]];
       auto key = std::make_pair(g.L, g.rules);
       const bool fresh = grammar_base.find(key) == grammar_base.end();
       const uint32_t nl = labels + (fresh ? uint32_t(g.L) : 0u);
@@ -134,7 +144,7 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
       const uint64_t lim = nl ? (0xFFFFFFFEull / nl) : nv;
       const bool over = nl > uint32_t(gg::kMaxLabels) || nv > lim || (max_vertices && nv > max_vertices);
       if (over && j > i) break;
-      if (over) return fail(b, GG_E_UNSUPPORTED, "graph " + std::to_string(j) + " alone exceeds the launch limits");
+      if (over) return fail(b, GG_E_UNSUPPORTED, "graph " + std::to_string(ord[j]) + " alone exceeds the launch limits");
       if (fresh) {
         grammar_base[key] = labels;
         labels = nl;
@@ -160,7 +170,7 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
     sg.n = uint32_t(verts);
     std::vector<uint32_t> v0(1, 0), l0, lg;
     for (size_t k = i; k < j; ++k) {
-      const auto& g = b->graphs[k];
+      const auto& g = b->graphs[ord[k]];
       const uint32_t base = v0.back(), lb = grammar_base[std::make_pair(g.L, g.rules)];
       for (size_t e2 = 0; e2 < g.src.size(); ++e2) {
         sg.src.push_back(g.src[e2] + base);
@@ -188,7 +198,7 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
     if ((s = gg::engine_slices(e, v0, l0, lg, counts, sums, b->err)) != GG_OK) return s;
     size_t c = 0;
     for (size_t k = i; k < j; ++k) {
-      auto& g = b->graphs[k];
+      auto& g = b->graphs[ord[k]];
       g.counts.assign(counts.begin() + long(c), counts.begin() + long(c + size_t(g.L)));
       c += size_t(g.L);
       g.total = 0;
