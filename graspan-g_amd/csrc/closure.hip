@@ -500,6 +500,25 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
 // ---------------------------------------------------------------------------
 // frontier pair lists for old_B x delta_C and the transpose of old_B
 
+// Inclusive prefix of the lanes' list lengths and its total, for the
+// flattened streams below. When no list reaches 2^26 elements the 64-lane sum
+// fits 32 bits and the scan runs on 32-bit DPP steps (half the VALU work of
+// the 64-bit scan); a wave with no list at all returns total 0 at once.
+__device__ inline unsigned long long stream_prefix(uint32_t len, unsigned long long& total) {
+  if (__ballot(len != 0) == 0) {
+    total = 0;
+    return 0;
+  }
+  if (__ballot(len >= (1u << 26)) == 0) {
+    const uint32_t incl = wave_incl_scan(len, 0);
+    total = lane63_u32(incl);
+    return incl;
+  }
+  const unsigned long long incl = wave_incl_scan_u64(len);
+  total = lane63_u64(incl);
+  return incl;
+}
+
 // Flattened list streams. A wave's lanes each hold one list (lane k: len
 // elements starting at `start` = the exclusive prefix of the lengths); all
 // lists form one element stream, cut into tiles of U*kWave elements, and lane
@@ -551,8 +570,8 @@ constexpr int kPosUnroll = 4;
 template <class F>
 __device__ inline void flat_positions(unsigned long long lb, uint32_t len, unsigned long long* tab,
                                       unsigned long long* tmask, int lane, F&& f) {
-  const unsigned long long incl = wave_incl_scan_u64(len);
-  const unsigned long long total = lane63_u64(incl);
+  unsigned long long total;
+  const unsigned long long incl = stream_prefix(len, total);
   if (total == 0) return;
   const unsigned long long start = incl - len;
   const uint32_t rank = stream_rank(len);
@@ -745,8 +764,8 @@ template <class T, class F>
 __device__ inline void expand_stream(const T* __restrict__ src, unsigned long long lb, uint32_t len, WaveLds& w,
                                      int lane, F&& elem) {
   constexpr uint32_t kTile = kWave * kUnroll;
-  const unsigned long long incl = wave_incl_scan_u64(len);
-  const unsigned long long total = lane63_u64(incl);
+  unsigned long long total;
+  const unsigned long long incl = stream_prefix(len, total);
   if (total == 0) return;
   const unsigned long long start = incl - len;
   const uint32_t rank = stream_rank(len);
@@ -865,8 +884,8 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
       visit(a.dl.col, ul && !upk ? b : 0ull, ul && !upk ? uint32_t(e - b) : 0u);
     }
     const unsigned long long klen = (uint32_t(lane) < ns && kind_l != kSegU) ? e - b : 0ull;
-    const unsigned long long incl = wave_incl_scan_u64(klen);
-    const unsigned long long total = lane63_u64(incl);
+    unsigned long long total;  // (a key list is a row: fewer than 2^32 keys)
+    const unsigned long long incl = stream_prefix(uint32_t(klen), total);
     for (unsigned long long t0 = 0; t0 < total; t0 += kWave) {
       const unsigned long long t = t0 + lane;
       // segment of key t: the first j whose inclusive end is past t
@@ -1462,8 +1481,8 @@ __device__ inline void flat_copy(const T* __restrict__ src, unsigned long long s
                                  unsigned long long db, uint32_t len, unsigned long long* tmask,
                                  unsigned long long* sbase, unsigned long long* dbase, int lane) {
   constexpr int kU = 4;
-  const unsigned long long incl = wave_incl_scan_u64(len);
-  const unsigned long long total = lane63_u64(incl);
+  unsigned long long total;
+  const unsigned long long incl = stream_prefix(len, total);
   if (total == 0) return;
   const unsigned long long start = incl - len;
   const uint32_t rank = stream_rank(len);
