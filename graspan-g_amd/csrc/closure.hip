@@ -66,7 +66,7 @@ constexpr uint64_t kDeferRatio = 128;   // pending edges kept apart while under 
 constexpr uint32_t kMaxCrossParts = 1024;  // partitions of the out-of-core schedule
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
 constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
-constexpr uint32_t kMergeBits = 8192;    // k_merge: rows spanning fewer ids are merged in an LDS bitmap
+constexpr uint32_t kMergeBits = 16384;   // k_merge: rows spanning fewer ids are merged in an LDS bitmap
 constexpr uint32_t kShortRow = 1024;     // k_merge: unchanged rows up to this length move in the block copy
 constexpr int kMaxRules = 64;
 constexpr int kScanItems = 8;
