@@ -118,9 +118,9 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
   gg_stats total{};
   b->groups = 0;
   const size_t G = b->graphs.size();
-  // Graphs of one grammar go into the same launch groups: a group's rows are
-  // its vertices times all its labels, and every round passes over them all,
-  // so mixing grammars would give each graph the other grammars' empty rows.
+  // Graphs are ordered by grammar before they are cut into launch groups: a
+  // group's rows are its vertices times all its labels, so a group that
+  // mixes fewer grammars carries fewer empty rows through every round.
   std::vector<size_t> ord(G);
   for (size_t k = 0; k < G; ++k) ord[k] = k;
   std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
@@ -165,32 +165,51 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
     gg::Grammar cg;
     int s = gg::grammar_from_rules(int(rules.size() / 3), rules.data(), int(std::max(labels, 1u)), cg, b->err);
     if (s != GG_OK) return s;
-    // super graph
+    // super graph: every graph's edges shifted to its vertex range, plus the
+    // eps loops its grammar asks for (sized first, then filled in place)
     gg::Graph sg;
     sg.n = uint32_t(verts);
+    uint64_t total_m = 0;
+    for (size_t k = i; k < j; ++k) {
+      const auto& g = b->graphs[ord[k]];
+      total_m += g.src.size();
+      for (size_t r = 0; r + 2 < g.rules.size(); r += 3)
+        if (g.rules[r + 1] == 0) total_m += g.n;
+    }
+    sg.src.resize(total_m);
+    sg.dst.resize(total_m);
+    sg.lab.resize(total_m);
+    uint64_t at = 0;
     std::vector<uint32_t> v0(1, 0), l0, lg;
     for (size_t k = i; k < j; ++k) {
       const auto& g = b->graphs[ord[k]];
       const uint32_t base = v0.back(), lb = grammar_base[std::make_pair(g.L, g.rules)];
-      for (size_t e2 = 0; e2 < g.src.size(); ++e2) {
-        sg.src.push_back(g.src[e2] + base);
-        sg.dst.push_back(g.dst[e2] + base);
-        sg.lab.push_back(uint8_t(g.lab[e2] + lb));
+      const size_t m = g.src.size();
+      uint32_t* ps = sg.src.data() + at;
+      uint32_t* pd = sg.dst.data() + at;
+      uint8_t* pl = sg.lab.data() + at;
+      for (size_t e2 = 0; e2 < m; ++e2) {
+        ps[e2] = g.src[e2] + base;
+        pd[e2] = g.dst[e2] + base;
+        pl[e2] = uint8_t(g.lab[e2] + lb);
       }
+      at += m;
       for (size_t r = 0; r + 2 < g.rules.size(); r += 3) {
         if (g.rules[r + 1] != 0) continue;
+        const uint8_t el = uint8_t(g.rules[r] + int32_t(lb));
         for (uint32_t v = 0; v < g.n; ++v) {
-          sg.src.push_back(base + v);
-          sg.dst.push_back(base + v);
-          sg.lab.push_back(uint8_t(g.rules[r] + int32_t(lb)));
+          sg.src[at + v] = base + v;
+          sg.dst[at + v] = base + v;
+          sg.lab[at + v] = el;
         }
+        at += g.n;
       }
       l0.push_back(lb);
       lg.push_back(uint32_t(g.L));
       v0.push_back(base + g.n);
     }
     if ((s = gg::engine_set_grammar(e, cg, b->err)) != GG_OK) return s;
-    if ((s = gg::engine_set_graph(e, sg, b->err)) != GG_OK) return s;
+    if ((s = gg::engine_set_graph(e, std::move(sg), b->err)) != GG_OK) return s;
     gg_stats one{};
     if ((s = gg::engine_run(e, 0, &one, b->err)) != GG_OK) return s;
     add_stats(total, one);

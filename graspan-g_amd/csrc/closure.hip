@@ -3190,7 +3190,8 @@ int engine_set_grammar(Engine* e, const Grammar& gr, std::string& err) {
   return GG_OK;
 }
 
-int engine_set_graph(Engine* e, const Graph& gr, std::string& err) {
+namespace {
+int upload_graph(Engine* e, const Graph& gr, std::string& err) {
   GG_HIP(hipSetDevice(e->device));
   e->n = gr.n;
   e->m = gr.m();
@@ -3203,11 +3204,25 @@ int engine_set_graph(Engine* e, const Graph& gr, std::string& err) {
     GG_HIP(hipMemcpyAsync(e->in_lab.p, gr.lab.data(), e->m, hipMemcpyHostToDevice, e->stream));
   }
   GG_HIP(hipStreamSynchronize(e->stream));
+  e->has_graph = true;
+  e->has_closure = false;
+  return GG_OK;
+}
+}  // namespace
+
+int engine_set_graph(Engine* e, const Graph& gr, std::string& err) {
+  GG_TRY(upload_graph(e, gr, err));
   e->h_src = gr.src;
   e->h_dst = gr.dst;
   e->h_lab = gr.lab;
-  e->has_graph = true;
-  e->has_closure = false;
+  return GG_OK;
+}
+
+int engine_set_graph(Engine* e, Graph&& gr, std::string& err) {
+  GG_TRY(upload_graph(e, gr, err));
+  e->h_src = std::move(gr.src);
+  e->h_dst = std::move(gr.dst);
+  e->h_lab = std::move(gr.lab);
   return GG_OK;
 }
 

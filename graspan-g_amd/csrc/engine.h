@@ -22,6 +22,8 @@ void engine_destroy(Engine* e);
 // edges stay resident on the device across runs.
 int engine_set_grammar(Engine* e, const Grammar& g, std::string& err);
 int engine_set_graph(Engine* e, const Graph& g, std::string& err);
+// Same, taking over the caller's host arrays instead of copying them.
+int engine_set_graph(Engine* e, Graph&& g, std::string& err);
 
 int engine_run(Engine* e, int filter, gg_stats* stats, std::string& err);
 // Options of the partitioned schedule: spill = partitions not in the current
