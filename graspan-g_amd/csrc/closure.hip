@@ -743,6 +743,7 @@ struct WaveLds {
   uint32_t fresh[kChunkWords];
   unsigned long long lbase[kWave];   // expand_stream: base address per non-empty list
   unsigned long long tmask[kUnroll];  // expand_stream: list starts inside the tile
+  unsigned long long acc[3];          // the wave's K, O and N counts (in LDS, not in registers for the whole kernel)
 #ifdef GG_EXPAND_STATS
   unsigned long long xs[4];
 #endif
@@ -1167,7 +1168,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
   if (lane < 4) w.xs[lane] = 0;
 #endif
   const uint32_t nactive = a.ctr->active;
-  unsigned long long kc = 0, orr = 0, ne = 0;
+  if (lane < 3) w.acc[lane] = 0;
   // rows are taken kGrab at a time and staging space kStageBlock words at a
   // time, so the two global counters see one atomic per many rows
   uint32_t next = 0, end = 0;
@@ -1346,16 +1347,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
     if (lane == 0) {
       a.seg_off[r] = pos;
       a.seg_cnt[r] = cnt;
-      kc += krow;
-      ne += cnt;
+      __hip_atomic_fetch_add(&w.acc[0], krow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(&w.acc[2], (unsigned long long)cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (MODE == 0 && krow) {
-        orr += (a.old.ptr[r + 1] - a.old.ptr[r]) + (a.dl.ptr[r + 1] - a.dl.ptr[r]);
+        unsigned long long orr = (a.old.ptr[r + 1] - a.old.ptr[r]) + (a.dl.ptr[r + 1] - a.dl.ptr[r]);
         if (PEND && a.pend.ptr) orr += a.pend.ptr[r + 1] - a.pend.ptr[r];
+        __hip_atomic_fetch_add(&w.acc[1], orr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
   }
+  wave_sync();
   if (lane == 0) {
     CtrSlot& cs = ctr_slot(a.ctr);
+    const unsigned long long kc = w.acc[0], orr = w.acc[1], ne = w.acc[2];
     if (kc) atomicAdd(&cs.cand, kc);
     if (orr) atomicAdd(&cs.oreads, orr);
     if (ne) atomicAdd(&cs.newe, ne);
