@@ -61,7 +61,7 @@ constexpr uint32_t kChunkWords = kChunkBits / 32;
 constexpr int kUnroll = 8;              // loads in flight per lane in the candidate stream
 constexpr uint32_t kGrab = 4;           // active rows taken per work-counter atomic
 constexpr uint32_t kXcds = 8;           // MI355X: 8 XCDs, each with its own L2
-constexpr int kExpandWaves = 6;         // k_expand occupancy target (waves per SIMD)
+constexpr int kExpandWaves = 7;         // k_expand occupancy target (waves per SIMD)
 constexpr uint64_t kDeferRatio = 128;   // pending edges kept apart while under 1/128 of old
 constexpr uint32_t kMaxCrossParts = 1024;  // partitions of the out-of-core schedule
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
