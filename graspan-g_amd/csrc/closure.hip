@@ -759,12 +759,16 @@ struct WaveLds {
 
 // Calls elem(x) for every element of the lists held by the lanes (lane i
 // holds src[lb .. lb+len)), as one flattened stream (see stream_tile) with
-// kUnroll loads in flight per lane before they are consumed. The per-rank
+// kUnroll loads (kUnroll/2 for 64-bit packed words) in flight per lane before
+// they are consumed. The per-rank
 // table holds each non-empty list's base address. All lanes take part.
 template <class T, class F>
 __device__ inline void expand_stream(const T* __restrict__ src, unsigned long long lb, uint32_t len, WaveLds& w,
                                      int lane, F&& elem) {
-  constexpr uint32_t kTile = kWave * kUnroll;
+  // loads in flight per lane: kUnroll ids, or half as many 64-bit packed
+  // words, so both kinds of stream hold the same registers
+  constexpr int kU = sizeof(T) == 8 ? kUnroll / 2 : kUnroll;
+  constexpr uint32_t kTile = kWave * kU;
   unsigned long long total;
   const unsigned long long incl = stream_prefix(len, total);
   if (total == 0) return;
@@ -773,27 +777,27 @@ __device__ inline void expand_stream(const T* __restrict__ src, unsigned long lo
   wave_sync();
   // element t of the stream lies at byte address lbase[rank of its list] + t*sizeof(T)
   if (len) w.lbase[rank] = reinterpret_cast<unsigned long long>(src) + (lb - start) * sizeof(T);
-  if (lane < kUnroll) w.tmask[lane] = 0;
+  if (lane < kU) w.tmask[lane] = 0;
   wave_sync();
   // full tiles first, with no bounds test, then the partial last tile
   const unsigned long long full = total / kTile * kTile;
   for (unsigned long long t0 = 0; t0 < full; t0 += kTile) {
     const unsigned long long tb = (t0 + uint32_t(lane)) * sizeof(T);
-    T x[kUnroll];
-    stream_tile<kUnroll>(start, len, t0, w.tmask, lane, [&](int q, uint32_t rk) {
+    T x[kU];
+    stream_tile<kU>(start, len, t0, w.tmask, lane, [&](int q, uint32_t rk) {
       x[q] = *reinterpret_cast<const __attribute__((address_space(1))) T*>(w.lbase[rk] + tb +
                                                                             uint32_t(q * kWave) * sizeof(T));
     });
 #pragma unroll
-    for (int q = 0; q < kUnroll; ++q) elem(x[q]);
+    for (int q = 0; q < kU; ++q) elem(x[q]);
   }
   if (full < total) {
     const unsigned long long t0 = full;
     const int lim = int(total - full);
     const int nq = (lim + kWave - 1) / kWave;
     const unsigned long long tb = (t0 + uint32_t(lane)) * sizeof(T);
-    T x[kUnroll];
-    stream_tile<kUnroll>(
+    T x[kU];
+    stream_tile<kU>(
         start, len, t0, w.tmask, lane,
         [&](int q, uint32_t rk) {
           x[q] = 0;
@@ -803,7 +807,7 @@ __device__ inline void expand_stream(const T* __restrict__ src, unsigned long lo
         },
         nq);
 #pragma unroll
-    for (int q = 0; q < kUnroll; ++q)
+    for (int q = 0; q < kU; ++q)
       if (q < nq && lane < lim - q * kWave) elem(x[q]);
   }
   wave_sync();
