@@ -1829,20 +1829,33 @@ __global__ void k_slice_counts(const unsigned long long* ptr, uint32_t n, DSlice
 
 __global__ __launch_bounds__(kBlock) void k_slice_checksum(DCsr c, uint32_t rows, uint32_t n, DSlices s,
                                                            unsigned long long* sums) {
+  // a wave takes a contiguous range of rows and adds its hashes into a
+  // graph's sum only when the graph changes (consecutive rows of a label
+  // belong to one graph), not once per row: per-row updates of the few
+  // per-graph sums serialise
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  for (uint32_t r = wave; r < rows; r += nwaves) {
+  const uint32_t per = (rows + nwaves - 1) / nwaves;
+  const uint32_t r0 = min(rows, wave * per), r1 = min(rows, r0 + per);
+  uint32_t gcur = 0xFFFFFFFFu;
+  unsigned long long acc = 0;  // this lane's part of graph gcur's sum
+  for (uint32_t r = r0; r < r1; ++r) {
     const unsigned long long b = c.ptr[r], e = c.ptr[r + 1];
     if (b == e) continue;
     const uint32_t l = r / n, u = r - l * n;
     const uint32_t g = s.vg[u];
+    if (g != gcur) {
+      const unsigned long long t = wave_sum_u64(acc);
+      if (lane == 0 && t) atomicAdd(&sums[gcur], t);
+      gcur = g;
+      acc = 0;
+    }
     const uint32_t base = s.v0[g], lab = l - s.l0[g] + 1;
-    unsigned long long h = 0;
-    for (unsigned long long i = b + lane; i < e; i += kWave) h += triple_hash(u - base, c.col[i] - base, lab);
-    h = wave_sum_u64(h);
-    if (lane == 0) atomicAdd(&sums[g], h);
+    for (unsigned long long i = b + lane; i < e; i += kWave) acc += triple_hash(u - base, c.col[i] - base, lab);
   }
+  const unsigned long long t = wave_sum_u64(acc);
+  if (lane == 0 && t) atomicAdd(&sums[gcur], t);
 }
 
 // ---------------------------------------------------------------------------
