@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.h"
@@ -179,34 +180,57 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
     sg.src.resize(total_m);
     sg.dst.resize(total_m);
     sg.lab.resize(total_m);
-    uint64_t at = 0;
+    // each graph's slice of the union: its vertex base, label base and first
+    // position, then the slices are filled by a few host threads
     std::vector<uint32_t> v0(1, 0), l0, lg;
+    std::vector<uint64_t> at0;
+    uint64_t at = 0;
     for (size_t k = i; k < j; ++k) {
       const auto& g = b->graphs[ord[k]];
-      const uint32_t base = v0.back(), lb = grammar_base[std::make_pair(g.L, g.rules)];
-      const size_t m = g.src.size();
-      uint32_t* ps = sg.src.data() + at;
-      uint32_t* pd = sg.dst.data() + at;
-      uint8_t* pl = sg.lab.data() + at;
-      for (size_t e2 = 0; e2 < m; ++e2) {
-        ps[e2] = g.src[e2] + base;
-        pd[e2] = g.dst[e2] + base;
-        pl[e2] = uint8_t(g.lab[e2] + lb);
-      }
-      at += m;
-      for (size_t r = 0; r + 2 < g.rules.size(); r += 3) {
-        if (g.rules[r + 1] != 0) continue;
-        const uint8_t el = uint8_t(g.rules[r] + int32_t(lb));
-        for (uint32_t v = 0; v < g.n; ++v) {
-          sg.src[at + v] = base + v;
-          sg.dst[at + v] = base + v;
-          sg.lab[at + v] = el;
-        }
-        at += g.n;
-      }
+      const uint32_t lb = grammar_base[std::make_pair(g.L, g.rules)];
+      at0.push_back(at);
+      at += g.src.size();
+      for (size_t r = 0; r + 2 < g.rules.size(); r += 3)
+        if (g.rules[r + 1] == 0) at += g.n;
       l0.push_back(lb);
       lg.push_back(uint32_t(g.L));
-      v0.push_back(base + g.n);
+      v0.push_back(v0.back() + g.n);
+    }
+    auto fill = [&](size_t k0, size_t k1) {
+      for (size_t k = k0; k < k1; ++k) {
+        const auto& g = b->graphs[ord[i + k]];
+        const uint32_t base = v0[k], lb = l0[k];
+        uint64_t p = at0[k];
+        const size_t m = g.src.size();
+        uint32_t* ps = sg.src.data() + p;
+        uint32_t* pd = sg.dst.data() + p;
+        uint8_t* pl = sg.lab.data() + p;
+        for (size_t e2 = 0; e2 < m; ++e2) {
+          ps[e2] = g.src[e2] + base;
+          pd[e2] = g.dst[e2] + base;
+          pl[e2] = uint8_t(g.lab[e2] + lb);
+        }
+        p += m;
+        for (size_t r = 0; r + 2 < g.rules.size(); r += 3) {
+          if (g.rules[r + 1] != 0) continue;
+          const uint8_t el = uint8_t(g.rules[r] + int32_t(lb));
+          for (uint32_t v = 0; v < g.n; ++v) {
+            sg.src[p + v] = base + v;
+            sg.dst[p + v] = base + v;
+            sg.lab[p + v] = el;
+          }
+          p += g.n;
+        }
+      }
+    };
+    const size_t ng = j - i;
+    const size_t nt = std::min<size_t>({ng, 16, std::max(1u, std::thread::hardware_concurrency())});
+    if (nt <= 1 || total_m < (1u << 20)) {
+      fill(0, ng);
+    } else {
+      std::vector<std::thread> pool;
+      for (size_t t = 0; t < nt; ++t) pool.emplace_back(fill, ng * t / nt, ng * (t + 1) / nt);
+      for (auto& th : pool) th.join();
     }
     if ((s = gg::engine_set_grammar(e, cg, b->err)) != GG_OK) return s;
     if ((s = gg::engine_set_graph(e, std::move(sg), b->err)) != GG_OK) return s;
