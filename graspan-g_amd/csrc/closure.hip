@@ -1236,7 +1236,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       } else if (a.n > kChunkBits) {
         uint32_t mn = 0xFFFFFFFFu, mx = 0;
         unsigned long long kk = 0;
-        for_each_list<MODE, false, PEND>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
+        // (packed lists through their entries: a packed row's ids are not
+        // kept during the fixpoint)
+        for_each_list<MODE, MODE == 0, PEND>(a, r, A, u, lane, [&](const uint32_t* col, unsigned long long lb, uint32_t len) {
           if (MODE == 1) {
             // raw rows are unsorted: reduce over every element
             const unsigned long long b = (unsigned long long)uint32_t(__builtin_amdgcn_readlane(int(uint32_t(lb)), 0)) |
@@ -1253,7 +1255,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
             mx = max(mx, col[lb + len - 1]);
             kk += len;
           }
-        }, [](const unsigned long long*, unsigned long long, uint32_t, uint32_t) {});
+        }, [&](const unsigned long long* ent, unsigned long long pb, uint32_t pl, uint32_t len) {
+          if (pl) {
+            mn = min(mn, uint32_t(ent[pb]) << 5);
+            mx = max(mx, min((uint32_t(ent[pb + pl - 1]) << 5) | 31u, a.n - 1));
+            kk += len;
+          }
+        });
         if (MODE == 2 && a.ro.kind == 2) {
           bool has_eps = false;
           for (int i = 0; i < a.g.ne; ++i) has_eps |= a.g.e_lhs[i] == A;
@@ -1557,10 +1565,12 @@ __device__ inline void wave_copy(const T* __restrict__ src, T* __restrict__ dst,
 // in one flattened copy. Long unchanged rows, rows with delta and (before old
 // is packed) the packed rows to build are appended to mlist for
 // k_merge_rows, one wave per row.
+// ids false: rows of packed labels (with out_pk) get packed entries only;
+// their ids are written once after the fixpoint (k_unpack).
 __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t rows, uint32_t n, const Counters* ctr,
                                                   DPacked plab, const unsigned long long* old_pk, uint32_t* out_col,
                                                   unsigned long long* out_pk, uint32_t* pk_len, uint32_t* mlist,
-                                                  uint32_t* mcnt) {
+                                                  uint32_t* mcnt, bool ids) {
   // ctr == nullptr: merge every row (no label is skipped)
   __shared__ unsigned long long s_tmask[kWavesPerBlock][4], s_sb[kWavesPerBlock][kWave], s_db[kWavesPerBlock][kWave];
   // listed rows gather per wave and take their place in mlist with one
@@ -1594,9 +1604,10 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
       de = dl.ptr[r + 1];
     }
     const bool same = live && de == db && oe - ob <= kShortRow;  // short, no delta: the row only shifts
+    const bool same_ids = same && (ids || !(out_pk && packed_label(plab, l)));
     // consecutive such rows are one contiguous range with one shift (no
     // delta between them): copied run by run, no per-id row search
-    unsigned long long runs = __ballot(same);
+    unsigned long long runs = __ballot(same_ids);
     while (runs) {
       const int k0 = __ffsll(runs) - 1;
       const unsigned long long from = runs >> k0;
@@ -1630,7 +1641,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(DCsr old, DCsr dl, uint32_t ro
 __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32_t n, DPacked plab,
                                                        const unsigned long long* old_pk, uint32_t* out_col,
                                                        unsigned long long* out_pk, uint32_t* pk_len,
-                                                       const uint32_t* mlist, const uint32_t* mcnt) {
+                                                       const uint32_t* mlist, const uint32_t* mcnt, bool ids) {
   __shared__ uint32_t s_win[kWavesPerBlock][kWave];
   __shared__ uint32_t s_bits[kWavesPerBlock][kMergeBits / 32];
   const int lane = threadIdx.x & (kWave - 1);
@@ -1652,12 +1663,17 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
     if (uint32_t(lane) < nb) {
       rr_l = mlist[i0 + lane];
       // packed entries of the old row (packed labels): the merge reads those
-      if (old_pk && out_pk && packed_label(plab, rr_l / n)) pl_l = pk_len[rr_l];
+      const bool opk = old_pk && out_pk && packed_label(plab, rr_l / n);
+      if (opk) pl_l = pk_len[rr_l];
       rob_l = old.ptr[rr_l];
       roe_l = old.ptr[rr_l + 1];
       rdb_l = dl.ptr[rr_l];
       rde_l = dl.ptr[rr_l + 1];
-      if (roe_l > rob_l) {
+      if (opk && pl_l) {
+        // the old row's first and last word (its ids may not be written)
+        lo_l = uint32_t(old_pk[rob_l]) << 5;
+        hi_l = min((uint32_t(old_pk[rob_l + pl_l - 1]) << 5) | 31u, n - 1);
+      } else if (!opk && roe_l > rob_l) {
         lo_l = old.col[rob_l];
         hi_l = old.col[roe_l - 1];
       }
@@ -1676,7 +1692,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
       const bool pkr = out_pk && packed_label(plab, lr);
       if (dln == 0 && (!pkr || old_pk)) {
         // a long unchanged row: ids and packed entries move as they are
-        wave_copy(old.col + rob, out, ol, lane);
+        if (!pkr || ids) wave_copy(old.col + rob, out, ol, lane);
         if (pkr) wave_copy(old_pk + rob, out_pk + rob + rdb, uint32_t(__builtin_amdgcn_readlane(int(pl_l), int(k))), lane);
         continue;
       }
@@ -1710,7 +1726,8 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
         // dense rows (4+ ids per word on average): two words per wave step,
         // one bit per lane, so each step stores its ids contiguously; sparse
         // rows: one word per lane, its bits stored one by one
-        const bool dense = uint64_t(ol) + dln >= 4ull * words;
+        const bool wid = !pkr || ids;  // the row's ids are written (else only its packed entries)
+        const bool dense = wid && uint64_t(ol) + dln >= 4ull * words;
         unsigned long long o = 0;
         uint32_t pe = 0;
         if (dense) {
@@ -1725,7 +1742,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
         }
         for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
           const uint32_t wd = w0 + lane < words ? bits[w0 + lane] : 0u;
-          if (!dense) {
+          if (!dense && wid) {
             const uint32_t pc = __popc(wd);
             const uint32_t incl = wave_incl_scan(pc, lane);
             uint32_t x = wd;
@@ -1747,6 +1764,53 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
         }
         if (pkr && lane == 0) pk_len[rr] = pe;
         wave_sync();
+        continue;
+      }
+      if (pkr && old_pk && !ids) {
+        // a wide packed row whose ids are not written: LDS windows of
+        // kMergeBits ids, each starting at the smallest id not merged yet;
+        // the old entries and delta ids inside a window are found by binary
+        // search (both are sorted), ORed into the bitmap and its non-zero
+        // words written as entries. Every window takes at least one element.
+        const unsigned long long below = (1ull << lane) - 1;
+        const uint32_t opl = uint32_t(__builtin_amdgcn_readlane(int(pl_l), int(k)));
+        unsigned long long io = rob, id = rdb;
+        const unsigned long long ie = rob + opl;
+        uint32_t pe = 0;
+        for (;;) {
+          uint32_t c0 = 0xFFFFFFFFu;  // no real window starts there: window starts are multiples of 32
+          if (io < ie) c0 = uint32_t(old_pk[io]) << 5;
+          if (id < rde) c0 = min(c0, dl.col[id] & ~31u);
+          c0 = uni_u32(c0);
+          if (c0 == 0xFFFFFFFFu) break;
+          const unsigned long long c1 = (unsigned long long)c0 + kMergeBits;
+          const unsigned long long jo = uni_u64(lower_pos_word(old_pk, io, ie, uint32_t(c1 >> 5)));
+          const unsigned long long jd = uni_u64(lower_pos(dl.col, id, rde, c1));
+          for (uint32_t t = lane; t < kMergeBits / 32; t += kWave) bits[t] = 0;
+          wave_sync();
+          for (unsigned long long i = io + lane; i < jo; i += kWave) {
+            const unsigned long long e = old_pk[i];
+            __hip_atomic_fetch_or(&bits[uint32_t(e) - (c0 >> 5)], uint32_t(e >> 32), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+          for (unsigned long long i = id + lane; i < jd; i += kWave) {
+            const uint32_t off = dl.col[i] - c0;
+            __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+          wave_sync();
+          for (uint32_t w0 = 0; w0 < kMergeBits / 32; w0 += kWave) {
+            const uint32_t wd = bits[w0 + lane];
+            const unsigned long long nz = __ballot(wd != 0);
+            if (wd)
+              out_pk[rob + rdb + pe + __popcll(nz & below)] =
+                  ((unsigned long long)wd << 32) | ((c0 >> 5) + w0 + uint32_t(lane));
+            pe += uint32_t(__popcll(nz));
+          }
+          wave_sync();
+          io = jo;
+          id = jd;
+        }
+        if (lane == 0) pk_len[rr] = pe;
         continue;
       }
       if (dln == 0) {
@@ -1771,19 +1835,53 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
   }
 }
 
+// Writes the ids of every row of a packed label from its packed entries (the
+// merges keep only the entries of those rows during the fixpoint). One wave
+// per row; lane t expands entry t at the prefix of the entries' popcounts,
+// which fills exactly the row's [ptr[r], ptr[r+1]).
+__global__ __launch_bounds__(kBlock) void k_unpack(const unsigned long long* ptr, uint32_t* col, DPacked plab,
+                                                   uint32_t rows, uint32_t n) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  for (uint32_t r = wave; r < rows; r += nwaves) {
+    if (!packed_label(plab, r / n)) continue;
+    const uint32_t pl = plab.len[r];
+    if (pl == 0) continue;
+    const unsigned long long b = ptr[r];
+    unsigned long long o = b;
+    for (uint32_t t0 = 0; t0 < pl; t0 += kWave) {
+      const uint32_t t = t0 + uint32_t(lane);
+      const unsigned long long e = t < pl ? plab.ent[b + t] : 0ull;
+      uint32_t m = uint32_t(e >> 32);
+      const uint32_t wbase = uint32_t(e) << 5;
+      const uint32_t pc = __popc(m);
+      const uint32_t incl = wave_incl_scan(pc, lane);
+      unsigned long long q = o + (incl - pc);
+      while (m) {
+        const int bit = __ffs(m) - 1;
+        m &= m - 1;
+        col[q++] = wbase + uint32_t(bit);
+      }
+      o += lane63_u32(incl);
+    }
+  }
+}
+
 // Labels whose delta is empty keep their rows: their id (and packed) segment
 // moves in one piece, by the label's constant delta rowptr.
 __global__ __launch_bounds__(kBlock) void k_copy_labels(DCsr old, DCsr dl, uint32_t n, const Counters* ctr, DPacked plab,
                                                         const unsigned long long* old_pk, uint32_t* out_col,
-                                                        unsigned long long* out_pk) {
+                                                        unsigned long long* out_pk, bool ids) {
   const uint32_t l = blockIdx.y;
   if (label_nonempty(ctr, l)) return;
   const unsigned long long b = old.ptr[uint64_t(l) * n], e = old.ptr[uint64_t(l + 1) * n];
   const unsigned long long shift = dl.ptr[uint64_t(l) * n];
   const bool pk = out_pk && old_pk && packed_label(plab, l);
+  const bool wid = ids || !pk;  // else the label's ids are not kept during the fixpoint
   for (unsigned long long i = b + blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < e;
        i += uint64_t(gridDim.x) * blockDim.x) {
-    out_col[i + shift] = old.col[i];
+    if (wid) out_col[i + shift] = old.col[i];
     if (pk) out_pk[i + shift] = old_pk[i];
   }
 }
@@ -1914,6 +2012,10 @@ class Engine {
   // a round's delta goes to pend while (pend + delta) * defer_ratio <= old
   // (0: never); GG_DEFER_RATIO overrides the default
   uint64_t defer_ratio = kDeferRatio;
+  // rows of packed labels keep their ids during the fixpoint (false: only
+  // their packed entries, the ids written once after the last round);
+  // GG_KEEP_PACKED_IDS=1 sets it
+  bool keep_ids = false;
   DevBuf raw_ptr, raw_col, staging, seg_off, seg_cnt, active, tiles, ctr;
   // frontier bitmaps and pair lists for old_B x delta_C
   DevBuf fr_bits, pr_ptr, pr_col, pr_cnt;
@@ -2148,10 +2250,11 @@ class Engine {
     }
     const DCsr oc{P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
     GG_TRY(merge_rows(oc, xc, R, n, flags, plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr,
-                      P<uint32_t>(o2_col), opk, P<uint32_t>(pk_len), err));
+                      P<uint32_t>(o2_col), opk, P<uint32_t>(pk_len), err, keep_ids));
     if (old_nnz && flags)
       k_copy_labels<<<dim3(uint32_t(cus), uint32_t(L)), kBlock, 0, stream>>>(
-          oc, xc, n, flags, plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr, P<uint32_t>(o2_col), opk);
+          oc, xc, n, flags, plab, pk_ready ? P<unsigned long long>(pk_ent) : nullptr, P<uint32_t>(o2_col), opk,
+          keep_ids);
     pk_ready = pack_any;
     if (pack_any) std::swap(pk_ent, pk2_ent);
     std::swap(old_ptr, o2_ptr);
@@ -2160,16 +2263,17 @@ class Engine {
     return GG_OK;
   }
 
+  // ids false: rows of packed labels get packed entries only (see k_merge)
   int merge_rows(DCsr oc, DCsr dc, uint32_t rows, uint32_t nn, const Counters* c, DPacked plab,
                  const unsigned long long* old_pk, uint32_t* out_col, unsigned long long* out_pk, uint32_t* pklen,
-                 std::string& err) {
+                 std::string& err, bool ids = true) {
     GG_TRY(grow(mlist, uint64_t(rows) * 4, err));
     GG_TRY(alloc(mcnt, 4, err));
     fill_u32(mcnt, 0u, 1);
     k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, dc, rows, nn, c, plab, old_pk, out_col, out_pk, pklen,
-                                                      P<uint32_t>(mlist), P<uint32_t>(mcnt));
+                                                      P<uint32_t>(mlist), P<uint32_t>(mcnt), ids);
     k_merge_rows<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, dc, nn, plab, old_pk, out_col, out_pk, pklen,
-                                                           P<uint32_t>(mlist), P<uint32_t>(mcnt));
+                                                           P<uint32_t>(mlist), P<uint32_t>(mcnt), ids);
     GG_HIP(hipGetLastError());
     return GG_OK;
   }
@@ -2231,6 +2335,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   pend_nnz = 0;
   defer_ratio = kDeferRatio;
   if (const char* dr = std::getenv("GG_DEFER_RATIO")) defer_ratio = std::strtoull(dr, nullptr, 10);
+  keep_ids = false;
+  if (const char* ki = std::getenv("GG_KEEP_PACKED_IDS")) keep_ids = std::atoi(ki) != 0;
   if (uint64_t(L) * n >= 0xFFFFFFFFull || uint64_t(g.nslot) * n >= 0xFFFFFFFFull) {
     err = "num_labels * num_vertices (and binary rules with a derived second label * num_vertices) must be < 2^32";
     return GG_E_UNSUPPORTED;
@@ -2477,7 +2583,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_TRY(merge_rows(pc, dc, R, n, C(), DPacked{}, nullptr, P<uint32_t>(p2_col), nullptr, nullptr, err));
       if (pend_nnz)
         k_copy_labels<<<dim3(uint32_t(cus), uint32_t(L)), kBlock, 0, stream>>>(pc, dc, n, C(), DPacked{}, nullptr,
-                                                                              P<uint32_t>(p2_col), nullptr);
+                                                                              P<uint32_t>(p2_col), nullptr, true);
       std::swap(pd_ptr, p2_ptr);
       std::swap(pd_col, p2_col);
       pend_nnz = p2n;
@@ -2506,6 +2612,17 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     GG_TRY(fold_into_old({P<unsigned long long>(pd_ptr), P<uint32_t>(pd_col)}, pend_nnz, nullptr, err));
     pend_nnz = 0;
     GG_HIP(hipEventRecord(mg.b, stream));
+  }
+  if (pk_ready && !keep_ids) {  // the packed labels' ids, from their entries
+    EventPair& mg = mk(ev_merge);
+    GG_HIP(hipEventRecord(mg.a, stream));
+    DPacked plab{};
+    for (int i = 0; i < 8; ++i) plab.labels[i] = pack_labels[i];
+    plab.len = P<uint32_t>(pk_len);
+    plab.ent = P<unsigned long long>(pk_ent);
+    k_unpack<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(P<unsigned long long>(old_ptr), P<uint32_t>(old_col), plab, R, n);
+    GG_HIP(hipEventRecord(mg.b, stream));
+    GG_HIP(hipGetLastError());
   }
   release(pd_ptr);
   release(pd_col);

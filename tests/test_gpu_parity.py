@@ -242,6 +242,17 @@ def test_gpu_deferred_merge_matches_oracle(built, monkeypatch, ratio):
     check_against_oracle(gen.alias(20_000, seed=24))
 
 
+def test_gpu_packed_ids_kept_matches_oracle(built, monkeypatch):
+    """The merges normally keep only the packed entries of packed labels'
+    rows and write their ids after the last round; GG_KEEP_PACKED_IDS=1 keeps
+    the ids in every merge. Both equal the oracle, wide rows included."""
+    monkeypatch.setenv("GG_KEEP_PACKED_IDS", "1")
+    say("packed ids kept: alias n=20000")
+    check_against_oracle(gen.alias(20_000, seed=31))
+    say("packed ids kept: wide rows n=120000")
+    check_against_oracle(_paired_blocks(120_000, seed=32), full_triples=False)
+
+
 def test_gpu_alias_1m_full_size(built):
     """The benchmark workload itself (alias generator, n = 1,000,000): per-label
     counts, checksum, round count and the P/K/O/N counters equal the oracle's."""
