@@ -1836,34 +1836,41 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
 }
 
 // Writes the ids of every row of a packed label from its packed entries (the
-// merges keep only the entries of those rows during the fixpoint). One wave
-// per row; lane t expands entry t at the prefix of the entries' popcounts,
-// which fills exactly the row's [ptr[r], ptr[r+1]).
+// merges keep only the entries of those rows during the fixpoint). Rows go
+// to the waves in batches of 64 (one row per lane: its offset and entry
+// count load at once); the wave then expands the batch's non-empty rows one
+// after another, in order, which fills exactly the row's [ptr[r], ptr[r+1]).
 __global__ __launch_bounds__(kBlock) void k_unpack(const unsigned long long* ptr, uint32_t* col, DPacked plab,
                                                    uint32_t rows, uint32_t n) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  for (uint32_t r = wave; r < rows; r += nwaves) {
-    if (!packed_label(plab, r / n)) continue;
-    const uint32_t pl = plab.len[r];
-    if (pl == 0) continue;
-    const unsigned long long b = ptr[r];
-    unsigned long long o = b;
-    for (uint32_t t0 = 0; t0 < pl; t0 += kWave) {
-      const uint32_t t = t0 + uint32_t(lane);
-      const unsigned long long e = t < pl ? plab.ent[b + t] : 0ull;
-      uint32_t m = uint32_t(e >> 32);
-      const uint32_t wbase = uint32_t(e) << 5;
-      const uint32_t pc = __popc(m);
-      const uint32_t incl = wave_incl_scan(pc, lane);
-      unsigned long long q = o + (incl - pc);
-      while (m) {
-        const int bit = __ffs(m) - 1;
-        m &= m - 1;
-        col[q++] = wbase + uint32_t(bit);
+  for (uint32_t r0 = wave * kWave; r0 < rows; r0 += nwaves * kWave) {
+    const uint32_t r = r0 + uint32_t(lane);
+    uint32_t pl = 0;
+    unsigned long long b = 0;
+    if (r < rows && packed_label(plab, r / n)) {
+      pl = plab.len[r];
+      if (pl) b = ptr[r];
+    }
+    unsigned long long todo = __ballot(pl != 0);
+    while (todo) {
+      const int k = __ffsll(todo) - 1;
+      todo &= todo - 1;
+      const uint32_t pk = uint32_t(__builtin_amdgcn_readlane(int(pl), k));
+      const unsigned long long bk = lane_u64(b, uint32_t(k));
+      // two entries per wave step, one bit per lane: each step stores its
+      // ids contiguously
+      const unsigned long long below = (1ull << lane) - 1;
+      unsigned long long o = bk;
+      for (uint32_t j = 0; j < pk; j += 2) {
+        const uint32_t ej = j + uint32_t(lane >> 5);
+        const unsigned long long e = ej < pk ? plab.ent[bk + ej] : 0ull;
+        const bool set = (uint32_t(e >> 32) >> (lane & 31)) & 1u;
+        const unsigned long long m = __ballot(set);
+        if (set) col[o + __popcll(m & below)] = (uint32_t(e) << 5) + uint32_t(lane & 31);
+        o += uint32_t(__popcll(m));
       }
-      o += lane63_u32(incl);
     }
   }
 }
