@@ -1837,19 +1837,22 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
 
 // Writes the ids of every row of a packed label from its packed entries (the
 // merges keep only the entries of those rows during the fixpoint). Rows go
-// to the waves in batches of 64 (one row per lane: its offset and entry
-// count load at once); the wave then expands the batch's non-empty rows one
+// to the waves in batches of up to 64 (one row per lane: its offset and
+// entry count load at once); the wave then expands the batch's non-empty rows one
 // after another, in order, which fills exactly the row's [ptr[r], ptr[r+1]).
 __global__ __launch_bounds__(kBlock) void k_unpack(const unsigned long long* ptr, uint32_t* col, DPacked plab,
                                                    uint32_t rows, uint32_t n) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  for (uint32_t r0 = wave * kWave; r0 < rows; r0 += nwaves * kWave) {
+  // up to 64 rows per batch, fewer when there are few rows (so that every
+  // wave gets some: few long rows must not queue on a few waves)
+  const uint32_t per = max(1u, min(uint32_t(kWave), rows / nwaves));
+  for (uint32_t r0 = wave * per; r0 < rows; r0 += nwaves * per) {
     const uint32_t r = r0 + uint32_t(lane);
     uint32_t pl = 0;
     unsigned long long b = 0;
-    if (r < rows && packed_label(plab, r / n)) {
+    if (uint32_t(lane) < per && r < rows && packed_label(plab, r / n)) {
       pl = plab.len[r];
       if (pl) b = ptr[r];
     }
