@@ -1396,11 +1396,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* active, const
                                                    DPacked plab, unsigned long long* out_pk, uint32_t* out_pklen) {
   // rows of packed labels are also packed, at the row's own offset (out_pk
   // null: no packing)
-  __shared__ uint32_t s_win[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_win[kWavesPerBlock][kWave], s_word[kWavesPerBlock][kWave];
   __shared__ unsigned long long s_tmask[kWavesPerBlock][4], s_sb[kWavesPerBlock][kWave], s_db[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wi = threadIdx.x / kWave;
   uint32_t* win = s_win[wi];
+  uint32_t* wword = s_word[wi];
+  const unsigned long long upto = (2ull << lane) - 1;  // lanes 0..lane (lane 63: all, by unsigned wrap)
   const uint32_t nactive = ctr->active;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
@@ -1429,6 +1431,28 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* active, const
       const uint32_t r = uint32_t(__builtin_amdgcn_readlane(int(r_l), k));
       const uint32_t c = uint32_t(__builtin_amdgcn_readlane(int(c_l), k));
       const unsigned long long s = lane_u64(s_l, uint32_t(k)), d = lane_u64(d_l, uint32_t(k));
+      if (c <= uint32_t(kWave)) {
+        // up to 64 sorted ids, one per lane: a lane whose word differs from
+        // the previous lane's starts an entry; the entry index is the number
+        // of starts up to the lane, the bits OR into that entry in LDS
+        const bool in = uint32_t(lane) < c;
+        const uint32_t x = in ? staging[s + uint32_t(lane)] : 0u;
+        const uint32_t wd = x >> 5;
+        const uint32_t prev = uint32_t(__shfl_up(int(wd), 1, kWave));
+        const bool head = in && (lane == 0 || prev != wd);
+        const unsigned long long hm = __ballot(head);
+        const uint32_t ei = uint32_t(__popcll(hm & upto)) - 1u;
+        win[lane] = 0;
+        wave_sync();
+        if (in) atomicOr(&win[ei], 1u << (x & 31));
+        if (head) wword[ei] = wd;
+        wave_sync();
+        const uint32_t ne = uint32_t(__popcll(hm));
+        if (uint32_t(lane) < ne) out_pk[d + uint32_t(lane)] = ((unsigned long long)win[lane] << 32) | wword[lane];
+        if (lane == 0) out_pklen[r] = ne;
+        wave_sync();
+        continue;
+      }
       const uint32_t pc = pack_union(staging, s, s + c, staging, s, s, out_pk + d, win, lane);
       if (lane == 0) out_pklen[r] = pc;
     }
