@@ -1886,17 +1886,24 @@ __global__ __launch_bounds__(kBlock) void k_unpack(const unsigned long long* ptr
       todo &= todo - 1;
       const uint32_t pk = uint32_t(__builtin_amdgcn_readlane(int(pl), k));
       const unsigned long long bk = lane_u64(b, uint32_t(k));
-      // two entries per wave step, one bit per lane: each step stores its
-      // ids contiguously
+      // the row's entries 64 at a time, one per lane; then two entries per
+      // wave step (moved to lanes 0-31 and 32-63 by shuffle), one bit per
+      // lane, so each step stores its ids contiguously
       const unsigned long long below = (1ull << lane) - 1;
       unsigned long long o = bk;
-      for (uint32_t j = 0; j < pk; j += 2) {
-        const uint32_t ej = j + uint32_t(lane >> 5);
-        const unsigned long long e = ej < pk ? plab.ent[bk + ej] : 0ull;
-        const bool set = (uint32_t(e >> 32) >> (lane & 31)) & 1u;
-        const unsigned long long m = __ballot(set);
-        if (set) col[o + __popcll(m & below)] = (uint32_t(e) << 5) + uint32_t(lane & 31);
-        o += uint32_t(__popcll(m));
+      for (uint32_t j0 = 0; j0 < pk; j0 += kWave) {
+        const uint32_t t = j0 + uint32_t(lane);
+        const unsigned long long ev = t < pk ? plab.ent[bk + t] : 0ull;
+        const uint32_t nloc = min(uint32_t(kWave), pk - j0);
+        for (uint32_t j = 0; j < nloc; j += 2) {
+          const uint32_t src = j + uint32_t(lane >> 5);  // <= 64: lane 64 wraps to 0 and is masked below
+          const uint32_t lo = uint32_t(__shfl(int(uint32_t(ev)), int(src), kWave));
+          const uint32_t hi = uint32_t(__shfl(int(uint32_t(ev >> 32)), int(src), kWave));
+          const bool set = src < nloc && ((hi >> (lane & 31)) & 1u);
+          const unsigned long long m = __ballot(set);
+          if (set) col[o + __popcll(m & below)] = (lo << 5) + uint32_t(lane & 31);
+          o += uint32_t(__popcll(m));
+        }
       }
     }
   }
