@@ -1706,7 +1706,25 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
         hi_l = max(hi_l, dl.col[rde_l - 1]);
       }
     }
+    // the next row's first 64 old entries (packed labels) and delta ids are
+    // loaded while the current row merges; a row that fits them uses them
+    // instead of loading its own
+    unsigned long long e_nxt = 0;
+    uint32_t d_nxt = 0;
+    auto prefetch = [&](uint32_t j) {
+      const uint32_t rj = uint32_t(__builtin_amdgcn_readlane(int(rr_l), int(j)));
+      const bool pj = old_pk && out_pk && packed_label(plab, rj / n);
+      const uint32_t oj = pj ? uint32_t(__builtin_amdgcn_readlane(int(pl_l), int(j))) : 0u;
+      const unsigned long long robj = lane_u64(rob_l, j), rdbj = lane_u64(rdb_l, j);
+      const uint32_t dj = uint32_t(lane_u64(rde_l, j) - rdbj);
+      e_nxt = uint32_t(lane) < oj ? old_pk[robj + uint32_t(lane)] : 0ull;
+      d_nxt = uint32_t(lane) < dj ? dl.col[rdbj + uint32_t(lane)] : 0u;
+    };
+    if (nb) prefetch(0);
     for (uint32_t k = 0; k < nb; ++k) {
+      const unsigned long long e_cur = e_nxt;
+      const uint32_t d_cur = d_nxt;
+      if (k + 1 < nb) prefetch(k + 1);
       const uint32_t rr = uint32_t(__builtin_amdgcn_readlane(int(rr_l), int(k)));
       const uint32_t lr = rr / n;
       const unsigned long long rob = lane_u64(rob_l, k), roe = lane_u64(roe_l, k);
@@ -1730,10 +1748,16 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
         if (pkr && old_pk) {
           // the old row word by word: 32 ids per entry
           const uint32_t opl = uint32_t(__builtin_amdgcn_readlane(int(pl_l), int(k)));
-          for (uint32_t t = lane; t < opl; t += kWave) {
-            const unsigned long long e = old_pk[rob + t];
-            __hip_atomic_fetch_or(&bits[uint32_t(e) - (lo >> 5)], uint32_t(e >> 32), __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (opl <= uint32_t(kWave)) {
+            if (uint32_t(lane) < opl)
+              __hip_atomic_fetch_or(&bits[uint32_t(e_cur) - (lo >> 5)], uint32_t(e_cur >> 32), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else {
+            for (uint32_t t = lane; t < opl; t += kWave) {
+              const unsigned long long e = old_pk[rob + t];
+              __hip_atomic_fetch_or(&bits[uint32_t(e) - (lo >> 5)], uint32_t(e >> 32), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
           }
         } else {
           for (uint32_t t = lane; t < ol; t += kWave) {
@@ -1741,9 +1765,16 @@ __global__ __launch_bounds__(kBlock) void k_merge_rows(DCsr old, DCsr dl, uint32
             __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         }
-        for (uint32_t t = lane; t < dln; t += kWave) {
-          const uint32_t off = dl.col[rdb + t] - lo;
-          __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (dln <= uint32_t(kWave)) {
+          if (uint32_t(lane) < dln) {
+            const uint32_t off = d_cur - lo;
+            __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        } else {
+          for (uint32_t t = lane; t < dln; t += kWave) {
+            const uint32_t off = dl.col[rdb + t] - lo;
+            __hip_atomic_fetch_or(&bits[off >> 5], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
         }
         wave_sync();
         const unsigned long long below = (1ull << lane) - 1;
