@@ -65,6 +65,8 @@ constexpr int kExpandWaves = 7;         // k_expand occupancy target (waves per 
 constexpr uint64_t kDeferRatio = 128;   // pending edges kept apart while under 1/128 of old
 constexpr uint32_t kMaxCrossParts = 1024;  // partitions of the out-of-core schedule
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
+constexpr uint32_t kPairBlock = 1024;   // k_pairs: pair-list words a wave reserves per arena atomic
+constexpr uint32_t kPairRowBuf = 8;     // k_pairs: pairs per row held in LDS before they are written
 constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
 constexpr uint32_t kMergeBits = 16384;   // k_merge: rows spanning fewer ids are merged in an LDS bitmap
 constexpr uint32_t kShortRow = 1024;     // k_merge: unchanged rows up to this length move in the block copy
@@ -99,6 +101,7 @@ struct alignas(128) CtrSlot {
   unsigned long long xs[4];    // built with GG_EXPAND_STATS: k_expand cycles in seeding, key lists, list streams, whole rows
   unsigned int misses;         // rows whose guessed window missed a candidate
   unsigned int pad;
+  unsigned long long pairs;    // k_pairs: words reserved in this slot's arena of the pair lists
 };
 struct alignas(128) WorkIdx {
   unsigned int v;  // one XCD's work index into the active list, on a line of its own
@@ -404,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_activate_raw(const unsigned long lon
 struct ActArgs {
   DCsr old, dl;
   const unsigned long long* pend_ptr;  // may be null (no pending edges)
-  const unsigned long long* b2ptr;  // may be null (no pair lists this round)
+  const unsigned long long* b2rng;  // pair-list bounds, two per row; may be null (no pair lists this round)
   uint32_t n;
   uint32_t nlhs;
   uint8_t lhs[256];
@@ -457,9 +460,9 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
             if (a.pend_ptr) pk += a.pend_ptr[rb + 1] - a.pend_ptr[rb];
           }
           const uint32_t slot = a.g.b_slot[i];
-          if (slot != 0xFFu && a.b2ptr) {
-            const uint32_t rs = slot * a.n + u;
-            act |= a.b2ptr[rs + 1] > a.b2ptr[rs];
+          if (slot != 0xFFu && a.b2rng) {
+            const uint64_t rs = uint64_t(slot) * a.n + u;
+            act |= a.b2rng[2 * rs + 1] > a.b2rng[2 * rs];
           }
         }
       }
@@ -625,20 +628,29 @@ __global__ __launch_bounds__(kBlock) void k_frontier(const DevGrammar g, uint32_
 }
 
 // Pair lists for old_B x delta_C: row (j, u) lists the v in old_B(u) whose
-// delta_C(v) is not empty (B = slot_b[j], C = slot_c[j]). A wave takes 64
-// rows and streams their old_B rows together, testing each v against the
-// slot's frontier bitmap; per-row counts live in LDS. COUNT pass: cnt[j*n+u]
-// = |pairs|; fill pass: the pairs, at ptr[j*n+u], in no particular order.
-template <bool FILL>
+// delta_C(v) is not empty (B = slot_b[j], C = slot_c[j]), in no particular
+// order; its bounds are rng[2r], rng[2r+1] (r = j*n+u), so rows need not be
+// contiguous. One pass: a wave takes 64 rows and streams their old_B rows
+// together, testing each v against the slot's frontier bitmap; per-row counts
+// live in LDS, and the first kPairRowBuf pairs of each row too. The wave then
+// reserves the group's words in its block's arena (arena blockIdx % kSpread,
+// `arena` words each; a wave takes kPairBlock words at a time from the arena's
+// cursor, so the atomics stay few and spread over 64 lines) and writes the
+// pairs out. A group with a row over kPairRowBuf pairs streams its rows a
+// second time and writes the pairs directly. When a cursor passes `arena`,
+// no pairs go past it and the host grows the arenas and runs the pass again.
 __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr old, DCsr pend, DPacked pk,
-                                                  const unsigned long long* bits, uint32_t words, uint32_t* cnt,
-                                                  const unsigned long long* ptr, uint32_t* col) {
+                                                  const unsigned long long* bits, uint32_t words, Counters* ctr,
+                                                  unsigned long long arena, unsigned long long* rng, uint32_t* col) {
   __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][kPosUnroll],
-      s_out[kWavesPerBlock][kWave];
-  __shared__ uint32_t s_cnt[kWavesPerBlock][kWave];
+      s_at[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_cnt[kWavesPerBlock][kWave], s_buf[kWavesPerBlock][kWave * kPairRowBuf];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * blockDim.x;
+  unsigned long long* cursor = &ctr_slot(ctr).pairs;
+  const unsigned long long a0 = uint64_t(blockIdx.x % kSpread) * arena;
+  unsigned long long blk = 0, blk_end = 0;  // the wave's reserved words, arena-relative (uniform)
   for (int j = 0; j < g.nslot; ++j) {
     const uint32_t bb = uint32_t(g.slot_b[j]) * n;
     const unsigned long long* fb = bits + uint64_t(j) * words;
@@ -646,53 +658,81 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
     const bool packed = pk.len && packed_label(pk, g.slot_b[j]);
     for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < n; base += stride) {
       const uint32_t u = base + lane;
-      unsigned long long b = 0;
-      uint32_t len = 0;
+      unsigned long long b = 0, qb = 0;
+      uint32_t len = 0, ql = 0;
       if (u < n) {
         b = old.ptr[bb + u];
         len = packed ? pk.len[bb + u] : uint32_t(old.ptr[bb + u + 1] - b);
-        if (FILL) s_out[wv][lane] = ptr[uint64_t(j) * n + u];
-      }
-      s_cnt[wv][lane] = 0;
-      wave_sync();
-      if (packed) {
-        flat_positions(b, len, s_tab[wv], s_tmask[wv], lane, [&](int k, unsigned long long pos) {
-          const unsigned long long e = pk.ent[pos];
-          const uint32_t w = uint32_t(e);
-          uint32_t hit = uint32_t(e >> 32) & uint32_t(fb[w >> 1] >> ((w & 1u) * 32u));
-          if (hit) {
-            uint32_t at = atomicAdd(&s_cnt[wv][k], uint32_t(__popc(hit)));
-            if (FILL)
-              while (hit) {
-                const int bit = __ffs(hit) - 1;
-                hit &= hit - 1;
-                col[s_out[wv][k] + at++] = w * 32u + uint32_t(bit);
-              }
-          }
-        });
-      }
-      // id rows: old_B(u) when it is not packed, and the pending edges of B(u)
-      auto ids = [&](const uint32_t* src, unsigned long long ib, uint32_t il) {
-        flat_positions(ib, il, s_tab[wv], s_tmask[wv], lane, [&](int k, unsigned long long pos) {
-          const uint32_t v = src[pos];
-          if ((fb[v / kWave] >> (v % kWave)) & 1ull) {
-            const uint32_t at = atomicAdd(&s_cnt[wv][k], 1u);
-            if (FILL) col[s_out[wv][k] + at] = v;
-          }
-        });
-      };
-      if (!packed) ids(old.col, b, len);
-      if (pend.ptr) {
-        unsigned long long qb = 0;
-        uint32_t ql = 0;
-        if (u < n) {
+        if (pend.ptr) {
           qb = pend.ptr[bb + u];
           ql = uint32_t(pend.ptr[bb + u + 1] - qb);
         }
-        ids(pend.col, qb, ql);
       }
-      wave_sync();
-      if (!FILL && u < n) cnt[uint64_t(j) * n + u] = s_cnt[wv][lane];
+      // direct: write pair `at` of row k to col[s_at[k] + at]; else keep it in LDS
+      auto stream = [&](bool direct) {
+        s_cnt[wv][lane] = 0;
+        wave_sync();
+        auto emit = [&](int k, uint32_t id0, uint32_t hit) {
+          uint32_t at = atomicAdd(&s_cnt[wv][k], uint32_t(__popc(hit)));
+          while (hit) {
+            const uint32_t id = id0 + uint32_t(__ffs(hit) - 1);
+            hit &= hit - 1;
+            if (direct)
+              col[s_at[wv][k] + at] = id;
+            else if (at < kPairRowBuf)
+              s_buf[wv][uint32_t(k) * kPairRowBuf + at] = id;
+            ++at;
+          }
+        };
+        if (packed)
+          flat_positions(b, len, s_tab[wv], s_tmask[wv], lane, [&](int k, unsigned long long pos) {
+            const unsigned long long e = pk.ent[pos];
+            const uint32_t w = uint32_t(e);
+            const uint32_t hit = uint32_t(e >> 32) & uint32_t(fb[w >> 1] >> ((w & 1u) * 32u));
+            if (hit) emit(k, w * 32u, hit);
+          });
+        // id rows: old_B(u) when it is not packed, and the pending edges of B(u)
+        auto ids = [&](const uint32_t* src, unsigned long long ib, uint32_t il) {
+          flat_positions(ib, il, s_tab[wv], s_tmask[wv], lane, [&](int k, unsigned long long pos) {
+            const uint32_t v = src[pos];
+            if ((fb[v / kWave] >> (v % kWave)) & 1ull) emit(k, v, 1u);
+          });
+        };
+        if (!packed) ids(old.col, b, len);
+        if (pend.ptr) ids(pend.col, qb, ql);
+        wave_sync();
+      };
+      stream(false);
+      const uint32_t c = s_cnt[wv][lane];
+      unsigned long long tot;
+      const unsigned long long ex = stream_prefix(c, tot) - c;
+      unsigned long long at0 = 0;
+      bool fits = true;
+      if (tot) {
+        if (blk + tot > blk_end) {
+          const unsigned long long grab = tot > kPairBlock ? tot : kPairBlock;
+          unsigned long long nb = 0;
+          if (lane == 0) nb = atomicAdd(cursor, grab);
+          blk = __shfl(nb, 0, kWave);
+          blk_end = blk + grab;
+        }
+        at0 = blk + ex;
+        fits = blk + tot <= arena;
+        blk += tot;
+      }
+      if (u < n) {
+        const unsigned long long lo = a0 + at0;
+        rng[2 * (uint64_t(j) * n + u)] = lo;
+        rng[2 * (uint64_t(j) * n + u) + 1] = lo + c;
+      }
+      if (tot && fits) {
+        if (__ballot(c > kPairRowBuf) == 0) {
+          for (uint32_t t = 0; t < c; ++t) col[a0 + at0 + t] = s_buf[wv][uint32_t(lane) * kPairRowBuf + t];
+        } else {
+          s_at[wv][lane] = a0 + at0;
+          stream(true);
+        }
+      }
       wave_sync();
     }
   }
@@ -726,7 +766,7 @@ struct ExpandArgs {
   RegionOp ro;
   DPacked pk;  // packed old rows
   DPacked dpk;  // packed delta rows (same labels; entries at the row's delta offset)
-  DCsr b2;  // rows slot*n + u: keys v with old_B(u,v) and delta_C(v) non-empty
+  DCsr b2;  // rows r = slot*n + u: keys v with old_B(u,v) and delta_C(v) non-empty, at col[ptr[2r], ptr[2r+1])
   uint32_t n;
   DevGrammar g;
   const uint32_t* active;
@@ -878,7 +918,8 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
     unsigned long long b = 0, e = 0;
     const uint32_t kind_l = s_desc & 3u;
     if (uint32_t(lane) < ns) {
-      const unsigned long long* ptr = kind_l == kSegP ? a.b2.ptr : a.dl.ptr;
+      // pair lists keep two bounds per row (rows are not contiguous)
+      const unsigned long long* ptr = kind_l == kSegP ? a.b2.ptr + s_row : a.dl.ptr;
       b = ptr[s_row];
       e = ptr[s_row + 1];
     }
@@ -2090,7 +2131,8 @@ class Engine {
   bool keep_ids = false;
   DevBuf raw_ptr, raw_col, staging, seg_off, seg_cnt, active, tiles, ctr;
   // frontier bitmaps and pair lists for old_B x delta_C
-  DevBuf fr_bits, pr_ptr, pr_col, pr_cnt;
+  DevBuf fr_bits, pr_ptr, pr_col;
+  uint64_t pr_arena = 0;  // words per arena of the pair lists (one arena per counter slot)
   // packed old rows (labels in pack_labels)
   DevBuf pk_len, pk_ent, pk2_ent;
   // packed delta rows (current and next round), entries at the row's delta offset
@@ -2115,7 +2157,7 @@ class Engine {
     DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
                      &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active,
                      &tiles,  &ctr,    &fr_bits,
-                     &pr_ptr, &pr_col, &pr_cnt, &pk_len, &pk_ent, &pk2_ent, &dpk_len, &dpk_ent, &npk_len, &npk_ent,
+                     &pr_ptr, &pr_col, &pk_len, &pk_ent, &pk2_ent, &dpk_len, &dpk_ent, &npk_len, &npk_ent,
                      &mlist,  &mcnt,   &pd_ptr, &pd_col, &p2_ptr, &p2_col};
     if (stream) (void)hipStreamSynchronize(stream);
     for (DevBuf* b : all)
@@ -2524,9 +2566,9 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   const uint64_t act_work = uint64_t(act.nlhs) * n;
   const uint64_t NS = uint64_t(g.nslot) * n;
   const uint32_t fr_words = (n + kWave - 1) / kWave;
+  pr_arena = 0;
   if (NS) {
-    GG_TRY(alloc(pr_ptr, (NS + 1) * 8, err));
-    GG_TRY(alloc(pr_cnt, NS * 4, err));
+    GG_TRY(alloc(pr_ptr, 2 * NS * 8, err));
     GG_TRY(alloc(fr_bits, uint64_t(g.nslot) * fr_words * 8, err));
   }
   pk_ready = false;
@@ -2551,14 +2593,24 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       DPacked opk = plabels();
       opk.len = pk_ready ? P<uint32_t>(pk_len) : nullptr;
       opk.ent = P<unsigned long long>(pk_ent);
-      k_pairs<false><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, pend_csr(), opk, P<unsigned long long>(fr_bits), fr_words,
-                                                         P<uint32_t>(pr_cnt), nullptr, nullptr);
-      GG_TRY(scan(pr_cnt, NS, pr_ptr, err));
-      uint64_t npairs = 0;
-      GG_TRY(read_u64(P<unsigned long long>(pr_ptr) + NS, npairs, err));
-      GG_TRY(grow(pr_col, std::max<uint64_t>(npairs, 1) * 4, err));
-      k_pairs<true><<<grid_for(n), kBlock, 0, stream>>>(g, n, oc, pend_csr(), opk, P<unsigned long long>(fr_bits), fr_words,
-                                                        nullptr, P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
+      // one pass into kSpread arenas; a pass whose arenas overflowed is run
+      // again with larger ones
+      const uint32_t pgrid = grid_for(n);
+      const uint64_t pwaves = uint64_t((pgrid + kSpread - 1) / kSpread) * kWavesPerBlock;
+      pr_arena = std::max<uint64_t>(pr_arena, 2 * pwaves * kPairBlock);
+      for (;;) {
+        GG_TRY(grow(pr_col, kSpread * pr_arena * 4, err));
+        pr_arena = pr_col.bytes / 4 / kSpread;
+        zero_ctr();
+        k_pairs<<<pgrid, kBlock, 0, stream>>>(g, n, oc, pend_csr(), opk, P<unsigned long long>(fr_bits), fr_words, C(),
+                                              pr_arena, P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
+        GG_HIP(hipGetLastError());
+        GG_TRY(read_ctr(err));
+        uint64_t most = 0;
+        for (const CtrSlot& q : h_ctr->slot) most = std::max<uint64_t>(most, q.pairs);
+        if (most <= pr_arena) break;
+        pr_arena = most + most / 4 + kPairBlock;
+      }
       GG_HIP(hipEventRecord(o.b, stream));
       pairs_ready = true;
     }
@@ -2573,7 +2625,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       k_label_flags<<<1, kBlock, 0, stream>>>(P<unsigned long long>(dl_ptr), n, uint32_t(L), C());
       act.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
       act.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
-      act.b2ptr = pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr;
+      act.b2rng = pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr;
       act.pend_ptr = pend_csr().ptr;
       if (act_work) k_activate<<<grid_for(n), kBlock, 0, stream>>>(act);
       // a round with few active rows launches only the blocks that can get
@@ -3652,7 +3704,7 @@ int engine_release(Engine* e) {
   DevBuf* all[] = {&e->old_ptr, &e->old_col, &e->dl_ptr,  &e->dl_col,   &e->nx_ptr, &e->nx_col,   &e->o2_ptr,
                    &e->o2_col,  &e->staging, &e->seg_off, &e->seg_cnt,  &e->active, &e->tiles,
                    &e->fr_bits,
-                   &e->pr_ptr,  &e->pr_col,  &e->pr_cnt,  &e->pk_len, &e->pk_ent, &e->pk2_ent,
+                   &e->pr_ptr,  &e->pr_col,  &e->pk_len, &e->pk_ent, &e->pk2_ent,
                    &e->dpk_len, &e->dpk_ent, &e->npk_len, &e->npk_ent, &e->mlist, &e->mcnt,
                    &e->pd_ptr,  &e->pd_col,  &e->p2_ptr,  &e->p2_col};
   for (DevBuf* b : all) e->release(*b);
