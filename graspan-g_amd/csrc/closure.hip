@@ -639,8 +639,11 @@ __global__ __launch_bounds__(kBlock) void k_frontier(const DevGrammar g, uint32_
 // pairs out. A group with a row over kPairRowBuf pairs streams its rows a
 // second time and writes the pairs directly. When a cursor passes `arena`,
 // no pairs go past it and the host grows the arenas and runs the pass again.
+// A slot whose delta_C is empty (a terminal C after the first round) has no
+// pairs: its rows get empty bounds and old_B is not read.
 __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr old, DCsr pend, DPacked pk,
-                                                  const unsigned long long* bits, uint32_t words, Counters* ctr,
+                                                  const unsigned long long* dlp, const unsigned long long* bits,
+                                                  uint32_t words, Counters* ctr,
                                                   unsigned long long arena, unsigned long long* rng, uint32_t* col) {
   __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][kPosUnroll],
       s_at[kWavesPerBlock][kWave];
@@ -652,6 +655,14 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
   const unsigned long long a0 = uint64_t(blockIdx.x % kSpread) * arena;
   unsigned long long blk = 0, blk_end = 0;  // the wave's reserved words, arena-relative (uniform)
   for (int j = 0; j < g.nslot; ++j) {
+    const uint64_t cb = uint64_t(g.slot_c[j]) * n;
+    if (dlp[cb + n] == dlp[cb]) {
+      for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += stride) {
+        rng[2 * (uint64_t(j) * n + u)] = 0;
+        rng[2 * (uint64_t(j) * n + u) + 1] = 0;
+      }
+      continue;
+    }
     const uint32_t bb = uint32_t(g.slot_b[j]) * n;
     const unsigned long long* fb = bits + uint64_t(j) * words;
     // packed old_B rows: 32 ids per entry, tested against the bitmap at once
@@ -2602,7 +2613,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
         GG_TRY(grow(pr_col, kSpread * pr_arena * 4, err));
         pr_arena = pr_col.bytes / 4 / kSpread;
         zero_ctr();
-        k_pairs<<<pgrid, kBlock, 0, stream>>>(g, n, oc, pend_csr(), opk, P<unsigned long long>(fr_bits), fr_words, C(),
+        k_pairs<<<pgrid, kBlock, 0, stream>>>(g, n, oc, pend_csr(), opk, dlc.ptr, P<unsigned long long>(fr_bits), fr_words, C(),
                                               pr_arena, P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
         GG_HIP(hipGetLastError());
         GG_TRY(read_ctr(err));
