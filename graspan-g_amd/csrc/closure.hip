@@ -636,8 +636,8 @@ __global__ __launch_bounds__(kBlock) void k_frontier(const DevGrammar g, uint32_
 // reserves the group's words in its block's arena (arena blockIdx % kSpread,
 // `arena` words each; a wave takes kPairBlock words at a time from the arena's
 // cursor, so the atomics stay few and spread over 64 lines) and writes the
-// pairs out. A group with a row over kPairRowBuf pairs streams its rows a
-// second time and writes the pairs directly. When a cursor passes `arena`,
+// pairs out. Rows with more than kPairRowBuf pairs are streamed a second
+// time, alone, and write their pairs directly. When a cursor passes `arena`,
 // no pairs go past it and the host grows the arenas and runs the pass again.
 // A slot whose delta_C is empty (a terminal C after the first round) has no
 // pairs: its rows get empty bounds and old_B is not read.
@@ -737,10 +737,13 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
         rng[2 * (uint64_t(j) * n + u) + 1] = lo + c;
       }
       if (tot && fits) {
-        if (__ballot(c > kPairRowBuf) == 0) {
+        const bool big = c > kPairRowBuf;
+        if (!big)
           for (uint32_t t = 0; t < c; ++t) col[a0 + at0 + t] = s_buf[wv][uint32_t(lane) * kPairRowBuf + t];
-        } else {
+        if (__ballot(big)) {
+          // only the rows that did not fit are streamed again
           s_at[wv][lane] = a0 + at0;
+          if (!big) len = ql = 0;
           stream(true);
         }
       }
@@ -2619,8 +2622,12 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
         GG_TRY(read_ctr(err));
         uint64_t most = 0;
         for (const CtrSlot& q : h_ctr->slot) most = std::max<uint64_t>(most, q.pairs);
-        if (most <= pr_arena) break;
-        pr_arena = most + most / 4 + kPairBlock;
+        if (most <= pr_arena) {
+          // pair lists grow over the first rounds: size the next round ahead
+          if (most * 4 > pr_arena * 3) pr_arena = 2 * most;
+          break;
+        }
+        pr_arena = 2 * most + kPairBlock;
       }
       GG_HIP(hipEventRecord(o.b, stream));
       pairs_ready = true;
