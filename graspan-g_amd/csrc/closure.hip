@@ -565,14 +565,17 @@ __device__ inline void stream_tile(unsigned long long start, uint32_t len, unsig
   wave_sync();
 }
 
-// Calls f(k, pos) for every element position pos of the lanes' lists
-// [lb, lb+len) (k = the lane that owns the list), as one flattened stream (see
-// stream_tile), kPosUnroll elements per lane per tile. tab: 64-entry LDS
-// table of the wave; tmask: kPosUnroll LDS words of the wave.
+// Walks the element positions of the lanes' lists [lb, lb+len) as one
+// flattened stream (see stream_tile), U elements per lane per tile, and calls
+// f(own, pos, ok) once per tile with U-entry arrays: ok[q] says that slice q
+// of this lane holds an element, at position pos[q] of the list of lane
+// own[q]. f gets a whole tile so that it can issue all U loads before it
+// uses any. tab: 64-entry LDS table of the wave; tmask: U LDS words of the
+// wave.
 constexpr int kPosUnroll = 4;
-template <class F>
-__device__ inline void flat_positions(unsigned long long lb, uint32_t len, unsigned long long* tab,
-                                      unsigned long long* tmask, int lane, F&& f) {
+template <int U, class F>
+__device__ inline void flat_tiles(unsigned long long lb, uint32_t len, unsigned long long* tab,
+                                  unsigned long long* tmask, int lane, F&& f) {
   unsigned long long total;
   const unsigned long long incl = stream_prefix(len, total);
   if (total == 0) return;
@@ -581,21 +584,33 @@ __device__ inline void flat_positions(unsigned long long lb, uint32_t len, unsig
   wave_sync();
   // owner lane in the top 6 bits; position of element t: low bits + t
   if (len) tab[rank] = ((lb - start) & ((1ull << 58) - 1)) | (uint64_t(lane) << 58);
-  if (lane < kPosUnroll) tmask[lane] = 0;
+  if (lane < U) tmask[lane] = 0;
   wave_sync();
-  for (unsigned long long t0 = 0; t0 < total; t0 += kPosUnroll * kWave) {
+  for (unsigned long long t0 = 0; t0 < total; t0 += U * kWave) {
     const unsigned long long rem = total - t0;
-    const int lim = rem < kPosUnroll * kWave ? int(rem) : kPosUnroll * kWave;
+    const int lim = rem < U * kWave ? int(rem) : U * kWave;
     const int nq = (lim + kWave - 1) / kWave;
-    stream_tile<kPosUnroll>(
+    int own[U];
+    unsigned long long pos[U];
+    bool ok[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      own[q] = 0;
+      pos[q] = 0;
+      ok[q] = false;
+    }
+    stream_tile<U>(
         start, len, t0, tmask, lane,
         [&](int q, uint32_t rk) {
           if (lane < lim - q * kWave) {
             const unsigned long long e = tab[rk];
-            f(int(e >> 58), ((e + t0 + uint32_t(q * kWave + lane)) & ((1ull << 58) - 1)));
+            own[q] = int(e >> 58);
+            pos[q] = (e + t0 + uint32_t(q * kWave + lane)) & ((1ull << 58) - 1);
+            ok[q] = true;
           }
         },
         nq);
+    f(own, pos, ok);
   }
 }
 
@@ -645,8 +660,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
                                                   const unsigned long long* dlp, const unsigned long long* bits,
                                                   uint32_t words, Counters* ctr,
                                                   unsigned long long arena, unsigned long long* rng, uint32_t* col) {
-  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][kPosUnroll],
-      s_at[kWavesPerBlock][kWave];
+  constexpr int U = kPosUnroll;
+  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][U], s_at[kWavesPerBlock][kWave];
   __shared__ uint32_t s_cnt[kWavesPerBlock][kWave], s_buf[kWavesPerBlock][kWave * kPairRowBuf];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
@@ -695,18 +710,38 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
             ++at;
           }
         };
+        // a tile's U entries are loaded, then their U bitmap words, then tested
         if (packed)
-          flat_positions(b, len, s_tab[wv], s_tmask[wv], lane, [&](int k, unsigned long long pos) {
-            const unsigned long long e = pk.ent[pos];
-            const uint32_t w = uint32_t(e);
-            const uint32_t hit = uint32_t(e >> 32) & uint32_t(fb[w >> 1] >> ((w & 1u) * 32u));
-            if (hit) emit(k, w * 32u, hit);
+          flat_tiles<U>(b, len, s_tab[wv], s_tmask[wv], lane, [&](const int* own, const unsigned long long* pos,
+                                                                   const bool* ok) {
+            unsigned long long e[U];
+            uint32_t f[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) e[q] = ok[q] ? pk.ent[pos[q]] : 0ull;
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+              const uint32_t w = uint32_t(e[q]);
+              f[q] = ok[q] ? uint32_t(fb[w >> 1] >> ((w & 1u) * 32u)) : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+              const uint32_t hit = uint32_t(e[q] >> 32) & f[q];
+              if (hit) emit(own[q], uint32_t(e[q]) * 32u, hit);
+            }
           });
         // id rows: old_B(u) when it is not packed, and the pending edges of B(u)
         auto ids = [&](const uint32_t* src, unsigned long long ib, uint32_t il) {
-          flat_positions(ib, il, s_tab[wv], s_tmask[wv], lane, [&](int k, unsigned long long pos) {
-            const uint32_t v = src[pos];
-            if ((fb[v / kWave] >> (v % kWave)) & 1ull) emit(k, v, 1u);
+          flat_tiles<U>(ib, il, s_tab[wv], s_tmask[wv], lane, [&](const int* own, const unsigned long long* pos,
+                                                                   const bool* ok) {
+            uint32_t v[U];
+            unsigned long long f[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) v[q] = ok[q] ? src[pos[q]] : 0u;
+#pragma unroll
+            for (int q = 0; q < U; ++q) f[q] = ok[q] ? fb[v[q] / kWave] : 0ull;
+#pragma unroll
+            for (int q = 0; q < U; ++q)
+              if ((f[q] >> (v[q] % kWave)) & 1ull) emit(own[q], v[q], 1u);
           });
         };
         if (!packed) ids(old.col, b, len);
