@@ -411,6 +411,7 @@ struct ActArgs {
   uint32_t n;
   uint32_t nlhs;
   uint8_t lhs[256];
+  uint16_t u_li[kMaxRules], b_li[kMaxRules];  // per rule: index of its lhs label in lhs[] (0xFFFF: none)
   DevGrammar g;
   uint32_t* active;
   Counters* ctr;
@@ -434,42 +435,77 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
   for (uint32_t bbase = blockIdx.x * blockDim.x; bbase < a.n; bbase += gridDim.x * blockDim.x) {
     const uint32_t u = bbase + uint32_t(wv) * kWave + uint32_t(lane);
     const bool valid = u < a.n;
-    uint32_t wcnt = 0;
-    for (uint32_t li = 0; li < a.nlhs; ++li) {
-      const uint32_t A = a.lhs[li];
-      bool any = false;
-      for (int i = 0; i < a.g.nu; ++i)
-        if (a.g.u_lhs[i] == A) any |= label_nonempty(a.ctr, a.g.u_rhs[i]);
-      for (int i = 0; i < a.g.nb; ++i)
-        if (a.g.b_lhs[i] == A) any |= label_nonempty(a.ctr, a.g.b_rhs1[i]) || label_nonempty(a.ctr, a.g.b_rhs2[i]);
-      bool act = false;
-      if (any && valid) {
-        for (int i = 0; i < a.g.nu; ++i) {
-          if (a.g.u_lhs[i] != A) continue;
+    for (uint32_t li = lane; li < a.nlhs; li += kWave) s_m[wv][li] = 0;
+    wave_sync();
+    // Rules go in chunks of kRuleChunk: every row-bound load of a chunk is
+    // issued before any is used, and each rule's ballot is ORed into the mask
+    // of its lhs label. A rule reads Δ_B(u) only when Δ_B is not empty, and
+    // old_B(u), its pending edges and its pair list only when Δ_C is not
+    // empty (each is empty otherwise).
+    constexpr int kRuleChunk = 4;
+    for (int i0 = 0; i0 < a.g.nu; i0 += kRuleChunk) {
+      unsigned long long lo[kRuleChunk], hi[kRuleChunk];
+#pragma unroll
+      for (int k = 0; k < kRuleChunk; ++k) {
+        const int i = i0 + k;
+        lo[k] = hi[k] = 0;
+        if (i < a.g.nu && a.u_li[i] != 0xFFFFu && valid && label_nonempty(a.ctr, a.g.u_rhs[i])) {
           const uint32_t rb = uint32_t(a.g.u_rhs[i]) * a.n + u;
-          act |= a.dl.ptr[rb + 1] > a.dl.ptr[rb];
+          lo[k] = a.dl.ptr[rb];
+          hi[k] = a.dl.ptr[rb + 1];
         }
-        for (int i = 0; i < a.g.nb; ++i) {
-          if (a.g.b_lhs[i] != A) continue;
+      }
+#pragma unroll
+      for (int k = 0; k < kRuleChunk; ++k) {
+        const int i = i0 + k;
+        if (i < a.g.nu && a.u_li[i] != 0xFFFFu) {
+          const unsigned long long m = __ballot(hi[k] > lo[k]);
+          if (lane == 0) s_m[wv][a.u_li[i]] |= m;
+        }
+      }
+    }
+    for (int i0 = 0; i0 < a.g.nb; i0 += kRuleChunk) {
+      unsigned long long d0[kRuleChunk], d1[kRuleChunk], o0[kRuleChunk], o1[kRuleChunk], q0[kRuleChunk],
+          q1[kRuleChunk], p0[kRuleChunk], p1[kRuleChunk];
+#pragma unroll
+      for (int k = 0; k < kRuleChunk; ++k) {
+        const int i = i0 + k;
+        d0[k] = d1[k] = o0[k] = o1[k] = q0[k] = q1[k] = p0[k] = p1[k] = 0;
+        if (i < a.g.nb && a.b_li[i] != 0xFFFFu && valid) {
           const uint32_t rb = uint32_t(a.g.b_rhs1[i]) * a.n + u;
-          const unsigned long long kl = a.dl.ptr[rb + 1] - a.dl.ptr[rb];
-          pk += kl;
-          act |= kl > 0;
-          if (label_nonempty(a.ctr, a.g.b_rhs2[i])) {
-            pk += a.old.ptr[rb + 1] - a.old.ptr[rb];
-            if (a.pend_ptr) pk += a.pend_ptr[rb + 1] - a.pend_ptr[rb];
+          if (label_nonempty(a.ctr, a.g.b_rhs1[i])) {
+            d0[k] = a.dl.ptr[rb];
+            d1[k] = a.dl.ptr[rb + 1];
           }
-          const uint32_t slot = a.g.b_slot[i];
-          if (slot != 0xFFu && a.b2rng) {
-            const uint64_t rs = uint64_t(slot) * a.n + u;
-            act |= a.b2rng[2 * rs + 1] > a.b2rng[2 * rs];
+          if (label_nonempty(a.ctr, a.g.b_rhs2[i])) {
+            o0[k] = a.old.ptr[rb];
+            o1[k] = a.old.ptr[rb + 1];
+            if (a.pend_ptr) {
+              q0[k] = a.pend_ptr[rb];
+              q1[k] = a.pend_ptr[rb + 1];
+            }
+            const uint32_t slot = a.g.b_slot[i];
+            if (slot != 0xFFu && a.b2rng) {
+              const uint64_t rs = uint64_t(slot) * a.n + u;
+              p0[k] = a.b2rng[2 * rs];
+              p1[k] = a.b2rng[2 * rs + 1];
+            }
           }
         }
       }
-      const unsigned long long m = __ballot(act);
-      if (lane == 0) s_m[wv][li] = m;
-      wcnt += uint32_t(__popcll(m));
+#pragma unroll
+      for (int k = 0; k < kRuleChunk; ++k) {
+        const int i = i0 + k;
+        if (i < a.g.nb && a.b_li[i] != 0xFFFFu) {
+          pk += (d1[k] - d0[k]) + (o1[k] - o0[k]) + (q1[k] - q0[k]);
+          const unsigned long long m = __ballot(d1[k] > d0[k] || p1[k] > p0[k]);
+          if (lane == 0) s_m[wv][a.b_li[i]] |= m;
+        }
+      }
     }
+    wave_sync();
+    uint32_t wcnt = 0;
+    for (uint32_t li = 0; li < a.nlhs; ++li) wcnt += uint32_t(__popcll(s_m[wv][li]));
     if (lane == 0) s_cnt[wv] = wcnt;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -2609,6 +2645,13 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   act.n = n;
   act.nlhs = uint32_t(lhs.size());
   for (size_t i = 0; i < lhs.size(); ++i) act.lhs[i] = lhs[i];
+  for (int i = 0; i < kMaxRules; ++i) act.u_li[i] = act.b_li[i] = 0xFFFF;
+  for (size_t li = 0; li < lhs.size(); ++li) {
+    for (int i = 0; i < g.nu; ++i)
+      if (g.u_lhs[i] == lhs[li]) act.u_li[i] = uint16_t(li);
+    for (int i = 0; i < g.nb; ++i)
+      if (g.b_lhs[i] == lhs[li]) act.b_li[i] = uint16_t(li);
+  }
   act.g = g;
   act.active = P<uint32_t>(active);
   act.ctr = C();
