@@ -66,7 +66,7 @@ constexpr uint64_t kDeferRatio = 128;   // pending edges kept apart while under 
 constexpr uint32_t kMaxCrossParts = 1024;  // partitions of the out-of-core schedule
 constexpr uint32_t kStageBlock = 4096;  // staging words reserved per cursor atomic
 constexpr uint32_t kPairBlock = 1024;   // k_pairs: pair-list words a wave reserves per arena atomic
-constexpr uint32_t kPairRowBuf = 8;     // k_pairs: pairs per row held in LDS before they are written
+constexpr uint32_t kPairRowBuf = 4;     // k_pairs: pairs per row and slot held in LDS before they are written
 constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
 constexpr uint32_t kMergeBits = 16384;   // k_merge: rows spanning fewer ids are merged in an LDS bitmap
 constexpr uint32_t kShortRow = 1024;     // k_merge: unchanged rows up to this length move in the block copy
@@ -678,27 +678,35 @@ __global__ __launch_bounds__(kBlock) void k_frontier(const DevGrammar g, uint32_
   }
 }
 
+// Slots that share their B label are paired, so that old_B is streamed once
+// for both: mate[j] = the slot streamed with j (0xFF: none), 0xFE for the
+// second slot of a pair (it is handled with its mate).
+struct PairPlan {
+  uint8_t mate[kMaxRules];
+};
+
 // Pair lists for old_B x delta_C: row (j, u) lists the v in old_B(u) whose
 // delta_C(v) is not empty (B = slot_b[j], C = slot_c[j]), in no particular
 // order; its bounds are rng[2r], rng[2r+1] (r = j*n+u), so rows need not be
 // contiguous. One pass: a wave takes 64 rows and streams their old_B rows
-// together, testing each v against the slot's frontier bitmap; per-row counts
-// live in LDS, and the first kPairRowBuf pairs of each row too. The wave then
-// reserves the group's words in its block's arena (arena blockIdx % kSpread,
-// `arena` words each; a wave takes kPairBlock words at a time from the arena's
-// cursor, so the atomics stay few and spread over 64 lines) and writes the
-// pairs out. Rows with more than kPairRowBuf pairs are streamed a second
-// time, alone, and write their pairs directly. When a cursor passes `arena`,
-// no pairs go past it and the host grows the arenas and runs the pass again.
-// A slot whose delta_C is empty (a terminal C after the first round) has no
-// pairs: its rows get empty bounds and old_B is not read.
-__global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n, DCsr old, DCsr pend, DPacked pk,
-                                                  const unsigned long long* dlp, const unsigned long long* bits,
-                                                  uint32_t words, Counters* ctr,
+// together, testing each v against the frontier bitmaps of the slot and its
+// mate; per-row counts live in LDS, and the first kPairRowBuf pairs of each
+// row too. The wave then reserves each slot's words in its block's arena
+// (arena blockIdx % kSpread, `arena` words each; a wave takes kPairBlock
+// words at a time from the arena's cursor, so the atomics stay few and spread
+// over 64 lines) and writes the pairs out. Rows with more than kPairRowBuf
+// pairs are streamed a second time, alone, and write their pairs directly.
+// When a cursor passes `arena`, no pairs go past it and the host grows the
+// arenas and runs the pass again. A slot whose delta_C is empty has no pairs:
+// its rows get empty bounds.
+__global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, const PairPlan plan, uint32_t n, DCsr old,
+                                                  DCsr pend, DPacked pk, const unsigned long long* dlp,
+                                                  const unsigned long long* bits, uint32_t words, Counters* ctr,
                                                   unsigned long long arena, unsigned long long* rng, uint32_t* col) {
   constexpr int U = kPosUnroll;
-  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][U], s_at[kWavesPerBlock][kWave];
-  __shared__ uint32_t s_cnt[kWavesPerBlock][kWave], s_buf[kWavesPerBlock][kWave * kPairRowBuf];
+  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][U],
+      s_at[kWavesPerBlock][2][kWave];
+  __shared__ uint32_t s_cnt[kWavesPerBlock][2][kWave], s_buf[kWavesPerBlock][2][kWave * kPairRowBuf];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -706,63 +714,85 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
   const unsigned long long a0 = uint64_t(blockIdx.x % kSpread) * arena;
   unsigned long long blk = 0, blk_end = 0;  // the wave's reserved words, arena-relative (uniform)
   for (int j = 0; j < g.nslot; ++j) {
-    const uint64_t cb = uint64_t(g.slot_c[j]) * n;
-    if (dlp[cb + n] == dlp[cb]) {
-      for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += stride) {
-        rng[2 * (uint64_t(j) * n + u)] = 0;
-        rng[2 * (uint64_t(j) * n + u) + 1] = 0;
-      }
-      continue;
+    if (plan.mate[j] == 0xFEu) continue;
+    int js[2] = {j, plan.mate[j] == 0xFFu ? -1 : int(plan.mate[j])};
+    bool live[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      live[h] = false;
+      if (js[h] < 0) continue;
+      const uint64_t cb = uint64_t(g.slot_c[js[h]]) * n;
+      live[h] = dlp[cb + n] != dlp[cb];
+      if (!live[h])
+        for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += stride) {
+          rng[2 * (uint64_t(js[h]) * n + u)] = 0;
+          rng[2 * (uint64_t(js[h]) * n + u) + 1] = 0;
+        }
     }
-    const uint32_t bb = uint32_t(g.slot_b[j]) * n;
-    const unsigned long long* fb = bits + uint64_t(j) * words;
-    // packed old_B rows: 32 ids per entry, tested against the bitmap at once
-    const bool packed = pk.len && packed_label(pk, g.slot_b[j]);
+    if (!live[0]) {
+      js[0] = js[1];
+      live[0] = live[1];
+      live[1] = false;
+    }
+    if (!live[0]) continue;
+    const bool two = live[1];
+    const uint32_t bb = uint32_t(g.slot_b[js[0]]) * n;
+    const unsigned long long* fb0 = bits + uint64_t(js[0]) * words;
+    const unsigned long long* fb1 = two ? bits + uint64_t(js[1]) * words : fb0;
+    // packed old_B rows: 32 ids per entry, tested against the bitmaps at once
+    const bool packed = pk.len && packed_label(pk, g.slot_b[js[0]]);
     for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < n; base += stride) {
       const uint32_t u = base + lane;
       unsigned long long b = 0, qb = 0;
-      uint32_t len = 0, ql = 0;
+      uint32_t len0 = 0, ql0 = 0;
       if (u < n) {
         b = old.ptr[bb + u];
-        len = packed ? pk.len[bb + u] : uint32_t(old.ptr[bb + u + 1] - b);
+        len0 = packed ? pk.len[bb + u] : uint32_t(old.ptr[bb + u + 1] - b);
         if (pend.ptr) {
           qb = pend.ptr[bb + u];
-          ql = uint32_t(pend.ptr[bb + u + 1] - qb);
+          ql0 = uint32_t(pend.ptr[bb + u + 1] - qb);
         }
       }
-      // direct: write pair `at` of row k to col[s_at[k] + at]; else keep it in LDS
-      auto stream = [&](bool direct) {
-        s_cnt[wv][lane] = 0;
+      uint32_t len = len0, ql = ql0;
+      // mode 0: both slots' pairs counted, the first kPairRowBuf of a row kept
+      // in LDS; mode 1 + h: slot h's pair `at` of row k written to
+      // col[s_at[h][k] + at]
+      auto stream = [&](int mode) {
+        if (mode == 0) s_cnt[wv][1][lane] = 0;
+        s_cnt[wv][mode == 2 ? 1 : 0][lane] = 0;
         wave_sync();
-        auto emit = [&](int k, uint32_t id0, uint32_t hit) {
-          uint32_t at = atomicAdd(&s_cnt[wv][k], uint32_t(__popc(hit)));
+        auto emit = [&](int h, int k, uint32_t id0, uint32_t hit) {
+          uint32_t at = atomicAdd(&s_cnt[wv][h][k], uint32_t(__popc(hit)));
           while (hit) {
             const uint32_t id = id0 + uint32_t(__ffs(hit) - 1);
             hit &= hit - 1;
-            if (direct)
-              col[s_at[wv][k] + at] = id;
+            if (mode != 0)
+              col[s_at[wv][h][k] + at] = id;
             else if (at < kPairRowBuf)
-              s_buf[wv][uint32_t(k) * kPairRowBuf + at] = id;
+              s_buf[wv][h][uint32_t(k) * kPairRowBuf + at] = id;
             ++at;
           }
         };
-        // a tile's U entries are loaded, then their U bitmap words, then tested
+        const bool t0 = mode != 2, t1 = two && mode != 1;
+        // a tile's U entries are loaded, then their bitmap words, then tested
         if (packed)
           flat_tiles<U>(b, len, s_tab[wv], s_tmask[wv], lane, [&](const int* own, const unsigned long long* pos,
                                                                    const bool* ok) {
             unsigned long long e[U];
-            uint32_t f[U];
+            uint32_t f0[U], f1[U];
 #pragma unroll
             for (int q = 0; q < U; ++q) e[q] = ok[q] ? pk.ent[pos[q]] : 0ull;
 #pragma unroll
             for (int q = 0; q < U; ++q) {
-              const uint32_t w = uint32_t(e[q]);
-              f[q] = ok[q] ? uint32_t(fb[w >> 1] >> ((w & 1u) * 32u)) : 0u;
+              const uint32_t w = uint32_t(e[q]), sh = (w & 1u) * 32u;
+              f0[q] = ok[q] && t0 ? uint32_t(fb0[w >> 1] >> sh) : 0u;
+              f1[q] = ok[q] && t1 ? uint32_t(fb1[w >> 1] >> sh) : 0u;
             }
 #pragma unroll
             for (int q = 0; q < U; ++q) {
-              const uint32_t hit = uint32_t(e[q] >> 32) & f[q];
-              if (hit) emit(own[q], uint32_t(e[q]) * 32u, hit);
+              const uint32_t m = uint32_t(e[q] >> 32), h0 = m & f0[q], h1 = m & f1[q];
+              if (h0) emit(0, own[q], uint32_t(e[q]) * 32u, h0);
+              if (h1) emit(1, own[q], uint32_t(e[q]) * 32u, h1);
             }
           });
         // id rows: old_B(u) when it is not packed, and the pending edges of B(u)
@@ -770,52 +800,63 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const DevGrammar g, uint32_t n
           flat_tiles<U>(ib, il, s_tab[wv], s_tmask[wv], lane, [&](const int* own, const unsigned long long* pos,
                                                                    const bool* ok) {
             uint32_t v[U];
-            unsigned long long f[U];
+            unsigned long long f0[U], f1[U];
 #pragma unroll
             for (int q = 0; q < U; ++q) v[q] = ok[q] ? src[pos[q]] : 0u;
 #pragma unroll
-            for (int q = 0; q < U; ++q) f[q] = ok[q] ? fb[v[q] / kWave] : 0ull;
+            for (int q = 0; q < U; ++q) {
+              f0[q] = ok[q] && t0 ? fb0[v[q] / kWave] : 0ull;
+              f1[q] = ok[q] && t1 ? fb1[v[q] / kWave] : 0ull;
+            }
 #pragma unroll
-            for (int q = 0; q < U; ++q)
-              if ((f[q] >> (v[q] % kWave)) & 1ull) emit(own[q], v[q], 1u);
+            for (int q = 0; q < U; ++q) {
+              if ((f0[q] >> (v[q] % kWave)) & 1ull) emit(0, own[q], v[q], 1u);
+              if ((f1[q] >> (v[q] % kWave)) & 1ull) emit(1, own[q], v[q], 1u);
+            }
           });
         };
         if (!packed) ids(old.col, b, len);
         if (pend.ptr) ids(pend.col, qb, ql);
         wave_sync();
       };
-      stream(false);
-      const uint32_t c = s_cnt[wv][lane];
-      unsigned long long tot;
-      const unsigned long long ex = stream_prefix(c, tot) - c;
-      unsigned long long at0 = 0;
-      bool fits = true;
-      if (tot) {
-        if (blk + tot > blk_end) {
-          const unsigned long long grab = tot > kPairBlock ? tot : kPairBlock;
-          unsigned long long nb = 0;
-          if (lane == 0) nb = atomicAdd(cursor, grab);
-          blk = __shfl(nb, 0, kWave);
-          blk_end = blk + grab;
+      stream(0);
+      const uint32_t c0 = s_cnt[wv][0][lane], c1 = two ? s_cnt[wv][1][lane] : 0u;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !two) break;
+        const uint32_t c = h ? c1 : c0;
+        unsigned long long tot;
+        const unsigned long long ex = stream_prefix(c, tot) - c;
+        unsigned long long at0 = 0;
+        bool fits = true;
+        if (tot) {
+          if (blk + tot > blk_end) {
+            const unsigned long long grab = tot > kPairBlock ? tot : kPairBlock;
+            unsigned long long nb = 0;
+            if (lane == 0) nb = atomicAdd(cursor, grab);
+            blk = __shfl(nb, 0, kWave);
+            blk_end = blk + grab;
+          }
+          at0 = blk + ex;
+          fits = blk + tot <= arena;
+          blk += tot;
         }
-        at0 = blk + ex;
-        fits = blk + tot <= arena;
-        blk += tot;
-      }
-      if (u < n) {
-        const unsigned long long lo = a0 + at0;
-        rng[2 * (uint64_t(j) * n + u)] = lo;
-        rng[2 * (uint64_t(j) * n + u) + 1] = lo + c;
-      }
-      if (tot && fits) {
-        const bool big = c > kPairRowBuf;
-        if (!big)
-          for (uint32_t t = 0; t < c; ++t) col[a0 + at0 + t] = s_buf[wv][uint32_t(lane) * kPairRowBuf + t];
-        if (__ballot(big)) {
-          // only the rows that did not fit are streamed again
-          s_at[wv][lane] = a0 + at0;
-          if (!big) len = ql = 0;
-          stream(true);
+        if (u < n) {
+          const unsigned long long lo = a0 + at0;
+          rng[2 * (uint64_t(js[h]) * n + u)] = lo;
+          rng[2 * (uint64_t(js[h]) * n + u) + 1] = lo + c;
+        }
+        if (tot && fits) {
+          const bool big = c > kPairRowBuf;
+          if (!big)
+            for (uint32_t t = 0; t < c; ++t) col[a0 + at0 + t] = s_buf[wv][h][uint32_t(lane) * kPairRowBuf + t];
+          if (__ballot(big)) {
+            // only the rows that did not fit are streamed again
+            s_at[wv][h][lane] = a0 + at0;
+            len = big ? len0 : 0u;
+            ql = big ? ql0 : 0u;
+            stream(1 + h);
+          }
         }
       }
       wave_sync();
@@ -2657,6 +2698,18 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   act.ctr = C();
   const uint64_t act_work = uint64_t(act.nlhs) * n;
   const uint64_t NS = uint64_t(g.nslot) * n;
+  // slots with the same B are streamed together, two at a time
+  PairPlan pair_plan{};
+  for (int j = 0; j < kMaxRules; ++j) pair_plan.mate[j] = 0xFF;
+  for (int j = 0; j < g.nslot; ++j) {
+    if (pair_plan.mate[j] != 0xFF) continue;
+    for (int j2 = j + 1; j2 < g.nslot; ++j2)
+      if (pair_plan.mate[j2] == 0xFF && g.slot_b[j2] == g.slot_b[j]) {
+        pair_plan.mate[j] = uint8_t(j2);
+        pair_plan.mate[j2] = 0xFE;
+        break;
+      }
+  }
   const uint32_t fr_words = (n + kWave - 1) / kWave;
   pr_arena = 0;
   if (NS) {
@@ -2694,7 +2747,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
         GG_TRY(grow(pr_col, kSpread * pr_arena * 4, err));
         pr_arena = pr_col.bytes / 4 / kSpread;
         zero_ctr();
-        k_pairs<<<pgrid, kBlock, 0, stream>>>(g, n, oc, pend_csr(), opk, dlc.ptr, P<unsigned long long>(fr_bits), fr_words, C(),
+        k_pairs<<<pgrid, kBlock, 0, stream>>>(g, pair_plan, n, oc, pend_csr(), opk, dlc.ptr, P<unsigned long long>(fr_bits),
+                                              fr_words, C(),
                                               pr_arena, P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
         GG_HIP(hipGetLastError());
         GG_TRY(read_ctr(err));
