@@ -2079,11 +2079,21 @@ __global__ __launch_bounds__(kBlock) void k_unpack(const unsigned long long* ptr
       if (pl) b = ptr[r];
     }
     unsigned long long todo = __ballot(pl != 0);
+    // a row's first 64 entries, one per lane; the next row's load while the
+    // current row expands
+    auto first = [&](int k) -> unsigned long long {
+      const uint32_t pk = uint32_t(__builtin_amdgcn_readlane(int(pl), k));
+      const unsigned long long bk = lane_u64(b, uint32_t(k));
+      return uint32_t(lane) < pk ? plab.ent[bk + uint32_t(lane)] : 0ull;
+    };
+    unsigned long long ev_next = todo ? first(__ffsll(todo) - 1) : 0ull;
     while (todo) {
       const int k = __ffsll(todo) - 1;
       todo &= todo - 1;
       const uint32_t pk = uint32_t(__builtin_amdgcn_readlane(int(pl), k));
       const unsigned long long bk = lane_u64(b, uint32_t(k));
+      const unsigned long long ev0 = ev_next;
+      if (todo) ev_next = first(__ffsll(todo) - 1);
       // the row's entries 64 at a time, one per lane; then two entries per
       // wave step (moved to lanes 0-31 and 32-63 by shuffle), one bit per
       // lane, so each step stores its ids contiguously
@@ -2091,7 +2101,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack(const unsigned long long* ptr
       unsigned long long o = bk;
       for (uint32_t j0 = 0; j0 < pk; j0 += kWave) {
         const uint32_t t = j0 + uint32_t(lane);
-        const unsigned long long ev = t < pk ? plab.ent[bk + t] : 0ull;
+        const unsigned long long ev = j0 == 0 ? ev0 : (t < pk ? plab.ent[bk + t] : 0ull);
         const uint32_t nloc = min(uint32_t(kWave), pk - j0);
         for (uint32_t j = 0; j < nloc; j += 2) {
           const uint32_t src = j + uint32_t(lane >> 5);  // <= 64: lane 64 wraps to 0 and is masked below
