@@ -2520,9 +2520,9 @@ class Engine {
     GG_TRY(grow(mlist, uint64_t(rows) * 4, err));
     GG_TRY(alloc(mcnt, 4, err));
     fill_u32(mcnt, 0u, 1);
-    k_merge<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, dc, rows, nn, c, plab, old_pk, out_col, out_pk, pklen,
+    k_merge<<<uint32_t(cus) * uint32_t(merge_blocks_per_cu), kBlock, 0, stream>>>(oc, dc, rows, nn, c, plab, old_pk, out_col, out_pk, pklen,
                                                       P<uint32_t>(mlist), P<uint32_t>(mcnt), ids);
-    k_merge_rows<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(oc, dc, nn, plab, old_pk, out_col, out_pk, pklen,
+    k_merge_rows<<<uint32_t(cus) * uint32_t(merge_rows_blocks_per_cu), kBlock, 0, stream>>>(oc, dc, nn, plab, old_pk, out_col, out_pk, pklen,
                                                            P<uint32_t>(mlist), P<uint32_t>(mcnt), ids);
     GG_HIP(hipGetLastError());
     return GG_OK;
@@ -2557,6 +2557,7 @@ class Engine {
   }
 
   int expand_blocks_per_cu = 4;
+  int merge_blocks_per_cu = 8, merge_rows_blocks_per_cu = 8;
   uint32_t expand_grid() const { return uint32_t(cus) * uint32_t(expand_blocks_per_cu); }
   uint32_t rows() const { return uint32_t(uint64_t(L) * n); }
 
@@ -3550,6 +3551,19 @@ Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std
           hipSuccess &&
       bpc > 0)
     en->expand_blocks_per_cu = bpc;
+  // the merge kernels hand each wave a fixed share of the rows, so their
+  // grids are what fits on the device at once (no second, late wave of blocks)
+  bpc = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_merge), kBlock, 0) ==
+          hipSuccess &&
+      bpc > 0)
+    en->merge_blocks_per_cu = bpc;
+  bpc = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_merge_rows), kBlock, 0) ==
+          hipSuccess &&
+      bpc > 0)
+    en->merge_rows_blocks_per_cu = bpc;
+  (void)hipGetLastError();
   status = GG_OK;
   return en;
 }
