@@ -2557,8 +2557,9 @@ class Engine {
   }
 
   int expand_blocks_per_cu = 4;
-  int merge_blocks_per_cu = 8, merge_rows_blocks_per_cu = 8;
+  int merge_blocks_per_cu = 8, merge_rows_blocks_per_cu = 8, gather_blocks_per_cu = 7;
   uint32_t expand_grid() const { return uint32_t(cus) * uint32_t(expand_blocks_per_cu); }
+  uint32_t gather_grid() const { return uint32_t(cus) * uint32_t(gather_blocks_per_cu); }
   uint32_t rows() const { return uint32_t(uint64_t(L) * n); }
 
   int run(int filter, gg_stats* st, std::string& err);
@@ -2679,7 +2680,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_TRY(grow(dpk_ent, std::max<uint64_t>(N0, 1) * 8, err));
       dpk_ready = true;
     }
-    k_gather<<<expand_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
+    k_gather<<<gather_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
                                                    P<unsigned long long>(seg_off), P<uint32_t>(staging),
                                                    P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col), n, plabels(),
                                                    dpk_ready ? P<unsigned long long>(dpk_ent) : nullptr,
@@ -2841,7 +2842,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_TRY(grow(npk_ent, std::max<uint64_t>(N, 1) * 8, err));
     }
     if (N)
-      k_gather<<<expand_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
+      k_gather<<<gather_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
                                                      P<unsigned long long>(seg_off), P<uint32_t>(staging),
                                                      P<unsigned long long>(nx_ptr), P<uint32_t>(nx_col), n, plabels(),
                                                      npk ? P<unsigned long long>(npk_ent) : nullptr,
@@ -3256,7 +3257,7 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     GG_TRY(scan(seg_cnt, rows, F.ptr, err));
     GG_TRY(grow(F.col, std::max<uint64_t>(N, 1) * 4, err));
     if (N)
-      k_gather<<<expand_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
+      k_gather<<<gather_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
                                                      P<unsigned long long>(seg_off), P<uint32_t>(staging),
                                                      P<unsigned long long>(F.ptr), P<uint32_t>(F.col), n, DPacked{},
                                                      nullptr, nullptr);
@@ -3551,7 +3552,7 @@ Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std
           hipSuccess &&
       bpc > 0)
     en->expand_blocks_per_cu = bpc;
-  // the merge kernels hand each wave a fixed share of the rows, so their
+  // the merge and gather kernels hand each wave a fixed share of the rows, so their
   // grids are what fits on the device at once (no second, late wave of blocks)
   bpc = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_merge), kBlock, 0) ==
@@ -3563,6 +3564,11 @@ Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std
           hipSuccess &&
       bpc > 0)
     en->merge_rows_blocks_per_cu = bpc;
+  bpc = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_gather), kBlock, 0) ==
+          hipSuccess &&
+      bpc > 0)
+    en->gather_blocks_per_cu = bpc;
   (void)hipGetLastError();
   status = GG_OK;
   return en;
