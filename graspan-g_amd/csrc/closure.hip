@@ -2590,6 +2590,10 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   if (const char* dr = std::getenv("GG_DEFER_RATIO")) defer_ratio = std::strtoull(dr, nullptr, 10);
   keep_ids = false;
   if (const char* ki = std::getenv("GG_KEEP_PACKED_IDS")) keep_ids = std::atoi(ki) != 0;
+  // words per pair arena for every round's first k_pairs pass (0: sized by
+  // use); a small value makes the passes overflow and rerun
+  uint64_t pair_arena_forced = 0;
+  if (const char* pa = std::getenv("GG_PAIR_ARENA")) pair_arena_forced = std::strtoull(pa, nullptr, 10);
   if (uint64_t(L) * n >= 0xFFFFFFFFull || uint64_t(g.nslot) * n >= 0xFFFFFFFFull) {
     err = "num_labels * num_vertices (and binary rules with a derived second label * num_vertices) must be < 2^32";
     return GG_E_UNSUPPORTED;
@@ -2755,10 +2759,10 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       // again with larger ones
       const uint32_t pgrid = grid_for(n);
       const uint64_t pwaves = uint64_t((pgrid + kSpread - 1) / kSpread) * kWavesPerBlock;
-      pr_arena = std::max<uint64_t>(pr_arena, 2 * pwaves * kPairBlock);
-      for (;;) {
+      pr_arena = pair_arena_forced ? pair_arena_forced : std::max<uint64_t>(pr_arena, 2 * pwaves * kPairBlock);
+      for (bool first = true;; first = false) {
         GG_TRY(grow(pr_col, kSpread * pr_arena * 4, err));
-        pr_arena = pr_col.bytes / 4 / kSpread;
+        if (!pair_arena_forced || !first) pr_arena = pr_col.bytes / 4 / kSpread;
         zero_ctr();
         k_pairs<<<pgrid, kBlock, 0, stream>>>(g, pair_plan, n, oc, pend_csr(), opk, dlc.ptr, P<unsigned long long>(fr_bits),
                                               fr_words, C(),

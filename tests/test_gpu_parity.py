@@ -253,6 +253,18 @@ def test_gpu_packed_ids_kept_matches_oracle(built, monkeypatch):
     check_against_oracle(_paired_blocks(120_000, seed=32), full_triples=False)
 
 
+def test_gpu_pair_arena_overflow_matches_oracle(built, monkeypatch):
+    """GG_PAIR_ARENA=64 starts every round's pair-list pass with 64-word
+    arenas: a wave reserves 1024 words at a time, so each pass with pairs
+    overflows, skips its writes past the arenas and runs again with grown
+    ones. The closure still equals the oracle, P/K/O/N counters included."""
+    monkeypatch.setenv("GG_PAIR_ARENA", "64")
+    for name, g in [("alias n=20000", gen.alias(20_000, seed=41)),
+                    ("dataflow n=3000", gen.dataflow(3_000, seed=42))]:
+        say("pair arenas forced small", name)
+        check_against_oracle(g)
+
+
 def test_gpu_alias_1m_full_size(built):
     """The benchmark workload itself (alias generator, n = 1,000,000): per-label
     counts, checksum, round count and the P/K/O/N counters equal the oracle's."""
