@@ -66,6 +66,7 @@ void add_stats(gg_stats& a, const gg_stats& b) {
   a.old_reads += b.old_reads;
   a.new_edges += b.new_edges;
   a.device_bytes_peak = std::max(a.device_bytes_peak, b.device_bytes_peak);
+  a.pair_reruns += b.pair_reruns;
 }
 
 }  // namespace
@@ -132,23 +133,36 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
   size_t i = 0;
   while (i < G) {
     // one launch group: graphs ord[i..j-1]
+    // A group's combined grammar must stay within the engine's limits: at
+    // most kMaxLabels labels, kMaxRulesPerKind unary and binary rules (eps
+    // rules become loops), and labels * vertices and pair slots * vertices
+    // below 2^32. A grammar that would pass one of them starts a new group.
     std::map<std::pair<int, std::vector<int32_t>>, uint32_t> grammar_base;
     uint32_t labels = 0;
+    int unary = 0, binary = 0, slots = 0;
     uint64_t verts = 0;
     size_t j = i;
     for (; j < G; ++j) {
       const auto& g = b->graphs[ord[j]];
       auto key = std::make_pair(g.L, g.rules);
       const bool fresh = grammar_base.find(key) == grammar_base.end();
+      gg::RuleKinds k{};
+      if (fresh) k = gg::rule_kinds(g.rules.data(), int(g.rules.size() / 3));
       const uint32_t nl = labels + (fresh ? uint32_t(g.L) : 0u);
+      const int nu = unary + k.unary, nb = binary + k.binary, ns = slots + k.slots;
       const uint64_t nv = verts + g.n;
       const uint64_t lim = nl ? (0xFFFFFFFEull / nl) : nv;
-      const bool over = nl > uint32_t(gg::kMaxLabels) || nv > lim || (max_vertices && nv > max_vertices);
+      const uint64_t slim = ns ? (0xFFFFFFFEull / uint64_t(ns)) : nv;
+      const bool over = nl > uint32_t(gg::kMaxLabels) || nu > gg::kMaxRulesPerKind || nb > gg::kMaxRulesPerKind ||
+                        nv > lim || nv > slim || (max_vertices && nv > max_vertices);
       if (over && j > i) break;
       if (over) return fail(b, GG_E_UNSUPPORTED, "graph " + std::to_string(ord[j]) + " alone exceeds the launch limits");
       if (fresh) {
         grammar_base[key] = labels;
         labels = nl;
+        unary = nu;
+        binary = nb;
+        slots = ns;
       }
       verts = nv;
     }

@@ -70,7 +70,7 @@ constexpr uint32_t kPairRowBuf = 4;     // k_pairs: pairs per row and slot held 
 constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
 constexpr uint32_t kMergeBits = 16384;   // k_merge: rows spanning fewer ids are merged in an LDS bitmap
 constexpr uint32_t kShortRow = 1024;     // k_merge: unchanged rows up to this length move in the block copy
-constexpr int kMaxRules = 64;
+constexpr int kMaxRules = kMaxRulesPerKind;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 
@@ -2778,6 +2778,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
           if (most * 4 > pr_arena * 3) pr_arena = 2 * most;
           break;
         }
+        S.pair_reruns++;
         pr_arena = 2 * most + kPairBlock;
       }
       GG_HIP(hipEventRecord(o.b, stream));
@@ -2964,6 +2965,16 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   if (!has_grammar || !has_graph) {
     err = "grammar and graph must be loaded before gg_closure_run";
     return GG_E_STATE;
+  }
+  // row ids (label * vertices + vertex) are uint32 in every region and in
+  // the assembled closure, which has all n vertices
+  if (uint64_t(L) * n >= 0xFFFFFFFFull) {
+    err = "num_labels * num_vertices must be < 2^32 (row ids are 32-bit)";
+    return GG_E_UNSUPPORTED;
+  }
+  if (parts.empty()) {
+    err = "the partitioned schedule needs at least one vertex interval";
+    return GG_E_ARG;
   }
   GG_HIP(hipSetDevice(device));
   struct HostBuf {

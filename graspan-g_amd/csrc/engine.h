@@ -13,6 +13,18 @@ namespace gg {
 
 class Engine;
 
+// The engine takes at most this many eps, unary and binary rules each.
+constexpr int kMaxRulesPerKind = 64;
+
+// Rules of each kind in a rule table (lhs, rhs1, rhs2 triples, 0 = absent)
+// and its pair-list slots: binary rules whose second label is the lhs of a
+// unary or binary rule. The engine holds one row per slot and vertex for the
+// pair lists, so slots * vertices must stay below 2^32 like labels * vertices.
+struct RuleKinds {
+  int eps = 0, unary = 0, binary = 0, slots = 0;
+};
+RuleKinds rule_kinds(const int32_t* rules, int num_rules);
+
 // Creates the device engine on `device` (-1 = most CUs). Returns nullptr and
 // sets status/err on failure.
 Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std::string& err);

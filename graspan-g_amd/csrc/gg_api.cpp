@@ -1,6 +1,7 @@
 // C ABI of the closure engine (declarations and reference citations in
 // include/graspan_g.h). Host C++; the device work lives in closure.hip.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <utility>
@@ -58,6 +59,17 @@ int push_graph(gg_ctx* c) {
 // reference's fixed DEVMEMUSED_MB = 10000 (src/kernel.h:26, src/kernel.cu:35);
 // a partition may hold heap/4 words of 8*elements + roundup32(vertices)*labels.
 // One partition that fits is split at the element midpoint.
+// GG_REF_HEAP_WORDS replaces the heap size (in 32-bit words), so that tests
+// can reach the multi-partition cut loop (src/main.cpp:145-155) with small
+// graphs.
+uint64_t reference_heap_words() {
+  if (const char* h = std::getenv("GG_REF_HEAP_WORDS")) {
+    const uint64_t w = std::strtoull(h, nullptr, 10);
+    if (w) return w;
+  }
+  return (10000ull / 8 * 7) * 1024 * 256;  // allocateElementPool, src/kernel.cu:35
+}
+
 int reference_partitions(const gg::Graph& gr, int L, std::vector<std::pair<uint32_t, uint32_t>>& parts,
                          std::string& err) {
   const uint32_t n = gr.n;
@@ -71,8 +83,7 @@ int reference_partitions(const gg::Graph& gr, int L, std::vector<std::pair<uint3
       elems[u] = uint64_t(std::unique(k.begin(), k.end()) - k.begin());
     }
   }
-  const uint64_t heap = (10000ull / 8 * 7) * 1024 * 256;
-  const uint64_t limit = heap / 4;
+  const uint64_t limit = reference_heap_words() / 4;
   auto rup32 = [](uint64_t x) { return (x + 31) / 32 * 32; };
   parts.clear();
   uint64_t t1 = 0, t3 = 0;
@@ -109,6 +120,28 @@ int reference_partitions(const gg::Graph& gr, int L, std::vector<std::pair<uint3
 }  // namespace
 
 namespace gg {
+RuleKinds rule_kinds(const int32_t* rules, int num_rules) {
+  RuleKinds k;
+  std::vector<int32_t> derived;  // lhs of unary and binary rules
+  for (int i = 0; i < num_rules; ++i) {
+    const int32_t* r = rules + 3 * i;
+    if (r[1] == 0) {
+      ++k.eps;
+    } else {
+      derived.push_back(r[0]);
+      if (r[2] == 0)
+        ++k.unary;
+      else
+        ++k.binary;
+    }
+  }
+  for (int i = 0; i < num_rules; ++i) {
+    const int32_t* r = rules + 3 * i;
+    if (r[1] != 0 && r[2] != 0 && std::find(derived.begin(), derived.end(), r[2]) != derived.end()) ++k.slots;
+  }
+  return k;
+}
+
 // The engine of a context, created on first use (batch.cpp).
 Engine* ctx_engine(gg_ctx* c, int& status, std::string*& err) {
   err = &c->err;

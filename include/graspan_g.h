@@ -76,6 +76,8 @@ typedef struct gg_stats {
   uint32_t pairs_skipped;   /* out-of-core: pairs with no crossing edges or nothing new */
   uint64_t spill_bytes;     /* out-of-core: bytes of evicted partitions moved between HBM and host DRAM
                                (out and back in; the input's first upload is not counted) */
+  uint32_t pair_reruns;     /* passes of the old x delta pair-list kernel re-run after growing its arenas */
+  uint32_t pad0;
 } gg_stats;
 
 /* ---- context ------------------------------------------------------------ */

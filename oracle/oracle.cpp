@@ -43,6 +43,7 @@
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <iterator>
 #include <string>
@@ -494,29 +495,51 @@ void merge_and_diff(State& s, int p) {
   }
 }
 
+// Planted mistakes, for the tests that show the hand-derived filter
+// fixtures can see them (ORC_MODE_R_MUTATION; 0 = the transcription as is):
+//   1  with filter != 0, skip delta_B x old_C and run old_B x delta_C instead
+//   2  drop spagpu's tmp x tmp launches
+//   3  run spagpu's loop once instead of until nothing changes
+int mutation() {
+  const char* m = std::getenv("ORC_MODE_R_MUTATION");
+  return m ? std::atoi(m) : 0;
+}
+
 void spagpu_s(State& s, int p) {
   Part& P = s.parts[p];
+  const bool swap = s.filter != 0 && mutation() == 1;
   P.has_tmp = true;
   if (!P.has_old) eps(s, p);
   if (s.filter == 0) unary(s, p, kDl);
   if (P.has_old) {
-    if (s.filter == 0) join(s, p, kOld, {{kDl, p}});
-    join(s, p, kDl, {{kOld, p}, {kDl, p}});
+    if (s.filter == 0 || swap) join(s, p, kOld, {{kDl, p}});
+    if (swap)
+      join(s, p, kDl, {{kDl, p}});
+    else
+      join(s, p, kDl, {{kOld, p}, {kDl, p}});
   } else {
     join(s, p, kDl, {{kDl, p}});
   }
 }
 
 void spagpu_b(State& s, int i, int j) {
+  const bool swap = s.filter != 0 && mutation() == 1;
   s.parts[i].has_tmp = true;
   s.parts[j].has_tmp = true;
-  if (s.filter == 0) join(s, i, kOld, {{kDl, j}});
-  join(s, i, kDl, {{kOld, j}, {kDl, j}});
-  if (s.filter == 0) join(s, j, kOld, {{kDl, i}});
-  join(s, j, kDl, {{kOld, i}, {kDl, i}});
+  if (s.filter == 0 || swap) join(s, i, kOld, {{kDl, j}});
+  if (swap)
+    join(s, i, kDl, {{kDl, j}});
+  else
+    join(s, i, kDl, {{kOld, j}, {kDl, j}});
+  if (s.filter == 0 || swap) join(s, j, kOld, {{kDl, i}});
+  if (swap)
+    join(s, j, kDl, {{kDl, i}});
+  else
+    join(s, j, kDl, {{kOld, i}, {kDl, i}});
 }
 
 void spagpu(State& s, int i, int j) {
+  const int mut = mutation();
   for (;;) {
     s.changed = false;
     for (int side = 0; side < 2; ++side) {
@@ -526,9 +549,9 @@ void spagpu(State& s, int i, int j) {
       join(s, p, kTmp, {{kOld, p}, {kDl, p}});
       join(s, p, kTmp, {{kOld, q}, {kDl, q}});
       if (s.filter == 0) unary(s, p, kTmp);
-      join(s, p, kTmp, {{kTmp, i}, {kTmp, j}});
+      if (mut != 2) join(s, p, kTmp, {{kTmp, i}, {kTmp, j}});
     }
-    if (!s.changed) break;
+    if (!s.changed || mut == 3) break;
   }
 }
 
@@ -551,7 +574,11 @@ bool make_partitions(State& s, uint64_t m, const uint32_t* src, const uint32_t* 
       elems[u] = uint64_t(std::unique(k.begin(), k.end()) - k.begin());
     }
   }
-  const uint64_t heap = (10000ull / 8 * 7) * 1024 * 256;
+  // the heap of allocateElementPool (src/kernel.cu:35) in 32-bit words;
+  // GG_REF_HEAP_WORDS replaces it so that tests reach the multi-partition cut
+  uint64_t heap = (10000ull / 8 * 7) * 1024 * 256;
+  if (const char* h = std::getenv("GG_REF_HEAP_WORDS"))
+    if (std::strtoull(h, nullptr, 10)) heap = std::strtoull(h, nullptr, 10);
   const uint64_t limit = heap / 4;
   uint64_t t1 = 0, t3 = 0;
   auto rup32 = [](uint64_t x) { return (x + 31) / 32 * 32; };
@@ -601,7 +628,7 @@ bool make_partitions(State& s, uint64_t m, const uint32_t* src, const uint32_t* 
 }  // namespace moder
 
 Result* run_r(uint32_t n, int L, int R, const int32_t* rules, uint64_t m, const uint32_t* src, const uint32_t* dst,
-              const uint8_t* lab, int filter) {
+              const uint8_t* lab, int filter, bool parts_only = false) {
   auto* res = new Result();
   res->n = n;
   res->L = L;
@@ -619,6 +646,16 @@ Result* run_r(uint32_t n, int L, int R, const int32_t* rules, uint64_t m, const 
   s.g = &g;
   s.filter = filter;
   if (!moder::make_partitions(s, m, src, dst, lab, res->err)) return res;
+  if (parts_only) {
+    res->num_parts = uint32_t(s.parts.size());
+    for (auto& p : s.parts) {
+      res->part_first.push_back(p.first);
+      res->part_last.push_back(p.last);
+    }
+    res->rel.assign(L, Csr());
+    for (auto& c : res->rel) c.ptr.assign(size_t(n) + 1, 0);
+    return res;
+  }
   const size_t rows = size_t(L) * n;
   s.old.assign(rows, {});
   s.dl.assign(rows, {});
@@ -700,6 +737,12 @@ void* orc_run(uint32_t n, int L, int R, const int32_t* rules, uint64_t m, const 
 void* orc_run_r(uint32_t n, int L, int R, const int32_t* rules, uint64_t m, const uint32_t* src, const uint32_t* dst,
                 const uint8_t* lab, int filter) {
   return run_r(n, L, R, rules, m, src, dst, lab, filter);
+}
+
+// Mode R's vertex intervals only (no closure is computed).
+void* orc_parts_r(uint32_t n, int L, int R, const int32_t* rules, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                  const uint8_t* lab) {
+  return run_r(n, L, R, rules, m, src, dst, lab, 1, true);
 }
 
 // Mode R schedule facts: supersteps, partition count, partition bounds.

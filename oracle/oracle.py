@@ -53,6 +53,8 @@ def _load():
         lib.orc_run_r.restype = P
         lib.orc_run_r.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, P, ctypes.c_uint64, P, P, P,
                                   ctypes.c_int]
+        lib.orc_parts_r.restype = P
+        lib.orc_parts_r.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, P, ctypes.c_uint64, P, P, P]
         lib.orc_supersteps.restype = ctypes.c_uint32
         lib.orc_supersteps.argtypes = [P]
         lib.orc_num_parts.restype = ctypes.c_uint32
@@ -79,6 +81,7 @@ class OracleResult:
     checksum: int = 0
     supersteps: int = 0          # mode R only
     parts: Optional[list] = None  # mode R only: [(first, last), ...]
+    degrees: Optional[np.ndarray] = None  # uint64 [n]: closure out-degree per vertex over all labels
 
     @property
     def alg_bytes(self) -> int:
@@ -89,7 +92,7 @@ class OracleResult:
 
 def closure(n: int, rules: np.ndarray, src: np.ndarray, dst: np.ndarray, lab: np.ndarray,
             num_labels: int, filter_flag: int = 0, threads: int = 0, want_triples: bool = True,
-            max_rounds: int = 0) -> OracleResult:
+            max_rounds: int = 0, want_degrees: bool = False) -> OracleResult:
     lib = _load()
     rules = np.ascontiguousarray(rules, dtype=np.int32).reshape(-1, 3)
     src = np.ascontiguousarray(src, dtype=np.uint32)
@@ -112,8 +115,14 @@ def closure(n: int, rules: np.ndarray, src: np.ndarray, dst: np.ndarray, lab: np
         if want_triples:
             triples = np.empty((total, 3), dtype=np.uint32)
             lib.orc_export(h, _ptr(triples), total)
+        degrees = None
+        if want_degrees:
+            # the reference's per-vertex edge count of old (getDegree,
+            # src/kernel.cu:818-839) once the fixpoint is reached
+            degrees = np.zeros(n, dtype=np.uint64)
+            lib.orc_degrees(h, _ptr(degrees))
         return OracleResult(n, num_labels, int(total), int(lib.orc_input_unique(h)), counts, rounds,
-                            float(lib.orc_seconds(h)), triples, int(lib.orc_checksum(h)))
+                            float(lib.orc_seconds(h)), triples, int(lib.orc_checksum(h)), degrees=degrees)
     finally:
         lib.orc_free(h)
 
@@ -146,6 +155,29 @@ def closure_r(n: int, rules: np.ndarray, src: np.ndarray, dst: np.ndarray, lab: 
         return OracleResult(n, num_labels, int(total), 0, counts, np.zeros((0, 4), np.uint64),
                             float(lib.orc_seconds(h)), triples, int(lib.orc_checksum(h)),
                             int(lib.orc_supersteps(h)), list(zip(first.tolist(), last.tolist())))
+    finally:
+        lib.orc_free(h)
+
+
+def partitions_r(n: int, rules: np.ndarray, src: np.ndarray, dst: np.ndarray, lab: np.ndarray,
+                 num_labels: int) -> list:
+    """Mode R's vertex intervals [(first, last), ...] without running the
+    closure; raises ValueError where the reference's cut is degenerate."""
+    lib = _load()
+    rules = np.ascontiguousarray(rules, dtype=np.int32).reshape(-1, 3)
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    lab = np.ascontiguousarray(lab, dtype=np.uint8)
+    h = lib.orc_parts_r(n, num_labels, rules.shape[0], _ptr(rules), src.shape[0], _ptr(src), _ptr(dst), _ptr(lab))
+    try:
+        err = lib.orc_error(h).decode()
+        if err:
+            raise ValueError(err)
+        npart = lib.orc_num_parts(h)
+        first = np.zeros(npart, np.uint32)
+        last = np.zeros(npart, np.uint32)
+        lib.orc_parts(h, _ptr(first), _ptr(last))
+        return list(zip(first.tolist(), last.tolist()))
     finally:
         lib.orc_free(h)
 

@@ -178,6 +178,79 @@ def test_gpu_partitioned_schedule_matches_oracle(built, flt):
             np.testing.assert_array_equal(ref.triples, g_ref.triples)
 
 
+def _fixture_arrays(f):
+    e = np.asarray(f["edges"], np.int64).reshape(-1, 3)
+    return (np.asarray(f["rules"], np.int32).reshape(-1, 3), e[:, 0].astype(np.uint32), e[:, 1].astype(np.uint32),
+            e[:, 2].astype(np.uint8))
+
+
+def test_gpu_filter_fixtures_match_hand_derivation(built, monkeypatch):
+    """The hand-derived filter-1 closures (tests/golden/filter_fixtures.py):
+    the GPU's partitioned schedule gives exactly the derived set, intervals
+    and superstep count; with filter 0 every path gives the least fixpoint."""
+    sys.path.insert(0, GOLDEN)
+    from filter_fixtures import FIXTURES
+    for f in FIXTURES:
+        say("filter fixture", f["name"])
+        if f["heap_words"]:
+            monkeypatch.setenv("GG_REF_HEAP_WORDS", str(f["heap_words"]))
+        else:
+            monkeypatch.delenv("GG_REF_HEAP_WORDS", raising=False)
+        rules, src, dst, lab = _fixture_arrays(f)
+        c = gg.Context()
+        c.set_grammar(rules, f["num_labels"])
+        c.set_graph(f["n"], src, dst, lab)
+        assert c.reference_partitions() == f["parts"]
+        st = c.run(1)
+        np.testing.assert_array_equal(c.export(), np.asarray(f["filter1"], np.uint32))
+        assert st.supersteps == f["filter1_supersteps"] and st.partitions == len(f["parts"])
+        want0 = np.asarray(f["filter0"], np.uint32)
+        c.run_partitioned(0)
+        np.testing.assert_array_equal(c.export(), want0)
+        c.run(0)
+        np.testing.assert_array_equal(c.export(), want0)
+
+
+@pytest.mark.parametrize("flt", [1, 0])
+def test_gpu_multi_partition_schedule_matches_oracle(built, monkeypatch, flt):
+    """A small heap (GG_REF_HEAP_WORDS) makes initializePartitions' cut loop
+    produce several intervals (src/main.cpp:145-155); the GPU's schedule over
+    them equals mode R's: intervals, supersteps and the closure."""
+    for name, g, parts_wanted in [("alias1500", gen.alias(1500, seed=81), 4),
+                                  ("aliasdf4000", gen.alias_as_dataflow(4000, seed=82), 6),
+                                  ("alias3000", gen.alias(3000, seed=83), 3)]:
+        # elements per source as getGraphInfo counts them
+        keys = (g.src.astype(np.uint64) << np.uint64(40)) | ((g.dst // 192).astype(np.uint64) << np.uint64(8)) | g.lab
+        elems = np.unique(keys).shape[0]
+        heap = 4 * (8 * elems // parts_wanted + ((g.n // parts_wanted + 31) // 32 * 32) * g.num_labels) + 4
+        monkeypatch.setenv("GG_REF_HEAP_WORDS", str(heap))
+        ref = oracle.closure_r_of(g, filter_flag=flt)
+        say("multi-partition", name, "filter", flt, "intervals", len(ref.parts))
+        assert len(ref.parts) >= 3
+        c = gg.Context()
+        c.set_spec(g)
+        assert c.reference_partitions() == ref.parts
+        st = c.run(flt) if flt else c.run_partitioned(0)
+        assert st.supersteps == ref.supersteps and st.partitions == len(ref.parts)
+        np.testing.assert_array_equal(c.export(), ref.triples)
+
+
+def test_gpu_degrees_match_oracle(built):
+    """gg_closure_degrees (the reference's per-vertex degree[old] column,
+    getDegree / computeDegree, src/kernel.cu:782-839) equals the oracle's and
+    the out-degree counted from the exported triples."""
+    for g in (gen.alias(6000, seed=91), gen.alias_as_dataflow(9000, seed=92)):
+        say("degrees", g.name)
+        ref = oracle.closure_of(g, want_degrees=True)
+        c = gg.Context()
+        c.set_spec(g)
+        c.run(0)
+        deg = c.degrees()
+        np.testing.assert_array_equal(deg, ref.degrees)
+        np.testing.assert_array_equal(deg, np.bincount(c.export()[:, 0], minlength=g.n).astype(np.uint64))
+        assert int(deg.sum()) == ref.total
+
+
 def test_gpu_partitioned_degenerate_split_rejected(built):
     """No input edges: the reference's split has no second interval."""
     say("degenerate split")
@@ -262,7 +335,8 @@ def test_gpu_pair_arena_overflow_matches_oracle(built, monkeypatch):
     for name, g in [("alias n=20000", gen.alias(20_000, seed=41)),
                     ("dataflow n=3000", gen.dataflow(3_000, seed=42))]:
         say("pair arenas forced small", name)
-        check_against_oracle(g)
+        _, st, _ = check_against_oracle(g)
+        assert st.pair_reruns > 0  # the overflow-and-rerun path ran
 
 
 def test_gpu_alias_1m_full_size(built):
@@ -338,12 +412,17 @@ def test_gpu_server_results_equal_alone(built):
     assert len({r.slot for r, _ in results}) == 2  # both slots ran
     alone = gg.Context()
     for g, (r, per) in zip(graphs, results):
+        assert r.status == gg.GG_OK
         alone.set_spec(g)
         alone.run(0)
         p2, t2 = alone.counts()
         assert (r.total_edges, r.checksum) == (t2, alone.checksum())
         np.testing.assert_array_equal(per, p2)
         assert r.submit_ms <= r.start_ms <= r.done_ms
+        # and against the oracle, not only the engine's own solo run
+        ref = oracle.closure_of(g, want_triples=False)
+        assert (r.total_edges, r.checksum) == (ref.total, ref.checksum)
+        np.testing.assert_array_equal(per, ref.counts)
 
 
 @pytest.mark.parametrize("parts", [1, 4, 16])
