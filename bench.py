@@ -154,7 +154,7 @@ def main():
                          "device budget; dataflow: the 10,000-vertex dataflow closure")
     ap.add_argument("--batch-stride", type=int, default=1,
                     help="batch leg: close every k-th graph of the 4096-graph set (1 = all)")
-    ap.add_argument("--batch-max-vertices", type=int, default=20_000_000,
+    ap.add_argument("--batch-max-vertices", type=int, default=8_000_000,
                     help="batch leg: vertices per launch group (bounds the device memory of a group)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="serve leg: arrivals per second (Poisson)")

@@ -69,52 +69,8 @@ void add_stats(gg_stats& a, const gg_stats& b) {
   a.pair_reruns += b.pair_reruns;
 }
 
-}  // namespace
-
-extern "C" {
-
-int gg_batch_create(gg_ctx* ctx, gg_batch** out) {
-  if (!ctx || !out) return GG_E_ARG;
-  auto* b = new gg_batch();
-  b->ctx = ctx;
-  *out = b;
-  return GG_OK;
-}
-
-int gg_batch_destroy(gg_batch* b) {
-  if (!b) return GG_E_ARG;
-  delete b;
-  return GG_OK;
-}
-
-int gg_batch_add(gg_batch* b, int num_rules, const int32_t* rules, int num_labels, uint32_t n, uint64_t m,
-                 const uint32_t* src, const uint32_t* dst, const uint8_t* lab, int* graph_id) {
-  if (!b) return GG_E_ARG;
-  gg::Grammar tmp;
-  int s = gg::grammar_from_rules(num_rules, rules, num_labels, tmp, b->err);
-  if (s != GG_OK) return s;
-  if (n == 0) return fail(b, GG_E_ARG, "graph has no vertices");
-  if (m && (!src || !dst || !lab)) return fail(b, GG_E_ARG, "null edge array");
-  for (uint64_t i = 0; i < m; ++i) {
-    if (src[i] >= n || dst[i] >= n) return fail(b, GG_E_ARG, "edge " + std::to_string(i) + ": vertex out of range");
-    if (lab[i] < 1 || lab[i] > num_labels) return fail(b, GG_E_ARG, "edge " + std::to_string(i) + ": bad label");
-  }
-  gg_batch::G g;
-  g.rules.assign(rules, rules + 3 * num_rules);
-  g.L = num_labels;
-  g.n = n;
-  g.src.assign(src, src + m);
-  g.dst.assign(dst, dst + m);
-  g.lab.assign(lab, lab + m);
-  b->graphs.push_back(std::move(g));
-  if (graph_id) *graph_id = int(b->graphs.size() - 1);
-  return GG_OK;
-}
-
-int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) {
-  if (!b) return GG_E_ARG;
+int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std::string*& cerr) {
   int st = GG_OK;
-  std::string* cerr = nullptr;
   gg::Engine* e = gg::ctx_engine(b->ctx, st, cerr);
   if (!e) return fail(b, st, cerr ? *cerr : "no device");
   gg_stats total{};
@@ -130,6 +86,10 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
     const auto& gy = b->graphs[y];
     return std::make_pair(gx.L, gx.rules) < std::make_pair(gy.L, gy.rules);
   });
+  // A group that runs out of device memory is cut in half and run again
+  // (its closure size was not known in advance); later groups keep the
+  // smaller graph count.
+  size_t cap = G;
   size_t i = 0;
   while (i < G) {
     // one launch group: graphs ord[i..j-1]
@@ -142,7 +102,7 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
     int unary = 0, binary = 0, slots = 0;
     uint64_t verts = 0;
     size_t j = i;
-    for (; j < G; ++j) {
+    for (; j < G && j - i < cap; ++j) {
       const auto& g = b->graphs[ord[j]];
       auto key = std::make_pair(g.L, g.rules);
       const bool fresh = grammar_base.find(key) == grammar_base.end();
@@ -247,9 +207,15 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
       for (auto& th : pool) th.join();
     }
     if ((s = gg::engine_set_grammar(e, cg, b->err)) != GG_OK) return s;
-    if ((s = gg::engine_set_graph(e, std::move(sg), b->err)) != GG_OK) return s;
+    s = gg::engine_set_graph(e, std::move(sg), b->err);
     gg_stats one{};
-    if ((s = gg::engine_run(e, 0, &one, b->err)) != GG_OK) return s;
+    if (s == GG_OK) s = gg::engine_run(e, 0, &one, b->err);
+    if (s == GG_E_NOMEM && j - i > 1) {
+      gg::engine_release(e);
+      cap = (j - i) / 2;
+      continue;
+    }
+    if (s != GG_OK) return s;
     add_stats(total, one);
     std::vector<uint64_t> counts, sums;
     if ((s = gg::engine_slices(e, v0, l0, lg, counts, sums, b->err)) != GG_OK) return s;
@@ -276,6 +242,57 @@ int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) 
   }
   if (stats) *stats = total;
   return GG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gg_batch_create(gg_ctx* ctx, gg_batch** out) {
+  if (!ctx || !out) return GG_E_ARG;
+  auto* b = new gg_batch();
+  b->ctx = ctx;
+  *out = b;
+  return GG_OK;
+}
+
+int gg_batch_destroy(gg_batch* b) {
+  if (!b) return GG_E_ARG;
+  delete b;
+  return GG_OK;
+}
+
+int gg_batch_add(gg_batch* b, int num_rules, const int32_t* rules, int num_labels, uint32_t n, uint64_t m,
+                 const uint32_t* src, const uint32_t* dst, const uint8_t* lab, int* graph_id) {
+  if (!b) return GG_E_ARG;
+  gg::Grammar tmp;
+  int s = gg::grammar_from_rules(num_rules, rules, num_labels, tmp, b->err);
+  if (s != GG_OK) return s;
+  if (n == 0) return fail(b, GG_E_ARG, "graph has no vertices");
+  if (m && (!src || !dst || !lab)) return fail(b, GG_E_ARG, "null edge array");
+  for (uint64_t i = 0; i < m; ++i) {
+    if (src[i] >= n || dst[i] >= n) return fail(b, GG_E_ARG, "edge " + std::to_string(i) + ": vertex out of range");
+    if (lab[i] < 1 || lab[i] > num_labels) return fail(b, GG_E_ARG, "edge " + std::to_string(i) + ": bad label");
+  }
+  gg_batch::G g;
+  g.rules.assign(rules, rules + 3 * num_rules);
+  g.L = num_labels;
+  g.n = n;
+  g.src.assign(src, src + m);
+  g.dst.assign(dst, dst + m);
+  g.lab.assign(lab, lab + m);
+  b->graphs.push_back(std::move(g));
+  if (graph_id) *graph_id = int(b->graphs.size() - 1);
+  return GG_OK;
+}
+
+int gg_batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats) {
+  if (!b) return GG_E_ARG;
+  std::string* cerr = nullptr;
+  const int s = batch_run(b, max_vertices, keep, stats, cerr);
+  // the context's last error names what failed (gg_last_error)
+  if (s != GG_OK && cerr) *cerr = b->err;
+  return s;
 }
 
 int gg_batch_info(const gg_batch* b, int* ng, int* ngroups) {
