@@ -13,14 +13,18 @@ HBM. value = new closure edges per second = (|closure| - |dedup input|) / t,
 summed over the timed steps.
 
 roofline: the join/dedup/difference kernel (k_expand), timed live with HIP
-events on the engine's stream. achieved = its HBM traffic per launch
-(rocprofv3 FETCH_SIZE + WRITE_SIZE of the same workload, committed under
-profiles/) over its average launch time. The algorithmic byte models ride
-beside it: 4*P + 4*O + 4*N for this fused kernel (P probe reads: key elements
-scanned plus every candidate-list element read; O old/delta row elements read
-for the difference; N new edges written) and 4*P + 8*K + 4*O + 8*N for a
-design that materialises its K candidates; P, K, O, N are the engine's
-counters, which the parity tests hold equal to the oracle's.
+events on the engine's stream. Its VALU instructions and HBM bytes per launch
+come from the committed rocprofv3 counter summary of the same workload
+(profiles/expand_alias1m_counters.*). The bound is the resource it uses the
+larger share of: VALU issue (instructions x 64 lanes over the live launch
+time, against 39.3 T lane-ops/s) -- the kernel is bound there -- or HBM
+(bytes over the launch time, against 8 TB/s); both shares are reported. The
+algorithmic byte models ride beside them: 4*P + 4*O + 4*N for this fused
+kernel (P probe reads: key elements scanned plus every candidate-list element
+read; O old/delta row elements read for the difference; N new edges written)
+and 4*P + 8*K + 4*O + 8*N for a design that materialises its K candidates;
+P, K, O, N are the engine's counters, which the parity tests hold equal to
+the oracle's.
 
 batch (extra key): the 4096-graph batched set, closed in launch groups,
 against the same kernels one graph at a time (graphs/s and their ratio).
@@ -43,6 +47,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "graspan-g_amd"))
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # CUs x SIMDs x lanes x peak clock
 WORKLOAD_N = 1_000_000
 
 
@@ -116,23 +121,26 @@ def roofline(stats):
            "alg_bytes_per_launch_fused": fused, "alg_bytes_per_launch_4P8K4O8N": survey,
            "alg_gbs_fused": fused / avg_s / 1e9, "alg_gbs_4P8K4O8N": survey / avg_s / 1e9}
     c = counters_from_profiles()
-    traffic = c.get("hbm_bytes_per_launch") if c else None
-    out.update({"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": traffic})
-    if traffic:
-        out["achieved"] = traffic / avg_s / 1e9
-        out["frac"] = out["achieved"] / HBM_PEAK_GBS
+    if not c:
+        # no counter summary: the fused byte model against HBM peak
+        out.update({"bound": "hbm", "achieved": out["alg_gbs_fused"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": out["alg_gbs_fused"] / HBM_PEAK_GBS, "traffic": None})
+        return out
+    traffic = c["hbm_bytes_per_launch"]
+    hbm_gbs = traffic / avg_s / 1e9
+    # VALU work: wave64 instructions x 64 lanes per second, against 256 CUs x
+    # 4 SIMD16 at the 2.4 GHz peak engine clock
+    valu = c["valu_insts_per_launch"] * 64 / avg_s / 1e12
+    out.update({"hbm_achieved_gbs": hbm_gbs, "hbm_peak_gbs": HBM_PEAK_GBS, "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
+                "valu_achieved_tops": valu, "valu_peak_tops": VALU_PEAK_TOPS, "valu_frac": valu / VALU_PEAK_TOPS,
+                "traffic": traffic, "counters_source": c.get("source")})
+    # the bound is whichever resource the kernel uses the larger share of
+    if out["valu_frac"] >= out["hbm_frac"]:
+        out.update({"bound": "valu", "achieved": valu, "peak": VALU_PEAK_TOPS, "unit": "Tops/s (int32 VALU lane-ops)",
+                    "frac": out["valu_frac"]})
     else:
-        out["achieved"] = out["alg_gbs_fused"]
-        out["frac"] = out["achieved"] / HBM_PEAK_GBS
-    if c and c.get("valu_insts_per_launch"):
-        # a wave64 VALU instruction occupies its SIMD16 for 4 cycles; 4 SIMDs
-        # per CU at the peak engine clock
-        simd_cycles = c["cus"] * 4 * c["sclk_mhz"] * 1e6 * avg_s
-        out["valu_busy_frac"] = c["valu_insts_per_launch"] * 4 / simd_cycles
-        if out["valu_busy_frac"] > out["frac"]:
-            out["limiter"] = "valu"
-    if c:
-        out["counters_source"] = c.get("source")
+        out.update({"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": out["hbm_frac"]})
     return out
 
 
