@@ -162,8 +162,8 @@ def main():
                          "device budget; dataflow: the 10,000-vertex dataflow closure")
     ap.add_argument("--batch-stride", type=int, default=1,
                     help="batch leg: close every k-th graph of the 4096-graph set (1 = all)")
-    ap.add_argument("--batch-max-vertices", type=int, default=8_000_000,
-                    help="batch leg: vertices per launch group (bounds the device memory of a group)")
+    ap.add_argument("--batch-max-vertices", type=int, default=0,
+                    help="batch leg: vertices per launch group (0: planned from device memory by the library)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="serve leg: arrivals per second (Poisson)")
     ap.add_argument("--serve-min-n", type=int, default=1_000)
@@ -296,8 +296,9 @@ def bench_batch(args):
         if (tot, cs) != (bt, bc) or not np.array_equal(per, bp):
             raise SystemExit(f"graph {i * args.batch_stride}: batched result differs from the single-graph result")
     return {"workload": f"batched closure of {len(graphs)} graphs of the 4096-graph set "
-                        f"(every {args.batch_stride}th), {nv} vertices, launch groups of <= "
-                        f"{args.batch_max_vertices} vertices",
+                        f"(every {args.batch_stride}th), {nv} vertices, launch groups "
+                        + (f"of <= {args.batch_max_vertices} vertices" if args.batch_max_vertices
+                           else "planned from device memory"),
             "graphs": len(graphs), "vertices": int(nv), "launch_groups": groups,
             "closure_edges": int(st.total_edges), "ms": dt_b * 1e3,
             "graphs_per_s": len(graphs) / dt_b, "one_at_a_time_ms": dt_s * 1e3,

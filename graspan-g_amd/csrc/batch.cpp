@@ -10,6 +10,9 @@
 // the disjoint union of the per-graph closures, computed by one engine run
 // (one launch sequence per semi-naive round for the whole group).
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <string>
 #include <thread>
@@ -22,6 +25,7 @@
 struct gg_ctx;
 namespace gg {
 Engine* ctx_engine(gg_ctx* c, int& status, std::string*& err);
+uint64_t ctx_memory(gg_ctx* c);
 }
 
 struct gg_batch {
@@ -90,8 +94,32 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
   // (its closure size was not known in advance); later groups keep the
   // smaller graph count.
   size_t cap = G;
+  // With max_vertices == 0 a group's vertex count is planned from device
+  // memory: a group may use kGroupMemShare of it, at the peak device bytes
+  // per vertex measured on the last group of the same grammar (a grammar not
+  // yet seen starts with kFirstGroupVertices). Groups that run close to the
+  // device's limit spend their time re-allocating instead of computing.
+  constexpr double kGroupMemShare = 0.4;
+  constexpr uint64_t kFirstGroupVertices = 2u << 20;
+  const uint64_t mem = max_vertices ? 0 : gg::ctx_memory(b->ctx);
+  std::map<std::pair<int, std::vector<int32_t>>, double> bytes_per_vertex;
+  auto vertex_cap = [&](const std::pair<int, std::vector<int32_t>>& key) -> uint64_t {
+    if (max_vertices) return max_vertices;
+    auto it = bytes_per_vertex.find(key);
+    if (it == bytes_per_vertex.end() || !mem) return kFirstGroupVertices;
+    return std::max<uint64_t>(1, uint64_t(double(mem) * kGroupMemShare / std::max(it->second, 1.0)));
+  };
+  // GG_BATCH_TRACE=1: one stderr line per launch group (sizes and the host
+  // time of each phase)
+  const char* tr = std::getenv("GG_BATCH_TRACE");
+  const bool trace = tr && tr[0] == '1';
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
   size_t i = 0;
   while (i < G) {
+    const auto t_start = now();
     // one launch group: graphs ord[i..j-1]
     // A group's combined grammar must stay within the engine's limits: at
     // most kMaxLabels labels, kMaxRulesPerKind unary and binary rules (eps
@@ -101,6 +129,7 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
     uint32_t labels = 0;
     int unary = 0, binary = 0, slots = 0;
     uint64_t verts = 0;
+    uint64_t vcap = ~0ull;  // the smallest vertex cap of the group's grammars
     size_t j = i;
     for (; j < G && j - i < cap; ++j) {
       const auto& g = b->graphs[ord[j]];
@@ -113,11 +142,13 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
       const uint64_t nv = verts + g.n;
       const uint64_t lim = nl ? (0xFFFFFFFEull / nl) : nv;
       const uint64_t slim = ns ? (0xFFFFFFFEull / uint64_t(ns)) : nv;
-      const bool over = nl > uint32_t(gg::kMaxLabels) || nu > gg::kMaxRulesPerKind || nb > gg::kMaxRulesPerKind ||
-                        nv > lim || nv > slim || (max_vertices && nv > max_vertices);
-      if (over && j > i) break;
-      if (over) return fail(b, GG_E_UNSUPPORTED, "graph " + std::to_string(ord[j]) + " alone exceeds the launch limits");
+      const uint64_t ncap = std::min(vcap, vertex_cap(key));
+      const bool hard = nl > uint32_t(gg::kMaxLabels) || nu > gg::kMaxRulesPerKind || nb > gg::kMaxRulesPerKind ||
+                        nv > lim || nv > slim;
+      if ((hard || nv > ncap) && j > i) break;  // a graph over the vertex cap alone still runs alone
+      if (hard) return fail(b, GG_E_UNSUPPORTED, "graph " + std::to_string(ord[j]) + " alone exceeds the launch limits");
       if (fresh) {
+        vcap = ncap;
         grammar_base[key] = labels;
         labels = nl;
         unary = nu;
@@ -206,19 +237,36 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
       for (size_t t = 0; t < nt; ++t) pool.emplace_back(fill, ng * t / nt, ng * (t + 1) / nt);
       for (auto& th : pool) th.join();
     }
+    const auto t_built = now();
     if ((s = gg::engine_set_grammar(e, cg, b->err)) != GG_OK) return s;
+    const uint64_t group_m = sg.src.size();
     s = gg::engine_set_graph(e, std::move(sg), b->err);
+    const auto t_up = now();
     gg_stats one{};
     if (s == GG_OK) s = gg::engine_run(e, 0, &one, b->err);
+    const auto t_run = now();
     if (s == GG_E_NOMEM && j - i > 1) {
+      if (trace)
+        std::fprintf(stderr, "[gg-batch] group of %zu graphs, %llu vertices: out of device memory after %.0f ms; "
+                     "splitting\n", j - i, (unsigned long long)verts, ms(t_start, t_run));
       gg::engine_release(e);
       cap = (j - i) / 2;
       continue;
     }
     if (s != GG_OK) return s;
     add_stats(total, one);
+    for (auto& kv : grammar_base) bytes_per_vertex[kv.first] = double(one.device_bytes_peak) / double(verts);
     std::vector<uint64_t> counts, sums;
     if ((s = gg::engine_slices(e, v0, l0, lg, counts, sums, b->err)) != GG_OK) return s;
+    if (trace)
+      std::fprintf(stderr,
+                   "[gg-batch] group %d: %zu graphs, %llu vertices, %u labels, %llu edges -> %llu closure edges, "
+                   "%u rounds; build %.0f ms, upload %.0f ms, run %.0f ms (fixpoint %.0f: expand %.0f merge %.0f "
+                   "compact %.0f other %.0f), results %.0f ms, device peak %.1f GB\n",
+                   b->groups, j - i, (unsigned long long)verts, labels, (unsigned long long)group_m,
+                   (unsigned long long)one.total_edges, one.rounds, ms(t_start, t_built), ms(t_built, t_up),
+                   ms(t_up, t_run), one.fixpoint_ms, one.expand_ms, one.merge_ms, one.compact_ms, one.other_ms,
+                   ms(t_run, now()), double(one.device_bytes_peak) / 1e9);
     size_t c = 0;
     for (size_t k = i; k < j; ++k) {
       auto& g = b->graphs[ord[k]];

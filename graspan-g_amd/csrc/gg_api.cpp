@@ -150,6 +150,16 @@ Engine* ctx_engine(gg_ctx* c, int& status, std::string*& err) {
   c->graph_on_device = false;  // a batch replaces the context's device graph
   return c->engine;
 }
+
+// Device memory a batch may plan for: the context's budget, else the
+// device's total memory (0 if it cannot be read).
+uint64_t ctx_memory(gg_ctx* c) {
+  if (c->opts.device_budget) return c->opts.device_budget;
+  uint64_t total = 0, free_bytes = 0;
+  std::string err;
+  if (device_query(nullptr, c->opts.device, nullptr, nullptr, 0, &total, &free_bytes, err) != GG_OK) return 0;
+  return total;
+}
 }  // namespace gg
 
 extern "C" {

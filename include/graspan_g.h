@@ -219,10 +219,16 @@ int gg_batch_destroy(gg_batch *batch);
 int gg_batch_add(gg_batch *batch, int num_rules, const int32_t *rules, int num_labels, uint32_t num_vertices,
                  uint64_t num_edges, const uint32_t *src, const uint32_t *dst, const uint8_t *label, int *graph_id);
 
-/* Closes every added graph. Graphs are taken in order into launches of at
- * most max_vertices vertices in total (0 = as many as the 2^32 row limit
- * allows); keep_closures keeps each graph's triples on the host for
- * gg_batch_export. stats (may be NULL) sums the launches. */
+/* Closes every added graph. Graphs are ordered by grammar and cut into
+ * launch groups of at most max_vertices vertices in total; 0 plans each
+ * group from device memory instead (the context's budget, else the device's
+ * memory): a group may use 40% of it, at the peak bytes per vertex measured
+ * on the previous group of the same grammar (2^21 vertices for a grammar
+ * not yet seen). A single graph above the cap runs alone. A group that still
+ * runs out of device memory is cut in half and run again. The 2^32 row limit
+ * and the engine's rule limits always cut groups. keep_closures keeps each
+ * graph's triples on the host for gg_batch_export. stats (may be NULL) sums
+ * the launches. */
 int gg_batch_run(gg_batch *batch, uint64_t max_vertices, int keep_closures, gg_stats *stats);
 
 /* Number of graphs and of launch groups the last run used. */
