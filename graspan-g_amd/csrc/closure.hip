@@ -44,6 +44,7 @@
 #include <string>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -2292,6 +2293,20 @@ class Engine {
   bool has_closure = false;
   Counters* h_ctr = nullptr;  // pinned
 
+  // Fixed-region mode (the allocation-free entry point): every buffer is
+  // carved from [arena, arena + arena_cap), a device region the engine does
+  // not own, by first fit over a free list kept on the host; the stream, the
+  // input edges and the host copies of the counters belong to the caller or
+  // to this object. Nothing is allocated or freed through HIP in this mode,
+  // and no events are created (the statistics' times stay 0).
+  char* arena = nullptr;
+  size_t arena_cap = 0;
+  std::map<size_t, size_t> arena_free;  // offset -> bytes, neighbours merged
+  bool owns_stream = true;
+  Counters arena_h_ctr{};
+  unsigned long long arena_h_u64 = 0;
+  uint32_t expand_grid_cap = 0;  // 0: as many blocks as fit the device
+
   ~Engine() {
     DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
                      &o2_ptr, &o2_col, &raw_ptr, &raw_col, &staging, &seg_off, &seg_cnt, &active,
@@ -2299,12 +2314,31 @@ class Engine {
                      &pr_ptr, &pr_col, &pk_len, &pk_ent, &pk2_ent, &dpk_len, &dpk_ent, &npk_len, &npk_ent,
                      &mlist,  &mcnt,   &pd_ptr, &pd_col, &p2_ptr, &p2_col};
     if (stream) (void)hipStreamSynchronize(stream);
-    for (DevBuf* b : all)
-      if (b->p) (void)hipFree(b->p);
-    for (auto& kv : cache) (void)hipFree(kv.second);
-    if (h_ctr) (void)hipHostFree(h_ctr);
-    if (h_u64) (void)hipHostFree(h_u64);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (!arena) {
+      for (DevBuf* b : all)
+        if (b->p) (void)hipFree(b->p);
+      for (auto& kv : cache) (void)hipFree(kv.second);
+      if (h_ctr) (void)hipHostFree(h_ctr);
+      if (h_u64) (void)hipHostFree(h_u64);
+    }
+    if (stream && owns_stream) (void)hipStreamDestroy(stream);
+  }
+
+  // fixed-region mode: returns [off, off + bytes) to the free list
+  void arena_put(size_t off, size_t bytes) {
+    auto it = arena_free.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != arena_free.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      arena_free.erase(next);
+    }
+    if (it != arena_free.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        arena_free.erase(it);
+      }
+    }
   }
 
   // Device memory. Buffers that are released or outgrown go to a per-engine
@@ -2328,15 +2362,39 @@ class Engine {
 
   void put_block(DevBuf& b) {
     if (!b.p) return;
-    cache.emplace(b.bytes, b.p);
-    cached += b.bytes;
+    if (arena)
+      arena_put(size_t(static_cast<char*>(b.p) - arena), b.bytes);
+    else
+      cache.emplace(b.bytes, b.p);
+    if (!arena) cached += b.bytes;
     used -= b.bytes;
     b.p = nullptr;
     b.bytes = 0;
   }
 
+  int arena_alloc(DevBuf& b, size_t bytes, std::string& err) {
+    put_block(b);
+    bytes = (std::max<size_t>(bytes, 256) + 255) & ~size_t(255);
+    for (auto it = arena_free.begin(); it != arena_free.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, left = it->second - bytes;
+      arena_free.erase(it);
+      if (left) arena_free.emplace(off + bytes, left);
+      b.p = arena + off;
+      b.bytes = bytes;
+      used += bytes;
+      peak = std::max(peak, used);
+      return GG_OK;
+    }
+    err = "the fixed scratch region (" + std::to_string(arena_cap) + " bytes) cannot hold a further " +
+          std::to_string(bytes) + " bytes (" + std::to_string(used) + " in use): the graph or its closure is too "
+          "large for the allocation-free entry point";
+    return GG_E_NOMEM;
+  }
+
   int alloc(DevBuf& b, size_t bytes, std::string& err) {
     if (b.p && b.bytes >= bytes) return GG_OK;
+    if (arena) return arena_alloc(b, bytes, err);
     const auto a0 = std::chrono::steady_clock::now();
     struct Clock {
       double& ms;
@@ -2561,7 +2619,10 @@ class Engine {
 
   int expand_blocks_per_cu = 4;
   int merge_blocks_per_cu = 8, merge_rows_blocks_per_cu = 8, gather_blocks_per_cu = 7;
-  uint32_t expand_grid() const { return uint32_t(cus) * uint32_t(expand_blocks_per_cu); }
+  uint32_t expand_grid() const {
+    const uint32_t g = uint32_t(cus) * uint32_t(expand_blocks_per_cu);
+    return expand_grid_cap ? std::min(g, expand_grid_cap) : g;
+  }
   uint32_t gather_grid() const { return uint32_t(cus) * uint32_t(gather_blocks_per_cu); }
   uint32_t rows() const { return uint32_t(uint64_t(L) * n); }
 
@@ -2604,12 +2665,16 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   gg_stats S{};
   const uint32_t R = rows();
   std::vector<EventPair> ev_expand, ev_compact, ev_merge, ev_other;
+  // fixed-region mode creates no events: its statistics carry no times
   auto mk = [&](std::vector<EventPair>& v) -> EventPair& {
     v.emplace_back();
-    (void)hipEventCreate(&v.back().a);
-    (void)hipEventCreate(&v.back().b);
+    if (!arena) {
+      (void)hipEventCreate(&v.back().a);
+      (void)hipEventCreate(&v.back().b);
+    }
     return v.back();
   };
+  auto rec = [&](hipEvent_t ev) -> hipError_t { return ev ? hipEventRecord(ev, stream) : hipSuccess; };
   struct EvGuard {
     std::vector<EventPair>* vs[4];
     ~EvGuard() {
@@ -2638,7 +2703,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   // ---- initial delta: dedup(input U eps loops) ---------------------------
   {
     EventPair& e = mk(ev_other);
-    GG_HIP(hipEventRecord(e.a, stream));
+    GG_HIP(rec(e.a));
     GG_TRY(alloc(raw_ptr, (uint64_t(R) + 1) * 8, err));
     fill_u32(seg_cnt, 0, R);
     k_count_raw<<<grid_for(std::max<uint64_t>(m, uint64_t(g.ne) * n)), kBlock, 0, stream>>>(
@@ -2660,15 +2725,15 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     if (const char* sw = std::getenv("GG_STAGING_MIN_WORDS")) stage_words = std::max<uint64_t>(1, std::strtoull(sw, nullptr, 10));
     release(staging);
     GG_TRY(alloc(staging, stage_words * 4, err));
-    GG_HIP(hipEventRecord(e.b, stream));
+    GG_HIP(rec(e.b));
     for (;;) {
       EventPair& x = mk(ev_expand);
       zero_ctr();
       fill_u32(seg_cnt, 0, R);
       k_activate_raw<<<grid_for(R), kBlock, 0, stream>>>(P<unsigned long long>(raw_ptr), R, P<uint32_t>(active), C());
-      GG_HIP(hipEventRecord(x.a, stream));
+      GG_HIP(rec(x.a));
       k_expand<1><<<expand_grid(), kBlock, 0, stream>>>(expand_args(true));
-      GG_HIP(hipEventRecord(x.b, stream));
+      GG_HIP(rec(x.b));
       GG_HIP(hipGetLastError());
       GG_TRY(read_ctr(err));
       S.expand_launches++;
@@ -2678,7 +2743,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     }
     const uint64_t N0 = h_ctr->newe;
     EventPair& c = mk(ev_compact);
-    GG_HIP(hipEventRecord(c.a, stream));
+    GG_HIP(rec(c.a));
     GG_TRY(scan(seg_cnt, R, dl_ptr, err));
     GG_TRY(grow(dl_col, std::max<uint64_t>(N0, 1) * 4, err));
     dpk_ready = false;
@@ -2692,7 +2757,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
                                                    P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col), n, plabels(),
                                                    dpk_ready ? P<unsigned long long>(dpk_ent) : nullptr,
                                                    P<uint32_t>(dpk_len));
-    GG_HIP(hipEventRecord(c.b, stream));
+    GG_HIP(rec(c.b));
     release(raw_col);
     release(raw_ptr);
     dl_nnz = N0;
@@ -2750,7 +2815,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     pairs_ready = false;
     if (NS && old_nnz > 0) {
       EventPair& o = mk(ev_other);
-      GG_HIP(hipEventRecord(o.a, stream));
+      GG_HIP(rec(o.a));
       const DCsr dlc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
       const DCsr oc{P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
       k_frontier<<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, fr_words, P<unsigned long long>(fr_bits));
@@ -2781,7 +2846,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
         S.pair_reruns++;
         pr_arena = 2 * most + kPairBlock;
       }
-      GG_HIP(hipEventRecord(o.b, stream));
+      GG_HIP(rec(o.b));
       pairs_ready = true;
     }
     // staging: room for every wave's reservation block plus a guess of the
@@ -2789,7 +2854,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     GG_TRY(grow(staging, (uint64_t(expand_grid()) * kWavesPerBlock * kStageBlock * 2 + 4 * dl_nnz) * 4, err));
     for (;;) {
       EventPair& o = mk(ev_other);
-      GG_HIP(hipEventRecord(o.a, stream));
+      GG_HIP(rec(o.a));
       zero_ctr();
       fill_u32(seg_cnt, 0, R);
       k_label_flags<<<1, kBlock, 0, stream>>>(P<unsigned long long>(dl_ptr), n, uint32_t(L), C());
@@ -2804,14 +2869,14 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_TRY(read_ctr(err));
       const uint32_t want = uint32_t((uint64_t(h_ctr->active) + kGrab * kWavesPerBlock - 1) / (kGrab * kWavesPerBlock));
       const uint32_t grid = std::max(1u, std::min(expand_grid(), want));
-      GG_HIP(hipEventRecord(o.b, stream));
+      GG_HIP(rec(o.b));
       EventPair& x = mk(ev_expand);
-      GG_HIP(hipEventRecord(x.a, stream));
+      GG_HIP(rec(x.a));
       if (pend_nnz)
         k_expand<0, true><<<grid, kBlock, 0, stream>>>(expand_args(false));
       else
         k_expand<0><<<grid, kBlock, 0, stream>>>(expand_args(false));
-      GG_HIP(hipEventRecord(x.b, stream));
+      GG_HIP(rec(x.b));
       GG_HIP(hipGetLastError());
       GG_TRY(read_ctr(err));
       S.expand_launches++;
@@ -2832,7 +2897,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     S.new_edges += N;
     if (verbose) {
       float xms = 0;
-      (void)hipEventElapsedTime(&xms, ev_expand.back().a, ev_expand.back().b);
+      if (ev_expand.back().a) (void)hipEventElapsedTime(&xms, ev_expand.back().a, ev_expand.back().b);
       std::fprintf(stderr,
                    "[gg] round %u: active %u cand %llu new %llu guess %u misses %u expand %.2f ms"
                    " (words %llu ids %llu keys %llu rows %llu)\n",
@@ -2841,7 +2906,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     }
     // delta' from staging
     EventPair& c = mk(ev_compact);
-    GG_HIP(hipEventRecord(c.a, stream));
+    GG_HIP(rec(c.a));
     GG_TRY(scan(seg_cnt, R, nx_ptr, err));
     GG_TRY(grow(nx_col, std::max<uint64_t>(N, 1) * 4, err));
     const bool npk = pack_delta();
@@ -2855,12 +2920,12 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
                                                      P<unsigned long long>(nx_ptr), P<uint32_t>(nx_col), n, plabels(),
                                                      npk ? P<unsigned long long>(npk_ent) : nullptr,
                                                      P<uint32_t>(npk_len));
-    GG_HIP(hipEventRecord(c.b, stream));
+    GG_HIP(rec(c.b));
     // old' = old ∪ delta. While the deltas are small next to old they go to
     // the pending CSR instead (pend' = pend ∪ delta), so a round that finds
     // few edges does not rewrite all of old; the joins read old ∪ pend.
     EventPair& mg = mk(ev_merge);
-    GG_HIP(hipEventRecord(mg.a, stream));
+    GG_HIP(rec(mg.a));
     const bool defer = defer_ratio && old_nnz && (pend_nnz + dl_nnz) * defer_ratio <= old_nnz;
     if (defer || pend_nnz) {
       if (!pend_nnz) {
@@ -2891,7 +2956,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
         GG_TRY(fold_into_old({P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)}, dl_nnz, C(), err));
       }
     }
-    GG_HIP(hipEventRecord(mg.b, stream));
+    GG_HIP(rec(mg.b));
     GG_HIP(hipGetLastError());
     std::swap(dl_ptr, nx_ptr);
     std::swap(dl_col, nx_col);
@@ -2902,20 +2967,20 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   }
   if (pend_nnz) {  // the closure is old ∪ pend: rebuild old once
     EventPair& mg = mk(ev_merge);
-    GG_HIP(hipEventRecord(mg.a, stream));
+    GG_HIP(rec(mg.a));
     GG_TRY(fold_into_old({P<unsigned long long>(pd_ptr), P<uint32_t>(pd_col)}, pend_nnz, nullptr, err));
     pend_nnz = 0;
-    GG_HIP(hipEventRecord(mg.b, stream));
+    GG_HIP(rec(mg.b));
   }
   if (pk_ready && !keep_ids) {  // the packed labels' ids, from their entries
     EventPair& mg = mk(ev_merge);
-    GG_HIP(hipEventRecord(mg.a, stream));
+    GG_HIP(rec(mg.a));
     DPacked plab{};
     for (int i = 0; i < 8; ++i) plab.labels[i] = pack_labels[i];
     plab.len = P<uint32_t>(pk_len);
     plab.ent = P<unsigned long long>(pk_ent);
     k_unpack<<<uint32_t(cus) * 8, kBlock, 0, stream>>>(P<unsigned long long>(old_ptr), P<uint32_t>(old_col), plab, R, n);
-    GG_HIP(hipEventRecord(mg.b, stream));
+    GG_HIP(rec(mg.b));
     GG_HIP(hipGetLastError());
   }
   release(pd_ptr);
@@ -2929,7 +2994,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     double t = 0;
     for (auto& e : v) {
       float ms = 0;
-      if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) t += ms;
+      if (e.a && e.b && hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) t += ms;
     }
     return t;
   };
@@ -3895,6 +3960,119 @@ int engine_checksum(Engine* e, uint64_t* sum, std::string& err) {
   GG_HIP(hipGetLastError());
   GG_TRY(e->read_ctr(err));
   *sum = e->h_ctr->checksum;
+  return GG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free entry point (gg_closure_run_noalloc). Its buffers come
+// from `scratch`, a fixed device region that every GPU the library runs on
+// holds for the life of the process. In host code the array's name is not a
+// device address: scratch_region() looks the address up once per device, with
+// that device current, and only that address is used afterwards.
+
+constexpr size_t kScratchBytes = size_t(256) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+int scratch_region(int device, char** out, std::string& err) {
+  if (device < 0 || device >= kMaxScratchDevices) {
+    err = "device ordinal " + std::to_string(device) + " out of range for the scratch table";
+    return GG_E_DEVICE;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    GG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      err = "hipGetSymbolAddress returned no address for the scratch region";
+      return GG_E_DEVICE;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = static_cast<char*>(g_scratch_ptr[device]);
+  return GG_OK;
+}
+
+// Sets ctr->overflow when an input edge names a vertex >= n or a label
+// outside 1..L (the caller's device arrays cannot be checked on the host).
+__global__ void k_check_edges(const uint32_t* src, const uint32_t* dst, const uint8_t* lab, uint64_t m, uint32_t n,
+                              uint32_t L, Counters* ctr) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += uint64_t(gridDim.x) * blockDim.x)
+    if (src[i] >= n || dst[i] >= n || lab[i] < 1 || lab[i] > L) ctr->overflow = 1u;
+}
+
+}  // namespace
+
+size_t noalloc_scratch_bytes() { return kScratchBytes; }
+
+int engine_run_noalloc(const Grammar& gr, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                       const uint8_t* lab, void* stream, uint64_t* out_ptr, uint32_t* out_col, uint64_t col_cap,
+                       uint64_t* total, uint64_t* checksum, uint64_t* per_label, std::string& err) {
+  int dev = 0, cus = 0;
+  GG_HIP(hipGetDevice(&dev));
+  GG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  char* base = nullptr;
+  GG_TRY(scratch_region(dev, &base, err));
+  Engine e;
+  e.device = dev;
+  e.cus = cus;
+  e.stream = static_cast<hipStream_t>(stream);
+  e.owns_stream = false;
+  e.arena = base;
+  e.arena_cap = kScratchBytes;
+  e.arena_free.emplace(0, kScratchBytes);
+  e.h_ctr = &e.arena_h_ctr;
+  e.h_u64 = &e.arena_h_u64;
+  // a small join grid: every wave of it reserves staging space up front
+  e.expand_grid_cap = 128;
+  GG_TRY(engine_set_grammar(&e, gr, err));
+  const uint64_t R = uint64_t(e.L) * n;
+  if (R >= 0xFFFFFFFFull) {
+    err = "num_labels * num_vertices must be < 2^32";
+    return GG_E_UNSUPPORTED;
+  }
+  // the per-row arrays and the raw input alone: the least the closure needs
+  const uint64_t least = R * (8 * 5 + 4 * 4) + (m + uint64_t(e.g.ne) * n) * 8;
+  if (least > kScratchBytes) {
+    err = "the graph needs at least " + std::to_string(least) + " bytes of device memory; the allocation-free "
+          "entry point has a fixed scratch region of " + std::to_string(kScratchBytes) + " bytes";
+    return GG_E_NOMEM;
+  }
+  e.in_src.p = const_cast<uint32_t*>(src);
+  e.in_dst.p = const_cast<uint32_t*>(dst);
+  e.in_lab.p = const_cast<uint8_t*>(lab);
+  e.n = n;
+  e.m = m;
+  e.has_graph = true;
+  GG_TRY(e.alloc(e.ctr, sizeof(Counters), err));
+  e.zero_ctr();
+  if (m)
+    k_check_edges<<<e.grid_for(m), kBlock, 0, e.stream>>>(src, dst, lab, m, n, uint32_t(e.L), e.C());
+  GG_HIP(hipGetLastError());
+  GG_TRY(e.read_ctr(err));
+  if (e.h_ctr->overflow) {
+    err = "an input edge names a vertex >= num_vertices or a label outside 1..num_labels";
+    return GG_E_ARG;
+  }
+  gg_stats st{};
+  GG_TRY(e.run(0, &st, err));
+  const uint64_t tot = st.total_edges;
+  if (total) *total = tot;
+  if (tot > col_cap) {
+    err = "the closure has " + std::to_string(tot) + " edges; col_cap is " + std::to_string(col_cap);
+    return GG_E_ARG;
+  }
+  GG_HIP(hipMemcpyAsync(out_ptr, e.old_ptr.p, (R + 1) * 8, hipMemcpyDeviceToDevice, e.stream));
+  if (tot) GG_HIP(hipMemcpyAsync(out_col, e.old_col.p, tot * 4, hipMemcpyDeviceToDevice, e.stream));
+  GG_HIP(hipStreamSynchronize(e.stream));
+  if (per_label) GG_TRY(engine_counts(&e, per_label, nullptr, err));
+  if (checksum) GG_TRY(engine_checksum(&e, checksum, err));
+  GG_HIP(hipStreamSynchronize(e.stream));
   return GG_OK;
 }
 

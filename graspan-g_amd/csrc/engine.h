@@ -62,6 +62,16 @@ int engine_export_slice(Engine* e, uint32_t v0, uint32_t v1, uint32_t l0, uint32
                         uint64_t* count, std::string& err);
 int engine_release(Engine* e);
 
+// The closure of a graph whose edges are already in device memory, computed
+// with no device or pinned-host allocation: every buffer comes from a fixed
+// device region of noalloc_scratch_bytes() (the current device's own copy).
+// The result is written as a CSR (row = (label-1) * n + src, sorted dst ids)
+// into the caller's device buffers, on the caller's stream.
+size_t noalloc_scratch_bytes();
+int engine_run_noalloc(const Grammar& gr, uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                       const uint8_t* lab, void* stream, uint64_t* out_ptr, uint32_t* out_col, uint64_t col_cap,
+                       uint64_t* total, uint64_t* checksum, uint64_t* per_label, std::string& err);
+
 int device_query(int* count, int preferred, int* selected, char* name, size_t name_cap, uint64_t* total,
                  uint64_t* free_bytes, std::string& err);
 

@@ -390,6 +390,30 @@ int gg_closure_release(gg_ctx* c) {
   return GG_OK;
 }
 
+int gg_closure_run_noalloc(int num_rules, const int32_t* rules, int num_labels, uint32_t num_vertices,
+                           uint64_t num_edges, const uint32_t* src, const uint32_t* dst, const uint8_t* label,
+                           void* stream, uint64_t* row_ptr, uint32_t* col, uint64_t col_cap, uint64_t* total,
+                           uint64_t* checksum, uint64_t* per_label, char* err, size_t err_cap) {
+  std::string msg;
+  int s = GG_OK;
+  gg::Grammar g;
+  if (!row_ptr || (col_cap && !col) || num_vertices == 0 || (num_edges && (!src || !dst || !label))) {
+    s = GG_E_ARG;
+    msg = "null buffer or no vertices";
+  } else {
+    s = gg::grammar_from_rules(num_rules, rules, num_labels, g, msg);
+  }
+  if (s == GG_OK)
+    s = gg::engine_run_noalloc(g, num_vertices, num_edges, src, dst, label, stream, row_ptr, col, col_cap, total,
+                               checksum, per_label, msg);
+  if (err && err_cap) {
+    const size_t k = std::min(err_cap - 1, s == GG_OK ? size_t(0) : msg.size());
+    std::memcpy(err, msg.data(), k);
+    err[k] = 0;
+  }
+  return s;
+}
+
 int gg_device_info(gg_ctx* c, int* count, int* selected, char* name, size_t name_cap, uint64_t* total,
                    uint64_t* free_bytes) {
   if (!c) return GG_E_ARG;

@@ -32,7 +32,7 @@ EXPORTED = (
     "gg_grammar_load_file", "gg_grammar_set", "gg_grammar_info", "gg_grammar_rules", "gg_label_name", "gg_label_id",
     "gg_graph_load_file", "gg_graph_set_coo", "gg_graph_info", "gg_graph_edges",
     "gg_closure_run", "gg_closure_run_partitioned", "gg_closure_run_out_of_core", "gg_reference_partitions", "gg_closure_counts", "gg_closure_export", "gg_closure_degrees", "gg_closure_checksum",
-    "gg_triple_hash", "gg_closure_release", "gg_device_info",
+    "gg_triple_hash", "gg_closure_release", "gg_closure_run_noalloc", "gg_device_info",
     "gg_batch_create", "gg_batch_destroy", "gg_batch_add", "gg_batch_run", "gg_batch_info", "gg_batch_result",
     "gg_batch_export",
     "gg_server_create", "gg_server_submit", "gg_server_wait", "gg_server_now_ms", "gg_server_destroy",
@@ -96,6 +96,8 @@ def lib():
             "gg_closure_export": (I, [P, P, U64, P]), "gg_closure_degrees": (I, [P, P]),
             "gg_closure_checksum": (I, [P, P]), "gg_triple_hash": (U64, [U32, U32, U32]),
             "gg_closure_release": (I, [P]),
+            "gg_closure_run_noalloc": (I, [I, P, I, U32, U64, P, P, P, P, P, P, U64, P, P, P, ctypes.c_char_p,
+                                           ctypes.c_size_t]),
             "gg_device_info": (I, [P, P, P, ctypes.c_char_p, ctypes.c_size_t, P, P]),
             "gg_batch_create": (I, [P, P]), "gg_batch_destroy": (I, [P]),
             "gg_batch_add": (I, [P, I, P, I, U32, U64, P, P, P, P]),
@@ -422,6 +424,26 @@ class Batch:
         cnt = ctypes.c_uint64()
         self._check(self._L.gg_batch_export(self._h, gid, _ptr(buf), tot, ctypes.byref(cnt)), "gg_batch_export")
         return buf
+
+
+def closure_noalloc(rules, num_labels: int, n: int, m: int, src_ptr: int, dst_ptr: int, lab_ptr: int,
+                    row_ptr_ptr: int, col_ptr: int, col_cap: int, stream: int = 0):
+    """gg_closure_run_noalloc on device buffers given by address (the input
+    edges, the output CSR's row offsets and ids); runs on the current device
+    and `stream`. Returns (total, checksum, per_label counts)."""
+    L = lib()
+    rules = np.ascontiguousarray(rules, dtype=np.int32).reshape(-1, 3)
+    total, cs = ctypes.c_uint64(), ctypes.c_uint64()
+    per = np.zeros(num_labels, dtype=np.uint64)
+    err = ctypes.create_string_buffer(512)
+    st = L.gg_closure_run_noalloc(rules.shape[0], _ptr(rules), num_labels, n, m, src_ptr, dst_ptr, lab_ptr,
+                                  stream or None, row_ptr_ptr, col_ptr, col_cap, ctypes.byref(total),
+                                  ctypes.byref(cs), _ptr(per), err, 512)
+    if st != GG_OK:
+        e = GGError(st, "gg_closure_run_noalloc: " + err.value.decode())
+        e.total = total.value
+        raise e
+    return total.value, cs.value, per
 
 
 def closure_of(g, filter_flag: int = 0, device: int = -1) -> tuple:

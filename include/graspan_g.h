@@ -200,6 +200,30 @@ uint64_t gg_triple_hash(uint32_t src, uint32_t dst, uint32_t label);
 /* Releases the device state of the last closure (the input stays). */
 int gg_closure_release(gg_ctx *ctx);
 
+/* ---- allocation-free closure -------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. It computes what gg_closure_run(ctx, 0, ...) computes
+ * (the least fixpoint, filter 0) on the current device and the given stream
+ * (NULL: the null stream), with no hipMalloc, hipHostMalloc, hipFree or event:
+ * every working buffer comes from one fixed 256 MiB device region that the
+ * library holds on each GPU it runs on until the process ends. The caller
+ * passes the input edges in device memory (num_edges each; labels 1..L) and
+ * receives the closure in its own device buffers as a CSR: row_ptr holds
+ * num_labels * num_vertices + 1 uint64 offsets, row (label - 1) * num_vertices
+ * + src lists that row's dst ids in col, sorted ascending. *total (and, if
+ * not NULL, checksum as gg_closure_checksum and per_label counts, host
+ * memory) are written; col_cap smaller than the closure returns GG_E_ARG with
+ * *total set. A graph or closure that does not fit the region returns
+ * GG_E_NOMEM; an edge out of range returns GG_E_ARG. err (err_cap bytes, may
+ * be NULL) receives the message. Every call on a GPU shares that GPU's
+ * region: calls that overlap on one GPU race, so callers must serialise
+ * them per device. */
+int gg_closure_run_noalloc(int num_rules, const int32_t *rules, int num_labels, uint32_t num_vertices,
+                           uint64_t num_edges, const uint32_t *src, const uint32_t *dst, const uint8_t *label,
+                           void *stream, uint64_t *row_ptr, uint32_t *col, uint64_t col_cap, uint64_t *total,
+                           uint64_t *checksum, uint64_t *per_label, char *err, size_t err_cap);
+
 /* ---- batched closure ---------------------------------------------------
  * No counterpart in the reference, which closes one graph per process. Many
  * independent graphs (each with its own grammar) are closed together: the
