@@ -119,6 +119,7 @@ struct Counters {
   unsigned long long newe;
   unsigned long long checksum;
   unsigned int active;         // rows in the active list
+  unsigned int active_base;    // MODE 1 expand: its rows start here in the active list
   unsigned int work;           // work-stealing index
   unsigned int overflow;       // staging too small
   unsigned int misses;
@@ -364,6 +365,18 @@ __global__ __launch_bounds__(kBlock) void k_cross(DCsr c, uint32_t rows, const u
 }
 
 // Active list = every row 0..rows-1 (region operators).
+// Before rows are appended to a round's active list for a second expand
+// launch: that launch's rows start at the current end, and the work indices
+// start again.
+__global__ void k_mark_active_base(Counters* ctr) {
+  const int t = int(threadIdx.x);
+  if (t == 0) {
+    ctr->active_base = ctr->active;
+    ctr->work = 0;
+  }
+  if (t < int(kXcds)) ctr->work_xcd[t].v = 0;
+}
+
 __global__ void k_all_active(uint32_t rows, uint32_t* active, Counters* ctr) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += gridDim.x * blockDim.x) active[i] = i;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctr->active = rows;
@@ -1341,7 +1354,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
 #ifdef GG_EXPAND_STATS
   if (lane < 4) w.xs[lane] = 0;
 #endif
-  const uint32_t nactive = a.ctr->active;
+  // the first round's dedup (MODE 1) takes the active rows from active_base:
+  // graphs joining a running session append their rows after the join's
+  const uint32_t abase = MODE == 1 ? a.ctr->active_base : 0u;
+  const uint32_t nactive = a.ctr->active - abase;
   if (lane < 3) w.acc[lane] = 0;
   // rows are taken kGrab at a time and staging space kStageBlock words at a
   // time, so the two global counters see one atomic per many rows
@@ -1385,7 +1401,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       if (!got) break;
     }
     const uint32_t idx = next++;
-    const uint32_t r = uni_u32(a.active[idx]);
+    const uint32_t r = uni_u32(a.active[abase + idx]);
     GG_CLK(t_row);
     const uint32_t rn = MODE == 2 ? a.ro.np : a.n;
     const uint32_t A = r / rn, u = r - A * rn + (MODE == 2 ? a.ro.first : 0u);
@@ -2232,6 +2248,42 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+// The timing events of one run, by phase (none are created when disabled).
+struct RunEvents {
+  bool enabled = true;
+  std::vector<EventPair> expand, compact, merge, other;
+  EventPair& mk(std::vector<EventPair>& v) {
+    v.emplace_back();
+    if (enabled) {
+      (void)hipEventCreate(&v.back().a);
+      (void)hipEventCreate(&v.back().b);
+    }
+    return v.back();
+  }
+  ~RunEvents() {
+    for (auto* v : {&expand, &compact, &merge, &other})
+      for (auto& e : *v) {
+        if (e.a) (void)hipEventDestroy(e.a);
+        if (e.b) (void)hipEventDestroy(e.b);
+      }
+  }
+};
+
+// What the round loop needs that depends only on the grammar and the vertex
+// count (plan_rounds).
+struct RoundPlan {
+  ActArgs act{};
+  PairPlan pair_plan{};
+  uint64_t act_work = 0, NS = 0;
+  uint32_t fr_words = 0;
+  uint32_t n = 0;  // the vertex count it was made for
+  uint64_t pair_arena_forced = 0;
+};
+
 }  // namespace
 
 class Engine {
@@ -2306,6 +2358,11 @@ class Engine {
   Counters arena_h_ctr{};
   unsigned long long arena_h_u64 = 0;
   uint32_t expand_grid_cap = 0;  // 0: as many blocks as fit the device
+
+  // session state (a running fixpoint that graphs join at round boundaries)
+  RoundPlan sess_rp;
+  gg_stats sess_S{};
+  uint64_t sess_inject = 0;  // raw edges waiting in raw_ptr/raw_col for the next round
 
   ~Engine() {
     DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
@@ -2626,18 +2683,19 @@ class Engine {
   uint32_t gather_grid() const { return uint32_t(cus) * uint32_t(gather_blocks_per_cu); }
   uint32_t rows() const { return uint32_t(uint64_t(L) * n); }
 
+  bool pack_any_label() const {
+    bool any = false;
+    for (int i = 0; i < 8; ++i) any |= pack_labels[i] != 0;
+    return any;
+  }
+  hipError_t rec(hipEvent_t ev) { return ev ? hipEventRecord(ev, stream) : hipSuccess; }
+
   int run(int filter, gg_stats* st, std::string& err);
+  int plan_rounds(RoundPlan& rp, std::string& err);
+  int round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t inject, std::string& err);
   int run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& parts, const PartOpts& po, gg_stats* st,
             std::string& err);
 };
-
-namespace {
-
-struct EventPair {
-  hipEvent_t a = nullptr, b = nullptr;
-};
-
-}  // namespace
 
 int Engine::run(int filter, gg_stats* st, std::string& err) {
   if (!has_grammar || !has_graph) {
@@ -2664,27 +2722,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   GG_HIP(hipSetDevice(device));
   gg_stats S{};
   const uint32_t R = rows();
-  std::vector<EventPair> ev_expand, ev_compact, ev_merge, ev_other;
-  // fixed-region mode creates no events: its statistics carry no times
-  auto mk = [&](std::vector<EventPair>& v) -> EventPair& {
-    v.emplace_back();
-    if (!arena) {
-      (void)hipEventCreate(&v.back().a);
-      (void)hipEventCreate(&v.back().b);
-    }
-    return v.back();
-  };
-  auto rec = [&](hipEvent_t ev) -> hipError_t { return ev ? hipEventRecord(ev, stream) : hipSuccess; };
-  struct EvGuard {
-    std::vector<EventPair>* vs[4];
-    ~EvGuard() {
-      for (auto* v : vs)
-        for (auto& e : *v) {
-          if (e.a) (void)hipEventDestroy(e.a);
-          if (e.b) (void)hipEventDestroy(e.b);
-        }
-    }
-  } guard{{&ev_expand, &ev_compact, &ev_merge, &ev_other}};
+  RunEvents ev;
+  ev.enabled = !arena;  // fixed-region mode creates no events: its statistics carry no times
 
   peak = used;  // device_bytes_peak is per run
   GG_TRY(alloc(ctr, sizeof(Counters), err));
@@ -2702,7 +2741,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
 
   // ---- initial delta: dedup(input U eps loops) ---------------------------
   {
-    EventPair& e = mk(ev_other);
+    EventPair& e = ev.mk(ev.other);
     GG_HIP(rec(e.a));
     GG_TRY(alloc(raw_ptr, (uint64_t(R) + 1) * 8, err));
     fill_u32(seg_cnt, 0, R);
@@ -2727,7 +2766,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     GG_TRY(alloc(staging, stage_words * 4, err));
     GG_HIP(rec(e.b));
     for (;;) {
-      EventPair& x = mk(ev_expand);
+      EventPair& x = ev.mk(ev.expand);
       zero_ctr();
       fill_u32(seg_cnt, 0, R);
       k_activate_raw<<<grid_for(R), kBlock, 0, stream>>>(P<unsigned long long>(raw_ptr), R, P<uint32_t>(active), C());
@@ -2742,7 +2781,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
     }
     const uint64_t N0 = h_ctr->newe;
-    EventPair& c = mk(ev_compact);
+    EventPair& c = ev.mk(ev.compact);
     GG_HIP(rec(c.a));
     GG_TRY(scan(seg_cnt, R, dl_ptr, err));
     GG_TRY(grow(dl_col, std::max<uint64_t>(N0, 1) * 4, err));
@@ -2766,214 +2805,26 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   }
 
   // ---- semi-naive rounds -------------------------------------------------
-  ActArgs act{};
-  act.n = n;
-  act.nlhs = uint32_t(lhs.size());
-  for (size_t i = 0; i < lhs.size(); ++i) act.lhs[i] = lhs[i];
-  for (int i = 0; i < kMaxRules; ++i) act.u_li[i] = act.b_li[i] = 0xFFFF;
-  for (size_t li = 0; li < lhs.size(); ++li) {
-    for (int i = 0; i < g.nu; ++i)
-      if (g.u_lhs[i] == lhs[li]) act.u_li[i] = uint16_t(li);
-    for (int i = 0; i < g.nb; ++i)
-      if (g.b_lhs[i] == lhs[li]) act.b_li[i] = uint16_t(li);
-  }
-  act.g = g;
-  act.active = P<uint32_t>(active);
-  act.ctr = C();
-  const uint64_t act_work = uint64_t(act.nlhs) * n;
-  const uint64_t NS = uint64_t(g.nslot) * n;
-  // slots with the same B are streamed together, two at a time
-  PairPlan pair_plan{};
-  for (int j = 0; j < kMaxRules; ++j) pair_plan.mate[j] = 0xFF;
-  for (int j = 0; j < g.nslot; ++j) {
-    if (pair_plan.mate[j] != 0xFF) continue;
-    for (int j2 = j + 1; j2 < g.nslot; ++j2)
-      if (pair_plan.mate[j2] == 0xFF && g.slot_b[j2] == g.slot_b[j]) {
-        pair_plan.mate[j] = uint8_t(j2);
-        pair_plan.mate[j2] = 0xFE;
-        break;
-      }
-  }
-  const uint32_t fr_words = (n + kWave - 1) / kWave;
+  RoundPlan rp;
+  rp.pair_arena_forced = pair_arena_forced;
   pr_arena = 0;
-  if (NS) {
-    GG_TRY(alloc(pr_ptr, 2 * NS * 8, err));
-    GG_TRY(alloc(fr_bits, uint64_t(g.nslot) * fr_words * 8, err));
-  }
   pk_ready = false;
   guess_bits = kGuessMin;
-  bool pack_any = false;
-  for (int i = 0; i < 8; ++i) pack_any |= pack_labels[i] != 0;
-  if (pack_any) {
+  GG_TRY(plan_rounds(rp, err));
+  if (pack_any_label()) {
     GG_TRY(alloc(pk_len, uint64_t(R) * 4, err));
     fill_u32(pk_len, 0, R);
   }
-
-  while (dl_nnz > 0) {
-    uint64_t N = 0;
-    // frontier pair lists for old_B x delta_C (old is empty in the first round)
-    pairs_ready = false;
-    if (NS && old_nnz > 0) {
-      EventPair& o = mk(ev_other);
-      GG_HIP(rec(o.a));
-      const DCsr dlc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
-      const DCsr oc{P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
-      k_frontier<<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, fr_words, P<unsigned long long>(fr_bits));
-      DPacked opk = plabels();
-      opk.len = pk_ready ? P<uint32_t>(pk_len) : nullptr;
-      opk.ent = P<unsigned long long>(pk_ent);
-      // one pass into kSpread arenas; a pass whose arenas overflowed is run
-      // again with larger ones
-      const uint32_t pgrid = grid_for(n);
-      const uint64_t pwaves = uint64_t((pgrid + kSpread - 1) / kSpread) * kWavesPerBlock;
-      pr_arena = pair_arena_forced ? pair_arena_forced : std::max<uint64_t>(pr_arena, 2 * pwaves * kPairBlock);
-      for (bool first = true;; first = false) {
-        GG_TRY(grow(pr_col, kSpread * pr_arena * 4, err));
-        if (!pair_arena_forced || !first) pr_arena = pr_col.bytes / 4 / kSpread;
-        zero_ctr();
-        k_pairs<<<pgrid, kBlock, 0, stream>>>(g, pair_plan, n, oc, pend_csr(), opk, dlc.ptr, P<unsigned long long>(fr_bits),
-                                              fr_words, C(),
-                                              pr_arena, P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
-        GG_HIP(hipGetLastError());
-        GG_TRY(read_ctr(err));
-        uint64_t most = 0;
-        for (const CtrSlot& q : h_ctr->slot) most = std::max<uint64_t>(most, q.pairs);
-        if (most <= pr_arena) {
-          // pair lists grow over the first rounds: size the next round ahead
-          if (most * 4 > pr_arena * 3) pr_arena = 2 * most;
-          break;
-        }
-        S.pair_reruns++;
-        pr_arena = 2 * most + kPairBlock;
-      }
-      GG_HIP(rec(o.b));
-      pairs_ready = true;
-    }
-    // staging: room for every wave's reservation block plus a guess of the
-    // output; an overflow grows it and re-runs the round
-    GG_TRY(grow(staging, (uint64_t(expand_grid()) * kWavesPerBlock * kStageBlock * 2 + 4 * dl_nnz) * 4, err));
-    for (;;) {
-      EventPair& o = mk(ev_other);
-      GG_HIP(rec(o.a));
-      zero_ctr();
-      fill_u32(seg_cnt, 0, R);
-      k_label_flags<<<1, kBlock, 0, stream>>>(P<unsigned long long>(dl_ptr), n, uint32_t(L), C());
-      act.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
-      act.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
-      act.b2rng = pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr;
-      act.pend_ptr = pend_csr().ptr;
-      if (act_work) k_activate<<<grid_for(n), kBlock, 0, stream>>>(act);
-      // a round with few active rows launches only the blocks that can get
-      // some (kGrab rows per grab, 4 waves per block), not a full grid of
-      // waves that would only contend on the work counters
-      GG_TRY(read_ctr(err));
-      const uint32_t want = uint32_t((uint64_t(h_ctr->active) + kGrab * kWavesPerBlock - 1) / (kGrab * kWavesPerBlock));
-      const uint32_t grid = std::max(1u, std::min(expand_grid(), want));
-      GG_HIP(rec(o.b));
-      EventPair& x = mk(ev_expand);
-      GG_HIP(rec(x.a));
-      if (pend_nnz)
-        k_expand<0, true><<<grid, kBlock, 0, stream>>>(expand_args(false));
-      else
-        k_expand<0><<<grid, kBlock, 0, stream>>>(expand_args(false));
-      GG_HIP(rec(x.b));
-      GG_HIP(hipGetLastError());
-      GG_TRY(read_ctr(err));
-      S.expand_launches++;
-      if (!h_ctr->overflow) break;
-      S.retries++;
-      GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
-    }
-    N = h_ctr->newe;
-    const uint32_t used_guess = guess_bits;
-    if (guess_bits == 0)
-      guess_bits = kChunkBits;
-    else if (uint64_t(h_ctr->misses) * 8u > h_ctr->active)
-      guess_bits = guess_bits < kChunkBits ? kChunkBits : 0u;
-    S.rounds++;
-    S.probe_reads += h_ctr->pkeys + h_ctr->cand;
-    S.candidates += h_ctr->cand;
-    S.old_reads += h_ctr->oreads;
-    S.new_edges += N;
-    if (verbose) {
-      float xms = 0;
-      if (ev_expand.back().a) (void)hipEventElapsedTime(&xms, ev_expand.back().a, ev_expand.back().b);
-      std::fprintf(stderr,
-                   "[gg] round %u: active %u cand %llu new %llu guess %u misses %u expand %.2f ms"
-                   " (words %llu ids %llu keys %llu rows %llu)\n",
-                   S.rounds, h_ctr->active, (unsigned long long)h_ctr->cand, (unsigned long long)N, used_guess,
-                   h_ctr->misses, xms, h_ctr->xs[0], h_ctr->xs[1], h_ctr->xs[2], h_ctr->xs[3]);
-    }
-    // delta' from staging
-    EventPair& c = mk(ev_compact);
-    GG_HIP(rec(c.a));
-    GG_TRY(scan(seg_cnt, R, nx_ptr, err));
-    GG_TRY(grow(nx_col, std::max<uint64_t>(N, 1) * 4, err));
-    const bool npk = pack_delta();
-    if (npk) {
-      GG_TRY(grow(npk_len, uint64_t(R) * 4, err));
-      GG_TRY(grow(npk_ent, std::max<uint64_t>(N, 1) * 8, err));
-    }
-    if (N)
-      k_gather<<<gather_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
-                                                     P<unsigned long long>(seg_off), P<uint32_t>(staging),
-                                                     P<unsigned long long>(nx_ptr), P<uint32_t>(nx_col), n, plabels(),
-                                                     npk ? P<unsigned long long>(npk_ent) : nullptr,
-                                                     P<uint32_t>(npk_len));
-    GG_HIP(rec(c.b));
-    // old' = old ∪ delta. While the deltas are small next to old they go to
-    // the pending CSR instead (pend' = pend ∪ delta), so a round that finds
-    // few edges does not rewrite all of old; the joins read old ∪ pend.
-    EventPair& mg = mk(ev_merge);
-    GG_HIP(rec(mg.a));
-    const bool defer = defer_ratio && old_nnz && (pend_nnz + dl_nnz) * defer_ratio <= old_nnz;
-    if (defer || pend_nnz) {
-      if (!pend_nnz) {
-        GG_TRY(alloc(pd_ptr, (uint64_t(R) + 1) * 8, err));
-        fill_u64(pd_ptr, 0ull, uint64_t(R) + 1);
-      }
-      GG_TRY(alloc(p2_ptr, (uint64_t(R) + 1) * 8, err));
-      const uint64_t p2n = pend_nnz + dl_nnz;
-      GG_TRY(grow(p2_col, std::max<uint64_t>(p2n, 1) * 4, err));
-      k_add_ptr<<<grid_for(uint64_t(R) + 1), kBlock, 0, stream>>>(
-          P<unsigned long long>(p2_ptr), P<unsigned long long>(pd_ptr), P<unsigned long long>(dl_ptr), uint64_t(R) + 1);
-      const DCsr pc{P<unsigned long long>(pd_ptr), P<uint32_t>(pd_col)};
-      const DCsr dc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
-      GG_TRY(merge_rows(pc, dc, R, n, C(), DPacked{}, nullptr, P<uint32_t>(p2_col), nullptr, nullptr, err));
-      if (pend_nnz)
-        k_copy_labels<<<dim3(uint32_t(cus), uint32_t(L)), kBlock, 0, stream>>>(pc, dc, n, C(), DPacked{}, nullptr,
-                                                                              P<uint32_t>(p2_col), nullptr, true);
-      std::swap(pd_ptr, p2_ptr);
-      std::swap(pd_col, p2_col);
-      pend_nnz = p2n;
-    }
-    if (!defer) {
-      // pend (which now holds delta) or delta itself goes into old
-      if (pend_nnz) {
-        GG_TRY(fold_into_old({P<unsigned long long>(pd_ptr), P<uint32_t>(pd_col)}, pend_nnz, nullptr, err));
-        pend_nnz = 0;
-      } else {
-        GG_TRY(fold_into_old({P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)}, dl_nnz, C(), err));
-      }
-    }
-    GG_HIP(rec(mg.b));
-    GG_HIP(hipGetLastError());
-    std::swap(dl_ptr, nx_ptr);
-    std::swap(dl_col, nx_col);
-    std::swap(dpk_len, npk_len);
-    std::swap(dpk_ent, npk_ent);
-    dpk_ready = npk;
-    dl_nnz = N;
-  }
+  while (dl_nnz > 0) GG_TRY(round_step(rp, S, ev, 0, err));
   if (pend_nnz) {  // the closure is old ∪ pend: rebuild old once
-    EventPair& mg = mk(ev_merge);
+    EventPair& mg = ev.mk(ev.merge);
     GG_HIP(rec(mg.a));
     GG_TRY(fold_into_old({P<unsigned long long>(pd_ptr), P<uint32_t>(pd_col)}, pend_nnz, nullptr, err));
     pend_nnz = 0;
     GG_HIP(rec(mg.b));
   }
   if (pk_ready && !keep_ids) {  // the packed labels' ids, from their entries
-    EventPair& mg = mk(ev_merge);
+    EventPair& mg = ev.mk(ev.merge);
     GG_HIP(rec(mg.a));
     DPacked plab{};
     for (int i = 0; i < 8; ++i) plab.labels[i] = pack_labels[i];
@@ -2998,10 +2849,10 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     }
     return t;
   };
-  S.expand_ms = sum(ev_expand);
-  S.compact_ms = sum(ev_compact);
-  S.merge_ms = sum(ev_merge);
-  S.other_ms = sum(ev_other);
+  S.expand_ms = sum(ev.expand);
+  S.compact_ms = sum(ev.compact);
+  S.merge_ms = sum(ev.merge);
+  S.other_ms = sum(ev.other);
   S.total_edges = old_nnz;
   S.device_bytes_peak = peak;
   // drop per-round scratch; keep the closure (old) for queries
@@ -3012,6 +2863,231 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   dpk_ready = false;
   has_closure = true;
   if (st) *st = S;
+  return GG_OK;
+}
+
+// Per-size setup of the round loop: the activation and pair-list plans and
+// the per-row arrays for the current vertex count (a session that grows its
+// vertex range plans again).
+int Engine::plan_rounds(RoundPlan& rp, std::string& err) {
+  const uint32_t R = rows();
+  GG_TRY(alloc(nx_ptr, (uint64_t(R) + 1) * 8, err));
+  GG_TRY(alloc(o2_ptr, (uint64_t(R) + 1) * 8, err));
+  GG_TRY(alloc(seg_off, uint64_t(R) * 8, err));
+  GG_TRY(alloc(seg_cnt, uint64_t(R) * 4, err));
+  GG_TRY(alloc(active, uint64_t(R) * 4, err));
+  ActArgs& act = rp.act;
+  act = ActArgs{};
+  act.n = n;
+  act.nlhs = uint32_t(lhs.size());
+  for (size_t i = 0; i < lhs.size(); ++i) act.lhs[i] = lhs[i];
+  for (int i = 0; i < kMaxRules; ++i) act.u_li[i] = act.b_li[i] = 0xFFFF;
+  for (size_t li = 0; li < lhs.size(); ++li) {
+    for (int i = 0; i < g.nu; ++i)
+      if (g.u_lhs[i] == lhs[li]) act.u_li[i] = uint16_t(li);
+    for (int i = 0; i < g.nb; ++i)
+      if (g.b_lhs[i] == lhs[li]) act.b_li[i] = uint16_t(li);
+  }
+  act.g = g;
+  act.active = P<uint32_t>(active);
+  act.ctr = C();
+  rp.act_work = uint64_t(act.nlhs) * n;
+  rp.NS = uint64_t(g.nslot) * n;
+  const uint64_t NS = rp.NS;
+  // slots with the same B are streamed together, two at a time
+  PairPlan& pair_plan = rp.pair_plan;
+  pair_plan = PairPlan{};
+  for (int j = 0; j < kMaxRules; ++j) pair_plan.mate[j] = 0xFF;
+  for (int j = 0; j < g.nslot; ++j) {
+    if (pair_plan.mate[j] != 0xFF) continue;
+    for (int j2 = j + 1; j2 < g.nslot; ++j2)
+      if (pair_plan.mate[j2] == 0xFF && g.slot_b[j2] == g.slot_b[j]) {
+        pair_plan.mate[j] = uint8_t(j2);
+        pair_plan.mate[j2] = 0xFE;
+        break;
+      }
+  }
+  rp.fr_words = (n + kWave - 1) / kWave;
+  const uint32_t fr_words = rp.fr_words;
+  if (NS) {
+    GG_TRY(alloc(pr_ptr, 2 * NS * 8, err));
+    GG_TRY(alloc(fr_bits, uint64_t(g.nslot) * fr_words * 8, err));
+  }
+  rp.n = n;
+  return GG_OK;
+}
+
+// One semi-naive round: pair lists, activation, the fused join, compaction of
+// the new edges into the next delta, and old' = old ∪ delta (or the pending
+// CSR). inject > 0: the raw rows in raw_ptr/raw_col (inject edges, rows of
+// vertices that have no edges yet) are deduplicated in the same round, after
+// the join, so that they enter the next delta as new edges (graphs joining a
+// running session).
+int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t inject, std::string& err) {
+  const uint32_t R = rows();
+  uint64_t N = 0;
+  // frontier pair lists for old_B x delta_C (old is empty in the first round)
+  pairs_ready = false;
+  if (rp.NS && old_nnz > 0) {
+    EventPair& o = ev.mk(ev.other);
+    GG_HIP(rec(o.a));
+    const DCsr dlc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
+    const DCsr oc{P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
+    k_frontier<<<grid_for(n), kBlock, 0, stream>>>(g, n, dlc, rp.fr_words, P<unsigned long long>(fr_bits));
+    DPacked opk = plabels();
+    opk.len = pk_ready ? P<uint32_t>(pk_len) : nullptr;
+    opk.ent = P<unsigned long long>(pk_ent);
+    // one pass into kSpread arenas; a pass whose arenas overflowed is run
+    // again with larger ones
+    const uint32_t pgrid = grid_for(n);
+    const uint64_t pwaves = uint64_t((pgrid + kSpread - 1) / kSpread) * kWavesPerBlock;
+    pr_arena = rp.pair_arena_forced ? rp.pair_arena_forced : std::max<uint64_t>(pr_arena, 2 * pwaves * kPairBlock);
+    for (bool first = true;; first = false) {
+      GG_TRY(grow(pr_col, kSpread * pr_arena * 4, err));
+      if (!rp.pair_arena_forced || !first) pr_arena = pr_col.bytes / 4 / kSpread;
+      zero_ctr();
+      k_pairs<<<pgrid, kBlock, 0, stream>>>(g, rp.pair_plan, n, oc, pend_csr(), opk, dlc.ptr, P<unsigned long long>(fr_bits),
+                                            rp.fr_words, C(),
+                                            pr_arena, P<unsigned long long>(pr_ptr), P<uint32_t>(pr_col));
+      GG_HIP(hipGetLastError());
+      GG_TRY(read_ctr(err));
+      uint64_t most = 0;
+      for (const CtrSlot& q : h_ctr->slot) most = std::max<uint64_t>(most, q.pairs);
+      if (most <= pr_arena) {
+        // pair lists grow over the first rounds: size the next round ahead
+        if (most * 4 > pr_arena * 3) pr_arena = 2 * most;
+        break;
+      }
+      S.pair_reruns++;
+      pr_arena = 2 * most + kPairBlock;
+    }
+    GG_HIP(rec(o.b));
+    pairs_ready = true;
+  }
+  // staging: room for every wave's reservation block plus a guess of the
+  // output; an overflow grows it and re-runs the round
+  GG_TRY(grow(staging, (uint64_t(expand_grid()) * kWavesPerBlock * kStageBlock * 2 + 4 * dl_nnz + inject) * 4, err));
+  for (;;) {
+    EventPair& o = ev.mk(ev.other);
+    GG_HIP(rec(o.a));
+    zero_ctr();
+    fill_u32(seg_cnt, 0, R);
+    k_label_flags<<<1, kBlock, 0, stream>>>(P<unsigned long long>(dl_ptr), n, uint32_t(L), C());
+    rp.act.old = {P<unsigned long long>(old_ptr), P<uint32_t>(old_col)};
+    rp.act.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
+    rp.act.b2rng = pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr;
+    rp.act.pend_ptr = pend_csr().ptr;
+    if (rp.act_work) k_activate<<<grid_for(n), kBlock, 0, stream>>>(rp.act);
+    // a round with few active rows launches only the blocks that can get
+    // some (kGrab rows per grab, 4 waves per block), not a full grid of
+    // waves that would only contend on the work counters
+    GG_TRY(read_ctr(err));
+    const uint32_t want = uint32_t((uint64_t(h_ctr->active) + kGrab * kWavesPerBlock - 1) / (kGrab * kWavesPerBlock));
+    const uint32_t grid = std::max(1u, std::min(expand_grid(), want));
+    GG_HIP(rec(o.b));
+    EventPair& x = ev.mk(ev.expand);
+    GG_HIP(rec(x.a));
+    if (pend_nnz)
+      k_expand<0, true><<<grid, kBlock, 0, stream>>>(expand_args(false));
+    else
+      k_expand<0><<<grid, kBlock, 0, stream>>>(expand_args(false));
+    if (inject) {
+      // graphs joining a session: their raw rows go to the end of this
+      // round's active list and are deduplicated there (their old, delta and
+      // pending rows are empty, so the result is their first delta)
+      k_mark_active_base<<<1, kWave, 0, stream>>>(C());
+      k_activate_raw<<<grid_for(R), kBlock, 0, stream>>>(P<unsigned long long>(raw_ptr), R, P<uint32_t>(active), C());
+      k_expand<1><<<expand_grid(), kBlock, 0, stream>>>(expand_args(true));
+      S.expand_launches++;
+    }
+    GG_HIP(rec(x.b));
+    GG_HIP(hipGetLastError());
+    GG_TRY(read_ctr(err));
+    S.expand_launches++;
+    if (!h_ctr->overflow) break;
+    S.retries++;
+    GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
+  }
+  N = h_ctr->newe;
+  const uint32_t used_guess = guess_bits;
+  if (guess_bits == 0)
+    guess_bits = kChunkBits;
+  else if (uint64_t(h_ctr->misses) * 8u > h_ctr->active)
+    guess_bits = guess_bits < kChunkBits ? kChunkBits : 0u;
+  S.rounds++;
+  S.probe_reads += h_ctr->pkeys + h_ctr->cand;
+  S.candidates += h_ctr->cand;
+  S.old_reads += h_ctr->oreads;
+  S.new_edges += N;
+  if (verbose) {
+    float xms = 0;
+    if (ev.expand.back().a) (void)hipEventElapsedTime(&xms, ev.expand.back().a, ev.expand.back().b);
+    std::fprintf(stderr,
+                 "[gg] round %u: active %u cand %llu new %llu guess %u misses %u expand %.2f ms"
+                 " (words %llu ids %llu keys %llu rows %llu)\n",
+                 S.rounds, h_ctr->active, (unsigned long long)h_ctr->cand, (unsigned long long)N, used_guess,
+                 h_ctr->misses, xms, h_ctr->xs[0], h_ctr->xs[1], h_ctr->xs[2], h_ctr->xs[3]);
+  }
+  // delta' from staging
+  EventPair& c = ev.mk(ev.compact);
+  GG_HIP(rec(c.a));
+  GG_TRY(scan(seg_cnt, R, nx_ptr, err));
+  GG_TRY(grow(nx_col, std::max<uint64_t>(N, 1) * 4, err));
+  const bool npk = pack_delta();
+  if (npk) {
+    GG_TRY(grow(npk_len, uint64_t(R) * 4, err));
+    GG_TRY(grow(npk_ent, std::max<uint64_t>(N, 1) * 8, err));
+  }
+  if (N)
+    k_gather<<<gather_grid(), kBlock, 0, stream>>>(P<uint32_t>(active), C(), P<uint32_t>(seg_cnt),
+                                                   P<unsigned long long>(seg_off), P<uint32_t>(staging),
+                                                   P<unsigned long long>(nx_ptr), P<uint32_t>(nx_col), n, plabels(),
+                                                   npk ? P<unsigned long long>(npk_ent) : nullptr,
+                                                   P<uint32_t>(npk_len));
+  GG_HIP(rec(c.b));
+  // old' = old ∪ delta. While the deltas are small next to old they go to
+  // the pending CSR instead (pend' = pend ∪ delta), so a round that finds
+  // few edges does not rewrite all of old; the joins read old ∪ pend.
+  EventPair& mg = ev.mk(ev.merge);
+  GG_HIP(rec(mg.a));
+  const bool defer = defer_ratio && old_nnz && (pend_nnz + dl_nnz) * defer_ratio <= old_nnz;
+  if (defer || pend_nnz) {
+    if (!pend_nnz) {
+      GG_TRY(alloc(pd_ptr, (uint64_t(R) + 1) * 8, err));
+      fill_u64(pd_ptr, 0ull, uint64_t(R) + 1);
+    }
+    GG_TRY(alloc(p2_ptr, (uint64_t(R) + 1) * 8, err));
+    const uint64_t p2n = pend_nnz + dl_nnz;
+    GG_TRY(grow(p2_col, std::max<uint64_t>(p2n, 1) * 4, err));
+    k_add_ptr<<<grid_for(uint64_t(R) + 1), kBlock, 0, stream>>>(
+        P<unsigned long long>(p2_ptr), P<unsigned long long>(pd_ptr), P<unsigned long long>(dl_ptr), uint64_t(R) + 1);
+    const DCsr pc{P<unsigned long long>(pd_ptr), P<uint32_t>(pd_col)};
+    const DCsr dc{P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
+    GG_TRY(merge_rows(pc, dc, R, n, C(), DPacked{}, nullptr, P<uint32_t>(p2_col), nullptr, nullptr, err));
+    if (pend_nnz)
+      k_copy_labels<<<dim3(uint32_t(cus), uint32_t(L)), kBlock, 0, stream>>>(pc, dc, n, C(), DPacked{}, nullptr,
+                                                                            P<uint32_t>(p2_col), nullptr, true);
+    std::swap(pd_ptr, p2_ptr);
+    std::swap(pd_col, p2_col);
+    pend_nnz = p2n;
+  }
+  if (!defer) {
+    // pend (which now holds delta) or delta itself goes into old
+    if (pend_nnz) {
+      GG_TRY(fold_into_old({P<unsigned long long>(pd_ptr), P<uint32_t>(pd_col)}, pend_nnz, nullptr, err));
+      pend_nnz = 0;
+    } else {
+      GG_TRY(fold_into_old({P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)}, dl_nnz, C(), err));
+    }
+  }
+  GG_HIP(rec(mg.b));
+  GG_HIP(hipGetLastError());
+  std::swap(dl_ptr, nx_ptr);
+  std::swap(dl_col, nx_col);
+  std::swap(dpk_len, npk_len);
+  std::swap(dpk_ent, npk_ent);
+  dpk_ready = npk;
+  dl_nnz = N;
   return GG_OK;
 }
 
@@ -3962,6 +4038,292 @@ int engine_checksum(Engine* e, uint64_t* sum, std::string& err) {
   *sum = e->h_ctr->checksum;
   return GG_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Sessions (the serving loop): one running fixpoint that graphs join at round
+// boundaries. A joining graph takes the next vertex range: the per-row offset
+// arrays are re-laid out for the larger vertex count (rows are label-major,
+// so each label's block gains empty rows at its end and no id moves), its
+// edges are uploaded and its raw rows are deduplicated inside the next round
+// (round_step's inject), which makes them its first delta. A graph whose rows
+// get no new edges in a round has reached its fixpoint: its closure is its
+// rows of old ∪ pending, which later rounds do not touch. Sessions keep the
+// ids of packed labels in old, so results can be read while the fixpoint of
+// the other graphs goes on.
+
+namespace {
+
+// rows [rb, re) of one label of one graph: their first vertex u0, the
+// graph's vertex base v0 and the label in the graph's numbering
+struct SliceItem {
+  uint32_t rb, re, u0, v0, lab;
+};
+
+__global__ void k_remap_ptr(const unsigned long long* in, unsigned long long* out, uint32_t L, uint32_t n0,
+                            uint32_t n1) {
+  const uint64_t total = uint64_t(L) * n1 + 1;
+  for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < total; r += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t l = uint32_t(r / n1), u = uint32_t(r - uint64_t(l) * n1);
+    // past the old vertices (and the final entry) a row starts where its
+    // label's old block ended
+    out[r] = (l < L && u < n0) ? in[uint64_t(l) * n0 + u] : in[uint64_t(l < L ? l + 1 : L) * n0];
+  }
+}
+
+__global__ void k_remap_len(const uint32_t* in, uint32_t* out, uint32_t L, uint32_t n0, uint32_t n1) {
+  const uint64_t total = uint64_t(L) * n1;
+  for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < total; r += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t l = uint32_t(r / n1), u = uint32_t(r - uint64_t(l) * n1);
+    out[r] = u < n0 ? in[uint64_t(l) * n0 + u] : 0u;
+  }
+}
+
+__global__ void k_item_counts(const unsigned long long* ptr, const SliceItem* it, uint32_t ni,
+                              unsigned long long* out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ni; i += gridDim.x * blockDim.x)
+    out[i] = ptr[it[i].re] - ptr[it[i].rb];
+}
+
+// per item: edge count and checksum of its rows in c, added to cnt / sums
+__global__ __launch_bounds__(kBlock) void k_item_results(DCsr c, const SliceItem* it, uint32_t ni,
+                                                         unsigned long long* cnt, unsigned long long* sums) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t wv = threadIdx.x / kWave;
+  for (uint32_t i = blockIdx.x; i < ni; i += gridDim.x) {
+    const SliceItem x = it[i];
+    unsigned long long acc = 0;
+    for (uint32_t r = x.rb + wv; r < x.re; r += kWavesPerBlock) {
+      const uint32_t u = x.u0 + (r - x.rb) - x.v0;
+      const unsigned long long b = c.ptr[r], e = c.ptr[r + 1];
+      for (unsigned long long k = b + lane; k < e; k += kWave) acc += triple_hash(u, c.col[k] - x.v0, x.lab);
+    }
+    acc = wave_sum_u64(acc);
+    if (lane == 0 && acc) atomicAdd(&sums[i], acc);
+    if (threadIdx.x == 0) atomicAdd(&cnt[i], (unsigned long long)(c.ptr[x.re] - c.ptr[x.rb]));
+  }
+}
+
+std::vector<SliceItem> slice_items(const std::vector<SessionSlice>& sl, uint32_t n) {
+  std::vector<SliceItem> it;
+  for (const SessionSlice& q : sl)
+    for (uint32_t i = 0; i < q.lg; ++i) {
+      const uint32_t rb = (q.l0 + i) * n + q.v0;
+      it.push_back({rb, rb + q.n, q.v0, q.v0, i + 1});
+    }
+  return it;
+}
+
+}  // namespace
+
+int engine_session_begin(Engine* e, const Grammar& gr, std::string& err) {
+  GG_TRY(engine_set_grammar(e, gr, err));
+  if (e->g.ne) {
+    err = "a session takes eps rules as explicit self-loop edges";
+    return GG_E_ARG;
+  }
+  GG_HIP(hipSetDevice(e->device));
+  e->n = 0;
+  e->m = 0;
+  e->old_nnz = e->dl_nnz = e->pend_nnz = 0;
+  e->pk_ready = e->dpk_ready = e->pairs_ready = false;
+  e->keep_ids = true;
+  e->defer_ratio = kDeferRatio;
+  if (const char* dr = std::getenv("GG_DEFER_RATIO")) e->defer_ratio = std::strtoull(dr, nullptr, 10);
+  e->guess_bits = kGuessMin;
+  e->pr_arena = 0;
+  e->sess_rp = RoundPlan{};
+  e->sess_S = gg_stats{};
+  e->sess_inject = 0;
+  e->has_graph = true;
+  e->has_closure = false;
+  e->peak = e->used;
+  GG_TRY(e->alloc(e->ctr, sizeof(Counters), err));
+  if (!e->h_ctr) GG_HIP(hipHostMalloc((void**)&e->h_ctr, sizeof(Counters), hipHostMallocDefault));
+  if (!e->h_u64) GG_HIP(hipHostMalloc((void**)&e->h_u64, 8, hipHostMallocDefault));
+  return GG_OK;
+}
+
+int engine_session_admit(Engine* e, uint32_t new_n, const uint32_t* src, const uint32_t* dst, const uint8_t* lab,
+                         uint64_t m, std::string& err) {
+  GG_HIP(hipSetDevice(e->device));
+  const uint32_t n0 = e->n;
+  if (new_n < n0) {
+    err = "a session's vertex range only grows";
+    return GG_E_ARG;
+  }
+  const uint64_t L = uint64_t(e->L);
+  if (L * new_n >= 0xFFFFFFFFull || uint64_t(e->g.nslot) * new_n >= 0xFFFFFFFFull) {
+    err = "session too large: labels * vertices (and pair slots * vertices) must stay below 2^32";
+    return GG_E_UNSUPPORTED;
+  }
+  const uint64_t R1 = L * new_n;
+  if (n0 == 0) {
+    GG_TRY(e->alloc(e->old_ptr, (R1 + 1) * 8, err));
+    GG_TRY(e->alloc(e->dl_ptr, (R1 + 1) * 8, err));
+    e->fill_u64(e->old_ptr, 0, R1 + 1);
+    e->fill_u64(e->dl_ptr, 0, R1 + 1);
+    if (e->pack_any_label()) {
+      GG_TRY(e->alloc(e->pk_len, R1 * 4, err));
+      e->fill_u32(e->pk_len, 0, R1);
+    }
+  } else if (new_n > n0) {
+    auto remap_ptr = [&](DevBuf& b) -> int {
+      DevBuf t;
+      GG_TRY(e->alloc(t, (R1 + 1) * 8, err));
+      k_remap_ptr<<<e->grid_for(R1 + 1), kBlock, 0, e->stream>>>(
+          static_cast<unsigned long long*>(b.p), static_cast<unsigned long long*>(t.p), uint32_t(L), n0, new_n);
+      std::swap(b, t);
+      e->release(t);
+      return GG_OK;
+    };
+    auto remap_len = [&](DevBuf& b) -> int {
+      DevBuf t;
+      GG_TRY(e->alloc(t, R1 * 4, err));
+      k_remap_len<<<e->grid_for(R1), kBlock, 0, e->stream>>>(static_cast<uint32_t*>(b.p), static_cast<uint32_t*>(t.p),
+                                                             uint32_t(L), n0, new_n);
+      std::swap(b, t);
+      e->release(t);
+      return GG_OK;
+    };
+    GG_TRY(remap_ptr(e->old_ptr));
+    GG_TRY(remap_ptr(e->dl_ptr));
+    if (e->pend_nnz) GG_TRY(remap_ptr(e->pd_ptr));
+    if (e->pack_any_label()) GG_TRY(remap_len(e->pk_len));
+    if (e->dpk_ready) GG_TRY(remap_len(e->dpk_len));
+    GG_HIP(hipGetLastError());
+  }
+  e->n = new_n;
+  e->m = m;
+  if (!m) return GG_OK;
+  GG_TRY(e->alloc(e->in_src, m * 4, err));
+  GG_TRY(e->alloc(e->in_dst, m * 4, err));
+  GG_TRY(e->alloc(e->in_lab, m, err));
+  GG_HIP(hipMemcpyAsync(e->in_src.p, src, m * 4, hipMemcpyHostToDevice, e->stream));
+  GG_HIP(hipMemcpyAsync(e->in_dst.p, dst, m * 4, hipMemcpyHostToDevice, e->stream));
+  GG_HIP(hipMemcpyAsync(e->in_lab.p, lab, m, hipMemcpyHostToDevice, e->stream));
+  const uint32_t R = e->rows();
+  GG_TRY(e->alloc(e->seg_cnt, uint64_t(R) * 4, err));
+  GG_TRY(e->alloc(e->raw_ptr, (uint64_t(R) + 1) * 8, err));
+  GG_TRY(e->grow(e->raw_col, m * 4, err));
+  e->fill_u32(e->seg_cnt, 0, R);
+  k_count_raw<<<e->grid_for(m), kBlock, 0, e->stream>>>(static_cast<uint32_t*>(e->in_src.p),
+                                                        static_cast<uint8_t*>(e->in_lab.p), m, e->n, e->g,
+                                                        static_cast<uint32_t*>(e->seg_cnt.p));
+  GG_TRY(e->scan(e->seg_cnt, R, e->raw_ptr, err));
+  e->fill_u32(e->seg_cnt, 0, R);
+  k_scatter_raw<<<e->grid_for(m), kBlock, 0, e->stream>>>(
+      static_cast<uint32_t*>(e->in_src.p), static_cast<uint32_t*>(e->in_dst.p), static_cast<uint8_t*>(e->in_lab.p), m,
+      e->n, e->g, static_cast<unsigned long long*>(e->raw_ptr.p), static_cast<uint32_t*>(e->seg_cnt.p),
+      static_cast<uint32_t*>(e->raw_col.p));
+  GG_HIP(hipGetLastError());
+  e->sess_inject = m;
+  return GG_OK;
+}
+
+int engine_session_round(Engine* e, const std::vector<SessionSlice>& live, std::vector<uint64_t>& delta,
+                         std::string& err) {
+  GG_HIP(hipSetDevice(e->device));
+  delta.assign(live.size(), 0);
+  if (e->n == 0 || (e->dl_nnz == 0 && e->sess_inject == 0)) return GG_OK;
+  if (e->sess_rp.n != e->n) GG_TRY(e->plan_rounds(e->sess_rp, err));
+  RunEvents ev;
+  ev.enabled = false;
+  GG_TRY(e->round_step(e->sess_rp, e->sess_S, ev, e->sess_inject, err));
+  e->sess_inject = 0;
+  if (live.empty()) return GG_OK;
+  // per graph: its edges in the next delta
+  const std::vector<SliceItem> it = slice_items(live, e->n);
+  DevBuf d_it, d_out;
+  struct Guard {
+    Engine* e;
+    DevBuf* b[2];
+    ~Guard() {
+      for (DevBuf* x : b) e->release(*x);
+    }
+  } guard{e, {&d_it, &d_out}};
+  GG_TRY(e->alloc(d_it, it.size() * sizeof(SliceItem), err));
+  GG_TRY(e->alloc(d_out, it.size() * 8, err));
+  GG_HIP(hipMemcpyAsync(d_it.p, it.data(), it.size() * sizeof(SliceItem), hipMemcpyHostToDevice, e->stream));
+  k_item_counts<<<e->grid_for(it.size()), kBlock, 0, e->stream>>>(static_cast<unsigned long long*>(e->dl_ptr.p),
+                                                                  static_cast<SliceItem*>(d_it.p), uint32_t(it.size()),
+                                                                  static_cast<unsigned long long*>(d_out.p));
+  GG_HIP(hipGetLastError());
+  std::vector<uint64_t> out(it.size());
+  GG_HIP(hipMemcpyAsync(out.data(), d_out.p, it.size() * 8, hipMemcpyDeviceToHost, e->stream));
+  GG_HIP(hipStreamSynchronize(e->stream));
+  size_t k = 0;
+  for (size_t q = 0; q < live.size(); ++q)
+    for (uint32_t i = 0; i < live[q].lg; ++i) delta[q] += out[k++];
+  return GG_OK;
+}
+
+int engine_session_results(Engine* e, const std::vector<SessionSlice>& done, std::vector<uint64_t>& counts,
+                           std::vector<uint64_t>& sums, std::string& err) {
+  GG_HIP(hipSetDevice(e->device));
+  const std::vector<SliceItem> it = slice_items(done, e->n);
+  counts.assign(it.size(), 0);
+  sums.assign(done.size(), 0);
+  if (it.empty() || e->n == 0) return GG_OK;
+  DevBuf d_it, d_cnt, d_sum;
+  struct Guard {
+    Engine* e;
+    DevBuf* b[3];
+    ~Guard() {
+      for (DevBuf* x : b) e->release(*x);
+    }
+  } guard{e, {&d_it, &d_cnt, &d_sum}};
+  GG_TRY(e->alloc(d_it, it.size() * sizeof(SliceItem), err));
+  GG_TRY(e->alloc(d_cnt, it.size() * 8, err));
+  GG_TRY(e->alloc(d_sum, it.size() * 8, err));
+  GG_HIP(hipMemcpyAsync(d_it.p, it.data(), it.size() * sizeof(SliceItem), hipMemcpyHostToDevice, e->stream));
+  e->fill_u64(d_cnt, 0, it.size());
+  e->fill_u64(d_sum, 0, it.size());
+  const uint32_t grid = uint32_t(std::min<uint64_t>(it.size(), uint64_t(e->cus) * 8));
+  k_item_results<<<grid, kBlock, 0, e->stream>>>({static_cast<unsigned long long*>(e->old_ptr.p),
+                                                  static_cast<uint32_t*>(e->old_col.p)},
+                                                 static_cast<SliceItem*>(d_it.p), uint32_t(it.size()),
+                                                 static_cast<unsigned long long*>(d_cnt.p),
+                                                 static_cast<unsigned long long*>(d_sum.p));
+  if (e->pend_nnz)
+    k_item_results<<<grid, kBlock, 0, e->stream>>>({static_cast<unsigned long long*>(e->pd_ptr.p),
+                                                    static_cast<uint32_t*>(e->pd_col.p)},
+                                                   static_cast<SliceItem*>(d_it.p), uint32_t(it.size()),
+                                                   static_cast<unsigned long long*>(d_cnt.p),
+                                                   static_cast<unsigned long long*>(d_sum.p));
+  GG_HIP(hipGetLastError());
+  std::vector<uint64_t> s(it.size());
+  GG_HIP(hipMemcpyAsync(counts.data(), d_cnt.p, it.size() * 8, hipMemcpyDeviceToHost, e->stream));
+  GG_HIP(hipMemcpyAsync(s.data(), d_sum.p, it.size() * 8, hipMemcpyDeviceToHost, e->stream));
+  GG_HIP(hipStreamSynchronize(e->stream));
+  size_t k = 0;
+  for (size_t q = 0; q < done.size(); ++q)
+    for (uint32_t i = 0; i < done[q].lg; ++i) sums[q] += s[k++];
+  return GG_OK;
+}
+
+void engine_session_end(Engine* e, bool trim) {
+  (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->stream);
+  if (trim) {
+    engine_release(e);
+  } else {
+    DevBuf* all[] = {&e->old_ptr, &e->old_col, &e->dl_ptr, &e->dl_col, &e->nx_ptr, &e->nx_col, &e->o2_ptr,
+                     &e->o2_col, &e->staging, &e->pk_len, &e->pk_ent, &e->pk2_ent, &e->dpk_len, &e->dpk_ent,
+                     &e->npk_len, &e->npk_ent, &e->pd_ptr, &e->pd_col, &e->p2_ptr, &e->p2_col, &e->raw_ptr,
+                     &e->raw_col, &e->pr_col};
+    for (DevBuf* b : all) e->release(*b);
+  }
+  e->n = 0;
+  e->m = 0;
+  e->old_nnz = e->dl_nnz = e->pend_nnz = 0;
+  e->pk_ready = e->dpk_ready = false;
+  e->sess_rp = RoundPlan{};
+  e->sess_inject = 0;
+  e->has_graph = false;
+  e->has_closure = false;
+}
+
+uint64_t engine_device_bytes(Engine* e) { return e->used + e->cached; }
 
 // ---------------------------------------------------------------------------
 // The allocation-free entry point (gg_closure_run_noalloc). Its buffers come

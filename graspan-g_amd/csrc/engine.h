@@ -62,6 +62,31 @@ int engine_export_slice(Engine* e, uint32_t v0, uint32_t v1, uint32_t l0, uint32
                         uint64_t* count, std::string& err);
 int engine_release(Engine* e);
 
+// Sessions: one running fixpoint that graphs join at round boundaries and
+// leave when their rows stop changing (the serving loop). A graph occupies
+// vertices [v0, v0 + n) and the combined labels [l0, l0 + lg) (0-based) of
+// the session's grammar, which must have no eps rules (they are explicit
+// loop edges). engine_session_admit grows the vertex range to new_n and
+// queues the given edges (session numbering) for the next round;
+// engine_session_round runs one round and returns, per live graph, its edges
+// in the next delta (0: it has reached its fixpoint); engine_session_results
+// gives finished graphs' per-label counts (concatenated) and checksums in the
+// graphs' own numbering. engine_session_end drops the session's state (trim:
+// also hand the engine's cached device memory back).
+struct SessionSlice {
+  uint32_t v0, n, l0, lg;
+};
+int engine_session_begin(Engine* e, const Grammar& combined, std::string& err);
+int engine_session_admit(Engine* e, uint32_t new_n, const uint32_t* src, const uint32_t* dst, const uint8_t* lab,
+                         uint64_t m, std::string& err);
+int engine_session_round(Engine* e, const std::vector<SessionSlice>& live, std::vector<uint64_t>& delta,
+                         std::string& err);
+int engine_session_results(Engine* e, const std::vector<SessionSlice>& done, std::vector<uint64_t>& counts,
+                           std::vector<uint64_t>& sums, std::string& err);
+void engine_session_end(Engine* e, bool trim);
+// Device bytes the engine holds (in use plus its block cache).
+uint64_t engine_device_bytes(Engine* e);
+
 // The closure of a graph whose edges are already in device memory, computed
 // with no device or pinned-host allocation: every buffer comes from a fixed
 // device region of noalloc_scratch_bytes() (the current device's own copy).

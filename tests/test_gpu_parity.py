@@ -426,6 +426,29 @@ def test_gpu_server_results_equal_alone(built):
         np.testing.assert_array_equal(per, ref.counts)
 
 
+def test_gpu_server_joins_running_fixpoint(built):
+    """Continuous batching: with one slot, requests submitted while a large
+    one is being closed join its running fixpoint at a round boundary and
+    finish before it; every result equals the oracle's."""
+    import time
+    say("server: small requests join a running fixpoint")
+    big = gen.alias(400_000, seed=24)
+    small = [gen.alias(1500 + 100 * i, seed=25 + i) for i in range(6)]
+    with gg.Server(slots=1, budget_bytes=64 << 30) as srv:
+        rb = srv.submit_spec(big)
+        time.sleep(0.02)
+        rs = [srv.submit_spec(g) for g in small]
+        res_small = [srv.wait(r) for r in rs]
+        res_big = srv.wait(rb)
+    b, _ = res_big
+    joined = [r for r, _ in res_small if r.start_ms > b.start_ms and r.done_ms < b.done_ms]
+    assert joined, "no small request joined the running fixpoint and finished first"
+    for g, (r, per) in zip(small + [big], res_small + [res_big]):
+        ref = oracle.closure_of(g, want_triples=False)
+        assert (r.total_edges, r.checksum) == (ref.total, ref.checksum), g.name
+        np.testing.assert_array_equal(per, ref.counts)
+
+
 @pytest.mark.parametrize("parts", [1, 4, 16])
 def test_gpu_out_of_core_matches_oracle(built, parts, monkeypatch):
     """Partitions spilled to host DRAM (at most two resident, through the
