@@ -44,6 +44,7 @@
 #include <string>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -2286,6 +2287,51 @@ struct RoundPlan {
 
 }  // namespace
 
+// A device region carved by best fit, with its free list on the host; one
+// region may serve several engines (the serving loop's slots), so its list is
+// guarded by a mutex. `owned`: the region was allocated for this object and is
+// freed with it (the allocation-free entry point's region is a static array).
+struct DeviceRegion {
+  char* base = nullptr;
+  size_t cap = 0;
+  bool owned = false;
+  std::mutex mu;
+  std::map<size_t, size_t> free_list;  // offset -> bytes, neighbours merged
+  ~DeviceRegion() {
+    if (owned && base) (void)hipFree(base);
+  }
+  void put(size_t off, size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = free_list.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != free_list.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_list.erase(next);
+    }
+    if (it != free_list.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_list.erase(it);
+      }
+    }
+  }
+  // best fit: the smallest free block that holds the request, so that the
+  // large blocks stay whole for the buffers that grow every round
+  bool take(size_t bytes, size_t& off) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto best = free_list.end();
+    for (auto it = free_list.begin(); it != free_list.end(); ++it)
+      if (it->second >= bytes && (best == free_list.end() || it->second < best->second)) best = it;
+    if (best == free_list.end()) return false;
+    off = best->first;
+    const size_t left = best->second - bytes;
+    free_list.erase(best);
+    if (left) free_list.emplace(off + bytes, left);
+    return true;
+  }
+};
+
 class Engine {
  public:
   int device = 0;
@@ -2351,9 +2397,8 @@ class Engine {
   // input edges and the host copies of the counters belong to the caller or
   // to this object. Nothing is allocated or freed through HIP in this mode,
   // and no events are created (the statistics' times stay 0).
-  char* arena = nullptr;
-  size_t arena_cap = 0;
-  std::map<size_t, size_t> arena_free;  // offset -> bytes, neighbours merged
+  char* arena = nullptr;                 // region->base when buffers come from a region
+  std::shared_ptr<DeviceRegion> region;
   bool owns_stream = true;
   Counters arena_h_ctr{};
   unsigned long long arena_h_u64 = 0;
@@ -2363,6 +2408,7 @@ class Engine {
   RoundPlan sess_rp;
   gg_stats sess_S{};
   uint64_t sess_inject = 0;  // raw edges waiting in raw_ptr/raw_col for the next round
+  bool sess_timing = false;  // time a session's kernels with events (statistics only)
 
   ~Engine() {
     DevBuf* all[] = {&in_src, &in_dst, &in_lab, &old_ptr, &old_col, &dl_ptr, &dl_col,  &nx_ptr, &nx_col,
@@ -2375,28 +2421,18 @@ class Engine {
       for (DevBuf* b : all)
         if (b->p) (void)hipFree(b->p);
       for (auto& kv : cache) (void)hipFree(kv.second);
-      if (h_ctr) (void)hipHostFree(h_ctr);
-      if (h_u64) (void)hipHostFree(h_u64);
+    } else {
+      // a shared region outlives this engine: its blocks go back to it
+      for (DevBuf* b : all)
+        if (b->p && static_cast<char*>(b->p) >= arena && static_cast<char*>(b->p) < arena + region->cap)
+          region->put(size_t(static_cast<char*>(b->p) - arena), b->bytes);
     }
+    if (h_ctr && h_ctr != &arena_h_ctr) (void)hipHostFree(h_ctr);
+    if (h_u64 && h_u64 != &arena_h_u64) (void)hipHostFree(h_u64);
     if (stream && owns_stream) (void)hipStreamDestroy(stream);
   }
 
-  // fixed-region mode: returns [off, off + bytes) to the free list
-  void arena_put(size_t off, size_t bytes) {
-    auto it = arena_free.emplace(off, bytes).first;
-    auto next = std::next(it);
-    if (next != arena_free.end() && it->first + it->second == next->first) {
-      it->second += next->second;
-      arena_free.erase(next);
-    }
-    if (it != arena_free.begin()) {
-      auto prev = std::prev(it);
-      if (prev->first + prev->second == it->first) {
-        prev->second += it->second;
-        arena_free.erase(it);
-      }
-    }
-  }
+
 
   // Device memory. Buffers that are released or outgrown go to a per-engine
   // block cache and are handed out again for requests up to half their size:
@@ -2420,7 +2456,7 @@ class Engine {
   void put_block(DevBuf& b) {
     if (!b.p) return;
     if (arena)
-      arena_put(size_t(static_cast<char*>(b.p) - arena), b.bytes);
+      region->put(size_t(static_cast<char*>(b.p) - arena), b.bytes);
     else
       cache.emplace(b.bytes, b.p);
     if (!arena) cached += b.bytes;
@@ -2432,20 +2468,21 @@ class Engine {
   int arena_alloc(DevBuf& b, size_t bytes, std::string& err) {
     put_block(b);
     bytes = (std::max<size_t>(bytes, 256) + 255) & ~size_t(255);
-    for (auto it = arena_free.begin(); it != arena_free.end(); ++it) {
-      if (it->second < bytes) continue;
-      const size_t off = it->first, left = it->second - bytes;
-      arena_free.erase(it);
-      if (left) arena_free.emplace(off + bytes, left);
+    size_t off = 0;
+    if (region->take(bytes, off)) {
       b.p = arena + off;
       b.bytes = bytes;
       used += bytes;
       peak = std::max(peak, used);
       return GG_OK;
     }
-    err = "the fixed scratch region (" + std::to_string(arena_cap) + " bytes) cannot hold a further " +
-          std::to_string(bytes) + " bytes (" + std::to_string(used) + " in use): the graph or its closure is too "
-          "large for the allocation-free entry point";
+    if (region->owned)
+      err = "the device region (" + std::to_string(region->cap) + " bytes) cannot hold a further " +
+            std::to_string(bytes) + " bytes (" + std::to_string(used) + " in use by this engine)";
+    else
+      err = "the fixed scratch region (" + std::to_string(region->cap) + " bytes) cannot hold a further " +
+            std::to_string(bytes) + " bytes (" + std::to_string(used) + " in use): the graph or its closure is "
+            "too large for the allocation-free entry point";
     return GG_E_NOMEM;
   }
 
@@ -4054,10 +4091,13 @@ int engine_checksum(Engine* e, uint64_t* sum, std::string& err) {
 namespace {
 
 // rows [rb, re) of one label of one graph: their first vertex u0, the
-// graph's vertex base v0 and the label in the graph's numbering
+// graph's vertex base v0, the label in the graph's numbering and the index
+// of the (graph, label) item the rows belong to (results are cut into row
+// chunks of at most kResultChunk rows, so that large graphs use every CU)
 struct SliceItem {
-  uint32_t rb, re, u0, v0, lab;
+  uint32_t rb, re, u0, v0, lab, item;
 };
+constexpr uint32_t kResultChunk = 2048;
 
 __global__ void k_remap_ptr(const unsigned long long* in, unsigned long long* out, uint32_t L, uint32_t n0,
                             uint32_t n1) {
@@ -4084,7 +4124,8 @@ __global__ void k_item_counts(const unsigned long long* ptr, const SliceItem* it
     out[i] = ptr[it[i].re] - ptr[it[i].rb];
 }
 
-// per item: edge count and checksum of its rows in c, added to cnt / sums
+// per chunk of rows: its edge count and checksum in c, added to its item's
+// cnt / sums
 __global__ __launch_bounds__(kBlock) void k_item_results(DCsr c, const SliceItem* it, uint32_t ni,
                                                          unsigned long long* cnt, unsigned long long* sums) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -4098,17 +4139,24 @@ __global__ __launch_bounds__(kBlock) void k_item_results(DCsr c, const SliceItem
       for (unsigned long long k = b + lane; k < e; k += kWave) acc += triple_hash(u, c.col[k] - x.v0, x.lab);
     }
     acc = wave_sum_u64(acc);
-    if (lane == 0 && acc) atomicAdd(&sums[i], acc);
-    if (threadIdx.x == 0) atomicAdd(&cnt[i], (unsigned long long)(c.ptr[x.re] - c.ptr[x.rb]));
+    if (lane == 0 && acc) atomicAdd(&sums[x.item], acc);
+    if (threadIdx.x == 0) atomicAdd(&cnt[x.item], (unsigned long long)(c.ptr[x.re] - c.ptr[x.rb]));
   }
 }
 
-std::vector<SliceItem> slice_items(const std::vector<SessionSlice>& sl, uint32_t n) {
+// one item per (graph, label); chunked: each cut into row chunks
+std::vector<SliceItem> slice_items(const std::vector<SessionSlice>& sl, uint32_t n, bool chunked = false) {
   std::vector<SliceItem> it;
+  uint32_t item = 0;
   for (const SessionSlice& q : sl)
-    for (uint32_t i = 0; i < q.lg; ++i) {
-      const uint32_t rb = (q.l0 + i) * n + q.v0;
-      it.push_back({rb, rb + q.n, q.v0, q.v0, i + 1});
+    for (uint32_t i = 0; i < q.lg; ++i, ++item) {
+      const uint32_t rb = (q.l0 + i) * n + q.v0, re = rb + q.n;
+      if (!chunked) {
+        it.push_back({rb, re, q.v0, q.v0, i + 1, item});
+        continue;
+      }
+      for (uint32_t a = rb; a < re; a += kResultChunk)
+        it.push_back({a, std::min(re, a + kResultChunk), q.v0 + (a - rb), q.v0, i + 1, item});
     }
   return it;
 }
@@ -4227,9 +4275,24 @@ int engine_session_round(Engine* e, const std::vector<SessionSlice>& live, std::
   if (e->n == 0 || (e->dl_nnz == 0 && e->sess_inject == 0)) return GG_OK;
   if (e->sess_rp.n != e->n) GG_TRY(e->plan_rounds(e->sess_rp, err));
   RunEvents ev;
-  ev.enabled = false;
+  ev.enabled = e->sess_timing;
   GG_TRY(e->round_step(e->sess_rp, e->sess_S, ev, e->sess_inject, err));
   e->sess_inject = 0;
+  if (e->sess_timing) {
+    GG_HIP(hipStreamSynchronize(e->stream));
+    auto sum = [](std::vector<EventPair>& v) {
+      double t = 0;
+      for (auto& x : v) {
+        float ms = 0;
+        if (x.a && x.b && hipEventElapsedTime(&ms, x.a, x.b) == hipSuccess) t += ms;
+      }
+      return t;
+    };
+    e->sess_S.expand_ms += sum(ev.expand);
+    e->sess_S.compact_ms += sum(ev.compact);
+    e->sess_S.merge_ms += sum(ev.merge);
+    e->sess_S.other_ms += sum(ev.other);
+  }
   if (live.empty()) return GG_OK;
   // per graph: its edges in the next delta
   const std::vector<SliceItem> it = slice_items(live, e->n);
@@ -4260,8 +4323,10 @@ int engine_session_round(Engine* e, const std::vector<SessionSlice>& live, std::
 int engine_session_results(Engine* e, const std::vector<SessionSlice>& done, std::vector<uint64_t>& counts,
                            std::vector<uint64_t>& sums, std::string& err) {
   GG_HIP(hipSetDevice(e->device));
-  const std::vector<SliceItem> it = slice_items(done, e->n);
-  counts.assign(it.size(), 0);
+  const std::vector<SliceItem> it = slice_items(done, e->n, true);
+  uint32_t nitems = 0;
+  for (const SessionSlice& q : done) nitems += q.lg;
+  counts.assign(nitems, 0);
   sums.assign(done.size(), 0);
   if (it.empty() || e->n == 0) return GG_OK;
   DevBuf d_it, d_cnt, d_sum;
@@ -4273,11 +4338,11 @@ int engine_session_results(Engine* e, const std::vector<SessionSlice>& done, std
     }
   } guard{e, {&d_it, &d_cnt, &d_sum}};
   GG_TRY(e->alloc(d_it, it.size() * sizeof(SliceItem), err));
-  GG_TRY(e->alloc(d_cnt, it.size() * 8, err));
-  GG_TRY(e->alloc(d_sum, it.size() * 8, err));
+  GG_TRY(e->alloc(d_cnt, uint64_t(nitems) * 8, err));
+  GG_TRY(e->alloc(d_sum, uint64_t(nitems) * 8, err));
   GG_HIP(hipMemcpyAsync(d_it.p, it.data(), it.size() * sizeof(SliceItem), hipMemcpyHostToDevice, e->stream));
-  e->fill_u64(d_cnt, 0, it.size());
-  e->fill_u64(d_sum, 0, it.size());
+  e->fill_u64(d_cnt, 0, nitems);
+  e->fill_u64(d_sum, 0, nitems);
   const uint32_t grid = uint32_t(std::min<uint64_t>(it.size(), uint64_t(e->cus) * 8));
   k_item_results<<<grid, kBlock, 0, e->stream>>>({static_cast<unsigned long long*>(e->old_ptr.p),
                                                   static_cast<uint32_t*>(e->old_col.p)},
@@ -4291,9 +4356,9 @@ int engine_session_results(Engine* e, const std::vector<SessionSlice>& done, std
                                                    static_cast<unsigned long long*>(d_cnt.p),
                                                    static_cast<unsigned long long*>(d_sum.p));
   GG_HIP(hipGetLastError());
-  std::vector<uint64_t> s(it.size());
-  GG_HIP(hipMemcpyAsync(counts.data(), d_cnt.p, it.size() * 8, hipMemcpyDeviceToHost, e->stream));
-  GG_HIP(hipMemcpyAsync(s.data(), d_sum.p, it.size() * 8, hipMemcpyDeviceToHost, e->stream));
+  std::vector<uint64_t> s(nitems);
+  GG_HIP(hipMemcpyAsync(counts.data(), d_cnt.p, uint64_t(nitems) * 8, hipMemcpyDeviceToHost, e->stream));
+  GG_HIP(hipMemcpyAsync(s.data(), d_sum.p, uint64_t(nitems) * 8, hipMemcpyDeviceToHost, e->stream));
   GG_HIP(hipStreamSynchronize(e->stream));
   size_t k = 0;
   for (size_t q = 0; q < done.size(); ++q)
@@ -4324,6 +4389,49 @@ void engine_session_end(Engine* e, bool trim) {
 }
 
 uint64_t engine_device_bytes(Engine* e) { return e->used + e->cached; }
+
+void engine_session_timing(Engine* e, bool on) { e->sess_timing = on; }
+
+std::shared_ptr<DeviceRegion> device_region_create(int device, size_t bytes, std::string& err) {
+  if (hipSetDevice(device) != hipSuccess) {
+    (void)hipGetLastError();
+    err = "cannot use HIP device " + std::to_string(device);
+    return nullptr;
+  }
+  void* p = nullptr;
+  const hipError_t he = hipMalloc(&p, bytes);
+  if (he != hipSuccess) {
+    (void)hipGetLastError();
+    err = "hipMalloc(" + std::to_string(bytes) + ") for a device region: " + hipGetErrorString(he);
+    return nullptr;
+  }
+  auto r = std::make_shared<DeviceRegion>();
+  r->base = static_cast<char*>(p);
+  r->cap = bytes;
+  r->owned = true;
+  r->free_list.emplace(0, bytes);
+  return r;
+}
+
+int engine_device(Engine* e) { return e->device; }
+
+int engine_use_region(Engine* e, std::shared_ptr<DeviceRegion> r, std::string& err) {
+  if (e->arena) return GG_OK;
+  if (e->used || !e->cache.empty()) {
+    err = "the engine already holds device memory";
+    return GG_E_STATE;
+  }
+  e->region = std::move(r);
+  e->arena = e->region->base;
+  return GG_OK;
+}
+
+gg_stats engine_session_stats(Engine* e, double* alloc_ms) {
+  if (alloc_ms) *alloc_ms = e->alloc_ms;
+  gg_stats st = e->sess_S;
+  st.device_bytes_peak = e->peak;
+  return st;
+}
 
 // ---------------------------------------------------------------------------
 // The allocation-free entry point (gg_closure_run_noalloc). Its buffers come
@@ -4385,9 +4493,12 @@ int engine_run_noalloc(const Grammar& gr, uint32_t n, uint64_t m, const uint32_t
   e.cus = cus;
   e.stream = static_cast<hipStream_t>(stream);
   e.owns_stream = false;
+  auto reg = std::make_shared<DeviceRegion>();
+  reg->base = base;
+  reg->cap = kScratchBytes;
+  reg->free_list.emplace(0, kScratchBytes);
+  e.region = reg;
   e.arena = base;
-  e.arena_cap = kScratchBytes;
-  e.arena_free.emplace(0, kScratchBytes);
   e.h_ctr = &e.arena_h_ctr;
   e.h_u64 = &e.arena_h_u64;
   // a small join grid: every wave of it reserves staging space up front

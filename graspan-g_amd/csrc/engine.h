@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -86,6 +87,21 @@ int engine_session_results(Engine* e, const std::vector<SessionSlice>& done, std
 void engine_session_end(Engine* e, bool trim);
 // Device bytes the engine holds (in use plus its block cache).
 uint64_t engine_device_bytes(Engine* e);
+// Session statistics: rounds, P/K/O/N, kernel times by phase when timing is
+// on (events around each phase of every round), device peak; alloc_ms gets
+// the engine's host time spent allocating device memory since it was made.
+void engine_session_timing(Engine* e, bool on);
+// A device region allocated once and carved by best fit (freed when the last
+// engine using it is gone). Engines given one carve every buffer from it
+// instead of allocating per buffer, so long-running callers (the serving
+// loop) avoid the per-round hipMalloc/hipFree of growing buffers; several
+// engines may share one region. engine_use_region must come before the
+// engine allocates anything.
+struct DeviceRegion;
+std::shared_ptr<DeviceRegion> device_region_create(int device, size_t bytes, std::string& err);
+int engine_use_region(Engine* e, std::shared_ptr<DeviceRegion> r, std::string& err);
+int engine_device(Engine* e);
+gg_stats engine_session_stats(Engine* e, double* alloc_ms);
 
 // The closure of a graph whose edges are already in device memory, computed
 // with no device or pinned-host allocation: every buffer comes from a fixed

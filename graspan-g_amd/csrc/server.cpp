@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -71,6 +72,7 @@ struct Session {
   std::vector<Live> live;
   uint32_t rounds = 0, admitted = 0;
   double t0 = 0;
+  double t_admit = 0, t_round = 0, t_results = 0;  // host time in each engine call
 };
 
 }  // namespace
@@ -86,6 +88,7 @@ struct gg_server {
   int64_t next_id = 1;
   uint64_t in_flight = 0;  // footprint estimates of the requests in running sessions
   uint64_t epoch = 0;      // bumped by every submit and every release of in-flight memory
+  std::shared_ptr<gg::DeviceRegion> region;  // device memory of every slot's engine
   bool stopping = false;
   std::vector<std::thread> workers;
 
@@ -170,16 +173,37 @@ struct gg_server {
     std::string* cerr = nullptr;
     std::string err;
     gg::Engine* e = cs == GG_OK ? gg::ctx_engine(ctx, cs, cerr) : nullptr;
+    if (e && trace) gg::engine_session_timing(e, true);
     Session s;
     std::unordered_map<Request*, std::pair<double, uint32_t>> start;  // admission time, cohort size
     const uint64_t share = opt.budget_bytes / uint64_t(std::max(1, opt.slots));
+    // The whole budget is one device region, allocated once and shared by
+    // the slots: sessions grow their buffers every round, and per-buffer
+    // hipMalloc and hipFree of tens of GB cost seconds per session.
+    if (e) {
+      std::shared_ptr<gg::DeviceRegion> r;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!region) region = gg::device_region_create(gg::engine_device(e), opt.budget_bytes, err);
+        r = region;
+      }
+      cs = r ? gg::engine_use_region(e, r, err) : GG_E_NOMEM;
+      if (cs != GG_OK) e = nullptr;
+    }
     uint64_t seen = ~0ull;  // epoch at which this slot last found nothing to admit
     auto end_session = [&](bool trim) {
       if (s.active) {
-        if (trace)
+        if (trace) {
+          double alloc_ms = 0;
+          const gg_stats st = gg::engine_session_stats(e, &alloc_ms);
           std::fprintf(stderr,
-                       "[gg-server] slot %d: session of %u requests, %u vertices, %u labels, %u rounds, %.0f ms\n",
-                       slot, s.admitted, s.n, s.labels, s.rounds, now() - s.t0);
+                       "[gg-server] slot %d: session of %u requests, %u vertices, %u labels, %u rounds, %.0f ms "
+                       "(admit %.0f, rounds %.0f [expand %.0f merge %.0f compact %.0f other %.0f], results %.0f); "
+                       "%.1fe9 new edges, device peak %.1f GB, allocation host time so far %.0f ms\n",
+                       slot, s.admitted, s.n, s.labels, s.rounds, now() - s.t0, s.t_admit, s.t_round, st.expand_ms,
+                       st.merge_ms, st.compact_ms, st.other_ms, s.t_results, double(st.new_edges) / 1e9,
+                       double(st.device_bytes_peak) / 1e9, alloc_ms);
+        }
         gg::engine_session_end(e, trim || gg::engine_device_bytes(e) > share);
       }
       std::lock_guard<std::mutex> lk(mu);
@@ -241,12 +265,15 @@ struct gg_server {
             continue;
           }
           if (opt.max_batch_requests && take.size() >= opt.max_batch_requests) break;
-          // alone: nothing else is on the device; then even a request larger
-          // than the budget is admitted
-          const bool alone = s.live.empty() && take.empty() && in_flight == 0;
+          // A session holds at most its slot's share of the budget, so that
+          // one slot admits while the other drains; a request larger than a
+          // share runs alone in its session, and one larger than the budget
+          // only when nothing else is on the device.
+          const bool alone = s.live.empty() && take.empty() && s.est == 0;
           const bool rows_ok = uint64_t(s.labels) * (nv + r->n) < 0xFFFFFFFFull &&
                                uint64_t(s.slots) * (nv + r->n) < 0xFFFFFFFFull;
-          const bool mem_ok = in_flight + r->est <= opt.budget_bytes;
+          const bool mem_ok = (s.est + r->est <= share || alone) &&
+                              (in_flight + r->est <= opt.budget_bytes || in_flight == 0);
           if (r->retried && !alone) break;  // a request that ran out of memory once runs alone
           if (!rows_ok && s.live.empty() && take.empty()) {
             // too many rows for the engine even alone in a session
@@ -255,7 +282,7 @@ struct gg_server {
             it = queue.erase(it);
             continue;
           }
-          if (!rows_ok || (!mem_ok && !alone)) {
+          if (!rows_ok || !mem_ok) {
             if (!s.live.empty() || !take.empty()) s.draining = true;
             break;
           }
@@ -305,7 +332,9 @@ struct gg_server {
           start[r] = {t_adm, uint32_t(take.size())};
           v0 += r->n;
         }
+        const double a0 = now();
         st = gg::engine_session_admit(e, v0, src.data(), dst.data(), lab.data(), m, err);
+        s.t_admit += now() - a0;
         s.n = v0;
         s.admitted += uint32_t(take.size());
       }
@@ -313,7 +342,9 @@ struct gg_server {
       if (st == GG_OK) {
         std::vector<gg::SessionSlice> sl;
         for (auto& l : s.live) sl.push_back(l.slice);
+        const double a0 = now();
         st = gg::engine_session_round(e, sl, delta, err);
+        s.t_round += now() - a0;
         ++s.rounds;
       }
       if (st != GG_OK) {
@@ -348,7 +379,9 @@ struct gg_server {
         std::vector<gg::SessionSlice> sl;
         for (auto& l : done) sl.push_back(l.slice);
         std::vector<uint64_t> counts, sums;
+        const double a0 = now();
         st = gg::engine_session_results(e, sl, counts, sums, err);
+        s.t_results += now() - a0;
         {
           std::lock_guard<std::mutex> lk(mu);
           size_t c = 0;
@@ -368,6 +401,10 @@ struct gg_server {
         cv_done.notify_all();
       }
       s.live.swap(rest);
+      // a session that holds more than half its region takes no one else:
+      // its live graphs still grow, and finished graphs keep their memory
+      // until the session ends
+      if (!s.draining && gg::engine_device_bytes(e) > share / 2) s.draining = true;
       if (s.live.empty()) {
         // nothing left in the session: it ends unless queued requests can
         // still join it (its finished graphs' memory stays until it does)
@@ -376,7 +413,8 @@ struct gg_server {
           std::lock_guard<std::mutex> lk(mu);
           if (!s.draining)
             for (Request* r : queue)
-              if (s.base.count(r->key) && !r->retried && in_flight + r->est <= opt.budget_bytes) {
+              if (s.base.count(r->key) && !r->retried && s.est + r->est <= share &&
+                  in_flight + r->est <= opt.budget_bytes) {
                 more = true;
                 break;
               }

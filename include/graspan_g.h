@@ -267,36 +267,43 @@ int gg_batch_export(const gg_batch *batch, int graph_id, uint32_t *buf, uint64_t
 
 /* ---- serving loop ---------------------------------------------------------
  * No counterpart in the reference. A request is one graph with its grammar.
- * Worker slots (one context and HIP stream each) take requests from a FIFO
- * queue: whenever a slot is idle it admits queued requests in arrival order
- * while their estimated device footprint fits the budget left by the batches
- * other slots are running, closes them as one batch (gg_batch_*), publishes
- * each result, frees its device memory and looks at the queue again. Requests
- * that arrive while batches run are therefore admitted as slots and memory
- * free up, and the batches of different slots overlap on the device. */
+ * Continuous batching at the granularity of fixpoint rounds: each worker slot
+ * (one context and HIP stream) runs a session, a semi-naive fixpoint that
+ * requests join at round boundaries. Before every round the slot admits
+ * queued requests in arrival order while their estimated footprint fits (the
+ * slot's share, budget / slots, for its session, and the budget for
+ * everything in flight); an admitted request's edges enter the running
+ * fixpoint in that round. After every round, each request whose graph got no
+ * new edges has reached its fixpoint and its result is published at once,
+ * while the others go on. A session whose next request does not fit (or that
+ * holds more than half its share) admits no more and ends with its last
+ * request; the slot then starts another. The slots' sessions overlap on the
+ * device. All slots carve their device memory from one region of
+ * budget_bytes, allocated when the server starts. A session that still runs
+ * out of memory puts its unfinished requests back at the front of the queue
+ * once, to run alone; a second failure is reported in the result. */
 
 typedef struct gg_server gg_server;
 
 typedef struct gg_server_options {
   int device;                  /* as gg_options.device */
-  int slots;                   /* concurrent batches (>= 1) */
-  uint64_t budget_bytes;       /* device memory for all in-flight requests */
+  int slots;                   /* concurrent sessions (>= 1) */
+  uint64_t budget_bytes;       /* device memory for all in-flight requests (allocated once) */
   uint64_t bytes_per_vertex;   /* footprint estimate of a request: this times
-                                  its vertex count plus 64 bytes per input edge
-                                  (the engine needs ~24 bytes per closure edge);
-                                  a batch that still runs out of device memory
-                                  is split in halves and retried */
-  uint32_t max_batch_requests; /* cap on requests per batch (0 = no cap) */
+                                  its vertex count plus 64 bytes per input edge;
+                                  a request's estimate stays charged until its
+                                  session ends */
+  uint32_t max_batch_requests; /* cap on requests admitted at one round boundary (0 = no cap) */
 } gg_server_options;
 
 typedef struct gg_server_result {
-  int status;                  /* GG_OK or the error of the request's batch */
+  int status;                  /* GG_OK or the error of the request's session */
   uint64_t total_edges;
   uint64_t checksum;           /* as gg_closure_checksum, in the request's numbering */
   double submit_ms;            /* times on the server's clock (ms since creation) */
-  double start_ms;             /* its batch started */
+  double start_ms;             /* it joined a session */
   double done_ms;              /* its result was published */
-  uint32_t batch_size;         /* requests in its batch */
+  uint32_t batch_size;         /* requests admitted at the same round boundary */
   int slot;
 } gg_server_result;
 
