@@ -95,11 +95,11 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
   // smaller graph count.
   size_t cap = G;
   // With max_vertices == 0 a group's vertex count is planned from device
-  // memory: a group may use kGroupMemShare of it, at the peak device bytes
+  // memory: a group may use a tuned share of it (0.4 by default), at the peak device bytes
   // per vertex measured on the last group of the same grammar (a grammar not
   // yet seen starts with kFirstGroupVertices). Groups that run close to the
   // device's limit spend their time re-allocating instead of computing.
-  constexpr double kGroupMemShare = 0.4;
+  const double kGroupMemShare = gg::tuned_group_mem_share();
   constexpr uint64_t kFirstGroupVertices = 2u << 20;
   const uint64_t mem = max_vertices ? 0 : gg::ctx_memory(b->ctx);
   std::map<std::pair<int, std::vector<int32_t>>, double> bytes_per_vertex;

@@ -33,9 +33,11 @@
 // is found by an mbcnt rank over a per-tile bitmap of list starts
 // (stream_tile). Windows wider than kChunkBits are processed in chunks
 // (count pass, then write pass).
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -61,7 +63,6 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kChunkBits = 16384;  // LDS window per wave: 2 KiB seen + 2 KiB new
 constexpr uint32_t kChunkWords = kChunkBits / 32;
 constexpr int kUnroll = 8;              // loads in flight per lane in the candidate stream
-constexpr uint32_t kGrab = 4;           // active rows taken per work-counter atomic
 constexpr uint32_t kXcds = 8;           // MI355X: 8 XCDs, each with its own L2
 constexpr int kExpandWaves = 7;         // k_expand occupancy target (waves per SIMD)
 constexpr uint64_t kDeferRatio = 128;   // pending edges kept apart while under 1/128 of old
@@ -919,6 +920,7 @@ struct ExpandArgs {
   unsigned long long* seg_off;
   uint32_t* seg_cnt;
   uint32_t guess;  // ids of a window centred on the row's vertex tried first (0: none)
+  uint32_t grab;   // active rows a wave takes per work-counter atomic
 };
 
 struct WaveLds {
@@ -1363,6 +1365,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
   // rows are taken kGrab at a time and staging space kStageBlock words at a
   // time, so the two global counters see one atomic per many rows
   uint32_t next = 0, end = 0;
+  const uint32_t grab = max(a.grab, 1u);
   unsigned long long blk_pos = 0, blk_end = 0;
   auto reserve = [&](uint32_t cnt) -> unsigned long long {
     if (blk_pos + cnt > blk_end) {
@@ -1388,11 +1391,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
         const uint32_t sb = uint32_t((unsigned long long)nactive * seg / kXcds);
         const uint32_t se = uint32_t((unsigned long long)nactive * (seg + 1) / kXcds);
         uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(&a.ctr->work_xcd[seg].v, kGrab);
+        if (lane == 0) b = atomicAdd(&a.ctr->work_xcd[seg].v, grab);
         b = sb + uni_u32(b);
         if (b < se) {
           next = b;
-          end = min(b + kGrab, se);
+          end = min(b + grab, se);
           got = true;
           break;
         }
@@ -2249,6 +2252,77 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// Launch and schedule parameters that the autotuner (graspan_g.tune) picks
+// on the device and writes to tuned_gfx950.cfg beside the library ("name
+// value" lines, '#' comments). GG_TUNE_<NAME> environment variables override
+// the file, read at the start of every run, so a tuner can sweep in one
+// process. 0 for a *_bpc value keeps the occupancy the kernel allows.
+struct Tuning {
+  // k_expand takes active rows from the work counters grab_light at a time,
+  // or grab_heavy at a time in a round whose previous round joined more than
+  // heavy_cand candidates per active row (a few heavy rows per grab balance
+  // the waves better; many light rows per grab save counter atomics)
+  uint32_t grab_light = 4;
+  uint32_t grab_heavy = 4;
+  uint64_t heavy_cand = 4096;
+  uint32_t guess_min = 4096;   // k_expand: first guessed window (ids)
+  uint64_t defer_ratio = 128;  // deltas go to the pending CSR while (pending + delta) * ratio <= old
+  int expand_bpc = 0;          // blocks per CU of k_expand
+  int merge_bpc = 0;           // of k_merge
+  int merge_rows_bpc = 0;      // of k_merge_rows
+  int gather_bpc = 0;          // of k_gather
+  double group_mem_share = 0.4;  // batch: share of device memory a launch group plans for
+};
+
+// Sets one parameter from its name; false for an unknown name.
+bool set_tuning(Tuning& t, const std::string& name, const char* v) {
+  if (name == "grab_light") t.grab_light = uint32_t(std::strtoul(v, nullptr, 10));
+  else if (name == "grab_heavy") t.grab_heavy = uint32_t(std::strtoul(v, nullptr, 10));
+  else if (name == "heavy_cand") t.heavy_cand = std::strtoull(v, nullptr, 10);
+  else if (name == "guess_min") t.guess_min = uint32_t(std::strtoul(v, nullptr, 10));
+  else if (name == "defer_ratio") t.defer_ratio = std::strtoull(v, nullptr, 10);
+  else if (name == "expand_bpc") t.expand_bpc = std::atoi(v);
+  else if (name == "merge_bpc") t.merge_bpc = std::atoi(v);
+  else if (name == "merge_rows_bpc") t.merge_rows_bpc = std::atoi(v);
+  else if (name == "gather_bpc") t.gather_bpc = std::atoi(v);
+  else if (name == "group_mem_share") t.group_mem_share = std::atof(v);
+  else return false;
+  return true;
+}
+
+const char* const kTuningNames[] = {"grab_light", "grab_heavy",     "heavy_cand", "guess_min",      "defer_ratio",
+                                    "expand_bpc", "merge_bpc",      "merge_rows_bpc", "gather_bpc", "group_mem_share"};
+
+// The tuning file beside this library (read once), then GG_TUNE_* variables.
+Tuning load_tuning() {
+  static Tuning from_file;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    Dl_info info{};
+    if (!dladdr(reinterpret_cast<void*>(&load_tuning), &info) || !info.dli_fname) return;
+    std::string path(info.dli_fname);
+    const size_t slash = path.rfind('/');
+    path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuned_gfx950.cfg";
+    if (const char* f = std::getenv("GG_TUNE_FILE")) path = f;
+    FILE* fp = std::fopen(path.c_str(), "r");
+    if (!fp) return;
+    char line[256];
+    while (std::fgets(line, sizeof line, fp)) {
+      char name[64], value[64];
+      if (line[0] == '#' || std::sscanf(line, "%63s %63s", name, value) != 2) continue;
+      set_tuning(from_file, name, value);
+    }
+    std::fclose(fp);
+  });
+  Tuning t = from_file;
+  for (const char* n : kTuningNames) {
+    std::string env = "GG_TUNE_";
+    for (const char* c = n; *c; ++c) env += char(std::toupper(static_cast<unsigned char>(*c)));
+    if (const char* v = std::getenv(env.c_str())) set_tuning(t, n, v);
+  }
+  return t;
+}
+
 struct EventPair {
   hipEvent_t a = nullptr, b = nullptr;
 };
@@ -2403,6 +2477,8 @@ class Engine {
   Counters arena_h_ctr{};
   unsigned long long arena_h_u64 = 0;
   uint32_t expand_grid_cap = 0;  // 0: as many blocks as fit the device
+  Tuning tune;                   // the tuned parameters of the current run
+  int occ_expand = 4, occ_merge = 8, occ_merge_rows = 8, occ_gather = 7;  // blocks per CU the kernels allow
 
   // session state (a running fixpoint that graphs join at round boundaries)
   RoundPlan sess_rp;
@@ -2708,11 +2784,30 @@ class Engine {
     a.seg_off = P<unsigned long long>(seg_off);
     a.seg_cnt = P<uint32_t>(seg_cnt);
     a.guess = guess_bits;
+    a.grab = grab;
     return a;
   }
 
   int expand_blocks_per_cu = 4;
   int merge_blocks_per_cu = 8, merge_rows_blocks_per_cu = 8, gather_blocks_per_cu = 7;
+  // the tuned parameters for this run: file beside the library, then GG_TUNE_*
+  void apply_tuning() {
+    tune = load_tuning();
+    expand_blocks_per_cu = tune.expand_bpc > 0 ? std::min(tune.expand_bpc, occ_expand) : occ_expand;
+    merge_blocks_per_cu = tune.merge_bpc > 0 ? std::min(tune.merge_bpc, occ_merge) : occ_merge;
+    merge_rows_blocks_per_cu = tune.merge_rows_bpc > 0 ? std::min(tune.merge_rows_bpc, occ_merge_rows) : occ_merge_rows;
+    gather_blocks_per_cu = tune.gather_bpc > 0 ? std::min(tune.gather_bpc, occ_gather) : occ_gather;
+    tune.grab_light = std::max(1u, tune.grab_light);
+    tune.grab_heavy = std::max(1u, tune.grab_heavy);
+    grab = tune.grab_light;
+  }
+  // rows per work-counter grab in this round (see Tuning) and the join's
+  // candidates and active rows of the last round, which choose it
+  uint32_t grab = 4;
+  uint64_t last_cand = 0, last_active = 0;
+  void choose_grab() {
+    grab = (last_active && last_cand / last_active > tune.heavy_cand) ? tune.grab_heavy : tune.grab_light;
+  }
   uint32_t expand_grid() const {
     const uint32_t g = uint32_t(cus) * uint32_t(expand_blocks_per_cu);
     return expand_grid_cap ? std::min(g, expand_grid_cap) : g;
@@ -2744,7 +2839,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     return GG_E_ARG;
   }
   pend_nnz = 0;
-  defer_ratio = kDeferRatio;
+  apply_tuning();
+  defer_ratio = tune.defer_ratio;
   if (const char* dr = std::getenv("GG_DEFER_RATIO")) defer_ratio = std::strtoull(dr, nullptr, 10);
   keep_ids = false;
   if (const char* ki = std::getenv("GG_KEEP_PACKED_IDS")) keep_ids = std::atoi(ki) != 0;
@@ -2846,7 +2942,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   rp.pair_arena_forced = pair_arena_forced;
   pr_arena = 0;
   pk_ready = false;
-  guess_bits = kGuessMin;
+  guess_bits = tune.guess_min;
+  last_cand = last_active = 0;
   GG_TRY(plan_rounds(rp, err));
   if (pack_any_label()) {
     GG_TRY(alloc(pk_len, uint64_t(R) * 4, err));
@@ -3001,6 +3098,7 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
     GG_HIP(rec(o.b));
     pairs_ready = true;
   }
+  choose_grab();
   // staging: room for every wave's reservation block plus a guess of the
   // output; an overflow grows it and re-runs the round
   GG_TRY(grow(staging, (uint64_t(expand_grid()) * kWavesPerBlock * kStageBlock * 2 + 4 * dl_nnz + inject) * 4, err));
@@ -3019,7 +3117,7 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
     // some (kGrab rows per grab, 4 waves per block), not a full grid of
     // waves that would only contend on the work counters
     GG_TRY(read_ctr(err));
-    const uint32_t want = uint32_t((uint64_t(h_ctr->active) + kGrab * kWavesPerBlock - 1) / (kGrab * kWavesPerBlock));
+    const uint32_t want = uint32_t((uint64_t(h_ctr->active) + grab * kWavesPerBlock - 1) / (grab * kWavesPerBlock));
     const uint32_t grid = std::max(1u, std::min(expand_grid(), want));
     GG_HIP(rec(o.b));
     EventPair& x = ev.mk(ev.expand);
@@ -3046,6 +3144,8 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
     GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
   }
   N = h_ctr->newe;
+  last_cand = h_ctr->cand;
+  last_active = h_ctr->active;
   const uint32_t used_guess = guess_bits;
   if (guess_bits == 0)
     guess_bits = kChunkBits;
@@ -3150,6 +3250,7 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     err = "num_labels * num_vertices must be < 2^32 (row ids are 32-bit)";
     return GG_E_UNSUPPORTED;
   }
+  apply_tuning();
   if (parts.empty()) {
     err = "the partitioned schedule needs at least one vertex interval";
     return GG_E_ARG;
@@ -3416,6 +3517,7 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     ExpandArgs a{};
     a.n = n;
     a.g = g;
+    a.grab = grab;
     a.active = P<uint32_t>(active);
     a.ctr = C();
     a.seg_off = P<unsigned long long>(seg_off);
@@ -3747,24 +3849,24 @@ Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_expand<0>), kBlock, 0) ==
           hipSuccess &&
       bpc > 0)
-    en->expand_blocks_per_cu = bpc;
+    en->expand_blocks_per_cu = en->occ_expand = bpc;
   // the merge and gather kernels hand each wave a fixed share of the rows, so their
   // grids are what fits on the device at once (no second, late wave of blocks)
   bpc = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_merge), kBlock, 0) ==
           hipSuccess &&
       bpc > 0)
-    en->merge_blocks_per_cu = bpc;
+    en->merge_blocks_per_cu = en->occ_merge = bpc;
   bpc = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_merge_rows), kBlock, 0) ==
           hipSuccess &&
       bpc > 0)
-    en->merge_rows_blocks_per_cu = bpc;
+    en->merge_rows_blocks_per_cu = en->occ_merge_rows = bpc;
   bpc = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void*>(k_gather), kBlock, 0) ==
           hipSuccess &&
       bpc > 0)
-    en->gather_blocks_per_cu = bpc;
+    en->gather_blocks_per_cu = en->occ_gather = bpc;
   (void)hipGetLastError();
   status = GG_OK;
   return en;
@@ -4175,9 +4277,11 @@ int engine_session_begin(Engine* e, const Grammar& gr, std::string& err) {
   e->old_nnz = e->dl_nnz = e->pend_nnz = 0;
   e->pk_ready = e->dpk_ready = e->pairs_ready = false;
   e->keep_ids = true;
-  e->defer_ratio = kDeferRatio;
+  e->apply_tuning();
+  e->defer_ratio = e->tune.defer_ratio;
   if (const char* dr = std::getenv("GG_DEFER_RATIO")) e->defer_ratio = std::strtoull(dr, nullptr, 10);
-  e->guess_bits = kGuessMin;
+  e->guess_bits = e->tune.guess_min;
+  e->last_cand = e->last_active = 0;
   e->pr_arena = 0;
   e->sess_rp = RoundPlan{};
   e->sess_S = gg_stats{};
@@ -4414,6 +4518,8 @@ std::shared_ptr<DeviceRegion> device_region_create(int device, size_t bytes, std
 }
 
 int engine_device(Engine* e) { return e->device; }
+
+double tuned_group_mem_share() { return load_tuning().group_mem_share; }
 
 int engine_use_region(Engine* e, std::shared_ptr<DeviceRegion> r, std::string& err) {
   if (e->arena) return GG_OK;

@@ -101,6 +101,8 @@ struct DeviceRegion;
 std::shared_ptr<DeviceRegion> device_region_create(int device, size_t bytes, std::string& err);
 int engine_use_region(Engine* e, std::shared_ptr<DeviceRegion> r, std::string& err);
 int engine_device(Engine* e);
+// The share of device memory a batch launch group plans for (tuned).
+double tuned_group_mem_share();
 gg_stats engine_session_stats(Engine* e, double* alloc_ms);
 
 // The closure of a graph whose edges are already in device memory, computed
