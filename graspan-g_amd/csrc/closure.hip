@@ -126,6 +126,11 @@ struct Counters {
   unsigned int overflow;       // staging too small
   unsigned int misses;
   unsigned int label_flags[8]; // bit l: delta of label l is not empty
+  // k_region_span: per destination region d of a partition operator, the
+  // local vertices with a non-empty row span [np - span_lo_c[d], span_hi[d])
+  // (both 0: no row is non-empty)
+  unsigned int span_lo_c[2];
+  unsigned int span_hi[2];
   unsigned long long xs[4];
   WorkIdx work_xcd[8];         // per-XCD work indices into the active list
   CtrSlot slot[kSpread];
@@ -377,11 +382,6 @@ __global__ void k_mark_active_base(Counters* ctr) {
     ctr->work = 0;
   }
   if (t < int(kXcds)) ctr->work_xcd[t].v = 0;
-}
-
-__global__ void k_all_active(uint32_t rows, uint32_t* active, Counters* ctr) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += gridDim.x * blockDim.x) active[i] = i;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->active = rows;
 }
 
 // Active list of the initial dedup: rows with raw entries.
@@ -923,6 +923,145 @@ struct ExpandArgs {
   uint32_t grab;   // active rows a wave takes per work-counter atomic
 };
 
+// The span of destination region d of a partition operator: global vertex
+// ids [lo, hi) outside which its rows are all empty (lo >= hi: all empty).
+// Keys outside the span have no candidates.
+__device__ inline void dst_span(const ExpandArgs& a, int d, uint32_t& lo, uint32_t& hi) {
+  lo = a.ro.dfirst[d] + (a.ro.dn[d] - a.ctr->span_lo_c[d]);
+  hi = a.ro.dfirst[d] + a.ctr->span_hi[d];
+}
+
+// Span of the local vertices of a region (rows label * np + local vertex)
+// with a non-empty row of any label, into ctr->span_lo_c[d] / span_hi[d],
+// which start at zero. Block y covers label y; each thread reduces its own
+// rows, then the block, so a launch makes two counter updates per block.
+__global__ __launch_bounds__(kBlock) void k_region_span(DCsr reg, uint32_t np, int d, Counters* ctr) {
+  __shared__ uint32_t s_mn[kWavesPerBlock], s_mx[kWavesPerBlock];
+  const unsigned long long* ptr = reg.ptr + (unsigned long long)blockIdx.y * np;
+  uint32_t mn = 0xFFFFFFFFu, mx = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < np; v += gridDim.x * blockDim.x)
+    if (ptr[v + 1] > ptr[v]) {
+      mn = min(mn, v);
+      mx = v + 1;  // v grows along the loop
+    }
+  mn = wave_min_u32(mn);
+  mx = wave_max_u32(mx);
+  const int wv = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_mn[wv] = mn;
+    s_mx[wv] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kWavesPerBlock; ++i) {
+      mn = min(mn, s_mn[i]);
+      mx = max(mx, s_mx[i]);
+    }
+    if (mx) {
+      atomicMax(&ctr->span_lo_c[d], np - mn);
+      atomicMax(&ctr->span_hi[d], mx);
+    }
+  }
+}
+
+// The same search by a whole wave (b, e and x uniform; uniform result): each
+// step samples 64 evenly spaced ids and keeps the gap where they cross x, so
+// a row of length len takes about log64(len) dependent loads.
+__device__ inline unsigned long long wave_lower_bound(const uint32_t* col, unsigned long long b, unsigned long long e,
+                                                      uint32_t x, int lane) {
+  while (e - b > kWave) {
+    const unsigned long long step = (e - b + kWave - 1) / kWave;
+    const unsigned long long k = b + uint32_t(lane) * step;
+    // the lanes whose sample is below x are a prefix of the wave
+    const uint32_t c = uint32_t(__popcll(__ballot(k < e && col[k] < x)));
+    if (c == 0) return b;
+    // col[b + (c-1)*step] < x, and col[b + c*step] >= x or b + c*step >= e
+    const unsigned long long nb = b + (c - 1) * step + 1;
+    e = min(e, b + c * step);
+    b = nb;
+  }
+  const unsigned long long k = b + uint32_t(lane);
+  return b + uint32_t(__popcll(__ballot(k < e && col[k] < x)));
+}
+
+// First index k in [b, e) with col[k] >= x (e if none), col ascending.
+__device__ inline unsigned long long lower_bound_u32(const uint32_t* col, unsigned long long b, unsigned long long e,
+                                                     uint32_t x) {
+  while (b < e) {
+    const unsigned long long m = b + (e - b) / 2;
+    if (col[m] < x)
+      b = m + 1;
+    else
+      e = m;
+  }
+  return b;
+}
+
+// The active list of one partition operator: the rows of the partition that
+// can receive candidates (see RegionOp). kind 3 and kind 1 rows need a
+// non-empty key row, kind 2 rows an eps rule, kind 0 rows a key id inside a
+// destination's span (ids in key rows are ascending: the first and last id
+// decide most rows, a binary search the rest). Rows gather per wave in LDS
+// and take their place in the list with one counter update per kBuf rows.
+__global__ __launch_bounds__(kBlock) void k_region_active(ExpandArgs a, uint32_t rows) {
+  constexpr uint32_t kBuf = 512;
+  __shared__ uint32_t s_buf[kWavesPerBlock][kBuf];
+  const RegionOp& ro = a.ro;
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t* buf = s_buf[threadIdx.x / kWave];
+  uint32_t nbuf = 0;  // uniform
+  uint32_t* out = const_cast<uint32_t*>(a.active);  // the list this launch writes
+  uint32_t lo[2] = {0, 0}, hi[2] = {0, 0};
+  for (int d = 0; d < ro.ndst; ++d) dst_span(a, d, lo[d], hi[d]);
+  auto flush = [&]() {
+    wave_sync();
+    uint32_t pos = 0;
+    if (lane == 0) pos = atomicAdd(&a.ctr->active, nbuf);
+    pos = uni_u32(pos);  // lane 0: every lane is active here
+    for (uint32_t i = uint32_t(lane); i < nbuf; i += kWave) out[pos + i] = buf[i];
+    nbuf = 0;
+    wave_sync();
+  };
+  for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~uint32_t(kWave - 1); base < rows;
+       base += gridDim.x * blockDim.x) {
+    const uint32_t r = base + lane;
+    bool act = false;
+    if (r < rows) {
+      const uint32_t A = r / ro.np, ul = r - A * ro.np;
+      if (ro.kind == 3) {
+        act = ro.key.ptr[r + 1] > ro.key.ptr[r];
+      } else if (ro.kind == 2) {
+        for (int i = 0; i < a.g.ne; ++i) act |= a.g.e_lhs[i] == A;
+      } else if (ro.kind == 1) {
+        for (int i = 0; i < a.g.nu && !act; ++i) {
+          if (a.g.u_lhs[i] != A) continue;
+          const uint32_t rk = uint32_t(a.g.u_rhs[i]) * ro.np + ul;
+          act = ro.key.ptr[rk + 1] > ro.key.ptr[rk];
+        }
+      } else {
+        for (int i = 0; i < a.g.nb && !act; ++i) {
+          if (a.g.b_lhs[i] != A) continue;
+          const uint32_t rk = uint32_t(a.g.b_rhs1[i]) * ro.np + ul;
+          const unsigned long long kb = ro.key.ptr[rk], ke = ro.key.ptr[rk + 1];
+          if (kb == ke) continue;
+          const uint32_t first = ro.key.col[kb], last = ro.key.col[ke - 1];
+          for (int d = 0; d < ro.ndst && !act; ++d) {
+            if (lo[d] >= hi[d] || last < lo[d] || first >= hi[d]) continue;
+            const unsigned long long k = first >= lo[d] ? kb : lower_bound_u32(ro.key.col, kb + 1, ke, lo[d]);
+            act = k < ke && ro.key.col[k] < hi[d];
+          }
+        }
+      }
+    }
+    const unsigned long long m = __ballot(act);
+    if (!m) continue;
+    if (nbuf + kWave > kBuf) flush();
+    if (act) buf[nbuf + __popcll(m & ((1ull << lane) - 1))] = r;
+    nbuf += uint32_t(__popcll(m));
+  }
+  if (nbuf) flush();
+}
+
 struct WaveLds {
   uint32_t seen[kChunkWords];
   uint32_t fresh[kChunkWords];
@@ -1026,18 +1165,36 @@ __device__ inline void for_each_list(const ExpandArgs& a, uint32_t r, uint32_t A
         visit(ro.key.col, lane == 0 ? b : 0ull, lane == 0 ? uint32_t(e - b) : 0u);
       }
     } else if (ro.kind == 0) {
+      // keys outside every destination's span have no candidates: long key
+      // rows are cut to [klo, khi) first
+      uint32_t slo[2] = {0, 0}, sw[2] = {0, 0};
+      uint32_t klo = 0xFFFFFFFFu, khi = 0;
+      for (int d = 0; d < ro.ndst; ++d) {
+        uint32_t hi = 0;
+        dst_span(a, d, slo[d], hi);
+        if (slo[d] < hi) {
+          sw[d] = hi - slo[d];
+          klo = min(klo, slo[d]);
+          khi = max(khi, hi);
+        }
+      }
+      if (klo >= khi) return;
       for (int i = 0; i < a.g.nb; ++i) {
         if (a.g.b_lhs[i] != A) continue;
         const uint32_t B = a.g.b_rhs1[i], C = a.g.b_rhs2[i];
         const uint32_t rk = B * ro.np + ul;
-        const unsigned long long kb = ro.key.ptr[rk], ke = ro.key.ptr[rk + 1];
+        unsigned long long kb = ro.key.ptr[rk], ke = ro.key.ptr[rk + 1];
+        if (ke - kb > 2 * kWave) {
+          kb = wave_lower_bound(ro.key.col, kb, ke, klo, lane);
+          ke = wave_lower_bound(ro.key.col, kb, ke, khi, lane);
+        }
         for (unsigned long long k0 = kb; k0 < ke; k0 += kWave) {
           const unsigned long long k = k0 + lane;
           const uint32_t v = k < ke ? ro.key.col[k] : 0u;
           for (int d = 0; d < ro.ndst; ++d) {
             unsigned long long lb = 0;
             uint32_t len = 0;
-            if (k < ke && v - ro.dfirst[d] < ro.dn[d]) {
+            if (k < ke && v - slo[d] < sw[d]) {
               const uint32_t rc = C * ro.dn[d] + (v - ro.dfirst[d]);
               lb = ro.dst[d].ptr[rc];
               len = uint32_t(ro.dst[d].ptr[rc + 1] - lb);
@@ -1365,7 +1522,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
   // rows are taken kGrab at a time and staging space kStageBlock words at a
   // time, so the two global counters see one atomic per many rows
   uint32_t next = 0, end = 0;
-  const uint32_t grab = max(a.grab, 1u);
+  // partition operators take at least 1/8 of a wave's even share per grab:
+  // their rows are many and mostly light
+  const uint32_t grab =
+      MODE == 2 ? max(max(a.grab, 1u), nactive / (gridDim.x * kWavesPerBlock * 8u)) : max(a.grab, 1u);
   unsigned long long blk_pos = 0, blk_end = 0;
   auto reserve = [&](uint32_t cnt) -> unsigned long long {
     if (blk_pos + cnt > blk_end) {
@@ -3270,7 +3430,10 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     Region r[3];  // old, delta, tmp
     bool has[3] = {false, false, false};
     bool resident = false;
-    bool evicted = false;  // has been spilled at least once
+    bool evicted = false;      // has been spilled at least once
+    bool in_pending = false;   // resident, its upload still in flight (the compute stream has not waited for it)
+    bool out_pending = false;  // spilled, its copy-out in flight: the device buffers are kept until it ends
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
   };
   enum { kO = 0, kD = 1, kT = 2 };
   const int NP = int(parts.size());
@@ -3289,13 +3452,17 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     std::function<void()> f;
     ~Cleanup() { f(); }
   } cl{[&]() {
-    for (auto& q : pt)
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    for (auto& q : pt) {
       for (auto& r : q.r) {
         release(r.ptr);
         release(r.col);
         if (r.hptr.p) (void)hipHostFree(r.hptr.p);
         if (r.hcol.p) (void)hipHostFree(r.hcol.p);
       }
+      if (q.ev_in) (void)hipEventDestroy(q.ev_in);
+      if (q.ev_out) (void)hipEventDestroy(q.ev_out);
+    }
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
   }};
   const uint64_t max_rows = uint64_t(L) * max_np;
@@ -3320,7 +3487,11 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   auto host_alloc = [&](HostBuf& h, size_t bytes) -> int {
     bytes = std::max<size_t>(bytes, 256);
     if (h.bytes >= bytes) return GG_OK;
-    if (h.p) (void)hipHostFree(h.p);
+    if (h.p) {
+      // a copy on the copy stream may still read or write the old buffer
+      GG_HIP(hipStreamSynchronize(copy_stream));
+      (void)hipHostFree(h.p);
+    }
     h.p = nullptr;
     h.bytes = 0;
     if (bytes > (1u << 20)) bytes += bytes / 4;
@@ -3329,11 +3500,20 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     return GG_OK;
   };
   // Residency. A partition is resident (its three regions in HBM) or spilled
-  // (in pinned host DRAM). Partitions come in when an operator needs them and
-  // leave only under pressure: when an allocation would pass the device
-  // budget (or hipMalloc fails), the resident partition that the current
-  // step does not hold and whose next use in the sweep is farthest is copied
-  // out over the copy stream.
+  // (in pinned host DRAM). Copies run on the copy stream and are ordered with
+  // the compute stream by events only, so they overlap the operators:
+  //  * before each step (a pair, or one partition's superstep-0 closure or
+  //    merge) the partitions of the next steps of the sweep are uploaded
+  //    ahead when they fit the budget next to the step's working memory
+  //    (the most any step has needed so far); the compute stream waits for
+  //    an upload only when the partition is first used;
+  //  * when they do not fit, partitions whose next use is farthest ahead are
+  //    copied out ahead, and their device buffers go back once the copy-out
+  //    has ended (checked before each step);
+  //  * an allocation that still does not fit (the allocator's pressure
+  //    callback) first waits for a copy-out in flight, then copies out the
+  //    farthest partition the current step does not hold and waits for it.
+  // The sweep order is known, so "farthest next use" is Belady's choice.
   // GG_OOC_MAX_RESIDENT (a test hook) also caps the number of resident
   // partitions, so that small graphs go through the spill path.
   hipEvent_t ev = nullptr;
@@ -3345,12 +3525,38 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   int hold[2] = {-1, -1};
   int max_resident = 0;
   if (const char* mr = std::getenv("GG_OOC_MAX_RESIDENT")) max_resident = std::max(2, std::atoi(mr));
-  double copy_ms = 0;
-  auto spill = [&](int p) -> int {
+  double copy_ms = 0;  // host time spent issuing copies and waiting for them
+  auto make_event = [&](hipEvent_t& e) -> int {
+    if (!e) GG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return GG_OK;
+  };
+  // the device buffers of a copied-out partition go back once its copy has
+  // ended (wait: block until it has)
+  auto finish_out = [&](int p, bool wait) -> bool {
+    Part& q = pt[p];
+    if (!q.out_pending) return false;
+    if (wait) {
+      if (hipEventSynchronize(q.ev_out) != hipSuccess) return false;
+    } else if (hipEventQuery(q.ev_out) != hipSuccess) {
+      (void)hipGetLastError();  // not ready is not an error
+      return false;
+    }
+    for (auto& r : q.r) {
+      release(r.ptr);
+      release(r.col);
+    }
+    q.out_pending = false;
+    return true;
+  };
+  auto reap = [&]() {
+    for (int p = 0; p < NP; ++p) finish_out(p, false);
+  };
+  auto spill = [&](int p, bool wait) -> int {
     Part& q = pt[p];
     if (!po.spill || !q.resident) return GG_OK;
     const auto c0 = std::chrono::steady_clock::now();
     const uint64_t rows = rows_of(q);
+    // after the partition's last use on the compute stream
     GG_HIP(hipEventRecord(ev, stream));
     GG_HIP(hipStreamWaitEvent(copy_stream, ev, 0));
     for (auto& r : q.r) {
@@ -3361,47 +3567,92 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
       GG_HIP(hipMemcpyAsync(r.hcol.p, r.col.p, r.nnz * 4, hipMemcpyDeviceToHost, copy_stream));
       S.spill_bytes += (rows + 1) * 8 + r.nnz * 4;
     }
-    GG_HIP(hipStreamSynchronize(copy_stream));
-    for (auto& r : q.r) {
-      release(r.ptr);
-      release(r.col);
-    }
+    GG_TRY(make_event(q.ev_out));
+    GG_HIP(hipEventRecord(q.ev_out, copy_stream));
     q.resident = false;
     q.evicted = true;
+    q.in_pending = false;
+    q.out_pending = true;
+    if (wait) finish_out(p, true);
     copy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
     return GG_OK;
   };
-  // The sweeps visit the partitions in index order, so the next use of
-  // partition q is known: at q if the sweep has not passed it yet, else in
-  // the next sweep. The victim is the partition used farthest ahead
-  // (Belady's choice), not the least recently used one, which for a cyclic
-  // sweep is the one needed next.
-  int sweep_pos = 0;
+  // Position of the next step that uses partition q, from the current step:
+  // superstep 0 closes partition s at step s; a pair superstep runs pair
+  // (i, j) at step i * (NP + 1) + j and merges partition i at step
+  // i * (NP + 1) + NP. Partition q is used in row i < q as j and in its own
+  // row q. Skipped pairs are ignored.
+  int cur_i = 0, cur_j = 0;
+  bool in_pairs = false;
+  const long kRow = long(NP) + 1, kSweep = long(NP) * (NP + 1);
+  auto next_use = [&](int q) -> long {
+    if (!in_pairs) return q > cur_i ? long(q) : long(NP) + (q == 0 ? 1 : long(q));
+    const long now = long(cur_i) * kRow + cur_j;
+    long t;
+    if (q > cur_j && q > cur_i) t = long(cur_i) * kRow + q;                       // later in this row
+    else if (q > cur_i + 1) t = long(cur_i + 1) * kRow + q;                        // next row, as j
+    else if (q == cur_i + 1) t = long(q) * kRow + q + 1;                           // next row, as i
+    else t = kSweep + (q == 0 ? 1 : long(q));                                      // next superstep
+    return t - now;
+  };
   auto evict_one = [&]() -> bool {
+    // a copy-out in flight frees its buffers first
+    for (int p = 0; p < NP; ++p)
+      if (pt[p].out_pending) return finish_out(p, true);
     int best = -1;
     long best_next = -1;
     for (int p = 0; p < NP; ++p) {
       if (!pt[p].resident || p == hold[0] || p == hold[1]) continue;
-      const long next_use = p >= sweep_pos ? long(p) : long(NP) + p;
-      if (next_use > best_next) {
-        best_next = next_use;
+      const long nu = next_use(p);
+      if (nu > best_next) {
+        best_next = nu;
         best = p;
       }
     }
-    return best >= 0 && spill(best) == GG_OK;
+    return best >= 0 && spill(best, true) == GG_OK;
   };
   struct PressureGuard {
     Engine* e;
     ~PressureGuard() { e->on_pressure = nullptr; }
   } pg{this};
   if (po.spill) on_pressure = evict_one;
-  auto fetch = [&](int p) -> int {
+  auto resident_count = [&]() {
+    int res = 0;
+    for (auto& x : pt) res += x.resident;
+    return res;
+  };
+  // device bytes a partition takes when resident (alloc rounds large blocks
+  // to 2 MiB)
+  auto part_bytes = [&](const Part& q) {
+    const uint64_t rows = rows_of(q);
+    uint64_t b = 0;
+    for (auto& r : q.r) b += (rows + 1) * 8 + std::max<uint64_t>(r.nnz, 1) * 4 + (4u << 20);
+    return b;
+  };
+  // Makes partition p resident. ahead: an upload ahead of its use (the
+  // compute stream waits for it when p is fetched for use).
+  auto fetch = [&](int p, bool ahead = false) -> int {
     Part& q = pt[p];
-    if (q.resident) return GG_OK;
+    if (q.out_pending) {
+      // needed again while its copy-out is in flight: the device buffers
+      // still hold it
+      const auto c0 = std::chrono::steady_clock::now();
+      GG_HIP(hipEventSynchronize(q.ev_out));
+      q.out_pending = false;
+      q.resident = true;
+      copy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
+      return GG_OK;
+    }
+    if (q.resident) {
+      if (q.in_pending && !ahead) {
+        GG_HIP(hipStreamWaitEvent(stream, q.ev_in, 0));
+        q.in_pending = false;
+      }
+      return GG_OK;
+    }
     if (po.spill && max_resident) {
-      int res = 0;
-      for (auto& x : pt) res += x.resident;
-      while (res >= max_resident && evict_one()) --res;
+      int res = resident_count();
+      while (res >= max_resident && evict_one()) res = resident_count();
     }
     const auto c0 = std::chrono::steady_clock::now();
     const uint64_t rows = rows_of(q);
@@ -3420,19 +3671,74 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
       GG_HIP(hipMemcpyAsync(r.col.p, r.hcol.p, r.nnz * 4, hipMemcpyHostToDevice, copy_stream));
       if (q.evicted) S.spill_bytes += (rows + 1) * 8 + r.nnz * 4;  // the first upload is the input
     }
-    GG_HIP(hipEventRecord(ev, copy_stream));
-    GG_HIP(hipStreamWaitEvent(stream, ev, 0));
+    GG_TRY(make_event(q.ev_in));
+    GG_HIP(hipEventRecord(q.ev_in, copy_stream));
+    if (!ahead) GG_HIP(hipStreamWaitEvent(stream, q.ev_in, 0));
+    q.in_pending = ahead;
     q.resident = true;
     copy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
     return GG_OK;
   };
+  // Working memory of a step beyond its resident partitions: the most any
+  // step has needed so far (the device peak within the step above the
+  // bytes in use when it started).
+  uint64_t step_extra = 0, step_base = 0, run_peak = 0;
+  auto step_begin = [&]() {
+    run_peak = std::max(run_peak, peak);
+    step_base = used;
+    peak = used;
+  };
+  auto step_end = [&]() {
+    step_extra = std::max(step_extra, peak > step_base ? peak - step_base : 0);
+    run_peak = std::max(run_peak, peak);
+  };
+  // Before a step: finished copy-outs give their buffers back, then the
+  // partitions the next steps will fetch (upcoming, in order) are uploaded
+  // ahead while they fit, and the first that does not fit gets room by
+  // copying out partitions used later than it.
+  auto prepare = [&](const std::vector<int>& upcoming) -> int {
+    if (!po.spill) return GG_OK;
+    reap();
+    for (int p : upcoming) {
+      if (pt[p].resident || pt[p].out_pending) continue;
+      const uint64_t need = part_bytes(pt[p]);
+      int res = resident_count();
+      if ((!budget || used + need + step_extra <= budget) && (!max_resident || res < max_resident)) {
+        GG_TRY(fetch(p, true));
+        S.uploads_ahead++;
+        continue;
+      }
+      // room for p: copy out the partitions used after p, farthest first
+      uint64_t pending = 0;
+      for (auto& x : pt)
+        if (x.out_pending) pending += part_bytes(x);
+      const long p_next = next_use(p);
+      for (;;) {
+        const bool fits_mem = !budget || used - std::min(used, pending) + need + step_extra <= budget;
+        const bool fits_count = !max_resident || res < max_resident;
+        if (fits_mem && fits_count) break;
+        int best = -1;
+        long best_next = p_next;
+        for (int v = 0; v < NP; ++v) {
+          if (!pt[v].resident || v == hold[0] || v == hold[1]) continue;
+          const long nu = next_use(v);
+          if (nu > best_next) {
+            best_next = nu;
+            best = v;
+          }
+        }
+        if (best < 0) break;
+        pending += part_bytes(pt[best]);
+        GG_TRY(spill(best, false));
+        S.spills_ahead++;
+        --res;
+      }
+      break;
+    }
+    return GG_OK;
+  };
   auto elapsed_s = [&]() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  };
-  auto resident_count = [&]() {
-    int res = 0;
-    for (auto& x : pt) res += x.resident;
-    return res;
   };
   // initial delta of every partition: its input edges, deduplicated, built
   // on the host in one pass (global CSR by row, then per-partition slices)
@@ -3538,7 +3844,13 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
       a.staging_cap = staging.bytes / 4;
       zero_ctr();
       fill_u32(seg_cnt, 0, rows);
-      k_all_active<<<grid_for(rows), kBlock, 0, stream>>>(uint32_t(rows), P<uint32_t>(active), C());
+      // only rows that can have candidates are expanded: for a join, rows
+      // with a key where a destination region has rows
+      for (size_t d = 0; d < dsts.size(); ++d) {
+        const uint32_t bx = std::max<uint32_t>(1, std::min<uint32_t>(grid_for(a.ro.dn[d]), uint32_t(cus) / L));
+        k_region_span<<<dim3(bx, uint32_t(L)), kBlock, 0, stream>>>(a.ro.dst[d], a.ro.dn[d], int(d), C());
+      }
+      k_region_active<<<std::min<uint32_t>(grid_for(rows), uint32_t(cus)), kBlock, 0, stream>>>(a, uint32_t(rows));
       k_expand<2><<<expand_grid(), kBlock, 0, stream>>>(a);
       GG_HIP(hipGetLastError());
       GG_TRY(read_ctr(err));
@@ -3657,12 +3969,46 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     for (int p = 0; p < NP; ++p) GG_TRY(fetch(p));
   if (verbose)
     std::fprintf(stderr, "[gg] partitioned: %d partitions, input on the host after %.1f s\n", NP, elapsed_s());
+  // with the DDM schedule a pair runs only when edges cross between the two
+  // partitions and one of them has new edges (delta or tmp) to push
+  auto dirty = [&](int p) { return pt[p].has[kD] || pt[p].r[kT].nnz > 0; };
+  auto pair_runs = [&](int i, int j) { return !po.ddm || (ddm[i][j] + ddm[j][i] != 0 && (dirty(i) || dirty(j))); };
+  // the partitions, not resident now, that the next two steps of the sweep
+  // fetch, in order (superstep 0 looks no further than its end)
+  constexpr int kAhead = 2;
+  auto upcoming = [&]() {
+    std::vector<int> up;
+    int steps = 0;
+    auto add = [&](int q) {
+      if (!pt[q].resident && q != hold[0] && q != hold[1] && std::find(up.begin(), up.end(), q) == up.end())
+        up.push_back(q);
+    };
+    if (!in_pairs) {
+      for (int q = cur_i + 1; q < NP && q <= cur_i + kAhead; ++q) add(q);
+      return up;
+    }
+    for (int i = cur_i; i < NP && steps < kAhead; ++i) {
+      for (int j = i == cur_i ? cur_j + 1 : i + 1; j < NP && steps < kAhead; ++j)
+        if (pair_runs(i, j)) {
+          add(i);
+          add(j);
+          ++steps;
+        }
+      if (i > cur_i && steps < kAhead && (pt[i].has[kD] || pt[i].has[kT])) {
+        add(i);  // row i's merge step
+        ++steps;
+      }
+    }
+    return up;
+  };
   // superstep 0 (precomputation)
   for (int p = 0; p < NP; ++p) {
     hold[0] = p;
     hold[1] = -1;
-    sweep_pos = p;
+    cur_i = p;
     GG_TRY(fetch(p));
+    GG_TRY(prepare(upcoming()));
+    step_begin();
     for (;;) {
       Part& q = pt[p];
       q.has[kT] = true;
@@ -3685,33 +4031,35 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
         break;
       }
     }
+    step_end();
     GG_TRY(cross(p));
     if (verbose)
       std::fprintf(stderr,
                    "[gg] superstep 0: %d/%d partitions, %u rounds, %u launches, %u retries, closure %llu, "
-                   "%d resident, %.2f GB used, %.2f GB moved, %llu allocs (%.1f s), %.1f s\n",
+                   "%d resident, %.2f GB used, %.2f GB moved (%.1f s), %llu allocs (%.1f s), %.1f s\n",
                    p + 1, NP, S.rounds, S.expand_launches, S.retries,
                    (unsigned long long)(pt[p].r[kD].nnz + pt[p].r[kO].nnz), resident_count(), used / 1e9,
-                   S.spill_bytes / 1e9, (unsigned long long)alloc_calls, alloc_ms / 1e3, elapsed_s());
+                   S.spill_bytes / 1e9, copy_ms / 1e3, (unsigned long long)alloc_calls, alloc_ms / 1e3, elapsed_s());
   }
-  // supersteps 1.. (run_computation); with the DDM schedule a pair runs only
-  // when edges cross between the two partitions and one of them has new
-  // edges (delta or tmp) to push
+  // supersteps 1.. (run_computation)
   uint32_t supersteps = 0;
-  auto dirty = [&](int p) { return pt[p].has[kD] || pt[p].r[kT].nnz > 0; };
+  in_pairs = true;
   for (;;) {
     ++supersteps;
     for (int i = 0; i < NP; ++i) {
-      sweep_pos = i;
       for (int j = i + 1; j < NP; ++j) {
-        if (po.ddm && (ddm[i][j] + ddm[j][i] == 0 || !(dirty(i) || dirty(j)))) {
+        if (!pair_runs(i, j)) {
           S.pairs_skipped++;
           continue;
         }
         hold[0] = i;
         hold[1] = j;
+        cur_i = i;
+        cur_j = j;
         GG_TRY(fetch(i));
         GG_TRY(fetch(j));
+        GG_TRY(prepare(upcoming()));
+        step_begin();
         S.pairs_run++;
         pt[i].has[kT] = true;
         pt[j].has[kT] = true;
@@ -3733,6 +4081,7 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
           S.rounds++;
           if (!changed) break;
         }
+        step_end();
         GG_TRY(cross(i));
         GG_TRY(cross(j));
       }
@@ -3740,20 +4089,34 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
       if (pt[i].has[kD] || pt[i].has[kT]) {
         hold[0] = i;
         hold[1] = -1;
+        cur_i = i;
+        cur_j = NP;
         GG_TRY(fetch(i));
+        GG_TRY(prepare(upcoming()));
+        step_begin();
         GG_TRY(merge_and_diff(i));
+        step_end();
         GG_TRY(cross(i));
       }
     }
     bool done = true;
     for (auto& q : pt) done &= !q.has[kD];
     if (verbose)
-      std::fprintf(stderr, "[gg] superstep %u: pairs run %u skipped %u, %d resident, %.2f GB moved (%.1f s), %.1f s\n",
+      std::fprintf(stderr,
+                   "[gg] superstep %u: pairs run %u skipped %u, %d resident, %.2f GB moved (%.1f s), %u uploads and "
+                   "%u copy-outs ahead, %.1f s\n",
                    supersteps, S.pairs_run, S.pairs_skipped, resident_count(), S.spill_bytes / 1e9, copy_ms / 1e3,
-                   elapsed_s());
+                   S.uploads_ahead, S.spills_ahead, elapsed_s());
     if (done) break;
   }
   hold[0] = hold[1] = -1;
+  // copies still in flight end before the closure is assembled
+  GG_HIP(hipStreamSynchronize(copy_stream));
+  for (int p = 0; p < NP; ++p) {
+    finish_out(p, true);
+    pt[p].in_pending = false;
+  }
+  peak = std::max(run_peak, peak);
   supersteps_r = supersteps;
   // the closure is every partition's old region: assemble the global CSR
   GG_HIP(hipStreamSynchronize(stream));

@@ -77,6 +77,8 @@ typedef struct gg_stats {
   uint64_t spill_bytes;     /* out-of-core: bytes of evicted partitions moved between HBM and host DRAM
                                (out and back in; the input's first upload is not counted) */
   uint32_t pair_reruns;     /* passes of the old x delta pair-list kernel re-run after growing its arenas */
+  uint32_t uploads_ahead;   /* out-of-core: partitions uploaded during the step before their first use */
+  uint32_t spills_ahead;    /* out-of-core: partitions copied out during a step to make room for the next */
   uint32_t pad0;
 } gg_stats;
 

@@ -484,6 +484,7 @@ def test_gpu_out_of_core_budget_pressure(built):
     s0 = free.run_out_of_core(16)
     ref = (free.counts()[1], free.checksum())
     assert s0.spill_bytes == 0
+    assert s0.uploads_ahead > 0  # partitions come in during the step before their use
     free.release()
     budget = int(s0.device_bytes_peak * 0.75)
     c = gg.Context(device_budget=budget)
@@ -491,6 +492,7 @@ def test_gpu_out_of_core_budget_pressure(built):
     st = c.run_out_of_core(16)
     assert (c.counts()[1], c.checksum()) == ref
     assert st.spill_bytes > 0 and st.device_bytes_peak <= budget
+    assert st.uploads_ahead + st.spills_ahead > 0
     inc = gg.Context()
     inc.set_spec(g)
     inc.run(0)
