@@ -291,13 +291,25 @@ def bench_batch(args):
     one = gg.Context()
     t0 = time.perf_counter()
     single = []
+    solo_ms = []
     for g in graphs:
+        t1 = time.perf_counter()
         one.set_spec(g)
         one.run(0)
         single.append((one.counts(), one.checksum()))
+        solo_ms.append((time.perf_counter() - t1) * 1e3)
     dt_s = time.perf_counter() - t0
     one.release()
     log(f"one graph at a time: {dt_s * 1e3:.0f} ms")
+    # where the one-at-a-time time goes, by graph size: small graphs are
+    # bound by per-round costs that batching shares, large ones by the same
+    # kernels either way
+    by_size = []
+    for lo, hi in ((0, 10_000), (10_000, 50_000), (50_000, 10**9)):
+        idx = [i for i, g in enumerate(graphs) if lo <= g.n < hi]
+        by_size.append({"n_from": lo, "n_to": hi if hi < 10**9 else None, "graphs": len(idx),
+                        "one_at_a_time_ms": float(sum(solo_ms[i] for i in idx)),
+                        "closure_edges": int(sum(int(single[i][0][1]) for i in idx))})
     for i, ((per, tot), cs) in enumerate(single):
         bp, bt, bc = batched[i]
         if (tot, cs) != (bt, bc) or not np.array_equal(per, bp):
@@ -310,6 +322,7 @@ def bench_batch(args):
             "closure_edges": int(st.total_edges), "ms": dt_b * 1e3,
             "graphs_per_s": len(graphs) / dt_b, "one_at_a_time_ms": dt_s * 1e3,
             "one_at_a_time_graphs_per_s": len(graphs) / dt_s, "speedup": dt_s / dt_b,
+            "one_at_a_time_by_size": by_size,
             "closure_edges_per_s": (st.total_edges - st.input_unique) / dt_b,
             "device_bytes_peak": int(st.device_bytes_peak),
             "checked": "every graph: batched per-label counts and checksum equal its one-at-a-time result"}

@@ -248,7 +248,8 @@ int gg_batch_add(gg_batch *batch, int num_rules, const int32_t *rules, int num_l
 /* Closes every added graph. Graphs are ordered by grammar and cut into
  * launch groups of at most max_vertices vertices in total; 0 plans each
  * group from device memory instead (the context's budget, else the device's
- * memory): a group may use 40% of it, at the peak bytes per vertex measured
+ * memory): a group may use a tuned share of it (group_mem_share in
+ * tuned_gfx950.cfg beside the library, 0.4), at the peak bytes per vertex measured
  * on the previous group of the same grammar (2^21 vertices for a grammar
  * not yet seen). A single graph above the cap runs alone. A group that still
  * runs out of device memory is cut in half and run again. The 2^32 row limit
