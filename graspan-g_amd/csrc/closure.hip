@@ -3321,7 +3321,7 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
     if (ev.expand.back().a) (void)hipEventElapsedTime(&xms, ev.expand.back().a, ev.expand.back().b);
     std::fprintf(stderr,
                  "[gg] round %u: active %u cand %llu new %llu guess %u misses %u expand %.2f ms"
-                 " (words %llu ids %llu keys %llu rows %llu)\n",
+                 " (cycles: seeding %llu lists %llu streams %llu rows %llu)\n",
                  S.rounds, h_ctr->active, (unsigned long long)h_ctr->cand, (unsigned long long)N, used_guess,
                  h_ctr->misses, xms, h_ctr->xs[0], h_ctr->xs[1], h_ctr->xs[2], h_ctr->xs[3]);
   }

@@ -31,6 +31,14 @@ int fail(gg_ctx* c, int status, const std::string& msg) {
   return status;
 }
 
+// Row ids (label * vertices + vertex) are 32-bit in every schedule: a graph
+// with num_labels * num_vertices >= 2^32 is refused before any device work.
+int check_row_ids(gg_ctx* c) {
+  if (uint64_t(c->grammar.num_labels()) * c->graph.n >= 0xFFFFFFFFull)
+    return fail(c, GG_E_UNSUPPORTED, "num_labels * num_vertices must be < 2^32 (row ids are 32-bit)");
+  return GG_OK;
+}
+
 // Creates the engine on first use (the only place that touches the GPU).
 int ensure_engine(gg_ctx* c) {
   if (c->engine) return GG_OK;
@@ -295,6 +303,7 @@ int gg_graph_edges(const gg_ctx* c, uint32_t* src, uint32_t* dst, uint8_t* lab) 
 int gg_closure_run_partitioned(gg_ctx* c, int filter_flag, gg_stats* stats) {
   if (!c) return GG_E_ARG;
   if (!c->has_grammar || !c->has_graph) return fail(c, GG_E_STATE, "load a grammar and a graph first");
+  if (check_row_ids(c) != GG_OK) return GG_E_UNSUPPORTED;
   std::vector<std::pair<uint32_t, uint32_t>> parts;
   int s = reference_partitions(c->graph, c->grammar.num_labels(), parts, c->err);
   if (s != GG_OK) return s;
@@ -310,6 +319,7 @@ int gg_closure_run_partitioned(gg_ctx* c, int filter_flag, gg_stats* stats) {
 int gg_closure_run_out_of_core(gg_ctx* c, int num_partitions, gg_stats* stats) {
   if (!c) return GG_E_ARG;
   if (!c->has_grammar || !c->has_graph) return fail(c, GG_E_STATE, "load a grammar and a graph first");
+  if (check_row_ids(c) != GG_OK) return GG_E_UNSUPPORTED;
   const uint32_t n = c->graph.n;
   if (num_partitions < 1 || uint32_t(num_partitions) > n) return fail(c, GG_E_ARG, "bad partition count");
   std::vector<std::pair<uint32_t, uint32_t>> parts;
@@ -349,6 +359,7 @@ int gg_closure_run(gg_ctx* c, int filter_flag, gg_stats* stats) {
   if (!c) return GG_E_ARG;
   if (!c->has_grammar || !c->has_graph) return fail(c, GG_E_STATE, "load a grammar and a graph first");
   if (filter_flag != 0) return gg_closure_run_partitioned(c, filter_flag, stats);
+  if (check_row_ids(c) != GG_OK) return GG_E_UNSUPPORTED;
   int s = ensure_engine(c);
   if (s != GG_OK) return s;
   if (!c->graph_on_device) {

@@ -139,6 +139,23 @@ def test_run_without_gpu_fails_loudly():
     assert ei.value.status == gg.GG_E_DEVICE
 
 
+def test_row_id_limit_refused_before_device_work():
+    """num_labels * num_vertices >= 2^32 cannot be addressed by 32-bit row
+    ids: the in-core, partitioned and out-of-core entry points all refuse it
+    with GG_E_UNSUPPORTED before they look for a device (so this holds with
+    or without a GPU)."""
+    g = gen.alias(50, seed=1)
+    c = gg.Context()
+    c.set_grammar(g.rules, g.num_labels)
+    n = (1 << 32) // g.num_labels + 1
+    c.set_graph(n, np.array([0], np.uint32), np.array([1], np.uint32), np.array([2], np.uint8))
+    for call in (lambda: c.run(0), lambda: c.run(1), lambda: c.run_out_of_core(4)):
+        with pytest.raises(gg.GGError) as ei:
+            call()
+        assert ei.value.status == gg.GG_E_UNSUPPORTED
+        assert "2^32" in str(ei.value)
+
+
 def test_comp_cli_without_gpu_exits_nonzero():
     if _gpu_present():
         pytest.skip("a GPU is present")

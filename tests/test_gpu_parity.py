@@ -400,6 +400,42 @@ def test_gpu_batch_groups_and_alone_equal(built):
     assert alone.counts()[1] == whole[3][1] and alone.checksum() == whole[3][2]
 
 
+def _relabelled(g, order, tag):
+    """The same graph under its grammar's rule lines taken in another order:
+    label ids follow first appearance, so the rule triples (and the batch's
+    grammar key) differ while the closure is the same up to label names."""
+    import dataclasses
+    lines = tuple(g.rule_lines[i] for i in order)
+    old_ids, new_ids = gen.label_ids(g.rule_lines), gen.label_ids(lines)
+    lut = np.zeros(256, np.uint8)
+    for name, i in old_ids.items():
+        lut[i] = new_ids[name]
+    return dataclasses.replace(g, rule_lines=lines, lab=lut[g.lab], name=f"{g.name}-{tag}")
+
+
+def test_gpu_batch_many_distinct_grammars(built):
+    """Twelve alias graphs under twelve distinct label numberings: 84 binary
+    rules, more than one launch group may hold (64), so the batch must start
+    new groups on the rule limit; every result equals the oracle's."""
+    say("batch of 12 distinct alias grammars")
+    rng = np.random.default_rng(23)
+    graphs = []
+    for k in range(12):
+        base = gen.alias(int(rng.integers(200, 900)), seed=230 + k)
+        graphs.append(_relabelled(base, rng.permutation(len(base.rule_lines)), f"p{k}"))
+    assert len({tuple(map(tuple, g.rules.tolist())) for g in graphs}) == 12
+    c = gg.Context()
+    b = gg.Batch(c)
+    ids = [b.add_spec(g) for g in graphs]
+    b.run()
+    assert b.info()[1] >= 2
+    for g, i in zip(graphs, ids):
+        ref = oracle.closure_of(g, want_triples=False)
+        per, tot, cs = b.result(i)
+        assert (tot, cs) == (ref.total, ref.checksum), g.name
+        np.testing.assert_array_equal(per, ref.counts)
+
+
 def test_gpu_server_results_equal_alone(built):
     """Requests served in a mix (two slots, batches formed from whatever is
     queued) give the same closures as each request closed alone."""
