@@ -19,6 +19,17 @@
 //   --budget-mb N      device memory budget
 //   --device N         HIP device ordinal
 //   --verbose          per-round progress on stderr
+//   --partition-files DIR
+//                      after the run, DIR/Partition<k> for each of the
+//                      reference's vertex intervals (initializePartitions,
+//                      src/main.cpp:141-175), in the record layout of its
+//                      compressed partition store (storeElements_c /
+//                      elements2Edges, src/main.cpp:510-600): per source vertex
+//                      in the interval, per label, the destinations ascending,
+//                      each as 5 bytes (u32 dst little-endian, u8 label id). At
+//                      the fixpoint the delta and tmp regions are empty, so a
+//                      file holds the interval's closure edges.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -41,7 +52,49 @@ int die(gg_ctx* c, const char* what, int status) {
 void usage() {
   std::fprintf(stderr,
                "usage: comp <graph_file> <grammar_file> [filter_flag] [--dump FILE] [--dump-text FILE]\n"
-               "            [--counts] [--stats-json FILE] [--budget-mb N] [--device N] [--verbose]\n");
+               "            [--counts] [--stats-json FILE] [--budget-mb N] [--device N] [--verbose]\n"
+               "            [--partition-files DIR]\n");
+}
+
+// Writes dir/Partition<k> for the reference's intervals from the sorted
+// (src, dst, label) triples t (see the option's description above).
+int write_partition_files(gg_ctx* c, const char* dir, const std::vector<uint32_t>& t, uint64_t cnt) {
+  int np = 0;
+  int s = gg_reference_partitions(c, nullptr, nullptr, 0, &np);
+  if (s != GG_OK) return s;
+  std::vector<uint32_t> first(static_cast<size_t>(np)), last(static_cast<size_t>(np));
+  if ((s = gg_reference_partitions(c, first.data(), last.data(), np, &np)) != GG_OK) return s;
+  uint64_t i = 0;
+  std::vector<uint64_t> order;
+  std::vector<unsigned char> rec;
+  for (int k = 0; k < np; ++k) {
+    rec.clear();
+    while (i < cnt && t[3 * i] < first[size_t(k)]) ++i;
+    // one source vertex at a time: its edges by (label, dst); the triples
+    // are sorted by (src, dst, label), so a stable sort by label is enough
+    while (i < cnt && t[3 * i] <= last[size_t(k)]) {
+      uint64_t j = i;
+      while (j < cnt && t[3 * j] == t[3 * i]) ++j;
+      order.resize(j - i);
+      for (uint64_t q = i; q < j; ++q) order[q - i] = q;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](uint64_t a, uint64_t b) { return t[3 * a + 2] < t[3 * b + 2]; });
+      for (uint64_t q : order) {
+        const uint32_t dst = t[3 * q + 1];
+        const unsigned char b[5] = {static_cast<unsigned char>(dst), static_cast<unsigned char>(dst >> 8),
+                                    static_cast<unsigned char>(dst >> 16), static_cast<unsigned char>(dst >> 24),
+                                    static_cast<unsigned char>(t[3 * q + 2])};
+        rec.insert(rec.end(), b, b + 5);
+      }
+      i = j;
+    }
+    const std::string path = std::string(dir) + "/Partition" + std::to_string(k);
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return GG_E_IO;
+    const bool ok = rec.empty() || std::fwrite(rec.data(), 1, rec.size(), f) == rec.size();
+    if (std::fclose(f) != 0 || !ok) return GG_E_IO;
+  }
+  return GG_OK;
 }
 
 }  // namespace
@@ -49,7 +102,7 @@ void usage() {
 int main(int argc, char** argv) {
   const clock_t start = clock();
   std::vector<const char*> pos;
-  const char *dump = nullptr, *dump_text = nullptr, *stats_json = nullptr;
+  const char *dump = nullptr, *dump_text = nullptr, *stats_json = nullptr, *part_dir = nullptr;
   bool counts = false;
   gg_options opts{-1, 0, 0};
   for (int i = 1; i < argc; ++i) {
@@ -65,6 +118,8 @@ int main(int argc, char** argv) {
       dump = need("--dump");
     else if (a == "--dump-text")
       dump_text = need("--dump-text");
+    else if (a == "--partition-files")
+      part_dir = need("--partition-files");
     else if (a == "--stats-json")
       stats_json = need("--stats-json");
     else if (a == "--counts")
@@ -173,10 +228,11 @@ int main(int argc, char** argv) {
                  (unsigned long long)st.device_bytes_peak);
     std::fclose(f);
   }
-  if (dump || dump_text) {
+  if (dump || dump_text || part_dir) {
     std::vector<uint32_t> buf(size_t(total) * 3);
     uint64_t cnt = 0;
     if ((s = gg_closure_export(c, buf.data(), total, &cnt)) != GG_OK) return die(c, "export", s);
+    if (part_dir && (s = write_partition_files(c, part_dir, buf, cnt)) != GG_OK) return die(c, "partition-files", s);
     if (dump) {
       FILE* f = std::fopen(dump, "wb");
       if (!f || std::fwrite(buf.data(), 4, buf.size(), f) != buf.size()) return die(c, "dump", GG_E_IO);

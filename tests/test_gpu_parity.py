@@ -291,6 +291,41 @@ def test_comp_cli_report_and_dump(built, tmp_path):
     assert p2.returncode == 0 and f"FINAL NUM EDGES: {len(d['expected'])}" in p2.stdout
 
 
+def _partition_records(triples, parts):
+    """Expected Partition<k> bytes: per interval, its closure edges by
+    (src, label, dst), 5 bytes each (u32 dst little-endian, u8 label)."""
+    out = []
+    for a, b in parts:
+        t = triples[(triples[:, 0] >= a) & (triples[:, 0] <= b)]
+        t = t[np.lexsort((t[:, 1], t[:, 2], t[:, 0]))]
+        rec = np.zeros((len(t), 5), np.uint8)
+        rec[:, :4] = t[:, 1].astype("<u4").view(np.uint8).reshape(-1, 4)
+        rec[:, 4] = t[:, 2]
+        out.append(rec.tobytes())
+    return out
+
+
+def test_comp_partition_files(built, tmp_path):
+    """--partition-files writes one Partition<k> file per interval of the
+    reference's initializePartitions, holding the interval's closure edges
+    in the record layout of its compressed partition store."""
+    say("comp --partition-files sample")
+    d = load_golden(os.path.join(GOLDEN, "sample.json"))
+    out = tmp_path / "parts"
+    out.mkdir()
+    p = subprocess.run([gg.COMP_PATH, os.path.join(GOLDEN, "sample_graph.txt"),
+                        os.path.join(GOLDEN, "sample_grammar.txt"), "0", "--partition-files", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    parts = oracle.partitions_r(d["n"], d["rules_arr"], d["src"], d["dst"], d["lab"], d["num_labels"])
+    assert len(parts) == 2  # the sample takes the small-graph split
+    want = _partition_records(np.asarray(d["expected_arr"]), parts)
+    assert sorted(os.listdir(out)) == [f"Partition{k}" for k in range(len(parts))]
+    for k, w in enumerate(want):
+        assert (out / f"Partition{k}").read_bytes() == w, k
+    assert sum(len(w) for w in want) == 5 * len(d["expected"])
+
+
 def test_gpu_unpacked_path_matches(built, monkeypatch):
     """The same closure with packed old rows switched off (every candidate as a
     separate id)."""
