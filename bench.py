@@ -12,10 +12,12 @@ One step = one full fixpoint of that graph with its input already resident in
 HBM. value = new closure edges per second = (|closure| - |dedup input|) / t,
 summed over the timed steps.
 
-roofline: the join/dedup/difference kernel (k_expand), timed live with HIP
-events on the engine's stream. Its VALU instructions and HBM bytes per launch
-come from the committed rocprofv3 counter summary of the same workload
-(profiles/expand_alias1m_counters.*). The bound is the resource it uses the
+roofline: the expand stage -- the join/dedup/difference kernels (k_expand_light,
+one lane per row, in rounds with few candidates per row, then k_expand, one
+wave per row, for every other row) -- timed live with HIP events on the
+engine's stream around each launch. Its VALU instructions and HBM bytes per
+launch come from the committed rocprofv3 counter summary of the same workload
+(profiles/expand_alias1m_counters.*, made by summarize_counters.py). The bound is the resource it uses the
 larger share of: VALU issue (instructions x 64 lanes over the live launch
 time, against 39.3 T lane-ops/s) -- the kernel is bound there -- or HBM
 (bytes over the launch time, against 8 TB/s); both shares are reported. The
@@ -59,7 +61,7 @@ def log(*a):
 
 
 def counters_from_profiles():
-    """Per-launch hardware counters of k_expand on this workload, from the
+    """Per-launch hardware counters of the expand stage on this workload, from the
     committed rocprofv3 summary (profiles/expand_alias1m_counters.json), or
     None if it is absent."""
     p = os.path.join(ROOT, "profiles", "expand_alias1m_counters.json")
@@ -106,8 +108,9 @@ def cpu_baseline(g):
 
 
 def roofline(stats):
-    """Roofline object for k_expand. Its time is measured live (HIP events on
-    the engine's stream around every launch); its HBM traffic and VALU work
+    """Roofline object for the expand stage (k_expand_light and k_expand). Its
+    time is measured live (HIP events on the engine's stream around every
+    launch); its HBM traffic and VALU work
     per launch come from the committed rocprofv3 counter summary of the same
     workload. Both byte models are reported: the fused kernel's own
     (4P + 4O + 4N: probe reads, old/delta reads for the difference, new edges
@@ -119,7 +122,7 @@ def roofline(stats):
     fused = sum(4 * s["probe_reads"] + 4 * s["old_reads"] + 4 * s["new_edges"] for s in stats) / max(1, launches)
     survey = sum(4 * s["probe_reads"] + 8 * s["candidates"] + 4 * s["old_reads"] + 8 * s["new_edges"]
                  for s in stats) / max(1, launches)
-    out = {"kernel": "k_expand (join+dedup+difference)", "avg_launch_ms": avg_s * 1e3,
+    out = {"kernel": "expand stage: k_expand_light + k_expand (join+dedup+difference)", "avg_launch_ms": avg_s * 1e3,
            "launches_per_step": launches / max(1, len(stats)),
            "alg_bytes_per_launch_fused": fused, "alg_bytes_per_launch_4P8K4O8N": survey,
            "alg_gbs_fused": fused / avg_s / 1e9, "alg_gbs_4P8K4O8N": survey / avg_s / 1e9}

@@ -71,6 +71,7 @@ void add_stats(gg_stats& a, const gg_stats& b) {
   a.new_edges += b.new_edges;
   a.device_bytes_peak = std::max(a.device_bytes_peak, b.device_bytes_peak);
   a.pair_reruns += b.pair_reruns;
+  a.light_rows += b.light_rows;
 }
 
 int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std::string*& cerr) {

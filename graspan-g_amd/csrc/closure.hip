@@ -122,6 +122,7 @@ struct Counters {
   unsigned long long checksum;
   unsigned int active;         // rows in the active list
   unsigned int active_base;    // MODE 1 expand: its rows start here in the active list
+  unsigned int deferred;       // rows the light pass left to the wave kernel (k_expand_light)
   unsigned int work;           // work-stealing index
   unsigned int overflow;       // staging too small
   unsigned int misses;
@@ -921,6 +922,9 @@ struct ExpandArgs {
   uint32_t* seg_cnt;
   uint32_t guess;  // ids of a window centred on the row's vertex tried first (0: none)
   uint32_t grab;   // active rows a wave takes per work-counter atomic
+  // MODE 0: when set, the rows to expand are the ctr->deferred rows listed
+  // here (those the light pass left), not the active list
+  const uint32_t* defer;
 };
 
 // The span of destination region d of a partition operator: global vertex
@@ -1503,6 +1507,184 @@ __device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t spa
   }
 }
 
+// ---------------------------------------------------------------------------
+// Light rows: one lane per row. In rounds whose rows have few candidates the
+// wave-per-row kernel spends a row's time on its chain of dependent loads
+// (segment bounds, keys, key rows, lists, seed rows), one row per wave at a
+// time. Here each lane follows that chain for a row of its own, so a wave
+// keeps 64 rows in flight. A lane gathers its row's candidates into its
+// column of an LDS buffer (at most kLightCap), sorts them, and keeps each
+// distinct id absent from old_A(u), delta_A(u) and the pending edges (binary
+// searches that only move forward, the ids being sorted). A row with more
+// candidates goes to the deferred list (ctr->deferred rows at `defer`),
+// which the wave kernel expands afterwards. The lists are those of the wave
+// kernel (for_each_list, MODE 0) and K, O and N count as it does.
+template <bool PEND, uint32_t kLightCap>
+__global__ __launch_bounds__(kBlock) void k_expand_light(const ExpandArgs a, uint32_t* defer) {
+  __shared__ uint32_t s_buf[kLightCap][kBlock];  // lane t's candidates in column t
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t t = threadIdx.x;
+  const uint32_t n = a.n;
+  const uint32_t nactive = a.ctr->active;
+  const bool pend = PEND && a.pend.ptr;
+  unsigned long long k_sum = 0, o_sum = 0, n_sum = 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nactive; base += gridDim.x * blockDim.x) {
+    const uint32_t idx = base + t;
+    const bool live = idx < nactive;
+    const uint32_t r = live ? a.active[idx] : 0u;
+    const uint32_t A = r / n, u = r - A * n;
+    uint32_t cnt = 0;
+    bool over = false;
+    unsigned long long krow = 0;
+    auto put = [&](uint32_t x) {
+      if (cnt < kLightCap) s_buf[cnt][t] = x;
+      ++cnt;
+    };
+    // ids col[b, e)
+    auto take_ids = [&](const uint32_t* col, unsigned long long b, unsigned long long e) {
+      krow += e - b;
+      if (over || e == b) return;
+      if (cnt + (e - b) > kLightCap) {
+        over = true;
+        return;
+      }
+      for (unsigned long long k = b; k < e; ++k) put(col[k]);
+    };
+    // a packed row: pl entries at ent[b ..] standing for `ids` ids
+    auto take_words = [&](const unsigned long long* ent, unsigned long long b, uint32_t pl, unsigned long long ids) {
+      krow += ids;
+      if (over || ids == 0) return;
+      if (cnt + ids > kLightCap) {
+        over = true;
+        return;
+      }
+      for (uint32_t q = 0; q < pl; ++q) {
+        const unsigned long long e = ent[b + q];
+        const uint32_t w = uint32_t(e) << 5;
+        for (uint32_t bits = uint32_t(e >> 32); bits; bits &= bits - 1) put(w + uint32_t(__ffs(bits) - 1));
+      }
+      over = cnt > kLightCap;
+    };
+    if (live) {
+      for (int i = 0; i < a.g.nu; ++i)
+        if (a.g.u_lhs[i] == A) {
+          const uint32_t row = uint32_t(a.g.u_rhs[i]) * n + u;
+          take_ids(a.dl.col, a.dl.ptr[row], a.dl.ptr[row + 1]);
+        }
+      for (int i = 0; i < a.g.nb && !over; ++i) {
+        if (a.g.b_lhs[i] != A) continue;
+        const uint32_t B = a.g.b_rhs1[i], C = a.g.b_rhs2[i];
+        const bool pkc = a.pk.len && packed_label(a.pk, C);
+        // keys delta_B(u): lists old_C(v), delta_C(v) and the pending C(v)
+        const uint32_t kr = B * n + u;
+        const unsigned long long ke = a.dl.ptr[kr + 1];
+        for (unsigned long long k = a.dl.ptr[kr]; k < ke && !over; ++k) {
+          const uint32_t rc = C * n + a.dl.col[k];
+          const unsigned long long ob = a.old.ptr[rc], oe = a.old.ptr[rc + 1];
+          if (pkc)
+            take_words(a.pk.ent, ob, a.pk.len[rc], oe - ob);
+          else
+            take_ids(a.old.col, ob, oe);
+          take_ids(a.dl.col, a.dl.ptr[rc], a.dl.ptr[rc + 1]);
+          if (pend) take_ids(a.pend.col, a.pend.ptr[rc], a.pend.ptr[rc + 1]);
+        }
+        // keys of the frontier pair list: lists delta_C(v)
+        const uint32_t slot = a.g.b_slot[i];
+        if (slot != 0xFFu && a.b2.ptr) {
+          const unsigned long long pr = (unsigned long long)slot * n + u;
+          const unsigned long long pe = a.b2.ptr[2 * pr + 1];
+          for (unsigned long long k = a.b2.ptr[2 * pr]; k < pe && !over; ++k) {
+            const uint32_t rc = C * n + a.b2.col[k];
+            take_ids(a.dl.col, a.dl.ptr[rc], a.dl.ptr[rc + 1]);
+          }
+        }
+      }
+    }
+    uint32_t c = 0;
+    if (live && !over && cnt) {
+      for (uint32_t i = 1; i < cnt; ++i) {
+        const uint32_t x = s_buf[i][t];
+        uint32_t j = i;
+        for (; j > 0 && s_buf[j - 1][t] > x; --j) s_buf[j][t] = s_buf[j - 1][t];
+        s_buf[j][t] = x;
+      }
+      const bool pka = a.pk.len && packed_label(a.pk, A);
+      unsigned long long ob = a.old.ptr[r];
+      const unsigned long long oe = pka ? ob + a.pk.len[r] : a.old.ptr[r + 1];
+      unsigned long long db = a.dl.ptr[r];
+      const unsigned long long de = a.dl.ptr[r + 1];
+      unsigned long long qb = 0, qe = 0;
+      if (pend) {
+        qb = a.pend.ptr[r];
+        qe = a.pend.ptr[r + 1];
+      }
+      // ids are below n < 2^32 - 1, so no id equals the first `prev`
+      uint32_t prev = 0xFFFFFFFFu;
+      for (uint32_t i = 0; i < cnt; ++i) {
+        const uint32_t x = s_buf[i][t];
+        if (x == prev) continue;
+        prev = x;
+        bool seen;
+        if (pka) {
+          ob = lower_pos_word(a.pk.ent, ob, oe, x >> 5);
+          seen = ob < oe && uint32_t(a.pk.ent[ob]) == (x >> 5) && ((a.pk.ent[ob] >> (32 + (x & 31))) & 1ull);
+        } else {
+          ob = lower_pos(a.old.col, ob, oe, x);
+          seen = ob < oe && a.old.col[ob] == x;
+        }
+        if (!seen) {
+          db = lower_pos(a.dl.col, db, de, x);
+          seen = db < de && a.dl.col[db] == x;
+        }
+        if (!seen && pend) {
+          qb = lower_pos(a.pend.col, qb, qe, x);
+          seen = qb < qe && a.pend.col[qb] == x;
+        }
+        if (!seen) s_buf[c++][t] = x;
+      }
+    }
+    // the wave's new ids take one staging reservation
+    const uint32_t incl = wave_incl_scan(c, lane);
+    const uint32_t tot = lane63_u32(incl);
+    unsigned long long wb = 0;
+    if (tot) {
+      if (lane == 0) wb = atomicAdd(&a.ctr->cursor, (unsigned long long)tot);
+      wb = uni_u64(wb);
+    }
+    if (tot && wb + tot > a.staging_cap) {
+      if (lane == 0) atomicExch(&a.ctr->overflow, 1u);
+    } else if (live && !over) {
+      const unsigned long long pos = wb + (incl - c);
+      for (uint32_t i = 0; i < c; ++i) a.staging[pos + i] = s_buf[i][t];
+      a.seg_off[r] = pos;
+      a.seg_cnt[r] = c;
+      k_sum += krow;
+      n_sum += c;
+      if (krow) {
+        o_sum += (a.old.ptr[r + 1] - a.old.ptr[r]) + (a.dl.ptr[r + 1] - a.dl.ptr[r]);
+        if (pend) o_sum += a.pend.ptr[r + 1] - a.pend.ptr[r];
+      }
+    }
+    // rows with too many candidates go to the wave kernel
+    const unsigned long long dm = __ballot(live && over);
+    if (dm) {
+      uint32_t dpos = 0;
+      if (lane == 0) dpos = atomicAdd(&a.ctr->deferred, uint32_t(__popcll(dm)));
+      dpos = uni_u32(dpos);
+      if (live && over) defer[dpos + uint32_t(__popcll(dm & ((1ull << lane) - 1)))] = r;
+    }
+  }
+  k_sum = wave_sum_u64(k_sum);
+  o_sum = wave_sum_u64(o_sum);
+  n_sum = wave_sum_u64(n_sum);
+  if (lane == 0) {
+    CtrSlot& cs = ctr_slot(a.ctr);
+    if (k_sum) atomicAdd(&cs.cand, k_sum);
+    if (o_sum) atomicAdd(&cs.oreads, o_sum);
+    if (n_sum) atomicAdd(&cs.newe, n_sum);
+  }
+}
+
 // PEND: the round reads pending edges beside old (a separate instance, so
 // rounds without them carry none of that code)
 template <int MODE, bool PEND = false>
@@ -1517,7 +1699,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
   // the first round's dedup (MODE 1) takes the active rows from active_base:
   // graphs joining a running session append their rows after the join's
   const uint32_t abase = MODE == 1 ? a.ctr->active_base : 0u;
-  const uint32_t nactive = a.ctr->active - abase;
+  const bool from_defer = MODE == 0 && a.defer != nullptr;
+  const uint32_t* rows_in = from_defer ? a.defer : a.active + abase;
+  const uint32_t nactive = from_defer ? a.ctr->deferred : a.ctr->active - abase;
   if (lane < 3) w.acc[lane] = 0;
   // rows are taken kGrab at a time and staging space kStageBlock words at a
   // time, so the two global counters see one atomic per many rows
@@ -1565,7 +1749,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
       if (!got) break;
     }
     const uint32_t idx = next++;
-    const uint32_t r = uni_u32(a.active[abase + idx]);
+    const uint32_t r = uni_u32(rows_in[idx]);
     GG_CLK(t_row);
     const uint32_t rn = MODE == 2 ? a.ro.np : a.n;
     const uint32_t A = r / rn, u = r - A * rn + (MODE == 2 ? a.ro.first : 0u);
@@ -2432,6 +2616,11 @@ struct Tuning {
   int merge_rows_bpc = 0;      // of k_merge_rows
   int gather_bpc = 0;          // of k_gather
   double group_mem_share = 0.4;  // batch: share of device memory a launch group plans for
+  // a round whose previous round joined at most light_cand candidates per
+  // active row (or the first round) runs the lane-per-row pass first
+  // (k_expand_light); 0: never
+  uint64_t light_cand = 32;
+  uint32_t light_cap = 32;  // candidates a lane holds in the light pass: 32 or 64
 };
 
 // Sets one parameter from its name; false for an unknown name.
@@ -2446,12 +2635,15 @@ bool set_tuning(Tuning& t, const std::string& name, const char* v) {
   else if (name == "merge_rows_bpc") t.merge_rows_bpc = std::atoi(v);
   else if (name == "gather_bpc") t.gather_bpc = std::atoi(v);
   else if (name == "group_mem_share") t.group_mem_share = std::atof(v);
+  else if (name == "light_cand") t.light_cand = std::strtoull(v, nullptr, 10);
+  else if (name == "light_cap") t.light_cap = uint32_t(std::strtoul(v, nullptr, 10));
   else return false;
   return true;
 }
 
 const char* const kTuningNames[] = {"grab_light", "grab_heavy",     "heavy_cand", "guess_min",      "defer_ratio",
-                                    "expand_bpc", "merge_bpc",      "merge_rows_bpc", "gather_bpc", "group_mem_share"};
+                                    "expand_bpc", "merge_bpc",      "merge_rows_bpc", "gather_bpc", "group_mem_share",
+                                    "light_cand", "light_cap"};
 
 // The tuning file beside this library (read once), then GG_TUNE_* variables.
 Tuning load_tuning() {
@@ -2637,6 +2829,7 @@ class Engine {
   Counters arena_h_ctr{};
   unsigned long long arena_h_u64 = 0;
   uint32_t expand_grid_cap = 0;  // 0: as many blocks as fit the device
+  uint64_t light_rows = 0;       // rows the current round's light pass expanded
   Tuning tune;                   // the tuned parameters of the current run
   int occ_expand = 4, occ_merge = 8, occ_merge_rows = 8, occ_gather = 7;  // blocks per CU the kernels allow
 
@@ -2965,6 +3158,11 @@ class Engine {
   // candidates and active rows of the last round, which choose it
   uint32_t grab = 4;
   uint64_t last_cand = 0, last_active = 0;
+  // the lane-per-row pass runs in the first round and after rounds with
+  // few candidates per active row
+  bool light_round() const {
+    return tune.light_cand && (last_active == 0 || last_cand <= tune.light_cand * last_active);
+  }
   void choose_grab() {
     grab = (last_active && last_cand / last_active > tune.heavy_cand) ? tune.grab_heavy : tune.grab_light;
   }
@@ -3219,6 +3417,7 @@ int Engine::plan_rounds(RoundPlan& rp, std::string& err) {
 // running session).
 int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t inject, std::string& err) {
   const uint32_t R = rows();
+  size_t ex0 = ev.expand.size();  // this round's expand event pairs start here
   uint64_t N = 0;
   // frontier pair lists for old_B x delta_C (old is empty in the first round)
   pairs_ready = false;
@@ -3280,30 +3479,77 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
     const uint32_t want = uint32_t((uint64_t(h_ctr->active) + grab * kWavesPerBlock - 1) / (grab * kWavesPerBlock));
     const uint32_t grid = std::max(1u, std::min(expand_grid(), want));
     GG_HIP(rec(o.b));
-    EventPair& x = ev.mk(ev.expand);
-    GG_HIP(rec(x.a));
-    if (pend_nnz)
-      k_expand<0, true><<<grid, kBlock, 0, stream>>>(expand_args(false));
-    else
-      k_expand<0><<<grid, kBlock, 0, stream>>>(expand_args(false));
+    ex0 = ev.expand.size();
+    // each launch gets its own event pair, so that the host's read-back
+    // between the light pass and the wave kernel is not timed as expand
+    EventPair* xp = &ev.mk(ev.expand);
+    GG_HIP(rec(xp->a));
+    light_rows = 0;
+    if (light_round()) {
+      // one lane per row first; the rows with more candidates than a lane
+      // holds are then expanded by the wave kernel
+      GG_TRY(grow(mlist, uint64_t(R) * 4, err));
+      ExpandArgs la = expand_args(false);
+      const uint32_t lg = grid_for(h_ctr->active);
+      uint32_t* dl = P<uint32_t>(mlist);
+      if (tune.light_cap > 32) {
+        if (pend_nnz)
+          k_expand_light<true, 64><<<lg, kBlock, 0, stream>>>(la, dl);
+        else
+          k_expand_light<false, 64><<<lg, kBlock, 0, stream>>>(la, dl);
+      } else if (pend_nnz) {
+        k_expand_light<true, 32><<<lg, kBlock, 0, stream>>>(la, dl);
+      } else {
+        k_expand_light<false, 32><<<lg, kBlock, 0, stream>>>(la, dl);
+      }
+      GG_HIP(rec(xp->b));
+      GG_HIP(hipGetLastError());
+      S.expand_launches++;
+      GG_TRY(read_ctr(err));
+      light_rows = h_ctr->active - h_ctr->deferred;
+      xp = nullptr;
+      if (h_ctr->deferred && !h_ctr->overflow) {
+        la.defer = P<uint32_t>(mlist);
+        const uint32_t dwant =
+            uint32_t((uint64_t(h_ctr->deferred) + grab * kWavesPerBlock - 1) / (grab * kWavesPerBlock));
+        const uint32_t dgrid = std::max(1u, std::min(expand_grid(), dwant));
+        xp = &ev.mk(ev.expand);
+        GG_HIP(rec(xp->a));
+        if (pend_nnz)
+          k_expand<0, true><<<dgrid, kBlock, 0, stream>>>(la);
+        else
+          k_expand<0><<<dgrid, kBlock, 0, stream>>>(la);
+        S.expand_launches++;
+      }
+    } else {
+      if (pend_nnz)
+        k_expand<0, true><<<grid, kBlock, 0, stream>>>(expand_args(false));
+      else
+        k_expand<0><<<grid, kBlock, 0, stream>>>(expand_args(false));
+      S.expand_launches++;
+    }
     if (inject) {
       // graphs joining a session: their raw rows go to the end of this
       // round's active list and are deduplicated there (their old, delta and
       // pending rows are empty, so the result is their first delta)
+      if (!xp) {
+        xp = &ev.mk(ev.expand);
+        GG_HIP(rec(xp->a));
+      }
       k_mark_active_base<<<1, kWave, 0, stream>>>(C());
       k_activate_raw<<<grid_for(R), kBlock, 0, stream>>>(P<unsigned long long>(raw_ptr), R, P<uint32_t>(active), C());
       k_expand<1><<<expand_grid(), kBlock, 0, stream>>>(expand_args(true));
       S.expand_launches++;
     }
-    GG_HIP(rec(x.b));
+    if (xp) GG_HIP(rec(xp->b));
     GG_HIP(hipGetLastError());
     GG_TRY(read_ctr(err));
-    S.expand_launches++;
     if (!h_ctr->overflow) break;
     S.retries++;
     GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
   }
   N = h_ctr->newe;
+  S.light_rows += light_rows;
   last_cand = h_ctr->cand;
   last_active = h_ctr->active;
   const uint32_t used_guess = guess_bits;
@@ -3318,7 +3564,10 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
   S.new_edges += N;
   if (verbose) {
     float xms = 0;
-    if (ev.expand.back().a) (void)hipEventElapsedTime(&xms, ev.expand.back().a, ev.expand.back().b);
+    for (size_t i = ex0; i < ev.expand.size(); ++i) {
+      float t = 0;
+      if (ev.expand[i].a && hipEventElapsedTime(&t, ev.expand[i].a, ev.expand[i].b) == hipSuccess) xms += t;
+    }
     std::fprintf(stderr,
                  "[gg] round %u: active %u cand %llu new %llu guess %u misses %u expand %.2f ms"
                  " (cycles: seeding %llu lists %llu streams %llu rows %llu)\n",

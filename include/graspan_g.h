@@ -80,6 +80,7 @@ typedef struct gg_stats {
   uint32_t uploads_ahead;   /* out-of-core: partitions uploaded during the step before their first use */
   uint32_t spills_ahead;    /* out-of-core: partitions copied out during a step to make room for the next */
   uint32_t pad0;
+  uint64_t light_rows;      /* rows expanded one per lane (rounds with few candidates per row) */
 } gg_stats;
 
 /* ---- context ------------------------------------------------------------ */
