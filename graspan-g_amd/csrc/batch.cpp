@@ -324,7 +324,9 @@ int gg_batch_add(gg_batch* b, int num_rules, const int32_t* rules, int num_label
     if (lab[i] < 1 || lab[i] > num_labels) return fail(b, GG_E_ARG, "edge " + std::to_string(i) + ": bad label");
   }
   gg_batch::G g;
-  g.rules.assign(rules, rules + 3 * num_rules);
+  // the graph is closed to its least fixpoint: its rules as planned for it
+  // (eps rules stay, and become loop edges when the group is built)
+  g.rules = gg::plan_fixpoint_rules(rules, num_rules, gg::kMaxRulesPerKind);
   g.L = num_labels;
   g.n = n;
   g.src.assign(src, src + m);

@@ -2774,6 +2774,21 @@ class Engine {
   int L = 0;
   std::vector<uint8_t> lhs;  // labels that are the lhs of a unary/binary rule
   bool has_grammar = false;
+  // The grammar's rule table as given (the partitioned schedule runs it)
+  // and as planned for the least fixpoint (plan_fixpoint_rules: same
+  // closure, fewer join candidates). A run loads one of them into g, lhs
+  // and pack_labels before it starts.
+  struct RuleSet {
+    DevGrammar g{};
+    std::vector<uint8_t> lhs;
+    uint32_t pack[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  };
+  RuleSet rules_given, rules_planned;
+  void use_rules(const RuleSet& r) {
+    g = r.g;
+    lhs = r.lhs;
+    for (int i = 0; i < 8; ++i) pack_labels[i] = r.pack[i];
+  }
 
   uint32_t n = 0;
   uint64_t m = 0;
@@ -3196,6 +3211,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     err = "the semi-naive engine computes the least fixpoint only; filter_flag != 0 runs the partitioned schedule";
     return GG_E_ARG;
   }
+  use_rules(rules_planned);
   pend_nnz = 0;
   apply_tuning();
   defer_ratio = tune.defer_ratio;
@@ -3653,6 +3669,9 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     err = "grammar and graph must be loaded before gg_closure_run";
     return GG_E_STATE;
   }
+  // the reference's schedule runs the rule table as given: with filter != 0
+  // its result depends on the rules, not only on their least fixpoint
+  use_rules(rules_given);
   // row ids (label * vertices + vertex) are uint32 in every region and in
   // the assembled closure, which has all n vertices
   if (uint64_t(L) * n >= 0xFFFFFFFFull) {
@@ -4486,10 +4505,14 @@ Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std
 
 void engine_destroy(Engine* e) { delete e; }
 
-int engine_set_grammar(Engine* e, const Grammar& gr, std::string& err) {
+namespace {
+// Device form of a rule table: rules by kind, pair-list slots, the labels
+// that are the lhs of a unary or binary rule, and the labels kept packed.
+int build_rule_set(const std::vector<std::array<int32_t, 3>>& rules, int num_labels, Engine::RuleSet& rs,
+                   std::string& err) {
   DevGrammar g{};
-  std::vector<bool> is_lhs(size_t(gr.num_labels()) + 1, false);
-  for (auto& r : gr.rules) {
+  std::vector<bool> is_lhs(size_t(num_labels) + 1, false);
+  for (auto& r : rules) {
     if (r[1] == 0) {
       if (g.ne >= kMaxRules) {
         err = "more than 64 eps rules";
@@ -4524,19 +4547,35 @@ int engine_set_grammar(Engine* e, const Grammar& gr, std::string& err) {
     g.slot_b[g.nslot] = uint8_t(B - 1);
     g.b_slot[i] = uint8_t(g.nslot++);
   }
-  for (int i = 0; i < 8; ++i) e->pack_labels[i] = 0;
-  auto mark_pack = [&](int l0) { e->pack_labels[l0 >> 5] |= 1u << (l0 & 31); };
+  for (int i = 0; i < 8; ++i) rs.pack[i] = 0;
+  auto mark_pack = [&](int l0) { rs.pack[l0 >> 5] |= 1u << (l0 & 31); };
   for (int i = 0; i < g.nb; ++i) mark_pack(g.b_rhs2[i]);
-  for (int l = 1; l <= gr.num_labels(); ++l)
+  for (int l = 1; l <= num_labels; ++l)
     if (is_lhs[l]) mark_pack(l - 1);
   if (const char* np = std::getenv("GG_NO_PACK"))
     if (np[0] == '1')
-      for (int i = 0; i < 8; ++i) e->pack_labels[i] = 0;
-  e->g = g;
+      for (int i = 0; i < 8; ++i) rs.pack[i] = 0;
+  rs.g = g;
+  rs.lhs.clear();
+  for (int l = 1; l <= num_labels; ++l)
+    if (is_lhs[l]) rs.lhs.push_back(uint8_t(l - 1));
+  return GG_OK;
+}
+}  // namespace
+
+int engine_set_grammar(Engine* e, const Grammar& gr, std::string& err) {
+  Engine::RuleSet given, planned;
+  GG_TRY(build_rule_set(gr.rules, gr.num_labels(), given, err));
+  std::vector<int32_t> flat;
+  for (auto& r : gr.rules) flat.insert(flat.end(), r.begin(), r.end());
+  const std::vector<int32_t> pl = plan_fixpoint_rules(flat.data(), int(gr.rules.size()), kMaxRules);
+  std::vector<std::array<int32_t, 3>> prules;
+  for (size_t i = 0; i + 2 < pl.size(); i += 3) prules.push_back({pl[i], pl[i + 1], pl[i + 2]});
+  GG_TRY(build_rule_set(prules, gr.num_labels(), planned, err));
+  e->rules_given = given;
+  e->rules_planned = planned;
+  e->use_rules(e->rules_planned);
   e->L = gr.num_labels();
-  e->lhs.clear();
-  for (int l = 1; l <= e->L; ++l)
-    if (is_lhs[l]) e->lhs.push_back(uint8_t(l - 1));
   e->has_grammar = true;
   e->has_closure = false;
   return GG_OK;

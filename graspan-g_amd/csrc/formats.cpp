@@ -17,6 +17,7 @@
 //   * more than 21 rules is accepted (the reference exits, src/kernel.cu:24-27).
 #include "formats.h"
 
+#include <algorithm>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -180,6 +181,72 @@ int grammar_from_rules(int num_rules, const int32_t* rules, int num_labels, Gram
     g.rules.push_back({a, b, c});
   }
   return GG_OK;
+}
+
+namespace {
+// Generators of the star label y (see plan_fixpoint_rules), or empty when y
+// is not one.
+std::vector<int32_t> star_generators(const int32_t* rules, int num_rules, int32_t y) {
+  bool eps = false, left = true, right = true;
+  std::vector<int32_t> left_gen, right_gen;
+  for (int i = 0; i < num_rules; ++i) {
+    const int32_t* r = rules + 3 * i;
+    if (r[0] != y) continue;
+    if (r[1] == 0) {
+      eps = true;
+      continue;
+    }
+    if (r[2] == 0) return {};  // a unary rule for y
+    left = left && r[1] == y && r[2] != y;
+    right = right && r[2] == y && r[1] != y;
+    left_gen.push_back(r[2]);
+    right_gen.push_back(r[1]);
+  }
+  std::vector<int32_t>& gen = left ? left_gen : right_gen;
+  if (!eps || gen.empty() || !(left || right)) return {};
+  std::sort(gen.begin(), gen.end());
+  gen.erase(std::unique(gen.begin(), gen.end()), gen.end());
+  return gen;
+}
+}  // namespace
+
+std::vector<int32_t> plan_fixpoint_rules(const int32_t* rules, int num_rules, int max_per_kind) {
+  std::vector<int32_t> out(rules, rules + 3 * std::max(num_rules, 0));
+  const char* off = std::getenv("GG_STAR_REWRITE");
+  if (off && off[0] == '0') return out;
+  int unary = 0, binary = 0;
+  for (int i = 0; i < num_rules; ++i)
+    if (rules[3 * i + 1] != 0) ++(rules[3 * i + 2] ? binary : unary);
+  out.clear();
+  for (int i = 0; i < num_rules; ++i) {
+    const int32_t x = rules[3 * i], b = rules[3 * i + 1], c = rules[3 * i + 2];
+    bool done = false;
+    if (b != 0 && c != 0 && x != b && x != c) {
+      int own = 0;
+      for (int k = 0; k < num_rules; ++k) own += rules[3 * k] == x;
+      const std::vector<int32_t> gb = own == 1 ? star_generators(rules, num_rules, b) : std::vector<int32_t>{};
+      const std::vector<int32_t> gc =
+          own == 1 && gb.empty() ? star_generators(rules, num_rules, c) : std::vector<int32_t>{};
+      const std::vector<int32_t>& gen = gb.empty() ? gc : gb;
+      const bool ok = !gen.empty() && std::find(gen.begin(), gen.end(), x) == gen.end() &&
+                      unary + 1 <= max_per_kind && binary - 1 + int(gen.size()) <= max_per_kind;
+      if (ok) {
+        const bool left = !gb.empty();
+        out.insert(out.end(), {x, left ? c : b, 0});  // X ::= C  (or X ::= B)
+        for (int32_t p : gen) {
+          if (left)
+            out.insert(out.end(), {x, p, x});  // X ::= P_k X
+          else
+            out.insert(out.end(), {x, x, p});  // X ::= X P_k
+        }
+        unary += 1;
+        binary += int(gen.size()) - 1;
+        done = true;
+      }
+    }
+    if (!done) out.insert(out.end(), {x, b, c});
+  }
+  return out;
 }
 
 int parse_graph_file(const char* path, const Grammar& g, Graph& out, std::string& err) {

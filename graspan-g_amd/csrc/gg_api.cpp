@@ -234,6 +234,19 @@ int gg_grammar_rules(const gg_ctx* c, int32_t* out) {
   return GG_OK;
 }
 
+int gg_grammar_plan_fixpoint(int num_rules, const int32_t* rules, int num_labels, int32_t* out, int cap,
+                             int* out_rules) {
+  gg::Grammar tmp;
+  std::string err;
+  if (!out_rules || (cap > 0 && !out) || gg::grammar_from_rules(num_rules, rules, num_labels, tmp, err) != GG_OK)
+    return GG_E_ARG;
+  const std::vector<int32_t> p = gg::plan_fixpoint_rules(rules, num_rules, gg::kMaxRulesPerKind);
+  *out_rules = int(p.size() / 3);
+  if (*out_rules > cap) return GG_E_ARG;
+  std::copy(p.begin(), p.end(), out);
+  return GG_OK;
+}
+
 const char* gg_label_name(const gg_ctx* c, int id) {
   if (!c || !c->has_grammar || id < 1 || id > c->grammar.num_labels()) return nullptr;
   return c->grammar.names[size_t(id - 1)].c_str();

@@ -449,7 +449,9 @@ int gg_server_submit(gg_server* s, int num_rules, const int32_t* rules, int num_
   for (uint64_t i = 0; i < m; ++i)
     if (src[i] >= n || dst[i] >= n || lab[i] < 1 || lab[i] > num_labels) return GG_E_ARG;
   auto* r = new Request();
-  r->key = {num_labels, std::vector<int32_t>(rules, rules + 3 * num_rules)};
+  // requests are closed to their least fixpoint: their rules as planned for
+  // it (eps rules stay, and become loop edges when the request is admitted)
+  r->key = {num_labels, gg::plan_fixpoint_rules(rules, num_rules, gg::kMaxRulesPerKind)};
   r->n = n;
   r->src.assign(src, src + m);
   r->dst.assign(dst, dst + m);

@@ -29,8 +29,8 @@ _STATUS = {GG_E_ARG: "GG_E_ARG", GG_E_IO: "GG_E_IO", GG_E_FORMAT: "GG_E_FORMAT",
 # Every function declared in include/graspan_g.h.
 EXPORTED = (
     "gg_ctx_create", "gg_ctx_destroy", "gg_last_error", "gg_version",
-    "gg_grammar_load_file", "gg_grammar_set", "gg_grammar_info", "gg_grammar_rules", "gg_label_name", "gg_label_id",
-    "gg_graph_load_file", "gg_graph_set_coo", "gg_graph_info", "gg_graph_edges",
+    "gg_grammar_load_file", "gg_grammar_set", "gg_grammar_info", "gg_grammar_rules", "gg_grammar_plan_fixpoint", "gg_label_name",
+    "gg_label_id", "gg_graph_load_file", "gg_graph_set_coo", "gg_graph_info", "gg_graph_edges",
     "gg_closure_run", "gg_closure_run_partitioned", "gg_closure_run_out_of_core", "gg_reference_partitions", "gg_closure_counts", "gg_closure_export", "gg_closure_degrees", "gg_closure_checksum",
     "gg_triple_hash", "gg_closure_release", "gg_closure_run_noalloc", "gg_device_info",
     "gg_batch_create", "gg_batch_destroy", "gg_batch_add", "gg_batch_run", "gg_batch_info", "gg_batch_result",
@@ -88,6 +88,7 @@ def lib():
             "gg_version": (ctypes.c_char_p, []),
             "gg_grammar_load_file": (I, [P, ctypes.c_char_p]), "gg_grammar_set": (I, [P, I, P, I]),
             "gg_grammar_info": (I, [P, P, P]), "gg_grammar_rules": (I, [P, P]),
+            "gg_grammar_plan_fixpoint": (I, [I, P, I, P, I, P]),
             "gg_label_name": (ctypes.c_char_p, [P, I]), "gg_label_id": (I, [P, ctypes.c_char_p]),
             "gg_graph_load_file": (I, [P, ctypes.c_char_p]), "gg_graph_set_coo": (I, [P, U32, U64, P, P, P]),
             "gg_graph_info": (I, [P, P, P]), "gg_graph_edges": (I, [P, P, P, P]),
@@ -123,6 +124,19 @@ def _ptr(a: np.ndarray):
 
 def triple_hash(s: int, d: int, l: int) -> int:
     return int(lib().gg_triple_hash(s, d, l))
+
+
+def plan_fixpoint_rules(rules, num_labels: int) -> np.ndarray:
+    """The rule triples the least-fixpoint engine evaluates for `rules`
+    (gg_grammar_plan_fixpoint; host only): int32 [R', 3]."""
+    rules = np.ascontiguousarray(rules, dtype=np.int32).reshape(-1, 3)
+    cap = 4 * rules.shape[0] + 64
+    out = np.zeros((cap, 3), dtype=np.int32)
+    cnt = ctypes.c_int()
+    st = lib().gg_grammar_plan_fixpoint(rules.shape[0], _ptr(rules), num_labels, _ptr(out), cap, ctypes.byref(cnt))
+    if st != GG_OK:
+        raise GGError(st, "gg_grammar_plan_fixpoint")
+    return out[:cnt.value].copy()
 
 
 @dataclass

@@ -107,6 +107,22 @@ int gg_grammar_set(gg_ctx *ctx, int num_rules, const int32_t *rules, int num_lab
 int gg_grammar_info(const gg_ctx *ctx, int *num_rules, int *num_labels);
 /* Writes the rule triples (3 * num_rules int32). */
 int gg_grammar_rules(const gg_ctx *ctx, int32_t *rules_out);
+/* The rule triples the least-fixpoint engine evaluates for a rule table (host
+ * only; no GPU). No counterpart in the reference, which evaluates its rule
+ * table as given (src/kernel.cu:403-464). A label Y whose rules are exactly
+ * Y ::= eps and Y ::= Y P_k (or all Y ::= P_k Y) is P* for P = the union of
+ * its P_k; a binary rule X ::= Y C whose lhs has no other rule is evaluated
+ * as X ::= C plus X ::= P_k X (X ::= B Y as X ::= B plus X ::= X P_k). Every
+ * label keeps its least fixpoint (Arden's lemma), with far fewer join
+ * candidates when Y is dense and P sparse. gg_closure_run (filter 0), the
+ * batched closure, the serving loop and the allocation-free closure use it;
+ * the partitioned and out-of-core schedules run the table as given.
+ * GG_STAR_REWRITE=0 in the environment turns it off. Writes up to cap
+ * triples to out (3 * cap int32); *out_rules receives their number
+ * (GG_E_ARG, nothing written, when cap is smaller). */
+int gg_grammar_plan_fixpoint(int num_rules, const int32_t *rules, int num_labels, int32_t *out, int cap,
+                             int *out_rules);
+
 /* Name of label id (1..L); NULL if out of range. */
 const char *gg_label_name(const gg_ctx *ctx, int label_id);
 /* Id of a label name, 0 if unknown. */
