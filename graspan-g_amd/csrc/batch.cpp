@@ -14,6 +14,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -45,6 +47,15 @@ struct gg_batch {
   std::vector<G> graphs;
   int groups = 0;
   std::string err;
+  // the two lanes' engines (GG_BATCH_LANES=2) and the device region they
+  // carve their buffers from: made by the first run, kept for the next run,
+  // freed with the batch
+  std::vector<gg::Engine*> lanes;
+  std::shared_ptr<gg::DeviceRegion> region;
+  ~gg_batch() {
+    for (auto* e : lanes) gg::engine_destroy(e);
+    region.reset();
+  }
 };
 
 namespace {
@@ -74,59 +85,48 @@ void add_stats(gg_stats& a, const gg_stats& b) {
   a.light_rows += b.light_rows;
 }
 
-int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std::string*& cerr) {
-  int st = GG_OK;
-  gg::Engine* e = gg::ctx_engine(b->ctx, st, cerr);
-  if (!e) return fail(b, st, cerr ? *cerr : "no device");
-  gg_stats total{};
-  b->groups = 0;
-  const size_t G = b->graphs.size();
-  // Graphs are ordered by grammar before they are cut into launch groups: a
-  // group's rows are its vertices times all its labels, so a group that
-  // mixes fewer grammars carries fewer empty rows through every round.
-  std::vector<size_t> ord(G);
-  for (size_t k = 0; k < G; ++k) ord[k] = k;
-  std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
-    const auto& gx = b->graphs[x];
-    const auto& gy = b->graphs[y];
-    return std::make_pair(gx.L, gx.rules) < std::make_pair(gy.L, gy.rules);
-  });
-  // A group that runs out of device memory is cut in half and run again
-  // (its closure size was not known in advance); later groups keep the
-  // smaller graph count.
-  size_t cap = G;
-  // With max_vertices == 0 a group's vertex count is planned from device
-  // memory: a group may use a tuned share of it (0.4 by default), at the peak device bytes
-  // per vertex measured on the last group of the same grammar (a grammar not
-  // yet seen starts with kFirstGroupVertices). Groups that run close to the
-  // device's limit spend their time re-allocating instead of computing.
-  const double kGroupMemShare = gg::tuned_group_mem_share();
-  constexpr uint64_t kFirstGroupVertices = 2u << 20;
-  const uint64_t mem = max_vertices ? 0 : gg::ctx_memory(b->ctx);
+// Launch groups are planned one at a time, each when a lane is free to
+// build it, so that the vertex cap of a group uses the device bytes per
+// vertex of the groups that have finished. With two lanes (each an engine
+// with its own HIP stream) one lane's fixpoint runs while the other reads
+// back its last group's results, builds its next group on the host and
+// uploads it; the fixpoints themselves take turns (one device-wide launch
+// sequence already fills the GPU).
+struct Planner {
+  gg_batch* b;
+  uint64_t max_vertices;
+  uint64_t mem;        // device bytes a group may plan against (0: use kFirstGroupVertices)
+  double share;        // of mem, per group
+  std::vector<size_t> ord;
+  size_t next = 0;     // first graph (in ord) not yet planned
+  size_t cap;          // graphs per group at most (halved after a group ran out of memory)
+  std::vector<std::pair<size_t, size_t>> redo;  // halves of groups that ran out of memory
   std::map<std::pair<int, std::vector<int32_t>>, double> bytes_per_vertex;
-  auto vertex_cap = [&](const std::pair<int, std::vector<int32_t>>& key) -> uint64_t {
+  static constexpr uint64_t kFirstGroupVertices = 2u << 20;
+
+  uint64_t vertex_cap(const std::pair<int, std::vector<int32_t>>& key) const {
     if (max_vertices) return max_vertices;
     auto it = bytes_per_vertex.find(key);
     if (it == bytes_per_vertex.end() || !mem) return kFirstGroupVertices;
-    return std::max<uint64_t>(1, uint64_t(double(mem) * kGroupMemShare / std::max(it->second, 1.0)));
-  };
-  // GG_BATCH_TRACE=1: one stderr line per launch group (sizes and the host
-  // time of each phase)
-  const char* tr = std::getenv("GG_BATCH_TRACE");
-  const bool trace = tr && tr[0] == '1';
-  auto now = [] { return std::chrono::steady_clock::now(); };
-  auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
-    return std::chrono::duration<double, std::milli>(b - a).count();
-  };
-  size_t i = 0;
-  while (i < G) {
-    const auto t_start = now();
-    // one launch group: graphs ord[i..j-1]
-    // A group's combined grammar must stay within the engine's limits: at
-    // most kMaxLabels labels, kMaxRulesPerKind unary and binary rules (eps
-    // rules become loops), and labels * vertices and pair slots * vertices
-    // below 2^32. A grammar that would pass one of them starts a new group.
-    std::map<std::pair<int, std::vector<int32_t>>, uint32_t> grammar_base;
+    return std::max<uint64_t>(1, uint64_t(double(mem) * share / std::max(it->second, 1.0)));
+  }
+
+  // The next group [i, j) of ord; false when every graph is planned. A
+  // group's combined grammar must stay within the engine's limits: at most
+  // kMaxLabels labels, kMaxRulesPerKind unary and binary rules (eps rules
+  // become loops), and labels * vertices and pair slots * vertices below
+  // 2^32. A grammar that would pass one of them starts a new group.
+  int take(size_t& i0, size_t& j0, std::string& err) {
+    if (!redo.empty()) {
+      i0 = redo.back().first;
+      j0 = redo.back().second;
+      redo.pop_back();
+      return 1;
+    }
+    const size_t G = ord.size();
+    if (next >= G) return 0;
+    const size_t i = next;
+    std::map<std::pair<int, std::vector<int32_t>>, uint32_t> seen;
     uint32_t labels = 0;
     int unary = 0, binary = 0, slots = 0;
     uint64_t verts = 0;
@@ -135,7 +135,7 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
     for (; j < G && j - i < cap; ++j) {
       const auto& g = b->graphs[ord[j]];
       auto key = std::make_pair(g.L, g.rules);
-      const bool fresh = grammar_base.find(key) == grammar_base.end();
+      const bool fresh = seen.find(key) == seen.end();
       gg::RuleKinds k{};
       if (fresh) k = gg::rule_kinds(g.rules.data(), int(g.rules.size() / 3));
       const uint32_t nl = labels + (fresh ? uint32_t(g.L) : 0u);
@@ -147,10 +147,13 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
       const bool hard = nl > uint32_t(gg::kMaxLabels) || nu > gg::kMaxRulesPerKind || nb > gg::kMaxRulesPerKind ||
                         nv > lim || nv > slim;
       if ((hard || nv > ncap) && j > i) break;  // a graph over the vertex cap alone still runs alone
-      if (hard) return fail(b, GG_E_UNSUPPORTED, "graph " + std::to_string(ord[j]) + " alone exceeds the launch limits");
+      if (hard) {
+        err = "graph " + std::to_string(ord[j]) + " alone exceeds the launch limits";
+        return -1;
+      }
       if (fresh) {
         vcap = ncap;
-        grammar_base[key] = labels;
+        seen[key] = labels;
         labels = nl;
         unary = nu;
         binary = nb;
@@ -158,137 +161,296 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
       }
       verts = nv;
     }
-    // combined grammar
-    std::vector<int32_t> rules;
-    for (auto& kv : grammar_base) {
-      const auto& r = kv.first.second;
-      for (size_t k = 0; k + 2 < r.size(); k += 3) {
-        if (r[k + 1] == 0) continue;  // eps: explicit loops below
-        rules.push_back(r[k] + int32_t(kv.second));
-        rules.push_back(r[k + 1] + int32_t(kv.second));
-        rules.push_back(r[k + 2] ? r[k + 2] + int32_t(kv.second) : 0);
-      }
-    }
-    gg::Grammar cg;
-    int s = gg::grammar_from_rules(int(rules.size() / 3), rules.data(), int(std::max(labels, 1u)), cg, b->err);
-    if (s != GG_OK) return s;
-    // super graph: every graph's edges shifted to its vertex range, plus the
-    // eps loops its grammar asks for (sized first, then filled in place)
-    gg::Graph sg;
-    sg.n = uint32_t(verts);
-    uint64_t total_m = 0;
-    for (size_t k = i; k < j; ++k) {
-      const auto& g = b->graphs[ord[k]];
-      total_m += g.src.size();
-      for (size_t r = 0; r + 2 < g.rules.size(); r += 3)
-        if (g.rules[r + 1] == 0) total_m += g.n;
-    }
-    sg.src.resize(total_m);
-    sg.dst.resize(total_m);
-    sg.lab.resize(total_m);
-    // each graph's slice of the union: its vertex base, label base and first
-    // position, then the slices are filled by a few host threads
-    std::vector<uint32_t> v0(1, 0), l0, lg;
-    std::vector<uint64_t> at0;
-    uint64_t at = 0;
-    for (size_t k = i; k < j; ++k) {
-      const auto& g = b->graphs[ord[k]];
-      const uint32_t lb = grammar_base[std::make_pair(g.L, g.rules)];
-      at0.push_back(at);
-      at += g.src.size();
-      for (size_t r = 0; r + 2 < g.rules.size(); r += 3)
-        if (g.rules[r + 1] == 0) at += g.n;
-      l0.push_back(lb);
-      lg.push_back(uint32_t(g.L));
-      v0.push_back(v0.back() + g.n);
-    }
-    auto fill = [&](size_t k0, size_t k1) {
-      for (size_t k = k0; k < k1; ++k) {
-        const auto& g = b->graphs[ord[i + k]];
-        const uint32_t base = v0[k], lb = l0[k];
-        uint64_t p = at0[k];
-        const size_t m = g.src.size();
-        uint32_t* ps = sg.src.data() + p;
-        uint32_t* pd = sg.dst.data() + p;
-        uint8_t* pl = sg.lab.data() + p;
-        for (size_t e2 = 0; e2 < m; ++e2) {
-          ps[e2] = g.src[e2] + base;
-          pd[e2] = g.dst[e2] + base;
-          pl[e2] = uint8_t(g.lab[e2] + lb);
-        }
-        p += m;
-        for (size_t r = 0; r + 2 < g.rules.size(); r += 3) {
-          if (g.rules[r + 1] != 0) continue;
-          const uint8_t el = uint8_t(g.rules[r] + int32_t(lb));
-          for (uint32_t v = 0; v < g.n; ++v) {
-            sg.src[p + v] = base + v;
-            sg.dst[p + v] = base + v;
-            sg.lab[p + v] = el;
-          }
-          p += g.n;
-        }
-      }
-    };
-    const size_t ng = j - i;
-    const size_t nt = std::min<size_t>({ng, 16, std::max(1u, std::thread::hardware_concurrency())});
-    if (nt <= 1 || total_m < (1u << 20)) {
-      fill(0, ng);
-    } else {
-      std::vector<std::thread> pool;
-      for (size_t t = 0; t < nt; ++t) pool.emplace_back(fill, ng * t / nt, ng * (t + 1) / nt);
-      for (auto& th : pool) th.join();
-    }
-    const auto t_built = now();
-    if ((s = gg::engine_set_grammar(e, cg, b->err)) != GG_OK) return s;
-    const uint64_t group_m = sg.src.size();
-    s = gg::engine_set_graph(e, std::move(sg), b->err);
-    const auto t_up = now();
-    gg_stats one{};
-    if (s == GG_OK) s = gg::engine_run(e, 0, &one, b->err);
-    const auto t_run = now();
-    if (s == GG_E_NOMEM && j - i > 1) {
-      if (trace)
-        std::fprintf(stderr, "[gg-batch] group of %zu graphs, %llu vertices: out of device memory after %.0f ms; "
-                     "splitting\n", j - i, (unsigned long long)verts, ms(t_start, t_run));
-      gg::engine_release(e);
-      cap = (j - i) / 2;
-      continue;
-    }
-    if (s != GG_OK) return s;
-    add_stats(total, one);
-    for (auto& kv : grammar_base) bytes_per_vertex[kv.first] = double(one.device_bytes_peak) / double(verts);
-    std::vector<uint64_t> counts, sums;
-    if ((s = gg::engine_slices(e, v0, l0, lg, counts, sums, b->err)) != GG_OK) return s;
-    if (trace)
-      std::fprintf(stderr,
-                   "[gg-batch] group %d: %zu graphs, %llu vertices, %u labels, %llu edges -> %llu closure edges, "
-                   "%u rounds; build %.0f ms, upload %.0f ms, run %.0f ms (fixpoint %.0f: expand %.0f merge %.0f "
-                   "compact %.0f other %.0f), results %.0f ms, device peak %.1f GB\n",
-                   b->groups, j - i, (unsigned long long)verts, labels, (unsigned long long)group_m,
-                   (unsigned long long)one.total_edges, one.rounds, ms(t_start, t_built), ms(t_built, t_up),
-                   ms(t_up, t_run), one.fixpoint_ms, one.expand_ms, one.merge_ms, one.compact_ms, one.other_ms,
-                   ms(t_run, now()), double(one.device_bytes_peak) / 1e9);
-    size_t c = 0;
-    for (size_t k = i; k < j; ++k) {
-      auto& g = b->graphs[ord[k]];
-      g.counts.assign(counts.begin() + long(c), counts.begin() + long(c + size_t(g.L)));
-      c += size_t(g.L);
-      g.total = 0;
-      for (uint64_t x : g.counts) g.total += x;
-      g.checksum = sums[k - i];
-      g.triples.clear();
-      if (keep) {
-        g.triples.resize(size_t(g.total) * 3);
-        uint64_t cnt = 0;
-        if ((s = gg::engine_export_slice(e, v0[k - i], v0[k - i + 1], l0[k - i], lg[k - i], g.triples.data(), g.total,
-                                         &cnt, b->err)) != GG_OK)
-          return s;
-      }
-      g.done = true;
-    }
-    ++b->groups;
-    i = j;
+    next = j;
+    i0 = i;
+    j0 = j;
+    return 1;
   }
+};
+
+// Builds, uploads, closes and reads back the launch group ord[i..j) on one
+// engine. GG_E_NOMEM is returned to the caller (which splits the group).
+int run_group(gg_batch* b, gg::Engine* e, const std::vector<size_t>& ord, size_t i, size_t j, int keep,
+              gg_stats& one, double& bytes_per_vertex, bool trace, int lane, std::mutex* turn, std::string& err) {
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point c) {
+    return std::chrono::duration<double, std::milli>(c - a).count();
+  };
+  const auto t_start = now();
+  // combined grammar: each distinct grammar (in order of first appearance in
+  // the group) gets the next range of label ids
+  std::map<std::pair<int, std::vector<int32_t>>, uint32_t> grammar_base;
+  std::vector<std::pair<int, std::vector<int32_t>>> order;
+  uint32_t labels = 0;
+  uint64_t verts = 0;
+  for (size_t k = i; k < j; ++k) {
+    const auto& g = b->graphs[ord[k]];
+    auto key = std::make_pair(g.L, g.rules);
+    if (grammar_base.emplace(key, labels).second) {
+      labels += uint32_t(g.L);
+      order.push_back(key);
+    }
+    verts += g.n;
+  }
+  std::vector<int32_t> rules;
+  for (auto& key : order) {
+    const uint32_t lb = grammar_base[key];
+    const auto& r = key.second;
+    for (size_t k = 0; k + 2 < r.size(); k += 3) {
+      if (r[k + 1] == 0) continue;  // eps: explicit loops below
+      rules.push_back(r[k] + int32_t(lb));
+      rules.push_back(r[k + 1] + int32_t(lb));
+      rules.push_back(r[k + 2] ? r[k + 2] + int32_t(lb) : 0);
+    }
+  }
+  gg::Grammar cg;
+  int s = gg::grammar_from_rules(int(rules.size() / 3), rules.data(), int(std::max(labels, 1u)), cg, err);
+  if (s != GG_OK) return s;
+  // super graph: every graph's edges shifted to its vertex range, plus the
+  // eps loops its grammar asks for (sized first, then filled in place)
+  gg::Graph sg;
+  sg.n = uint32_t(verts);
+  // each graph's slice of the union: its vertex base, label base and first
+  // position, then the slices are filled by a few host threads
+  std::vector<uint32_t> v0(1, 0), l0, lg;
+  std::vector<uint64_t> at0;
+  uint64_t at = 0;
+  for (size_t k = i; k < j; ++k) {
+    const auto& g = b->graphs[ord[k]];
+    at0.push_back(at);
+    at += g.src.size();
+    for (size_t r = 0; r + 2 < g.rules.size(); r += 3)
+      if (g.rules[r + 1] == 0) at += g.n;
+    l0.push_back(grammar_base[std::make_pair(g.L, g.rules)]);
+    lg.push_back(uint32_t(g.L));
+    v0.push_back(v0.back() + g.n);
+  }
+  sg.src.resize(at);
+  sg.dst.resize(at);
+  sg.lab.resize(at);
+  auto fill = [&](size_t k0, size_t k1) {
+    for (size_t k = k0; k < k1; ++k) {
+      const auto& g = b->graphs[ord[i + k]];
+      const uint32_t base = v0[k], lb = l0[k];
+      uint64_t p = at0[k];
+      const size_t m = g.src.size();
+      uint32_t* ps = sg.src.data() + p;
+      uint32_t* pd = sg.dst.data() + p;
+      uint8_t* pl = sg.lab.data() + p;
+      for (size_t e2 = 0; e2 < m; ++e2) {
+        ps[e2] = g.src[e2] + base;
+        pd[e2] = g.dst[e2] + base;
+        pl[e2] = uint8_t(g.lab[e2] + lb);
+      }
+      p += m;
+      for (size_t r = 0; r + 2 < g.rules.size(); r += 3) {
+        if (g.rules[r + 1] != 0) continue;
+        const uint8_t el = uint8_t(g.rules[r] + int32_t(lb));
+        for (uint32_t v = 0; v < g.n; ++v) {
+          sg.src[p + v] = base + v;
+          sg.dst[p + v] = base + v;
+          sg.lab[p + v] = el;
+        }
+        p += g.n;
+      }
+    }
+  };
+  const size_t ng = j - i;
+  const size_t nt = std::min<size_t>({ng, 8, std::max(1u, std::thread::hardware_concurrency())});
+  if (nt <= 1 || at < (1u << 20)) {
+    fill(0, ng);
+  } else {
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nt; ++t) pool.emplace_back(fill, ng * t / nt, ng * (t + 1) / nt);
+    for (auto& th : pool) th.join();
+  }
+  const auto t_built = now();
+  if ((s = gg::engine_set_grammar(e, cg, err)) != GG_OK) return s;
+  const uint64_t group_m = sg.src.size();
+  s = gg::engine_set_graph(e, std::move(sg), err);
+  const auto t_up = now();
+  one = gg_stats{};
+  // lanes take turns at the fixpoint; reading results, building and
+  // uploading overlap the other lane's turn
+  std::unique_lock<std::mutex> lk;
+  if (turn && s == GG_OK) lk = std::unique_lock<std::mutex>(*turn);
+  const auto t_turn = now();
+  if (s == GG_OK) s = gg::engine_run(e, 0, &one, err);
+  if (lk.owns_lock()) lk.unlock();
+  const auto t_run = now();
+  if (s != GG_OK) {
+    if (trace && s == GG_E_NOMEM)
+      std::fprintf(stderr, "[gg-batch] lane %d: group of %zu graphs, %llu vertices: out of device memory after %.0f ms\n",
+                   lane, j - i, (unsigned long long)verts, ms(t_start, t_run));
+    gg::engine_release(e);
+    return s;
+  }
+  bytes_per_vertex = double(one.device_bytes_peak) / double(verts);
+  std::vector<uint64_t> counts, sums;
+  if ((s = gg::engine_slices(e, v0, l0, lg, counts, sums, err)) != GG_OK) return s;
+  size_t c = 0;
+  for (size_t k = i; k < j; ++k) {
+    auto& g = b->graphs[ord[k]];
+    g.counts.assign(counts.begin() + long(c), counts.begin() + long(c + size_t(g.L)));
+    c += size_t(g.L);
+    g.total = 0;
+    for (uint64_t x : g.counts) g.total += x;
+    g.checksum = sums[k - i];
+    g.triples.clear();
+    if (keep) {
+      g.triples.resize(size_t(g.total) * 3);
+      uint64_t cnt = 0;
+      if ((s = gg::engine_export_slice(e, v0[k - i], v0[k - i + 1], l0[k - i], lg[k - i], g.triples.data(), g.total,
+                                       &cnt, err)) != GG_OK)
+        return s;
+    }
+    g.done = true;
+  }
+  if (trace)
+    std::fprintf(stderr,
+                 "[gg-batch] lane %d: %zu graphs, %llu vertices, %u labels, %llu edges -> %llu closure edges, "
+                 "%u rounds; build %.0f ms, upload %.0f ms, wait %.0f ms, run %.0f ms, results %.0f ms, "
+                 "device peak %.1f GB\n",
+                 lane, j - i, (unsigned long long)verts, labels, (unsigned long long)group_m,
+                 (unsigned long long)one.total_edges, one.rounds, ms(t_start, t_built), ms(t_built, t_up),
+                 ms(t_up, t_turn), ms(t_turn, t_run), ms(t_run, now()), double(one.device_bytes_peak) / 1e9);
+  return GG_OK;
+}
+
+int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std::string*& cerr) {
+  int st = GG_OK;
+  gg::Engine* e = gg::ctx_engine(b->ctx, st, cerr);
+  if (!e) return fail(b, st, cerr ? *cerr : "no device");
+  b->groups = 0;
+  const size_t G = b->graphs.size();
+  Planner pl{b, max_vertices, max_vertices ? 0 : gg::ctx_memory(b->ctx), gg::tuned_group_mem_share(), {}, 0, G,
+             {}, {}};
+  // Graphs are ordered by grammar before they are cut into launch groups: a
+  // group's rows are its vertices times all its labels, so a group that
+  // mixes fewer grammars carries fewer empty rows through every round.
+  pl.ord.resize(G);
+  for (size_t k = 0; k < G; ++k) pl.ord[k] = k;
+  std::stable_sort(pl.ord.begin(), pl.ord.end(), [&](size_t x, size_t y) {
+    const auto& gx = b->graphs[x];
+    const auto& gy = b->graphs[y];
+    return std::make_pair(gx.L, gx.rules) < std::make_pair(gy.L, gy.rules);
+  });
+  // GG_BATCH_TRACE=1: one stderr line per launch group (sizes and the host
+  // time of each phase)
+  const char* tr = std::getenv("GG_BATCH_TRACE");
+  const bool trace = tr && tr[0] == '1';
+  // GG_BATCH_LANES: launch groups in flight at once (default 2; 1 runs the
+  // groups one after another on the context's own engine, host work and
+  // device work in turn)
+  int lanes = 2;
+  if (const char* ln = std::getenv("GG_BATCH_LANES")) lanes = std::max(1, std::min(2, std::atoi(ln)));
+  if (G < 2) lanes = 1;
+  // Two lanes carve their buffers from one device region, allocated once
+  // (per-buffer hipMalloc/hipFree of tens of GB would synchronise the lanes
+  // and costs seconds once memory has been freed before). A lane frees its
+  // closure as soon as its results are read, so the lane whose fixpoint runs
+  // may use nearly all of the region: groups are planned as for one lane.
+  std::vector<gg::Engine*> eng{e};
+  if (lanes > 1 && b->lanes.empty()) {
+    std::string err;
+    const int dev = gg::engine_device(e);
+    // the context's engine gives its cached blocks back to the device
+    gg::engine_release(e);
+    gg::engine_trim(e);
+    uint64_t total = 0, free_bytes = 0;
+    if (gg::device_query(nullptr, dev, nullptr, nullptr, 0, &total, &free_bytes, err) == GG_OK) {
+      uint64_t bytes = uint64_t(double(free_bytes) * 0.92) & ~uint64_t((2u << 20) - 1);
+      const uint64_t mem = gg::ctx_memory(b->ctx);
+      if (mem) bytes = std::min<uint64_t>(bytes, mem);
+      b->region = gg::device_region_create(dev, bytes, err);
+    }
+    for (int l = 0; l < lanes && b->region; ++l) {
+      int s = GG_OK;
+      gg::Engine* x = gg::engine_create(dev, 0, 0, s, err);
+      if (!x) return fail(b, s, err);
+      b->lanes.push_back(x);
+      if ((s = gg::engine_use_region(x, b->region, err)) != GG_OK) return fail(b, s, err);
+    }
+  }
+  if (lanes > 1 && b->lanes.size() == size_t(lanes)) {
+    eng = b->lanes;
+    // the same group sizes as one lane on the whole device
+    if (pl.mem) {
+      const uint64_t bytes = gg::region_bytes(b->region.get());
+      pl.share = std::min(0.9, pl.share * double(pl.mem) / double(bytes));
+      pl.mem = bytes;
+    }
+  } else {
+    lanes = 1;
+  }
+  std::mutex mu, turn;
+  gg_stats total{};
+  int first_err = GG_OK;
+  std::string err_msg;
+  auto worker = [&](int lane) {
+    std::string err;
+    for (;;) {
+      size_t i = 0, j = 0;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (first_err != GG_OK) return;
+        const int t = pl.take(i, j, err);
+        if (t < 0) {
+          first_err = GG_E_UNSUPPORTED;
+          err_msg = err;
+          return;
+        }
+        if (t == 0) return;
+      }
+      gg_stats one{};
+      double bpv = 0;
+      const int s = run_group(b, eng[size_t(lane)], pl.ord, i, j, keep, one, bpv, trace, lane,
+                              lanes > 1 ? &turn : nullptr, err);
+      if (lanes > 1) gg::engine_release(eng[size_t(lane)]);  // the closure's memory, for the other lane
+      std::lock_guard<std::mutex> lk(mu);
+      if (s == GG_E_NOMEM && j - i > 1) {
+        // the closure size was not known in advance: cut the group in half
+        // and run both halves again; later groups keep the smaller count
+        const size_t mid = i + (j - i) / 2;
+        pl.cap = std::min(pl.cap, mid - i);
+        pl.redo.push_back({mid, j});
+        pl.redo.push_back({i, mid});
+        continue;
+      }
+      if (s != GG_OK) {
+        if (first_err == GG_OK) {
+          first_err = s;
+          err_msg = err;
+        }
+        return;
+      }
+      add_stats(total, one);
+      // a group of one grammar sets that grammar's bytes per vertex; a mixed
+      // group's figure is an average over its grammars, which may be far
+      // below the densest one's: it only raises figures already known
+      std::map<std::pair<int, std::vector<int32_t>>, int> kinds;
+      for (size_t k = i; k < j; ++k) {
+        const auto& g = b->graphs[pl.ord[k]];
+        kinds[std::make_pair(g.L, g.rules)] = 1;
+      }
+      for (auto& kv : kinds) {
+        auto it = pl.bytes_per_vertex.find(kv.first);
+        if (kinds.size() == 1)
+          pl.bytes_per_vertex[kv.first] = bpv;
+        else if (it != pl.bytes_per_vertex.end())
+          it->second = std::max(it->second, bpv);
+      }
+      ++b->groups;
+    }
+  };
+  if (lanes == 1) {
+    worker(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int l = 0; l < lanes; ++l) th.emplace_back(worker, l);
+    for (auto& t : th) t.join();
+  }
+  if (first_err != GG_OK) return fail(b, first_err, err_msg);
   if (stats) *stats = total;
   return GG_OK;
 }

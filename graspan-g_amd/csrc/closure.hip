@@ -2899,9 +2899,15 @@ class Engine {
 
   void put_block(DevBuf& b) {
     if (!b.p) return;
-    if (arena)
+    if (arena) {
+      // A shared region hands a block to whichever engine asks next, on its
+      // own stream: this engine's stream must be done with the block first
+      // (kernels launched before the release may still read or write it).
+      // The fixed scratch region of the allocation-free entry point has one
+      // engine per call on the caller's stream and needs no wait.
+      if (region->owned) (void)hipStreamSynchronize(stream);
       region->put(size_t(static_cast<char*>(b.p) - arena), b.bytes);
-    else
+    } else
       cache.emplace(b.bytes, b.p);
     if (!arena) cached += b.bytes;
     used -= b.bytes;
@@ -2913,6 +2919,12 @@ class Engine {
     put_block(b);
     bytes = (std::max<size_t>(bytes, 256) + 255) & ~size_t(255);
     size_t off = 0;
+    // engines that share a region may each be held to a budget of it
+    if (budget && used + bytes > budget) {
+      err = "device budget exceeded: need " + std::to_string(used + bytes) + " bytes of the region, budget " +
+            std::to_string(budget);
+      return GG_E_NOMEM;
+    }
     if (region->take(bytes, off)) {
       b.p = arena + off;
       b.bytes = bytes;
@@ -4505,6 +4517,10 @@ Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std
 
 void engine_destroy(Engine* e) { delete e; }
 
+void engine_trim(Engine* e) {
+  if (!e->arena) e->trim_cache();
+}
+
 namespace {
 // Device form of a rule table: rules by kind, pair-list slots, the labels
 // that are the lhs of a unary or binary rule, and the labels kept packed.
@@ -5169,6 +5185,8 @@ std::shared_ptr<DeviceRegion> device_region_create(int device, size_t bytes, std
 }
 
 int engine_device(Engine* e) { return e->device; }
+
+size_t region_bytes(const DeviceRegion* r) { return r ? r->cap : 0; }
 
 double tuned_group_mem_share() { return load_tuning().group_mem_share; }
 

@@ -30,6 +30,8 @@ RuleKinds rule_kinds(const int32_t* rules, int num_rules);
 // sets status/err on failure.
 Engine* engine_create(int device, uint64_t budget, int verbose, int& status, std::string& err);
 void engine_destroy(Engine* e);
+// Hands the engine's cached device blocks back to the device.
+void engine_trim(Engine* e);
 
 // Uploads the grammar rule table and the input edges (host copies). The
 // edges stay resident on the device across runs.
@@ -100,6 +102,7 @@ void engine_session_timing(Engine* e, bool on);
 struct DeviceRegion;
 std::shared_ptr<DeviceRegion> device_region_create(int device, size_t bytes, std::string& err);
 int engine_use_region(Engine* e, std::shared_ptr<DeviceRegion> r, std::string& err);
+size_t region_bytes(const DeviceRegion* r);
 int engine_device(Engine* e);
 // The share of device memory a batch launch group plans for (tuned).
 double tuned_group_mem_share();

@@ -451,6 +451,36 @@ def test_gpu_batch_groups_and_alone_equal(built):
     assert alone.counts()[1] == whole[3][1] and alone.checksum() == whole[3][2]
 
 
+@pytest.mark.parametrize("lanes", ["1", "2"])
+def test_gpu_batch_lanes_and_memory_splits(built, lanes, monkeypatch):
+    """Launch groups run one at a time (one lane) or two at once (two
+    engines on one device region); under a device budget far below the
+    first planned group, groups run out of memory and are split in half
+    until they fit. Every result equals the oracle's either way."""
+    monkeypatch.setenv("GG_BATCH_LANES", lanes)
+    graphs = _mixed_set(24, 5000, 20000, seed=24)
+    # each engine may hold 1.25x the largest graph's own peak: every graph
+    # fits alone, the whole set does not
+    peak1 = 0
+    for g in graphs:
+        one = gg.Context()
+        one.set_spec(g)
+        peak1 = max(peak1, one.run(0).device_bytes_peak)
+        one.release()
+    budget = int(peak1 * 1.25) * int(lanes)
+    say("batch lanes", lanes, "budget", budget)
+    c = gg.Context(device_budget=budget)
+    b = gg.Batch(c)
+    ids = [b.add_spec(g) for g in graphs]
+    b.run()
+    assert b.info()[1] > 2
+    for g, i in zip(graphs, ids):
+        ref = oracle.closure_of(g, want_triples=False)
+        per, tot, cs = b.result(i)
+        assert (tot, cs) == (ref.total, ref.checksum), g.name
+        np.testing.assert_array_equal(per, ref.counts)
+
+
 def _relabelled(g, order, tag):
     """The same graph under its grammar's rule lines taken in another order:
     label ids follow first appearance, so the rule triples (and the batch's
