@@ -271,11 +271,18 @@ int run_group(gg_batch* b, gg::Engine* e, const std::vector<size_t>& ord, size_t
   const auto t_up = now();
   one = gg_stats{};
   // lanes take turns at the fixpoint; reading results, building and
-  // uploading overlap the other lane's turn
+  // uploading overlap the other lane's turn. A fixpoint hands the turn on
+  // once its rounds have few active rows (its tail, which leaves most of the
+  // device idle): the next lane's first rounds overlap it.
   std::unique_lock<std::mutex> lk;
   if (turn && s == GG_OK) lk = std::unique_lock<std::mutex>(*turn);
   const auto t_turn = now();
+  if (lk.owns_lock())
+    gg::engine_on_round(e, [&lk](uint64_t active, uint64_t rows) {
+      if (lk.owns_lock() && active * 16 < rows) lk.unlock();
+    });
   if (s == GG_OK) s = gg::engine_run(e, 0, &one, err);
+  gg::engine_on_round(e, nullptr);
   if (lk.owns_lock()) lk.unlock();
   const auto t_run = now();
   if (s != GG_OK) {

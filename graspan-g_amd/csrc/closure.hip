@@ -2767,6 +2767,9 @@ class Engine {
   hipStream_t stream = nullptr;
   size_t used = 0, peak = 0;
   std::function<bool()> on_pressure;  // frees device memory, if it can, when an allocation does not fit
+  // called on the host after each semi-naive round with the round's active
+  // rows and all rows (batched lanes hand the device on in a closure's tail)
+  std::function<void(uint64_t, uint64_t)> on_round;
   double alloc_ms = 0;                // host time spent allocating and freeing device memory
   uint64_t alloc_calls = 0;
 
@@ -3662,6 +3665,7 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
   std::swap(dpk_ent, npk_ent);
   dpk_ready = npk;
   dl_nnz = N;
+  if (on_round) on_round(last_active, R);
   return GG_OK;
 }
 
@@ -5187,6 +5191,8 @@ std::shared_ptr<DeviceRegion> device_region_create(int device, size_t bytes, std
 int engine_device(Engine* e) { return e->device; }
 
 size_t region_bytes(const DeviceRegion* r) { return r ? r->cap : 0; }
+
+void engine_on_round(Engine* e, std::function<void(uint64_t, uint64_t)> f) { e->on_round = std::move(f); }
 
 double tuned_group_mem_share() { return load_tuning().group_mem_share; }
 

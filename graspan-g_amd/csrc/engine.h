@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -103,6 +104,9 @@ struct DeviceRegion;
 std::shared_ptr<DeviceRegion> device_region_create(int device, size_t bytes, std::string& err);
 int engine_use_region(Engine* e, std::shared_ptr<DeviceRegion> r, std::string& err);
 size_t region_bytes(const DeviceRegion* r);
+// f(active rows, all rows) runs on the host after every semi-naive round of
+// engine_run (empty f: none).
+void engine_on_round(Engine* e, std::function<void(uint64_t, uint64_t)> f);
 int engine_device(Engine* e);
 // The share of device memory a batch launch group plans for (tuned).
 double tuned_group_mem_share();
