@@ -10,7 +10,10 @@ V ::= L R; T ::= D V; M ::= T d), filter 0.
 
 One step = one full fixpoint of that graph with its input already resident in
 HBM. value = new closure edges per second = (|closure| - |dedup input|) / t,
-summed over the timed steps.
+summed over the timed steps. The engine evaluates the least-fixpoint rule plan
+(gg_grammar_plan_fixpoint: V ::= L R as V ::= R | A V | M V, same closure);
+rule_table_as_given (extra key) times the same closure with the rule table as
+given (GG_STAR_REWRITE=0).
 
 roofline: the expand stage -- the join/dedup/difference kernels (k_expand_light,
 one lane per row, in rounds with few candidates per row, then k_expand, one
@@ -239,7 +242,9 @@ def main():
         "roofline": roofline(stats),
         "breakdown_ms": {k: sum(s[k] for s in stats) / len(stats)
                          for k in ("fixpoint_ms", "expand_ms", "compact_ms", "merge_ms", "other_ms")},
+        "join_candidates": s0["candidates"],
     }
+    line["rule_table_as_given"] = as_given_leg(g, stats)
     if not args.no_batch:
         log("batched closure of the 4096-graph set")
         line["batch"] = bench_batch(args)
@@ -253,6 +258,34 @@ def main():
         log("cpu baseline: the oracle on the same n=%d graph" % g.n)
         line["cpu_baseline"] = cpu_baseline(g)
     print(json.dumps(line), flush=True)
+
+
+def as_given_leg(g, planned_stats):
+    """The same closure with the alias rule table evaluated as given
+    (GG_STAR_REWRITE=0: V ::= L R joined directly), for comparison with the
+    least-fixpoint rule plan the engine uses by default; one warm-up and two
+    timed runs. The closure must be the same."""
+    import graspan_g as gg
+
+    os.environ["GG_STAR_REWRITE"] = "0"
+    try:
+        ctx = gg.Context()
+        ctx.set_spec(g)
+        ctx.run(0)
+        t0 = time.perf_counter()
+        sts = [ctx.run(0) for _ in range(2)]
+        wall = time.perf_counter() - t0
+        ctx.release()
+    finally:
+        del os.environ["GG_STAR_REWRITE"]
+    st = sts[-1]
+    if st.total_edges != planned_stats[-1]["total_edges"]:
+        raise SystemExit("the rule table as given closes to a different edge count")
+    log(f"rule table as given: {wall * 1e3 / 2:.1f} ms per closure, {st.rounds} rounds, "
+        f"{st.candidates} join candidates")
+    return {"ms_per_step": wall * 1e3 / 2, "value": 2 * (st.total_edges - st.input_unique) / wall,
+            "rounds": st.rounds, "join_candidates": st.candidates,
+            "note": "GG_STAR_REWRITE=0: the alias rule table evaluated as given (V ::= L R), same closure"}
 
 
 def line_of(metric, leg, args):
