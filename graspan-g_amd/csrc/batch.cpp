@@ -273,14 +273,20 @@ int run_group(gg_batch* b, gg::Engine* e, const std::vector<size_t>& ord, size_t
   // lanes take turns at the fixpoint; reading results, building and
   // uploading overlap the other lane's turn. A fixpoint hands the turn on
   // once its rounds have few active rows (its tail, which leaves most of the
-  // device idle): the next lane's first rounds overlap it.
+  // device idle): the next lane's first rounds overlap it. Measured on the
+  // 4096-graph set: 1/8 of the rows 9.75 s, 1/16 9.94 s, never 9.98 s, 1/4
+  // 10.5 s (two closures at their peak no longer fit, groups split).
   std::unique_lock<std::mutex> lk;
   if (turn && s == GG_OK) lk = std::unique_lock<std::mutex>(*turn);
   const auto t_turn = now();
-  if (lk.owns_lock())
-    gg::engine_on_round(e, [&lk](uint64_t active, uint64_t rows) {
-      if (lk.owns_lock() && active * 16 < rows) lk.unlock();
+  if (lk.owns_lock()) {
+    // GG_BATCH_HANDOFF=d: hand on below rows / d active rows (default 8)
+    uint64_t d = 8;
+    if (const char* h = std::getenv("GG_BATCH_HANDOFF")) d = std::max<uint64_t>(1, std::strtoull(h, nullptr, 10));
+    gg::engine_on_round(e, [&lk, d](uint64_t active, uint64_t rows) {
+      if (lk.owns_lock() && active * d < rows) lk.unlock();
     });
+  }
   if (s == GG_OK) s = gg::engine_run(e, 0, &one, err);
   gg::engine_on_round(e, nullptr);
   if (lk.owns_lock()) lk.unlock();
