@@ -448,7 +448,6 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
   __shared__ uint32_t s_base;
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
-  unsigned long long pk = 0;
   for (uint32_t bbase = blockIdx.x * blockDim.x; bbase < a.n; bbase += gridDim.x * blockDim.x) {
     const uint32_t u = bbase + uint32_t(wv) * kWave + uint32_t(lane);
     const bool valid = u < a.n;
@@ -484,31 +483,22 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
       }
     }
     for (int i0 = 0; i0 < a.g.nb; i0 += kRuleChunk) {
-      unsigned long long d0[kRuleChunk], d1[kRuleChunk], o0[kRuleChunk], o1[kRuleChunk], q0[kRuleChunk],
-          q1[kRuleChunk], p0[kRuleChunk], p1[kRuleChunk];
+      unsigned long long d0[kRuleChunk], d1[kRuleChunk], p0[kRuleChunk], p1[kRuleChunk];
 #pragma unroll
       for (int k = 0; k < kRuleChunk; ++k) {
         const int i = i0 + k;
-        d0[k] = d1[k] = o0[k] = o1[k] = q0[k] = q1[k] = p0[k] = p1[k] = 0;
+        d0[k] = d1[k] = p0[k] = p1[k] = 0;
         if (i < a.g.nb && a.b_li[i] != 0xFFFFu && valid) {
           const uint32_t rb = uint32_t(a.g.b_rhs1[i]) * a.n + u;
           if (label_nonempty(a.ctr, a.g.b_rhs1[i])) {
             d0[k] = a.dl.ptr[rb];
             d1[k] = a.dl.ptr[rb + 1];
           }
-          if (label_nonempty(a.ctr, a.g.b_rhs2[i])) {
-            o0[k] = a.old.ptr[rb];
-            o1[k] = a.old.ptr[rb + 1];
-            if (a.pend_ptr) {
-              q0[k] = a.pend_ptr[rb];
-              q1[k] = a.pend_ptr[rb + 1];
-            }
-            const uint32_t slot = a.g.b_slot[i];
-            if (slot != 0xFFu && a.b2rng) {
-              const uint64_t rs = uint64_t(slot) * a.n + u;
-              p0[k] = a.b2rng[2 * rs];
-              p1[k] = a.b2rng[2 * rs + 1];
-            }
+          const uint32_t slot = a.g.b_slot[i];
+          if (slot != 0xFFu && a.b2rng && label_nonempty(a.ctr, a.g.b_rhs2[i])) {
+            const uint64_t rs = uint64_t(slot) * a.n + u;
+            p0[k] = a.b2rng[2 * rs];
+            p1[k] = a.b2rng[2 * rs + 1];
           }
         }
       }
@@ -516,7 +506,6 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
       for (int k = 0; k < kRuleChunk; ++k) {
         const int i = i0 + k;
         if (i < a.g.nb && a.b_li[i] != 0xFFFFu) {
-          pk += (d1[k] - d0[k]) + (o1[k] - o0[k]) + (q1[k] - q0[k]);
           const unsigned long long m = __ballot(d1[k] > d0[k] || p1[k] > p0[k]);
           if (lane == 0) s_m[wv][a.b_li[i]] |= m;
         }
@@ -542,17 +531,28 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
     }
     __syncthreads();  // s_m, s_cnt and s_base are rewritten by the next batch
   }
-  // one counter update per block, not per wave: updates of one address from
-  // every block serialise
-  pk = wave_sum_u64(pk);
-  __shared__ unsigned long long s_pk[kWavesPerBlock];
-  if (lane == 0) s_pk[wv] = pk;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < kWavesPerBlock; ++w) t += s_pk[w];
-    if (t) atomicAdd(&ctr_slot(a.ctr).pkeys, t);
+}
+
+// The round's key elements scanned (the probe counter's key part): per
+// binary rule A ::= B C, every element of delta_B when delta_B is not empty,
+// and every element of old_B and the pending B when delta_C is not empty
+// (the rows the join reads keys from), i.e. whole-label sizes: one lane per
+// rule, no per-vertex loads.
+__global__ void k_probe_keys(const DevGrammar g, const unsigned long long* old_ptr, const unsigned long long* dl_ptr,
+                             const unsigned long long* pend_ptr, uint32_t n, Counters* ctr) {
+  const int lane = threadIdx.x & (kWave - 1);
+  unsigned long long pk = 0;
+  for (int i = lane; i < g.nb; i += kWave) {
+    const uint64_t b0 = uint64_t(g.b_rhs1[i]) * n, b1 = b0 + n;
+    const uint64_t c0 = uint64_t(g.b_rhs2[i]) * n, c1 = c0 + n;
+    pk += dl_ptr[b1] - dl_ptr[b0];
+    if (dl_ptr[c1] > dl_ptr[c0]) {
+      pk += old_ptr[b1] - old_ptr[b0];
+      if (pend_ptr) pk += pend_ptr[b1] - pend_ptr[b0];
+    }
   }
+  pk = wave_sum_u64(pk);
+  if (lane == 0 && pk) atomicAdd(&ctr->slot[0].pkeys, pk);
 }
 
 // ---------------------------------------------------------------------------
@@ -3503,6 +3503,9 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
     rp.act.b2rng = pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr;
     rp.act.pend_ptr = pend_csr().ptr;
     if (rp.act_work) k_activate<<<grid_for(n), kBlock, 0, stream>>>(rp.act);
+    if (g.nb)
+      k_probe_keys<<<1, kWave, 0, stream>>>(g, P<unsigned long long>(old_ptr), P<unsigned long long>(dl_ptr),
+                                            pend_csr().ptr, n, C());
     // a round with few active rows launches only the blocks that can get
     // some (kGrab rows per grab, 4 waves per block), not a full grid of
     // waves that would only contend on the work counters
