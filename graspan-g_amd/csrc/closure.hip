@@ -4902,16 +4902,32 @@ __global__ void k_item_counts(const unsigned long long* ptr, const SliceItem* it
 
 // per chunk of rows: its edge count and checksum in c, added to its item's
 // cnt / sums
-__global__ __launch_bounds__(kBlock) void k_item_results(DCsr c, const SliceItem* it, uint32_t ni,
-                                                         unsigned long long* cnt, unsigned long long* sums) {
+// plab: rows of packed labels read their packed entries (plab.ent at the
+// row's own offset, plab.len[r] of them) when plab.len is set, their ids
+// otherwise.
+__global__ __launch_bounds__(kBlock) void k_item_results(DCsr c, DPacked plab, uint32_t n, const SliceItem* it,
+                                                         uint32_t ni, unsigned long long* cnt,
+                                                         unsigned long long* sums) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t wv = threadIdx.x / kWave;
   for (uint32_t i = blockIdx.x; i < ni; i += gridDim.x) {
     const SliceItem x = it[i];
     unsigned long long acc = 0;
+    const bool pk = plab.len && x.re > x.rb && packed_label(plab, x.rb / n);
     for (uint32_t r = x.rb + wv; r < x.re; r += kWavesPerBlock) {
       const uint32_t u = x.u0 + (r - x.rb) - x.v0;
-      const unsigned long long b = c.ptr[r], e = c.ptr[r + 1];
+      const unsigned long long b = c.ptr[r];
+      if (pk) {
+        const unsigned long long pe = b + plab.len[r];
+        for (unsigned long long k = b + lane; k < pe; k += kWave) {
+          const unsigned long long en = plab.ent[k];
+          const uint32_t w = uint32_t(en) << 5;
+          for (uint32_t bits = uint32_t(en >> 32); bits; bits &= bits - 1)
+            acc += triple_hash(u, w + uint32_t(__ffs(bits) - 1) - x.v0, x.lab);
+        }
+        continue;
+      }
+      const unsigned long long e = c.ptr[r + 1];
       for (unsigned long long k = b + lane; k < e; k += kWave) acc += triple_hash(u, c.col[k] - x.v0, x.lab);
     }
     acc = wave_sum_u64(acc);
@@ -4950,7 +4966,10 @@ int engine_session_begin(Engine* e, const Grammar& gr, std::string& err) {
   e->m = 0;
   e->old_nnz = e->dl_nnz = e->pend_nnz = 0;
   e->pk_ready = e->dpk_ready = e->pairs_ready = false;
-  e->keep_ids = true;
+  // rows of packed labels keep only their packed entries during the rounds
+  // (results read them directly); GG_KEEP_PACKED_IDS=1 keeps their ids too
+  e->keep_ids = false;
+  if (const char* ki = std::getenv("GG_KEEP_PACKED_IDS")) e->keep_ids = std::atoi(ki) != 0;
   e->apply_tuning();
   e->defer_ratio = e->tune.defer_ratio;
   if (const char* dr = std::getenv("GG_DEFER_RATIO")) e->defer_ratio = std::strtoull(dr, nullptr, 10);
@@ -5122,15 +5141,22 @@ int engine_session_results(Engine* e, const std::vector<SessionSlice>& done, std
   e->fill_u64(d_cnt, 0, nitems);
   e->fill_u64(d_sum, 0, nitems);
   const uint32_t grid = uint32_t(std::min<uint64_t>(it.size(), uint64_t(e->cus) * 8));
+  // packed labels' rows of old hold only their packed entries unless the
+  // ids are kept every round
+  DPacked opk{};
+  for (int i = 0; i < 8; ++i) opk.labels[i] = e->pack_labels[i];
+  opk.len = e->pk_ready && !e->keep_ids ? static_cast<uint32_t*>(e->pk_len.p) : nullptr;
+  opk.ent = static_cast<unsigned long long*>(e->pk_ent.p);
   k_item_results<<<grid, kBlock, 0, e->stream>>>({static_cast<unsigned long long*>(e->old_ptr.p),
                                                   static_cast<uint32_t*>(e->old_col.p)},
-                                                 static_cast<SliceItem*>(d_it.p), uint32_t(it.size()),
+                                                 opk, e->n, static_cast<SliceItem*>(d_it.p), uint32_t(it.size()),
                                                  static_cast<unsigned long long*>(d_cnt.p),
                                                  static_cast<unsigned long long*>(d_sum.p));
   if (e->pend_nnz)
     k_item_results<<<grid, kBlock, 0, e->stream>>>({static_cast<unsigned long long*>(e->pd_ptr.p),
                                                     static_cast<uint32_t*>(e->pd_col.p)},
-                                                   static_cast<SliceItem*>(d_it.p), uint32_t(it.size()),
+                                                   DPacked{}, e->n, static_cast<SliceItem*>(d_it.p),
+                                                   uint32_t(it.size()),
                                                    static_cast<unsigned long long*>(d_cnt.p),
                                                    static_cast<unsigned long long*>(d_sum.p));
   GG_HIP(hipGetLastError());

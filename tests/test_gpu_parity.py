@@ -517,10 +517,15 @@ def test_gpu_batch_many_distinct_grammars(built):
         np.testing.assert_array_equal(per, ref.counts)
 
 
-def test_gpu_server_results_equal_alone(built):
+@pytest.mark.parametrize("keep_ids", ["0", "1"])
+def test_gpu_server_results_equal_alone(built, keep_ids, monkeypatch):
     """Requests served in a mix (two slots, batches formed from whatever is
-    queued) give the same closures as each request closed alone."""
-    say("server mix")
+    queued) give the same closures as each request closed alone. Sessions
+    keep packed labels' rows as packed entries only and read results from
+    them (GG_KEEP_PACKED_IDS=0, the default) or keep their ids every round
+    (1)."""
+    monkeypatch.setenv("GG_KEEP_PACKED_IDS", keep_ids)
+    say("server mix, keep ids", keep_ids)
     graphs = _mixed_set(40, 100, 20_000, seed=23)
     with gg.Server(slots=2, budget_bytes=2 << 30, bytes_per_vertex=16 << 10, max_batch_requests=8) as srv:
         ids = [srv.submit_spec(g) for g in graphs]
