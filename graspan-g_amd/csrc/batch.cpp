@@ -385,6 +385,10 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
       if ((s = gg::engine_use_region(x, b->region, err)) != GG_OK) return fail(b, s, err);
     }
   }
+  // per-graph results are read from the packed entries: no id expansion of
+  // the packed labels after each group's last round (exports write them)
+  for (auto* x : eng) gg::engine_defer_unpack(x, true);
+  for (auto* x : b->lanes) gg::engine_defer_unpack(x, true);
   if (lanes > 1 && b->lanes.size() == size_t(lanes)) {
     eng = b->lanes;
     // the same group sizes as one lane on the whole device
@@ -463,6 +467,7 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
     for (int l = 0; l < lanes; ++l) th.emplace_back(worker, l);
     for (auto& t : th) t.join();
   }
+  gg::engine_defer_unpack(e, false);  // the context's own runs write every id
   if (first_err != GG_OK) return fail(b, first_err, err_msg);
   if (stats) *stats = total;
   return GG_OK;

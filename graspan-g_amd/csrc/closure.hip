@@ -2542,7 +2542,9 @@ __global__ void k_slice_counts(const unsigned long long* ptr, uint32_t n, DSlice
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_slice_checksum(DCsr c, uint32_t rows, uint32_t n, DSlices s,
+// plab.len set: rows of packed labels are read from their packed entries
+// (their ids are not written).
+__global__ __launch_bounds__(kBlock) void k_slice_checksum(DCsr c, DPacked plab, uint32_t rows, uint32_t n, DSlices s,
                                                            unsigned long long* sums) {
   // a wave takes a contiguous range of rows and adds its hashes into a
   // graph's sum only when the graph changes (consecutive rows of a label
@@ -2567,6 +2569,16 @@ __global__ __launch_bounds__(kBlock) void k_slice_checksum(DCsr c, uint32_t rows
       acc = 0;
     }
     const uint32_t base = s.v0[g], lab = l - s.l0[g] + 1;
+    if (plab.len && packed_label(plab, l)) {
+      const unsigned long long pe = b + plab.len[r];
+      for (unsigned long long i = b + lane; i < pe; i += kWave) {
+        const unsigned long long en = plab.ent[i];
+        const uint32_t w = uint32_t(en) << 5;
+        for (uint32_t bits = uint32_t(en >> 32); bits; bits &= bits - 1)
+          acc += triple_hash(u - base, w + uint32_t(__ffs(bits) - 1) - base, lab);
+      }
+      continue;
+    }
     for (unsigned long long i = b + lane; i < e; i += kWave) acc += triple_hash(u - base, c.col[i] - base, lab);
   }
   const unsigned long long t = wave_sum_u64(acc);
@@ -2793,6 +2805,7 @@ class Engine {
     for (int i = 0; i < 8; ++i) pack_labels[i] = r.pack[i];
   }
 
+
   uint32_t n = 0;
   uint64_t m = 0;
   DevBuf in_src, in_dst, in_lab;
@@ -2811,6 +2824,11 @@ class Engine {
   // their packed entries, the ids written once after the last round);
   // GG_KEEP_PACKED_IDS=1 sets it
   bool keep_ids = false;
+  // defer_unpack: run() leaves the packed labels' ids unwritten (ids_pending)
+  // until a caller needs them (ensure_ids); results that can be read from the
+  // packed entries (the batch's per-graph counts and checksums) never need
+  // them
+  bool defer_unpack = false, ids_pending = false;
   DevBuf raw_ptr, raw_col, staging, seg_off, seg_cnt, active, tiles, ctr;
   // frontier bitmaps and pair lists for old_B x delta_C
   DevBuf fr_bits, pr_ptr, pr_col;
@@ -3227,6 +3245,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     return GG_E_ARG;
   }
   use_rules(rules_planned);
+  ids_pending = false;
   pend_nnz = 0;
   apply_tuning();
   defer_ratio = tune.defer_ratio;
@@ -3346,7 +3365,8 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
     pend_nnz = 0;
     GG_HIP(rec(mg.b));
   }
-  if (pk_ready && !keep_ids) {  // the packed labels' ids, from their entries
+  ids_pending = pk_ready && !keep_ids && defer_unpack;
+  if (pk_ready && !keep_ids && !defer_unpack) {  // the packed labels' ids, from their entries
     EventPair& mg = ev.mk(ev.merge);
     GG_HIP(rec(mg.a));
     DPacked plab{};
@@ -3691,6 +3711,7 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
   // the reference's schedule runs the rule table as given: with filter != 0
   // its result depends on the rules, not only on their least fixpoint
   use_rules(rules_given);
+  ids_pending = false;
   // row ids (label * vertices + vertex) are uint32 in every region and in
   // the assembled closure, which has all n vertices
   if (uint64_t(L) * n >= 0xFFFFFFFFull) {
@@ -4675,6 +4696,27 @@ int need_closure(Engine* e, std::string& err) {
   }
   return GG_OK;
 }
+
+// writes the packed labels' ids that run() left pending
+int ensure_ids(Engine* e, std::string& err) {
+  if (!e->ids_pending) return GG_OK;
+  GG_HIP(hipSetDevice(e->device));
+  DPacked plab{};
+  for (int i = 0; i < 8; ++i) plab.labels[i] = e->pack_labels[i];
+  plab.len = static_cast<uint32_t*>(e->pk_len.p);
+  plab.ent = static_cast<unsigned long long*>(e->pk_ent.p);
+  k_unpack<<<uint32_t(e->cus) * 8, kBlock, 0, e->stream>>>(static_cast<unsigned long long*>(e->old_ptr.p),
+                                                          static_cast<uint32_t*>(e->old_col.p), plab, e->rows(), e->n);
+  GG_HIP(hipGetLastError());
+  e->ids_pending = false;
+  return GG_OK;
+}
+
+// need_closure, and the closure's ids written
+int need_ids(Engine* e, std::string& err) {
+  GG_TRY(need_closure(e, err));
+  return ensure_ids(e, err);
+}
 }  // namespace
 
 int engine_counts(Engine* e, uint64_t* per_label, uint64_t* total, std::string& err) {
@@ -4780,7 +4822,7 @@ static int export_slice(Engine* e, uint32_t v0, uint32_t v1, uint32_t l0, uint32
 }
 
 int engine_export(Engine* e, uint32_t* buf, uint64_t cap, uint64_t* count, std::string& err) {
-  GG_TRY(need_closure(e, err));
+  GG_TRY(need_ids(e, err));
   return export_slice(e, 0, e->n, 0, uint32_t(e->L), buf, cap, count, err);
 }
 
@@ -4822,8 +4864,13 @@ int engine_slices(Engine* e, const std::vector<uint32_t>& v0, const std::vector<
                    static_cast<uint32_t*>(d_lg.p), static_cast<uint32_t*>(d_coff.p), G};
   k_slice_counts<<<e->grid_for(G), kBlock, 0, e->stream>>>(static_cast<unsigned long long*>(e->old_ptr.p), e->n, sl,
                                                            static_cast<unsigned long long*>(d_cnt.p));
+  // ids left pending by the run: the packed labels' rows are read as entries
+  DPacked plab{};
+  for (int i = 0; i < 8; ++i) plab.labels[i] = e->pack_labels[i];
+  plab.len = e->ids_pending ? static_cast<uint32_t*>(e->pk_len.p) : nullptr;
+  plab.ent = static_cast<unsigned long long*>(e->pk_ent.p);
   k_slice_checksum<<<uint32_t(e->cus) * 8, kBlock, 0, e->stream>>>(
-      {static_cast<unsigned long long*>(e->old_ptr.p), static_cast<uint32_t*>(e->old_col.p)}, e->rows(), e->n, sl,
+      {static_cast<unsigned long long*>(e->old_ptr.p), static_cast<uint32_t*>(e->old_col.p)}, plab, e->rows(), e->n, sl,
       static_cast<unsigned long long*>(d_sum.p));
   GG_HIP(hipGetLastError());
   counts.assign(nc, 0);
@@ -4836,12 +4883,12 @@ int engine_slices(Engine* e, const std::vector<uint32_t>& v0, const std::vector<
 
 int engine_export_slice(Engine* e, uint32_t v0, uint32_t v1, uint32_t l0, uint32_t lg, uint32_t* buf, uint64_t cap,
                         uint64_t* count, std::string& err) {
-  GG_TRY(need_closure(e, err));
+  GG_TRY(need_ids(e, err));
   return export_slice(e, v0, v1, l0, lg, buf, cap, count, err);
 }
 
 int engine_checksum(Engine* e, uint64_t* sum, std::string& err) {
-  GG_TRY(need_closure(e, err));
+  GG_TRY(need_ids(e, err));
   GG_HIP(hipSetDevice(e->device));
   e->zero_ctr();
   k_checksum<<<uint32_t(e->cus) * 8, kBlock, 0, e->stream>>>(
@@ -5223,6 +5270,8 @@ size_t region_bytes(const DeviceRegion* r) { return r ? r->cap : 0; }
 
 void engine_on_round(Engine* e, std::function<void(uint64_t, uint64_t)> f) { e->on_round = std::move(f); }
 
+void engine_defer_unpack(Engine* e, bool on) { e->defer_unpack = on; }
+
 double tuned_group_mem_share() { return load_tuning().group_mem_share; }
 
 int engine_use_region(Engine* e, std::shared_ptr<DeviceRegion> r, std::string& err) {
@@ -5367,6 +5416,7 @@ int engine_release(Engine* e) {
                    &e->dpk_len, &e->dpk_ent, &e->npk_len, &e->npk_ent, &e->mlist, &e->mcnt,
                    &e->pd_ptr,  &e->pd_col,  &e->p2_ptr,  &e->p2_col};
   for (DevBuf* b : all) e->release(*b);
+  e->ids_pending = false;
   e->pend_nnz = 0;
   e->trim_cache();  // the memory goes back to the device (other contexts may need it)
   e->has_closure = false;

@@ -107,6 +107,9 @@ size_t region_bytes(const DeviceRegion* r);
 // f(active rows, all rows) runs on the host after every semi-naive round of
 // engine_run (empty f: none).
 void engine_on_round(Engine* e, std::function<void(uint64_t, uint64_t)> f);
+// on: engine_run leaves the packed labels' ids unwritten until an export or
+// checksum needs them; engine_slices reads the packed entries instead.
+void engine_defer_unpack(Engine* e, bool on);
 int engine_device(Engine* e);
 // The share of device memory a batch launch group plans for (tuned).
 double tuned_group_mem_share();
