@@ -404,6 +404,7 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
   gg_stats total{};
   int first_err = GG_OK;
   std::string err_msg;
+  std::vector<std::pair<size_t, size_t>> alone;  // lanes: single graphs to run again alone
   auto worker = [&](int lane) {
     std::string err;
     for (;;) {
@@ -427,11 +428,20 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
       std::lock_guard<std::mutex> lk(mu);
       if (s == GG_E_NOMEM && j - i > 1) {
         // the closure size was not known in advance: cut the group in half
-        // and run both halves again; later groups keep the smaller count
+        // and run both halves again. One lane: later groups keep the smaller
+        // count. Two lanes: the other lane's closure may have been on the
+        // device too, so the count stays (the bytes per vertex of the groups
+        // that finish size the next ones).
         const size_t mid = i + (j - i) / 2;
-        pl.cap = std::min(pl.cap, mid - i);
+        if (lanes == 1) pl.cap = std::min(pl.cap, mid - i);
         pl.redo.push_back({mid, j});
         pl.redo.push_back({i, mid});
+        continue;
+      }
+      if (s == GG_E_NOMEM && lanes > 1) {
+        // one graph that did not fit beside the other lane's closure runs
+        // again alone once both lanes are done
+        alone.push_back({i, j});
         continue;
       }
       if (s != GG_OK) {
@@ -466,6 +476,23 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
     std::vector<std::thread> th;
     for (int l = 0; l < lanes; ++l) th.emplace_back(worker, l);
     for (auto& t : th) t.join();
+    // single graphs that ran out of memory beside the other lane, now with
+    // the whole region
+    for (size_t k = 0; k < alone.size() && first_err == GG_OK; ++k) {
+      std::string err;
+      gg_stats one{};
+      double bpv = 0;
+      const int s = run_group(b, eng[0], pl.ord, alone[k].first, alone[k].second, keep, one, bpv, trace, 0, nullptr,
+                              err);
+      gg::engine_release(eng[0]);
+      if (s != GG_OK) {
+        first_err = s;
+        err_msg = err;
+        break;
+      }
+      add_stats(total, one);
+      ++b->groups;
+    }
   }
   gg::engine_defer_unpack(e, false);  // the context's own runs write every id
   if (first_err != GG_OK) return fail(b, first_err, err_msg);

@@ -1338,6 +1338,31 @@ __device__ inline unsigned long long lower_pos_word(const unsigned long long* en
   return b;
 }
 
+// lower_pos and lower_pos_word for a caller whose keys only grow: the first
+// position >= x found by galloping from b (steps 1, 2, 4, ... past b, then a
+// binary search inside the last step), so that a key just past the previous
+// one costs a few dependent loads, not a search over the whole rest.
+__device__ inline unsigned long long gallop_pos(const uint32_t* col, unsigned long long b, unsigned long long e,
+                                                unsigned long long x) {
+  if (b >= e || col[b] >= x) return b;
+  unsigned long long lo = b, step = 1;  // col[lo] < x
+  while (lo + step < e && col[lo + step] < x) {
+    lo += step;
+    step <<= 1;
+  }
+  return lower_pos(col, lo + 1, lo + step < e ? lo + step : e, x);
+}
+__device__ inline unsigned long long gallop_pos_word(const unsigned long long* ent, unsigned long long b,
+                                                     unsigned long long e, uint32_t wi) {
+  if (b >= e || uint32_t(ent[b]) >= wi) return b;
+  unsigned long long lo = b, step = 1;  // word of ent[lo] < wi
+  while (lo + step < e && uint32_t(ent[lo + step]) < wi) {
+    lo += step;
+    step <<= 1;
+  }
+  return lower_pos_word(ent, lo + 1, lo + step < e ? lo + step : e, wi);
+}
+
 // Clears the window [c0, c0+span), seeds it with old_A(u) U delta_A(u) and
 // streams every candidate through it. Returns the number of new edges (set
 // bits of w.fresh) and stores the candidate count in kcount.
@@ -1626,18 +1651,18 @@ __global__ __launch_bounds__(kBlock) void k_expand_light(const ExpandArgs a, uin
         prev = x;
         bool seen;
         if (pka) {
-          ob = lower_pos_word(a.pk.ent, ob, oe, x >> 5);
+          ob = gallop_pos_word(a.pk.ent, ob, oe, x >> 5);
           seen = ob < oe && uint32_t(a.pk.ent[ob]) == (x >> 5) && ((a.pk.ent[ob] >> (32 + (x & 31))) & 1ull);
         } else {
-          ob = lower_pos(a.old.col, ob, oe, x);
+          ob = gallop_pos(a.old.col, ob, oe, x);
           seen = ob < oe && a.old.col[ob] == x;
         }
         if (!seen) {
-          db = lower_pos(a.dl.col, db, de, x);
+          db = gallop_pos(a.dl.col, db, de, x);
           seen = db < de && a.dl.col[db] == x;
         }
         if (!seen && pend) {
-          qb = lower_pos(a.pend.col, qb, qe, x);
+          qb = gallop_pos(a.pend.col, qb, qe, x);
           seen = qb < qe && a.pend.col[qb] == x;
         }
         if (!seen) s_buf[c++][t] = x;
