@@ -438,82 +438,94 @@ struct ActArgs {
 // comes out vertex-major (rows of one vertex range stay together, which keeps
 // the rows they join against in one XCD's L2 when the expand kernel splits
 // the list by XCD).
-__global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
-  // A block takes 256 consecutive vertices. Each wave records one ballot per
-  // lhs label; the block then reserves its rows with a single atomic (one per
-  // block, not one per wave and label, which serialised on the counter) and
-  // every wave writes its rows label by label.
-  __shared__ unsigned long long s_m[kWavesPerBlock][256];
+__global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a, uint32_t sub) {
+  // A block takes sub-chunks of 256 consecutive vertices, U of them at a
+  // time (U <= sub, U*4*nlhs <= 1024 masks in LDS). Each wave records one
+  // ballot per sub-chunk and lhs label; the block then reserves the rows of
+  // all U sub-chunks with a single atomic (the counter is one address for
+  // the whole device: a reservation per 256 vertices serialised on it, 2.4e5
+  // of them per round at 6e7 vertices) and every wave writes its rows
+  // sub-chunk by sub-chunk, label by label.
+  constexpr uint32_t kMasks = 1024;
+  __shared__ unsigned long long s_m[kMasks];  // [U][wave][nlhs]
   __shared__ uint32_t s_cnt[kWavesPerBlock];
   __shared__ uint32_t s_base;
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
-  for (uint32_t bbase = blockIdx.x * blockDim.x; bbase < a.n; bbase += gridDim.x * blockDim.x) {
-    const uint32_t u = bbase + uint32_t(wv) * kWave + uint32_t(lane);
-    const bool valid = u < a.n;
-    for (uint32_t li = lane; li < a.nlhs; li += kWave) s_m[wv][li] = 0;
-    wave_sync();
-    // Rules go in chunks of kRuleChunk: every row-bound load of a chunk is
-    // issued before any is used, and each rule's ballot is ORed into the mask
-    // of its lhs label. A rule reads Δ_B(u) only when Δ_B is not empty, and
-    // old_B(u), its pending edges and its pair list only when Δ_C is not
-    // empty (each is empty otherwise). 8 rules per chunk: one round trip per
-    // vertex for grammars of up to 8 rules of a kind, at 4 waves per SIMD
-    // (measured faster than 4 rules per chunk at 5 waves).
-    constexpr int kRuleChunk = 8;
-    for (int i0 = 0; i0 < a.g.nu; i0 += kRuleChunk) {
-      unsigned long long lo[kRuleChunk], hi[kRuleChunk];
-#pragma unroll
-      for (int k = 0; k < kRuleChunk; ++k) {
-        const int i = i0 + k;
-        lo[k] = hi[k] = 0;
-        if (i < a.g.nu && a.u_li[i] != 0xFFFFu && valid && label_nonempty(a.ctr, a.g.u_rhs[i])) {
-          const uint32_t rb = uint32_t(a.g.u_rhs[i]) * a.n + u;
-          lo[k] = a.dl.ptr[rb];
-          hi[k] = a.dl.ptr[rb + 1];
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < kRuleChunk; ++k) {
-        const int i = i0 + k;
-        if (i < a.g.nu && a.u_li[i] != 0xFFFFu) {
-          const unsigned long long m = __ballot(hi[k] > lo[k]);
-          if (lane == 0) s_m[wv][a.u_li[i]] |= m;
-        }
-      }
-    }
-    for (int i0 = 0; i0 < a.g.nb; i0 += kRuleChunk) {
-      unsigned long long d0[kRuleChunk], d1[kRuleChunk], p0[kRuleChunk], p1[kRuleChunk];
-#pragma unroll
-      for (int k = 0; k < kRuleChunk; ++k) {
-        const int i = i0 + k;
-        d0[k] = d1[k] = p0[k] = p1[k] = 0;
-        if (i < a.g.nb && a.b_li[i] != 0xFFFFu && valid) {
-          const uint32_t rb = uint32_t(a.g.b_rhs1[i]) * a.n + u;
-          if (label_nonempty(a.ctr, a.g.b_rhs1[i])) {
-            d0[k] = a.dl.ptr[rb];
-            d1[k] = a.dl.ptr[rb + 1];
-          }
-          const uint32_t slot = a.g.b_slot[i];
-          if (slot != 0xFFu && a.b2rng && label_nonempty(a.ctr, a.g.b_rhs2[i])) {
-            const uint64_t rs = uint64_t(slot) * a.n + u;
-            p0[k] = a.b2rng[2 * rs];
-            p1[k] = a.b2rng[2 * rs + 1];
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < kRuleChunk; ++k) {
-        const int i = i0 + k;
-        if (i < a.g.nb && a.b_li[i] != 0xFFFFu) {
-          const unsigned long long m = __ballot(d1[k] > d0[k] || p1[k] > p0[k]);
-          if (lane == 0) s_m[wv][a.b_li[i]] |= m;
-        }
-      }
-    }
-    wave_sync();
+  const uint32_t nl = a.nlhs;
+  const uint32_t U = max(1u, min(sub, kMasks / (uint32_t(kWavesPerBlock) * max(nl, 1u))));
+  auto mask = [&](uint32_t k, uint32_t li) -> unsigned long long& {
+    return s_m[(k * kWavesPerBlock + uint32_t(wv)) * nl + li];
+  };
+  const uint64_t span = uint64_t(blockDim.x) * U;
+  for (uint64_t bbase = uint64_t(blockIdx.x) * span; bbase < a.n; bbase += uint64_t(gridDim.x) * span) {
     uint32_t wcnt = 0;
-    for (uint32_t li = 0; li < a.nlhs; ++li) wcnt += uint32_t(__popcll(s_m[wv][li]));
+    for (uint32_t k = 0; k < U; ++k) {
+      const uint64_t u64 = bbase + uint64_t(k) * blockDim.x + uint32_t(wv) * kWave + uint32_t(lane);
+      const bool valid = u64 < a.n;
+      const uint32_t u = valid ? uint32_t(u64) : 0u;
+      for (uint32_t li = lane; li < nl; li += kWave) mask(k, li) = 0;
+      wave_sync();
+      // Rules go in chunks of kRuleChunk: every row-bound load of a chunk is
+      // issued before any is used, and each rule's ballot is ORed into the
+      // mask of its lhs label. A rule reads delta_B(u) only when delta_B is
+      // not empty, and its pair list only when delta_C is not empty (each is
+      // empty otherwise). 8 rules per chunk: one round trip per vertex for
+      // grammars of up to 8 rules of a kind.
+      constexpr int kRuleChunk = 8;
+      for (int i0 = 0; i0 < a.g.nu; i0 += kRuleChunk) {
+        unsigned long long lo[kRuleChunk], hi[kRuleChunk];
+#pragma unroll
+        for (int q = 0; q < kRuleChunk; ++q) {
+          const int i = i0 + q;
+          lo[q] = hi[q] = 0;
+          if (i < a.g.nu && a.u_li[i] != 0xFFFFu && valid && label_nonempty(a.ctr, a.g.u_rhs[i])) {
+            const uint32_t rb = uint32_t(a.g.u_rhs[i]) * a.n + u;
+            lo[q] = a.dl.ptr[rb];
+            hi[q] = a.dl.ptr[rb + 1];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kRuleChunk; ++q) {
+          const int i = i0 + q;
+          if (i < a.g.nu && a.u_li[i] != 0xFFFFu) {
+            const unsigned long long m = __ballot(hi[q] > lo[q]);
+            if (lane == 0) mask(k, a.u_li[i]) |= m;
+          }
+        }
+      }
+      for (int i0 = 0; i0 < a.g.nb; i0 += kRuleChunk) {
+        unsigned long long d0[kRuleChunk], d1[kRuleChunk], p0[kRuleChunk], p1[kRuleChunk];
+#pragma unroll
+        for (int q = 0; q < kRuleChunk; ++q) {
+          const int i = i0 + q;
+          d0[q] = d1[q] = p0[q] = p1[q] = 0;
+          if (i < a.g.nb && a.b_li[i] != 0xFFFFu && valid) {
+            const uint32_t rb = uint32_t(a.g.b_rhs1[i]) * a.n + u;
+            if (label_nonempty(a.ctr, a.g.b_rhs1[i])) {
+              d0[q] = a.dl.ptr[rb];
+              d1[q] = a.dl.ptr[rb + 1];
+            }
+            const uint32_t slot = a.g.b_slot[i];
+            if (slot != 0xFFu && a.b2rng && label_nonempty(a.ctr, a.g.b_rhs2[i])) {
+              const uint64_t rs = uint64_t(slot) * a.n + u;
+              p0[q] = a.b2rng[2 * rs];
+              p1[q] = a.b2rng[2 * rs + 1];
+            }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kRuleChunk; ++q) {
+          const int i = i0 + q;
+          if (i < a.g.nb && a.b_li[i] != 0xFFFFu) {
+            const unsigned long long m = __ballot(d1[q] > d0[q] || p1[q] > p0[q]);
+            if (lane == 0) mask(k, a.b_li[i]) |= m;
+          }
+        }
+      }
+      wave_sync();
+      for (uint32_t li = 0; li < nl; ++li) wcnt += uint32_t(__popcll(mask(k, li)));
+    }
     if (lane == 0) s_cnt[wv] = wcnt;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -524,10 +536,13 @@ __global__ __launch_bounds__(kBlock) void k_activate(const ActArgs a) {
     __syncthreads();
     uint32_t pos = s_base;
     for (int w = 0; w < wv; ++w) pos += s_cnt[w];
-    for (uint32_t li = 0; li < a.nlhs; ++li) {
-      const unsigned long long m = s_m[wv][li];
-      if ((m >> lane) & 1ull) a.active[pos + __popcll(m & ((1ull << lane) - 1))] = uint32_t(a.lhs[li]) * a.n + u;
-      pos += uint32_t(__popcll(m));
+    for (uint32_t k = 0; k < U; ++k) {
+      const uint32_t u = uint32_t(bbase + uint64_t(k) * blockDim.x) + uint32_t(wv) * kWave + uint32_t(lane);
+      for (uint32_t li = 0; li < nl; ++li) {
+        const unsigned long long m = mask(k, li);
+        if ((m >> lane) & 1ull) a.active[pos + __popcll(m & ((1ull << lane) - 1))] = uint32_t(a.lhs[li]) * a.n + u;
+        pos += uint32_t(__popcll(m));
+      }
     }
     __syncthreads();  // s_m, s_cnt and s_base are rewritten by the next batch
   }
@@ -3547,7 +3562,12 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
     rp.act.dl = {P<unsigned long long>(dl_ptr), P<uint32_t>(dl_col)};
     rp.act.b2rng = pairs_ready ? P<unsigned long long>(pr_ptr) : nullptr;
     rp.act.pend_ptr = pend_csr().ptr;
-    if (rp.act_work) k_activate<<<grid_for(n), kBlock, 0, stream>>>(rp.act);
+    if (rp.act_work) {
+      // sub-chunks per counter reservation: 1 while a grid of 256-vertex
+      // blocks fits the device (4096 blocks), more beyond (at most 8)
+      const uint32_t sub = uint32_t(std::min<uint64_t>(8, std::max<uint64_t>(1, uint64_t(n) / (uint64_t(cus) * 16 * kBlock))));
+      k_activate<<<grid_for((n + sub - 1) / sub), kBlock, 0, stream>>>(rp.act, sub);
+    }
     if (g.nb)
       k_probe_keys<<<1, kWave, 0, stream>>>(g, P<unsigned long long>(old_ptr), P<unsigned long long>(dl_ptr),
                                             pend_csr().ptr, n, C());
