@@ -1568,6 +1568,8 @@ __global__ __launch_bounds__(kBlock) void k_expand_light(const ExpandArgs a, uin
   const uint32_t nactive = a.ctr->active;
   const bool pend = PEND && a.pend.ptr;
   unsigned long long k_sum = 0, o_sum = 0, n_sum = 0;
+  constexpr unsigned long long kLightStage = 1024;
+  unsigned long long blk_pos = 0, blk_end = 0;  // this wave's staging block
   for (uint32_t base = blockIdx.x * blockDim.x; base < nactive; base += gridDim.x * blockDim.x) {
     const uint32_t idx = base + t;
     const bool live = idx < nactive;
@@ -1683,13 +1685,25 @@ __global__ __launch_bounds__(kBlock) void k_expand_light(const ExpandArgs a, uin
         if (!seen) s_buf[c++][t] = x;
       }
     }
-    // the wave's new ids take one staging reservation
+    // the wave's new ids go to its current staging block; a wave takes
+    // kLightStage words of staging at a time (one atomic on the device-wide
+    // cursor per block, not per 64 rows: on large vertex counts those
+    // serialised), the rest of a block it leaves is a gap that the
+    // compaction skips (rows are copied by their own offsets)
     const uint32_t incl = wave_incl_scan(c, lane);
     const uint32_t tot = lane63_u32(incl);
     unsigned long long wb = 0;
     if (tot) {
-      if (lane == 0) wb = atomicAdd(&a.ctr->cursor, (unsigned long long)tot);
-      wb = uni_u64(wb);
+      if (blk_pos + tot > blk_end) {
+        const unsigned long long take = max<unsigned long long>(tot, kLightStage);
+        unsigned long long b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&a.ctr->cursor, take);
+        b0 = uni_u64(b0);
+        blk_pos = b0;
+        blk_end = b0 + take;
+      }
+      wb = blk_pos;
+      blk_pos += tot;
     }
     if (tot && wb + tot > a.staging_cap) {
       if (lane == 0) atomicExch(&a.ctr->overflow, 1u);
