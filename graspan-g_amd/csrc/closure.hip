@@ -2686,7 +2686,7 @@ struct Tuning {
   // active row (or the first round) runs the lane-per-row pass first
   // (k_expand_light); 0: never
   uint64_t light_cand = 32;
-  uint32_t light_cap = 32;  // candidates a lane holds in the light pass: 32 or 64
+  uint32_t light_cap = 32;  // candidates a lane holds in the light pass: 16, 32 or 64
 };
 
 // Sets one parameter from its name; false for an unknown name.
@@ -3610,6 +3610,12 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
           k_expand_light<true, 64><<<lg, kBlock, 0, stream>>>(la, dl);
         else
           k_expand_light<false, 64><<<lg, kBlock, 0, stream>>>(la, dl);
+      } else if (tune.light_cap <= 16) {
+        // half the LDS column per lane: twice the blocks per CU
+        if (pend_nnz)
+          k_expand_light<true, 16><<<lg, kBlock, 0, stream>>>(la, dl);
+        else
+          k_expand_light<false, 16><<<lg, kBlock, 0, stream>>>(la, dl);
       } else if (pend_nnz) {
         k_expand_light<true, 32><<<lg, kBlock, 0, stream>>>(la, dl);
       } else {
