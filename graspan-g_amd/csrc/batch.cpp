@@ -405,6 +405,9 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
   int first_err = GG_OK;
   std::string err_msg;
   std::vector<std::pair<size_t, size_t>> alone;  // lanes: single graphs to run again alone
+  // GG_BATCH_TURNS=0: the lanes' fixpoints run at the same time (no turns)
+  bool take_turns = true;
+  if (const char* tt = std::getenv("GG_BATCH_TURNS")) take_turns = tt[0] != '0';
   auto worker = [&](int lane) {
     std::string err;
     for (;;) {
@@ -423,7 +426,7 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
       gg_stats one{};
       double bpv = 0;
       const int s = run_group(b, eng[size_t(lane)], pl.ord, i, j, keep, one, bpv, trace, lane,
-                              lanes > 1 ? &turn : nullptr, err);
+                              lanes > 1 && take_turns ? &turn : nullptr, err);
       if (lanes > 1) gg::engine_release(eng[size_t(lane)]);  // the closure's memory, for the other lane
       std::lock_guard<std::mutex> lk(mu);
       if (s == GG_E_NOMEM && j - i > 1) {
