@@ -16,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -425,8 +426,16 @@ int batch_run(gg_batch* b, uint64_t max_vertices, int keep, gg_stats* stats, std
       }
       gg_stats one{};
       double bpv = 0;
-      const int s = run_group(b, eng[size_t(lane)], pl.ord, i, j, keep, one, bpv, trace, lane,
-                              lanes > 1 && take_turns ? &turn : nullptr, err);
+      int s = GG_OK;
+      try {
+        s = run_group(b, eng[size_t(lane)], pl.ord, i, j, keep, one, bpv, trace, lane,
+                      lanes > 1 && take_turns ? &turn : nullptr, err);
+      } catch (const std::bad_alloc&) {
+        // host memory for the union graph or the exported triples: nothing
+        // may escape a worker thread (or the C ABI)
+        s = GG_E_NOMEM;
+        err = "host memory exhausted building or reading a launch group";
+      }
       if (lanes > 1) gg::engine_release(eng[size_t(lane)]);  // the closure's memory, for the other lane
       std::lock_guard<std::mutex> lk(mu);
       if (s == GG_E_NOMEM && j - i > 1) {
