@@ -1756,6 +1756,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
   const bool from_defer = MODE == 0 && a.defer != nullptr;
   const uint32_t* rows_in = from_defer ? a.defer : a.active + abase;
   const uint32_t nactive = from_defer ? a.ctr->deferred : a.ctr->active - abase;
+  if (nactive == 0) return;  // (launched before the host knew the count)
   if (lane < 3) w.acc[lane] = 0;
   // rows are taken kGrab at a time and staging space kStageBlock words at a
   // time, so the two global counters see one atomic per many rows
@@ -3585,25 +3586,30 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
     if (g.nb)
       k_probe_keys<<<1, kWave, 0, stream>>>(g, P<unsigned long long>(old_ptr), P<unsigned long long>(dl_ptr),
                                             pend_csr().ptr, n, C());
-    // a round with few active rows launches only the blocks that can get
-    // some (kGrab rows per grab, 4 waves per block), not a full grid of
-    // waves that would only contend on the work counters
-    GG_TRY(read_ctr(err));
-    const uint32_t want = uint32_t((uint64_t(h_ctr->active) + grab * kWavesPerBlock - 1) / (grab * kWavesPerBlock));
-    const uint32_t grid = std::max(1u, std::min(expand_grid(), want));
+    // The first expand launch follows the activation without a read-back:
+    // its kernel reads the active row count on the device. The grid comes
+    // from the last round's active rows (a round with few of them launches
+    // only the blocks that can get some, kGrab rows per grab and 4 waves per
+    // block, not a full grid of waves that would only contend on the work
+    // counters); the waves take rows until none are left, whatever the
+    // grid.
+    const uint64_t est = std::max<uint64_t>(last_active, 1) * 2;
+    const uint32_t want = uint32_t(std::min<uint64_t>(expand_grid(), (est + grab * kWavesPerBlock - 1) /
+                                                                          (grab * kWavesPerBlock)));
+    const uint32_t grid = last_active ? std::max(1u, want) : expand_grid();
     GG_HIP(rec(o.b));
     ex0 = ev.expand.size();
-    // each launch gets its own event pair, so that the host's read-back
-    // between the light pass and the wave kernel is not timed as expand
     EventPair* xp = &ev.mk(ev.expand);
     GG_HIP(rec(xp->a));
     light_rows = 0;
-    if (light_round()) {
+    const bool light = light_round();
+    if (light) {
       // one lane per row first; the rows with more candidates than a lane
       // holds are then expanded by the wave kernel
       GG_TRY(grow(mlist, uint64_t(R) * 4, err));
       ExpandArgs la = expand_args(false);
-      const uint32_t lg = grid_for(h_ctr->active);
+      // lanes past the active count exit at once
+      const uint32_t lg = grid_for(last_active ? std::min<uint64_t>(R, 2 * last_active) : R);
       uint32_t* dl = P<uint32_t>(mlist);
       if (tune.light_cap > 32) {
         if (pend_nnz)
@@ -3624,8 +3630,10 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
       GG_HIP(rec(xp->b));
       GG_HIP(hipGetLastError());
       S.expand_launches++;
+      // the rows the light pass left go to the wave kernel, with a grid for
+      // their number; not after an overflow (the round runs again), whose
+      // further staging reservations would only inflate the regrowth
       GG_TRY(read_ctr(err));
-      light_rows = h_ctr->active - h_ctr->deferred;
       xp = nullptr;
       if (h_ctr->deferred && !h_ctr->overflow) {
         la.defer = P<uint32_t>(mlist);
@@ -3663,6 +3671,7 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
     if (xp) GG_HIP(rec(xp->b));
     GG_HIP(hipGetLastError());
     GG_TRY(read_ctr(err));
+    if (light) light_rows = h_ctr->active - h_ctr->deferred;
     if (!h_ctr->overflow) break;
     S.retries++;
     GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));
