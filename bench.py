@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-closure leg of the default line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the default line")
     ap.add_argument("--no-ooc", action="store_true", help="skip the out-of-core leg of the default line")
+    ap.add_argument("--no-as-given", action="store_true",
+                    help="skip the rule-table-as-given comparison of the default line")
     ap.add_argument("--verbose", action="store_true", help="per-round engine statistics on stderr")
     ap.add_argument("--mode", choices=("single", "batch", "serve", "ooc", "dataflow"), default="single",
                     help="single: the 1M-vertex alias closure, with the batched and serving legs as extra keys "
@@ -244,7 +246,8 @@ def main():
                          for k in ("fixpoint_ms", "expand_ms", "compact_ms", "merge_ms", "other_ms")},
         "join_candidates": s0["candidates"],
     }
-    line["rule_table_as_given"] = as_given_leg(g, stats)
+    if not args.no_as_given:
+        line["rule_table_as_given"] = as_given_leg(g, stats)
     if not args.no_batch:
         log("batched closure of the 4096-graph set")
         line["batch"] = bench_batch(args)
