@@ -73,6 +73,7 @@ constexpr uint32_t kPairRowBuf = 4;     // k_pairs: pairs per row and slot held 
 constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
 constexpr uint32_t kMergeBits = 16384;   // k_merge: rows spanning fewer ids are merged in an LDS bitmap
 constexpr uint32_t kShortRow = 1024;     // k_merge: unchanged rows up to this length move in the block copy
+constexpr uint32_t kLaneRow = 32;        // k_gather/k_merge_rows: rows up to this many elements take one lane each
 constexpr int kMaxRules = kMaxRulesPerKind;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -213,6 +214,14 @@ __device__ inline unsigned long long lane63_u64(unsigned long long x) {
 // Reductions: the last lane of the scan holds the result; it is read into a
 // scalar register, so the result is uniform.
 __device__ inline unsigned long long wave_sum_u64(unsigned long long x) { return lane63_u64(wave_incl_scan_u64(x)); }
+__device__ inline uint32_t wave_sum_u32(uint32_t x) { return lane63_u32(wave_incl_scan(x, 0)); }
+// The same for 64-bit values that are mostly small: when every lane's value
+// is below 2^26 their sum fits 32 bits, and the 32-bit scan (one DPP add per
+// step, against a carry chain of three instructions) gives it.
+__device__ inline unsigned long long wave_sum_u64_small(unsigned long long x) {
+  if (__ballot(x >= (1ull << 26)) == 0) return wave_sum_u32(uint32_t(x));
+  return wave_sum_u64(x);
+}
 
 __device__ inline uint32_t wave_min_u32(uint32_t x) {
   return lane63_u32(dpp_scan_u32(x, 0xFFFFFFFFu, [](uint32_t p, uint32_t q) { return p < q ? p : q; }));
@@ -1513,9 +1522,9 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
       wave_sync();
     }
   }
-  kcount = wave_sum_u64(kc);
+  kcount = wave_sum_u64_small(kc);
   miss = __ballot(miss_l) != 0;
-  next = wave_min_u32(nxt);
+  next = ranged ? wave_min_u32(nxt) : 0xFFFFFFFFu;  // (only ranged windows use it)
   wave_sync();
   uint32_t c = 0;
   for (uint32_t i = lane; i < words; i += kWave) {
@@ -1524,7 +1533,7 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
     c += __popc(f);
   }
   wave_sync();
-  return uint32_t(wave_sum_u64(c));
+  return wave_sum_u32(c);  // at most span
 }
 
 // Writes the set bits of w.fresh (window base c0) in ascending order at
@@ -1860,7 +1869,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
             kk += 1;
           }
         }
-        kk = wave_sum_u64(kk);
+        kk = wave_sum_u64_small(kk);
         if (kk == 0) {
           empty = true;
           break;
@@ -2013,7 +2022,36 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* active, const
     }
     flat_copy(staging, s_l, out_col, d_l, c_l, s_tmask[wi], s_sb[wi], s_db[wi], lane);
     if (!out_pk) continue;
-    unsigned long long todo = __ballot(c_l != 0 && packed_label(plab, r_l / n));
+    const bool pkr = c_l != 0 && packed_label(plab, r_l / n);
+    // rows of up to kLaneRow ids are packed by their own lane (the wave's
+    // rows at once: the loop runs as long as the longest of them), the
+    // others by the whole wave one after another below
+    const bool mine = pkr && c_l <= kLaneRow;
+    if (__ballot(mine)) {
+      if (mine) {
+        const uint32_t* ids = staging + s_l;
+        unsigned long long* ent = out_pk + d_l;
+        uint32_t wd = ids[0] >> 5, mask = 0, ne = 0;
+        for (uint32_t i = 0; i < c_l; i += 4) {
+          uint32_t x[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) x[q] = i + q < c_l ? ids[i + q] : 0u;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            if (i + q >= c_l) break;
+            if ((x[q] >> 5) != wd) {
+              ent[ne++] = ((unsigned long long)mask << 32) | wd;
+              wd = x[q] >> 5;
+              mask = 0;
+            }
+            mask |= 1u << (x[q] & 31);
+          }
+        }
+        ent[ne++] = ((unsigned long long)mask << 32) | wd;
+        out_pklen[r_l] = ne;
+      }
+    }
+    unsigned long long todo = __ballot(pkr && !mine);
     while (todo) {
       const int k = __ffsll(todo) - 1;
       todo &= todo - 1;
@@ -2310,11 +2348,52 @@ __global__ __launch_bounds__(kBlock, 7) void k_merge_rows(DCsr old, DCsr dl, uin
       e_nxt = uint32_t(lane) < oj ? old_pk[robj + uint32_t(lane)] : 0ull;
       d_nxt = uint32_t(lane) < dj ? dl.col[rdbj + uint32_t(lane)] : 0u;
     };
-    if (nb) prefetch(0);
-    for (uint32_t k = 0; k < nb; ++k) {
+    // packed rows with delta and at most kLaneRow old entries and delta ids
+    // (while their ids are not written) are merged by their own lane: a
+    // sorted merge of the entries and the ids' words
+    bool mine = false;
+    if (uint32_t(lane) < nb && old_pk && out_pk && !ids && rde_l > rdb_l)
+      mine = packed_label(plab, rr_l / n) && pl_l + (rde_l - rdb_l) <= kLaneRow;
+    if (__ballot(mine)) {
+      if (mine) {
+        const unsigned long long* oe = old_pk + rob_l;
+        const uint32_t* di = dl.col + rdb_l;
+        unsigned long long* out = out_pk + rob_l + rdb_l;
+        const uint32_t dn = uint32_t(rde_l - rdb_l);
+        uint32_t io = 0, id = 0, ne = 0;
+        unsigned long long e = pl_l ? oe[0] : 0ull;
+        uint32_t x = di[0];
+        while (io < pl_l || id < dn) {
+          // words are below 2^27: 0xFFFFFFFF marks an exhausted list
+          const uint32_t we = io < pl_l ? uint32_t(e) : 0xFFFFFFFFu;
+          const uint32_t wx = id < dn ? x >> 5 : 0xFFFFFFFFu;
+          const uint32_t wmin = min(we, wx);
+          uint32_t mask = 0;
+          if (we == wmin) {
+            mask = uint32_t(e >> 32);
+            if (++io < pl_l) e = oe[io];
+          }
+          while (id < dn && (x >> 5) == wmin) {
+            mask |= 1u << (x & 31);
+            if (++id < dn) x = di[id];
+          }
+          out[ne++] = ((unsigned long long)mask << 32) | wmin;
+        }
+        pk_len[rr_l] = ne;
+      }
+    }
+    // the other rows: one after another, the next one's first old entries
+    // and delta ids loading while the current one merges
+    unsigned long long rest = __ballot(uint32_t(lane) < nb && !mine);
+    int kn = rest ? __ffsll(rest) - 1 : -1;
+    if (kn >= 0) prefetch(uint32_t(kn));
+    for (int kk = kn; kk >= 0; kk = kn) {
+      const uint32_t k = uint32_t(kk);
+      rest &= rest - 1;  // (kk is its lowest bit)
+      kn = rest ? __ffsll(rest) - 1 : -1;
       const unsigned long long e_cur = e_nxt;
       const uint32_t d_cur = d_nxt;
-      if (k + 1 < nb) prefetch(k + 1);
+      if (kn >= 0) prefetch(uint32_t(kn));
       const uint32_t rr = uint32_t(__builtin_amdgcn_readlane(int(rr_l), int(k)));
       const uint32_t lr = rr / n;
       const unsigned long long rob = lane_u64(rob_l, k), roe = lane_u64(roe_l, k);
