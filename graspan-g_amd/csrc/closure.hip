@@ -73,7 +73,8 @@ constexpr uint32_t kPairRowBuf = 4;     // k_pairs: pairs per row and slot held 
 constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
 constexpr uint32_t kMergeBits = 16384;   // k_merge: rows spanning fewer ids are merged in an LDS bitmap
 constexpr uint32_t kShortRow = 1024;     // k_merge: unchanged rows up to this length move in the block copy
-constexpr uint32_t kLaneRow = 32;        // k_gather/k_merge_rows: rows up to this many elements take one lane each
+constexpr uint32_t kLaneGather = 32;     // k_gather: rows of up to this many ids are packed by one lane each
+constexpr uint32_t kLaneMerge = 16;      // k_merge_rows: rows of up to this many elements are merged by one lane each
 constexpr int kMaxRules = kMaxRulesPerKind;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -2023,10 +2024,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* active, const
     flat_copy(staging, s_l, out_col, d_l, c_l, s_tmask[wi], s_sb[wi], s_db[wi], lane);
     if (!out_pk) continue;
     const bool pkr = c_l != 0 && packed_label(plab, r_l / n);
-    // rows of up to kLaneRow ids are packed by their own lane (the wave's
+    // rows of up to kLaneGather ids are packed by their own lane (the wave's
     // rows at once: the loop runs as long as the longest of them), the
     // others by the whole wave one after another below
-    const bool mine = pkr && c_l <= kLaneRow;
+    const bool mine = pkr && c_l <= kLaneGather;
     if (__ballot(mine)) {
       if (mine) {
         const uint32_t* ids = staging + s_l;
@@ -2348,12 +2349,12 @@ __global__ __launch_bounds__(kBlock, 7) void k_merge_rows(DCsr old, DCsr dl, uin
       e_nxt = uint32_t(lane) < oj ? old_pk[robj + uint32_t(lane)] : 0ull;
       d_nxt = uint32_t(lane) < dj ? dl.col[rdbj + uint32_t(lane)] : 0u;
     };
-    // packed rows with delta and at most kLaneRow old entries and delta ids
+    // packed rows with delta and at most kLaneMerge old entries and delta ids
     // (while their ids are not written) are merged by their own lane: a
     // sorted merge of the entries and the ids' words
     bool mine = false;
     if (uint32_t(lane) < nb && old_pk && out_pk && !ids && rde_l > rdb_l)
-      mine = packed_label(plab, rr_l / n) && pl_l + (rde_l - rdb_l) <= kLaneRow;
+      mine = packed_label(plab, rr_l / n) && pl_l + (rde_l - rdb_l) <= kLaneMerge;
     if (__ballot(mine)) {
       if (mine) {
         const unsigned long long* oe = old_pk + rob_l;
