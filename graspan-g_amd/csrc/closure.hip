@@ -2677,47 +2677,124 @@ __global__ void k_slice_counts(const unsigned long long* ptr, uint32_t n, DSlice
   }
 }
 
+// Adds to acc the hashes of the lanes' rows, one row per lane: lane k's row
+// has len_p packed entries (at ent[b ..]) or len_i ids (at col[b ..]), its
+// part of the hash is crow = (src - base) * kHashS ^ label * kHashL, and
+// every dst is numbered from base (the same on every lane). The rows' elements
+// go as one flattened stream (flat_tiles). row: 64-entry LDS table of the
+// wave; bit[b] = b * kHashD.
+__device__ inline void hash_rows(const unsigned long long* __restrict__ ent, const uint32_t* __restrict__ col,
+                                 unsigned long long b, uint32_t len_p, uint32_t len_i, uint32_t base,
+                                 unsigned long long crow, unsigned long long* tab, unsigned long long* tmask,
+                                 unsigned long long* row, const unsigned long long* bit, int lane,
+                                 unsigned long long& acc) {
+  constexpr int U = kPosUnroll;
+  row[lane] = crow;
+  wave_sync();
+  flat_tiles<U>(b, len_p, tab, tmask, lane, [&](const int* own, const unsigned long long* pos, const bool* ok) {
+    unsigned long long en[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) en[q] = ok[q] ? ent[pos[q]] : 0ull;
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      if (!ok[q]) continue;
+      const unsigned long long cr = row[own[q]];
+      // (word * 32 - base) * kHashD modulo 2^64 (the word may start below
+      // base); each set bit b adds b * kHashD
+      const unsigned long long dk = (uint64_t(uint32_t(en[q])) * 32u - uint64_t(base)) * kHashD;
+      for (uint32_t bits = uint32_t(en[q] >> 32); bits; bits &= bits - 1)
+        acc += hash_mix(cr ^ (dk + bit[__ffs(bits) - 1]));
+    }
+  });
+  flat_tiles<U>(b, len_i, tab, tmask, lane, [&](const int* own, const unsigned long long* pos, const bool* ok) {
+    uint32_t x[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) x[q] = ok[q] ? col[pos[q]] : 0u;
+#pragma unroll
+    for (int q = 0; q < U; ++q)
+      if (ok[q]) acc += hash_mix(row[own[q]] ^ (uint64_t(x[q] - base) * kHashD));
+  });
+  wave_sync();
+}
+
 // plab.len set: rows of packed labels are read from their packed entries
 // (their ids are not written).
+// A wave takes 64 consecutive rows, one per lane (bounds, graph, the row's
+// part of the hash), and streams their elements as one flattened stream
+// (flat_tiles): packed entries first, then id rows. The lanes' sums belong to
+// one graph; they go to its checksum when the wave moves on to another graph.
+// A batch whose rows belong to several graphs (at a graph's first or last
+// rows) is taken row by row.
 __global__ __launch_bounds__(kBlock) void k_slice_checksum(DCsr c, DPacked plab, uint32_t rows, uint32_t n, DSlices s,
                                                            unsigned long long* sums) {
-  // a wave takes a contiguous range of rows and adds its hashes into a
-  // graph's sum only when the graph changes (consecutive rows of a label
-  // belong to one graph), not once per row: per-row updates of the few
-  // per-graph sums serialise
+  constexpr int U = kPosUnroll;
+  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][U], s_row[kWavesPerBlock][kWave];
+  __shared__ unsigned long long s_bit[32];  // b * kHashD: the hash's dst part steps by it within a word
+  if (threadIdx.x < 32) s_bit[threadIdx.x] = uint64_t(threadIdx.x) * kHashD;
+  __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
+  const int wi = threadIdx.x / kWave;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  const uint32_t per = (rows + nwaves - 1) / nwaves;
-  const uint32_t r0 = min(rows, wave * per), r1 = min(rows, r0 + per);
   uint32_t gcur = 0xFFFFFFFFu;
   unsigned long long acc = 0;  // this lane's part of graph gcur's sum
-  for (uint32_t r = r0; r < r1; ++r) {
-    const unsigned long long b = c.ptr[r], e = c.ptr[r + 1];
-    if (b == e) continue;
-    const uint32_t l = r / n, u = r - l * n;
-    const uint32_t g = s.vg[u];
-    if (g != gcur) {
-      const unsigned long long t = wave_sum_u64(acc);
-      if (lane == 0 && t) atomicAdd(&sums[gcur], t);
-      gcur = g;
-      acc = 0;
+  auto flush = [&](uint32_t g) {
+    if (g == gcur) return;
+    const unsigned long long t = wave_sum_u64(acc);
+    if (lane == 0 && t) atomicAdd(&sums[gcur], t);
+    gcur = g;
+    acc = 0;
+  };
+  for (uint32_t r0 = wave * kWave; r0 < rows; r0 += nwaves * kWave) {
+    const uint32_t r = r0 + uint32_t(lane);
+    unsigned long long b = 0, e = 0;
+    if (r < rows) {
+      b = c.ptr[r];
+      e = c.ptr[r + 1];
     }
-    const uint32_t base = s.v0[g], lab = l - s.l0[g] + 1;
-    if (plab.len && packed_label(plab, l)) {
-      const unsigned long long pe = b + plab.len[r];
-      for (unsigned long long i = b + lane; i < pe; i += kWave) {
-        const unsigned long long en = plab.ent[i];
-        const uint32_t w = uint32_t(en) << 5;
-        for (uint32_t bits = uint32_t(en >> 32); bits; bits &= bits - 1)
-          acc += triple_hash(u - base, w + uint32_t(__ffs(bits) - 1) - base, lab);
-      }
+    const bool live = e > b;
+    const uint32_t l = r / n, u = r - l * n;
+    uint32_t g = 0, base = 0, len_p = 0, len_i = 0;
+    unsigned long long crow = 0;
+    if (live) {
+      g = s.vg[u];
+      base = s.v0[g];
+      // the row's part of every hash: (u - base) * kHashS ^ label * kHashL
+      crow = uint64_t(u - base) * kHashS ^ uint64_t(l - s.l0[g] + 1) * kHashL;
+      if (plab.len && packed_label(plab, l))
+        len_p = plab.len[r];
+      else
+        len_i = uint32_t(e - b);
+    }
+    const unsigned long long lm = __ballot(live);
+    if (!lm) continue;
+    const int f = __ffsll(lm) - 1;
+    const uint32_t g0 = uint32_t(__builtin_amdgcn_readlane(int(g), f));
+    if (__ballot(live && g != g0) == 0) {
+      flush(g0);
+      hash_rows(plab.ent, c.col, b, len_p, len_i, uint32_t(__builtin_amdgcn_readlane(int(base), f)), crow, s_tab[wi],
+                s_tmask[wi], s_row[wi], s_bit, lane, acc);
       continue;
     }
-    for (unsigned long long i = b + lane; i < e; i += kWave) acc += triple_hash(u - base, c.col[i] - base, lab);
+    // rows of several graphs: one after another, all lanes on each row
+    for (unsigned long long todo = lm; todo; todo &= todo - 1) {
+      const int k = __ffsll(todo) - 1;
+      const uint32_t gk = uint32_t(__builtin_amdgcn_readlane(int(g), k));
+      flush(gk);
+      const uint32_t bk = uint32_t(__builtin_amdgcn_readlane(int(base), k));
+      const unsigned long long ck = lane_u64(crow, uint32_t(k)), rb = lane_u64(b, uint32_t(k));
+      const uint32_t lp = uint32_t(__builtin_amdgcn_readlane(int(len_p), k));
+      const uint32_t li = uint32_t(__builtin_amdgcn_readlane(int(len_i), k));
+      for (uint32_t i = uint32_t(lane); i < lp; i += kWave) {
+        const unsigned long long en = plab.ent[rb + i];
+        const unsigned long long dk = (uint64_t(uint32_t(en)) * 32u - uint64_t(bk)) * kHashD;
+        for (uint32_t bits = uint32_t(en >> 32); bits; bits &= bits - 1)
+          acc += hash_mix(ck ^ (dk + s_bit[__ffs(bits) - 1]));
+      }
+      for (uint32_t i = uint32_t(lane); i < li; i += kWave) acc += hash_mix(ck ^ (uint64_t(c.col[rb + i] - bk) * kHashD));
+    }
   }
-  const unsigned long long t = wave_sum_u64(acc);
-  if (lane == 0 && t) atomicAdd(&sums[gcur], t);
+  flush(0xFFFFFFFEu);
 }
 
 // ---------------------------------------------------------------------------
@@ -5109,27 +5186,35 @@ __global__ void k_item_counts(const unsigned long long* ptr, const SliceItem* it
 __global__ __launch_bounds__(kBlock) void k_item_results(DCsr c, DPacked plab, uint32_t n, const SliceItem* it,
                                                          uint32_t ni, unsigned long long* cnt,
                                                          unsigned long long* sums) {
+  // a wave takes 64 rows of the item at a time, one per lane, and streams
+  // their elements together (hash_rows)
+  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][kPosUnroll],
+      s_row[kWavesPerBlock][kWave];
+  __shared__ unsigned long long s_bit[32];
+  if (threadIdx.x < 32) s_bit[threadIdx.x] = uint64_t(threadIdx.x) * kHashD;
+  __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t wv = threadIdx.x / kWave;
   for (uint32_t i = blockIdx.x; i < ni; i += gridDim.x) {
     const SliceItem x = it[i];
     unsigned long long acc = 0;
     const bool pk = plab.len && x.re > x.rb && packed_label(plab, x.rb / n);
-    for (uint32_t r = x.rb + wv; r < x.re; r += kWavesPerBlock) {
-      const uint32_t u = x.u0 + (r - x.rb) - x.v0;
-      const unsigned long long b = c.ptr[r];
-      if (pk) {
-        const unsigned long long pe = b + plab.len[r];
-        for (unsigned long long k = b + lane; k < pe; k += kWave) {
-          const unsigned long long en = plab.ent[k];
-          const uint32_t w = uint32_t(en) << 5;
-          for (uint32_t bits = uint32_t(en >> 32); bits; bits &= bits - 1)
-            acc += triple_hash(u, w + uint32_t(__ffs(bits) - 1) - x.v0, x.lab);
+    for (uint32_t r0 = x.rb + wv * kWave; r0 < x.re; r0 += kWavesPerBlock * kWave) {
+      const uint32_t r = r0 + uint32_t(lane);
+      unsigned long long b = 0;
+      uint32_t lp = 0, li = 0;
+      if (r < x.re) {
+        b = c.ptr[r];
+        const unsigned long long e = c.ptr[r + 1];
+        if (e > b) {
+          if (pk)
+            lp = plab.len[r];
+          else
+            li = uint32_t(e - b);
         }
-        continue;
       }
-      const unsigned long long e = c.ptr[r + 1];
-      for (unsigned long long k = b + lane; k < e; k += kWave) acc += triple_hash(u, c.col[k] - x.v0, x.lab);
+      const unsigned long long crow = uint64_t(x.u0 + (r - x.rb) - x.v0) * kHashS ^ uint64_t(x.lab) * kHashL;
+      hash_rows(plab.ent, c.col, b, lp, li, x.v0, crow, s_tab[wv], s_tmask[wv], s_row[wv], s_bit, lane, acc);
     }
     acc = wave_sum_u64(acc);
     if (lane == 0 && acc) atomicAdd(&sums[x.item], acc);
