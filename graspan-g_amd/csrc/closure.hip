@@ -2638,45 +2638,6 @@ __global__ __launch_bounds__(kBlock) void k_copy_labels(DCsr old, DCsr dl, uint3
   }
 }
 
-// Sum of triple_hash over the closure.
-__global__ __launch_bounds__(kBlock) void k_checksum(DCsr c, uint32_t rows, uint32_t n, Counters* ctr) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  unsigned long long s = 0;
-  for (uint32_t r = wave; r < rows; r += nwaves) {
-    const uint32_t l = r / n, u = r - l * n;
-    const unsigned long long b = c.ptr[r], e = c.ptr[r + 1];
-    for (unsigned long long i = b + lane; i < e; i += kWave) s += triple_hash(u, c.col[i], l + 1);
-  }
-  s = wave_sum_u64(s);
-  if (lane == 0 && s) atomicAdd(&ctr_slot(ctr).checksum, s);
-}
-
-// ---------------------------------------------------------------------------
-// per-graph results of a batched closure: graph g owns vertices
-// [v0[g], v0[g+1]) and the combined labels [l0[g], l0[g] + lg[g]); in its own
-// numbering a vertex is (id - v0[g]) and a label (combined - l0[g] + 1).
-
-struct DSlices {
-  const uint32_t* v0;    // G + 1
-  const uint32_t* vg;    // graph of every vertex
-  const uint32_t* l0;    // G
-  const uint32_t* lg;    // G
-  const uint32_t* coff;  // G: offset of graph g's per-label counts
-  uint32_t G;
-};
-
-__global__ void k_slice_counts(const unsigned long long* ptr, uint32_t n, DSlices s, unsigned long long* out) {
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < s.G; g += gridDim.x * blockDim.x) {
-    const uint32_t a = s.v0[g], b = s.v0[g + 1];
-    for (uint32_t i = 0; i < s.lg[g]; ++i) {
-      const uint64_t base = uint64_t(s.l0[g] + i) * n;
-      out[s.coff[g] + i] = ptr[base + b] - ptr[base + a];
-    }
-  }
-}
-
 // Adds to acc the hashes of the lanes' rows, one row per lane: lane k's row
 // has len_p packed entries (at ent[b ..]) or len_i ids (at col[b ..]), its
 // part of the hash is crow = (src - base) * kHashS ^ label * kHashL, and
@@ -2715,6 +2676,59 @@ __device__ inline void hash_rows(const unsigned long long* __restrict__ ent, con
       if (ok[q]) acc += hash_mix(row[own[q]] ^ (uint64_t(x[q] - base) * kHashD));
   });
   wave_sync();
+}
+
+// Sum of triple_hash over the closure (its ids: rows of packed labels are
+// written first). A wave takes 64 consecutive rows, one per lane, and streams
+// their ids together (hash_rows).
+__global__ __launch_bounds__(kBlock) void k_checksum(DCsr c, uint32_t rows, uint32_t n, Counters* ctr) {
+  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][kPosUnroll],
+      s_row[kWavesPerBlock][kWave];
+  __shared__ unsigned long long s_bit[32];
+  if (threadIdx.x < 32) s_bit[threadIdx.x] = uint64_t(threadIdx.x) * kHashD;
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wi = threadIdx.x / kWave;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  unsigned long long acc = 0;
+  for (uint32_t r0 = wave * kWave; r0 < rows; r0 += nwaves * kWave) {
+    const uint32_t r = r0 + uint32_t(lane);
+    unsigned long long b = 0, e = 0;
+    if (r < rows) {
+      b = c.ptr[r];
+      e = c.ptr[r + 1];
+    }
+    const uint32_t l = r / n, u = r - l * n;
+    hash_rows(nullptr, c.col, b, 0u, uint32_t(e - b), 0u, uint64_t(u) * kHashS ^ uint64_t(l + 1) * kHashL, s_tab[wi],
+              s_tmask[wi], s_row[wi], s_bit, lane, acc);
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(&ctr_slot(ctr).checksum, acc);
+}
+
+// ---------------------------------------------------------------------------
+// per-graph results of a batched closure: graph g owns vertices
+// [v0[g], v0[g+1]) and the combined labels [l0[g], l0[g] + lg[g]); in its own
+// numbering a vertex is (id - v0[g]) and a label (combined - l0[g] + 1).
+
+struct DSlices {
+  const uint32_t* v0;    // G + 1
+  const uint32_t* vg;    // graph of every vertex
+  const uint32_t* l0;    // G
+  const uint32_t* lg;    // G
+  const uint32_t* coff;  // G: offset of graph g's per-label counts
+  uint32_t G;
+};
+
+__global__ void k_slice_counts(const unsigned long long* ptr, uint32_t n, DSlices s, unsigned long long* out) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < s.G; g += gridDim.x * blockDim.x) {
+    const uint32_t a = s.v0[g], b = s.v0[g + 1];
+    for (uint32_t i = 0; i < s.lg[g]; ++i) {
+      const uint64_t base = uint64_t(s.l0[g] + i) * n;
+      out[s.coff[g] + i] = ptr[base + b] - ptr[base + a];
+    }
+  }
 }
 
 // plab.len set: rows of packed labels are read from their packed entries
