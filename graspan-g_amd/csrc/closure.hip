@@ -382,6 +382,13 @@ __global__ __launch_bounds__(kBlock) void k_cross(DCsr c, uint32_t rows, const u
     if (hist[i]) atomicAdd(&out[i], hist[i]);
 }
 
+// The per-XCD work indices start again (a second expand launch over another
+// row list).
+__global__ void k_reset_work(Counters* ctr) {
+  const int t = int(threadIdx.x);
+  if (t < int(kXcds)) ctr->work_xcd[t].v = 0;
+}
+
 // Active list = every row 0..rows-1 (region operators).
 // Before rows are appended to a round's active list for a second expand
 // launch: that launch's rows start at the current end, and the work indices
@@ -1537,14 +1544,14 @@ __device__ __attribute__((always_inline)) inline uint32_t process_chunk(const Ex
   return wave_sum_u32(c);  // at most span
 }
 
-// Writes the set bits of w.fresh (window base c0) in ascending order at
-// staging[pos ..].
-__device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t span, unsigned long long pos, WaveLds& w,
-                                  int lane) {
+// Writes the set bits of the bitmap `fresh` (window base c0) in ascending
+// order at staging[pos ..].
+__device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t span, unsigned long long pos,
+                                  const uint32_t* fresh, int lane) {
   const uint32_t words = (span + 31) >> 5;
   for (uint32_t w0 = 0; w0 < words; w0 += kWave) {
     const uint32_t wi = w0 + lane;
-    uint32_t bits = wi < words ? w.fresh[wi] : 0u;
+    uint32_t bits = wi < words ? fresh[wi] : 0u;
     const uint32_t pc = __popc(bits);
     const uint32_t incl = wave_incl_scan(pc, lane);
     unsigned long long p = pos + (incl - pc);
@@ -1554,6 +1561,384 @@ __device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t spa
       a.staging[p++] = c0 + wi * 32 + uint32_t(b);
     }
     pos += lane63_u32(incl);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row groups: up to kGroupRows consecutive rows of the active list per wave,
+// in one guessed window [c0, c0 + span) shared by the group (centred on its
+// middle row's vertex; the active list is vertex-major, so the group's rows
+// are one or a few neighbouring vertices), each row with its own seen/fresh
+// bitmaps (words [j*words, (j+1)*words) for row j). The seed rows and the
+// lists of the whole group go through one set of flattened streams, every
+// list carrying its row's bitmap offset, so what the wave kernel pays per row
+// (segment bounds, key batches, stream set-ups, reductions) is paid once per
+// group. A row with a candidate outside the window goes to the deferred
+// list, which the wave kernel expands afterwards (its own windows); its
+// counts are left to that kernel.
+constexpr uint32_t kGroupRows = 4;
+
+struct GroupLds {
+  uint32_t seen[kChunkWords];
+  uint32_t fresh[kChunkWords];
+  unsigned long long lbase[kWave];      // stream: base address per non-empty list
+  uint32_t lwoff[kWave];                // stream: bitmap offset (words) per non-empty list
+  unsigned long long tmask[kUnroll];    // stream: list starts inside the tile
+  unsigned long long kacc[kGroupRows];  // candidates per row of the group
+  unsigned long long acc[3];            // the wave's K, O and N counts
+};
+
+// expand_stream with a bitmap offset per list: lane i's list
+// src[lb .. lb+len) (src may differ from lane to lane) carries woff, and
+// elem(x, woff) gets the offset of the element's list.
+template <class T, class F>
+__device__ inline void expand_stream_w(const T* src, unsigned long long lb, uint32_t len, uint32_t woff, GroupLds& w,
+                                       int lane, F&& elem) {
+  constexpr int kU = sizeof(T) == 8 ? kUnroll / 2 : kUnroll;
+  constexpr uint32_t kTile = kWave * kU;
+  unsigned long long total;
+  const unsigned long long incl = stream_prefix(len, total);
+  if (total == 0) return;
+  const unsigned long long start = incl - len;
+  const uint32_t rank = stream_rank(len);
+  wave_sync();
+  if (len) {
+    w.lbase[rank] = reinterpret_cast<unsigned long long>(src) + (lb - start) * sizeof(T);
+    w.lwoff[rank] = woff;
+  }
+  if (lane < kU) w.tmask[lane] = 0;
+  wave_sync();
+  const unsigned long long full = total / kTile * kTile;
+  for (unsigned long long t0 = 0; t0 < full; t0 += kTile) {
+    const unsigned long long tb = (t0 + uint32_t(lane)) * sizeof(T);
+    T x[kU];
+    uint32_t wo[kU];
+    stream_tile<kU>(start, len, t0, w.tmask, lane, [&](int q, uint32_t rk) {
+      x[q] = *reinterpret_cast<const __attribute__((address_space(1))) T*>(w.lbase[rk] + tb +
+                                                                            uint32_t(q * kWave) * sizeof(T));
+      wo[q] = w.lwoff[rk];
+    });
+#pragma unroll
+    for (int q = 0; q < kU; ++q) elem(x[q], wo[q]);
+  }
+  if (full < total) {
+    const unsigned long long t0 = full;
+    const int lim = int(total - full);
+    const int nq = (lim + kWave - 1) / kWave;
+    const unsigned long long tb = (t0 + uint32_t(lane)) * sizeof(T);
+    T x[kU];
+    uint32_t wo[kU];
+    stream_tile<kU>(
+        start, len, t0, w.tmask, lane,
+        [&](int q, uint32_t rk) {
+          x[q] = 0;
+          wo[q] = 0;
+          if (lane < lim - q * kWave) {
+            x[q] = *reinterpret_cast<const __attribute__((address_space(1))) T*>(w.lbase[rk] + tb +
+                                                                                  uint32_t(q * kWave) * sizeof(T));
+            wo[q] = w.lwoff[rk];
+          }
+        },
+        nq);
+#pragma unroll
+    for (int q = 0; q < kU; ++q)
+      if (q < nq && lane < lim - q * kWave) elem(x[q], wo[q]);
+  }
+  wave_sync();
+}
+
+// The candidate lists of the group's rows (lane j < G holds row j's A and
+// u): for_each_list's MODE 0 segments, every one tagged with its row, so
+// visit(col, lb, len, j) and visitw(ent, pb, plen, len, j) name the row j of
+// the lane's list.
+template <bool PEND, class V, class W>
+__device__ inline void for_each_list_group(const ExpandArgs& a, uint32_t G, uint32_t A_l, uint32_t u_l, int lane,
+                                           V&& visit, W&& visitw) {
+  const uint32_t n = a.n;
+  // a segment descriptor: kind (bits 0-1), packed rows (bit 2), C (bits
+  // 8-15), row of the group (bits 16-17)
+  enum { kSegU = 0, kSegD = 1, kSegP = 2 };
+  uint32_t s_desc = 0, s_row = 0;
+  uint32_t ns = 0;  // segments in the current batch (uniform)
+  auto run_group = [&]() {
+    unsigned long long b = 0, e = 0;
+    const uint32_t kind_l = s_desc & 3u, tag_l = (s_desc >> 16) & 3u;
+    if (uint32_t(lane) < ns) {
+      // pair lists keep two bounds per row (rows are not contiguous)
+      const unsigned long long* ptr = kind_l == kSegP ? a.b2.ptr + s_row : a.dl.ptr;
+      b = ptr[s_row];
+      e = ptr[s_row + 1];
+    }
+    {
+      const bool ul = uint32_t(lane) < ns && kind_l == kSegU;
+      const bool upk = ul && (s_desc & 4u) && a.dpk.len && e > b;
+      visitw(a.dpk.ent, upk ? b : 0ull, upk ? a.dpk.len[s_row] : 0u, upk ? uint32_t(e - b) : 0u, tag_l);
+      visit(a.dl.col, ul && !upk ? b : 0ull, ul && !upk ? uint32_t(e - b) : 0u, tag_l);
+    }
+    const unsigned long long klen = (uint32_t(lane) < ns && kind_l != kSegU) ? e - b : 0ull;
+    unsigned long long total;  // (a key list is a row: fewer than 2^32 keys)
+    const unsigned long long incl = stream_prefix(uint32_t(klen), total);
+    for (unsigned long long t0 = 0; t0 < total; t0 += kWave) {
+      const unsigned long long t = t0 + lane;
+      // segment of key t: the first j whose inclusive end is past t
+      int j = 0;
+      unsigned long long ex = 0;
+      for (uint32_t q = 0; q + 1 < ns; ++q) {
+        const unsigned long long iq =
+            (unsigned long long)uint32_t(__builtin_amdgcn_readlane(int(uint32_t(incl)), int(q))) |
+            ((unsigned long long)uint32_t(__builtin_amdgcn_readlane(int(uint32_t(incl >> 32)), int(q))) << 32);
+        if (iq <= t) {
+          j = int(q) + 1;
+          ex = iq;
+        }
+      }
+      const uint32_t desc = __shfl(s_desc, j, kWave);
+      const uint32_t kind = desc & 3u, C = (desc >> 8) & 0xFFu, tag = (desc >> 16) & 3u;
+      const bool pk = (desc & 4u) != 0;
+      const unsigned long long bj = __shfl(b, j, kWave);
+      unsigned long long ob = 0, db = 0, qb = 0;
+      uint32_t ol = 0, dlen = 0, pl = 0, dpl = 0, ql = 0;
+      bool is_pk = false, d_pk = false;
+      if (t < total) {
+        const unsigned long long k = bj + (t - ex);
+        const uint32_t v = kind == kSegP ? a.b2.col[k] : a.dl.col[k];
+        const uint32_t rc = C * n + v;
+        db = a.dl.ptr[rc];
+        dlen = uint32_t(a.dl.ptr[rc + 1] - db);
+        if (pk && a.dpk.len && dlen) {
+          d_pk = true;
+          dpl = a.dpk.len[rc];
+        }
+        if (kind == kSegD) {
+          ob = a.old.ptr[rc];
+          ol = uint32_t(a.old.ptr[rc + 1] - ob);
+          if (pk && a.pk.len) {
+            is_pk = true;
+            pl = a.pk.len[rc];
+          }
+          if (PEND && a.pend.ptr) {
+            qb = a.pend.ptr[rc];
+            ql = uint32_t(a.pend.ptr[rc + 1] - qb);
+          }
+        }
+      }
+      visitw(a.pk.ent, is_pk ? ob : 0ull, is_pk ? pl : 0u, is_pk ? ol : 0u, tag);
+      visitw(a.dpk.ent, d_pk ? db : 0ull, d_pk ? dpl : 0u, d_pk ? dlen : 0u, tag);
+      visit(a.old.col, is_pk ? 0ull : ob, is_pk ? 0u : ol, tag);
+      visit(a.dl.col, d_pk ? 0ull : db, d_pk ? 0u : dlen, tag);
+      if (PEND && a.pend.ptr) visit(a.pend.col, qb, ql, tag);
+    }
+    ns = 0;
+  };
+  auto add = [&](uint32_t kind, uint32_t label, uint32_t row, bool pk, uint32_t tag) {
+    if (uint32_t(lane) == ns) {
+      s_desc = kind | (pk ? 4u : 0u) | (label << 8) | (tag << 16);
+      s_row = row;
+    }
+    if (++ns == uint32_t(kWave)) run_group();
+  };
+  for (uint32_t j = 0; j < G; ++j) {
+    const uint32_t A = uint32_t(__builtin_amdgcn_readlane(int(A_l), int(j)));
+    const uint32_t u = uint32_t(__builtin_amdgcn_readlane(int(u_l), int(j)));
+    for (int i = 0; i < a.g.nu; ++i)
+      if (a.g.u_lhs[i] == A)
+        add(kSegU, 0, uint32_t(a.g.u_rhs[i]) * n + u, packed_label(a.pk, uint32_t(a.g.u_rhs[i])), j);
+    for (int i = 0; i < a.g.nb; ++i) {
+      if (a.g.b_lhs[i] != A) continue;
+      const uint32_t B = a.g.b_rhs1[i], C = a.g.b_rhs2[i];
+      add(kSegD, C, B * n + u, packed_label(a.pk, C), j);
+      const uint32_t slot = a.g.b_slot[i];
+      if (slot != 0xFFu && a.b2.ptr) add(kSegP, C, slot * n + u, false, j);
+    }
+  }
+  if (ns) run_group();
+}
+
+// The active rows (MODE 0, guessed windows of a.guess ids, a power of two
+// that leaves room for at least two rows' bitmaps) in groups, see above.
+// Rows that do not fit their group's window are appended to `defer`
+// (ctr->deferred of them, also counted as misses).
+template <bool PEND>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandWaves, kExpandWaves))) void
+k_expand_group(const ExpandArgs a, uint32_t* defer) {
+  __shared__ GroupLds s_lds[kWavesPerBlock];
+  const int lane = threadIdx.x & (kWave - 1);
+  GroupLds& w = s_lds[threadIdx.x / kWave];
+  const uint32_t nactive = a.ctr->active;
+  if (nactive == 0) return;  // (launched before the host knew the count)
+  if (lane < 3) w.acc[lane] = 0;
+  const uint32_t n = a.n;
+  const uint32_t span = a.guess, words = span >> 5, wshift = 31u - uint32_t(__clz(int(words)));
+  const uint32_t gmax = min(kGroupRows, kChunkWords / words);
+  unsigned long long blk_pos = 0, blk_end = 0;
+  auto reserve = [&](uint32_t cnt) -> unsigned long long {
+    if (blk_pos + cnt > blk_end) {
+      const unsigned long long take = max<unsigned long long>(cnt, kStageBlock);
+      unsigned long long b = 0;
+      if (lane == 0) b = atomicAdd(&a.ctr->cursor, take);
+      b = uni_u64(b);
+      blk_pos = b;
+      blk_end = b + take;
+    }
+    const unsigned long long p = blk_pos;
+    blk_pos += cnt;
+    return p;
+  };
+  // each XCD first drains its own eighth of the active list (see k_expand)
+  uint32_t seg = blockIdx.x % kXcds, visited = 0;
+  for (;;) {
+    uint32_t i0 = 0, G = 0;
+    while (visited < kXcds) {
+      const uint32_t sb = uint32_t((unsigned long long)nactive * seg / kXcds);
+      const uint32_t se = uint32_t((unsigned long long)nactive * (seg + 1) / kXcds);
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(&a.ctr->work_xcd[seg].v, gmax);
+      b = sb + uni_u32(b);
+      if (b < se) {
+        i0 = b;
+        G = min(gmax, se - b);
+        break;
+      }
+      seg = (seg + 1) % kXcds;
+      ++visited;
+    }
+    if (!G) break;
+    uint32_t r_l = 0, A_l = 0, u_l = 0;
+    if (uint32_t(lane) < G) {
+      r_l = a.active[i0 + lane];
+      A_l = r_l / n;
+      u_l = r_l - A_l * n;
+    }
+    // the shared window, word-aligned, inside [0, n) (n > span)
+    const uint32_t um = uint32_t(__builtin_amdgcn_readlane(int(u_l), int(G / 2)));
+    uint32_t c0 = (um > span / 2 ? um - span / 2 : 0u) & ~31u;
+    if (c0 + span > n) c0 = (n - span) & ~31u;
+    const uint32_t w0 = c0 >> 5;
+    for (uint32_t i = lane; i < G * words; i += kWave) {
+      w.seen[i] = 0;
+      w.fresh[i] = 0;
+    }
+    if (lane < int(kGroupRows)) w.kacc[lane] = 0;
+    wave_sync();
+    // seeds: row j's old row (lane j), delta row (lane G + j) and pending
+    // row (lane 2G + j); orl = that row's length in ids (the O count)
+    unsigned long long orl = 0;
+    {
+      const uint32_t k = uint32_t(lane);
+      const bool in_old = k < G, in_dl = k >= G && k < 2 * G, in_pd = PEND && a.pend.ptr && k >= 2 * G && k < 3 * G;
+      const uint32_t j = in_old ? k : (in_dl ? k - G : (in_pd ? k - 2 * G : 0u));
+      const uint32_t rj = uint32_t(__shfl(int(r_l), int(j), kWave));
+      const uint32_t Aj = uint32_t(__shfl(int(A_l), int(j), kWave));
+      unsigned long long pb = 0, ib = 0;
+      uint32_t plen = 0, ilen = 0;
+      const uint32_t* src = a.dl.col;
+      if (in_old) {
+        const unsigned long long ob = a.old.ptr[rj], oe = a.old.ptr[rj + 1];
+        orl = oe - ob;
+        if (a.pk.len && packed_label(a.pk, Aj)) {
+          pb = ob;
+          plen = a.pk.len[rj];
+        } else {
+          src = a.old.col;
+          ib = ob;
+          ilen = uint32_t(oe - ob);
+        }
+      } else if (in_dl) {
+        ib = a.dl.ptr[rj];
+        ilen = uint32_t(a.dl.ptr[rj + 1] - ib);
+        orl = ilen;
+      } else if (in_pd) {
+        src = a.pend.col;
+        ib = a.pend.ptr[rj];
+        ilen = uint32_t(a.pend.ptr[rj + 1] - ib);
+        orl = ilen;
+      }
+      const uint32_t woff = j * words;
+      expand_stream_w(a.pk.ent, pb, plen, woff, w, lane, [&](unsigned long long e, uint32_t wo) {
+        const uint32_t wi = uint32_t(e) - w0;
+        if (wi < words) atomicOr(&w.seen[wo + wi], uint32_t(e >> 32));
+      });
+      expand_stream_w(src, ib, ilen, woff, w, lane, [&](uint32_t x, uint32_t wo) {
+        const uint32_t off = x - c0;
+        if (off < span) atomicOr(&w.seen[wo + (off >> 5)], 1u << (off & 31));
+      });
+    }
+    wave_sync();
+    // candidates: plain LDS ORs into the rows' fresh bitmaps; a candidate
+    // outside the window marks its row missed
+    uint32_t miss_l = 0;
+    auto mark = [&](uint32_t x, uint32_t wo) {
+      const uint32_t off = x - c0;
+      if (off < span)
+        __hip_atomic_fetch_or(&w.fresh[wo + (off >> 5)], 1u << (off & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else
+        miss_l |= 1u << (wo >> wshift);
+    };
+    auto mark_word = [&](unsigned long long e, uint32_t wo) {
+      const uint32_t wi = uint32_t(e) - w0;
+      if (wi < words)
+        __hip_atomic_fetch_or(&w.fresh[wo + wi], uint32_t(e >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else
+        miss_l |= 1u << (wo >> wshift);
+    };
+    for_each_list_group<PEND>(
+        a, G, A_l, u_l, lane,
+        [&](const uint32_t* col, unsigned long long lb, uint32_t len, uint32_t tag) {
+          if (len) __hip_atomic_fetch_add(&w.kacc[tag], (unsigned long long)len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          expand_stream_w(col, lb, len, tag * words, w, lane, mark);
+        },
+        [&](const unsigned long long* ent, unsigned long long pb, uint32_t pl, uint32_t len, uint32_t tag) {
+          if (len) __hip_atomic_fetch_add(&w.kacc[tag], (unsigned long long)len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          expand_stream_w(ent, pb, pl, tag * words, w, lane, mark_word);
+        });
+    wave_sync();
+    // per row: new ids = fresh minus seen, then its output or its deferral
+    for (uint32_t j = 0; j < G; ++j) {
+      const uint32_t rj = uint32_t(__builtin_amdgcn_readlane(int(r_l), int(j)));
+      if (__ballot((miss_l >> j) & 1u)) {
+        if (lane == 0) {
+          defer[atomicAdd(&a.ctr->deferred, 1u)] = rj;
+          atomicAdd(&ctr_slot(a.ctr).misses, 1u);
+        }
+        continue;
+      }
+      uint32_t* fr = w.fresh + j * words;
+      const uint32_t* sn = w.seen + j * words;
+      uint32_t c = 0;
+      for (uint32_t i = lane; i < words; i += kWave) {
+        const uint32_t f = fr[i] & ~sn[i];
+        fr[i] = f;
+        c += __popc(f);
+      }
+      c = wave_sum_u32(c);
+      wave_sync();
+      unsigned long long pos = 0;
+      if (c) {
+        pos = reserve(c);
+        if (pos + c <= a.staging_cap)
+          emit_chunk(a, c0, span, pos, fr, lane);
+        else if (lane == 0)
+          atomicExch(&a.ctr->overflow, 1u);
+      }
+      const unsigned long long kj = w.kacc[j];
+      const unsigned long long oj = lane_u64(orl, j) + lane_u64(orl, G + j) + lane_u64(orl, min(2 * G + j, 63u));
+      if (lane == 0) {
+        a.seg_off[rj] = pos;
+        a.seg_cnt[rj] = c;
+        w.acc[0] += kj;
+        w.acc[2] += c;
+        if (kj) w.acc[1] += oj;
+      }
+    }
+    wave_sync();
+  }
+  wave_sync();
+  if (lane == 0) {
+    CtrSlot& cs = ctr_slot(a.ctr);
+    const unsigned long long kc = w.acc[0], orr = w.acc[1], ne = w.acc[2];
+    if (kc) atomicAdd(&cs.cand, kc);
+    if (orr) atomicAdd(&cs.oreads, orr);
+    if (ne) atomicAdd(&cs.newe, ne);
   }
 }
 
@@ -1934,12 +2319,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
             if (c) {
               pos = reserve(c);
               if (pos + c <= a.staging_cap)
-                emit_chunk(a, c0, span, pos, w, lane);
+                emit_chunk(a, c0, span, pos, w.fresh, lane);
               else if (lane == 0)
                 atomicExch(&a.ctr->overflow, 1u);
             }
           } else {
-            emit_chunk(a, c0, span, p, w, lane);
+            emit_chunk(a, c0, span, p, w.fresh, lane);
             p += c;
           }
           if (last) break;
@@ -3416,6 +3801,7 @@ class Engine {
   }
 
   int expand_blocks_per_cu = 4;
+  bool expand_group = true;  // rows in groups (k_expand_group) where the guessed windows allow
   int merge_blocks_per_cu = 8, merge_rows_blocks_per_cu = 8, gather_blocks_per_cu = 7;
   // the tuned parameters for this run: file beside the library, then GG_TUNE_*
   void apply_tuning() {
@@ -3424,6 +3810,7 @@ class Engine {
     merge_blocks_per_cu = tune.merge_bpc > 0 ? std::min(tune.merge_bpc, occ_merge) : occ_merge;
     merge_rows_blocks_per_cu = tune.merge_rows_bpc > 0 ? std::min(tune.merge_rows_bpc, occ_merge_rows) : occ_merge_rows;
     gather_blocks_per_cu = tune.gather_bpc > 0 ? std::min(tune.gather_bpc, occ_gather) : occ_gather;
+    if (const char* eg = std::getenv("GG_EXPAND_GROUP")) expand_group = eg[0] != '0';
     tune.grab_light = std::max(1u, tune.grab_light);
     tune.grab_heavy = std::max(1u, tune.grab_heavy);
     grab = tune.grab_light;
@@ -3434,6 +3821,12 @@ class Engine {
   uint64_t last_cand = 0, last_active = 0;
   // the lane-per-row pass runs in the first round and after rounds with
   // few candidates per active row
+  // k_expand_group: guessed windows of a power of two ids that leave room
+  // for two rows' bitmaps or more (GG_EXPAND_GROUP=0 turns it off)
+  bool group_rows() const {
+    if (!expand_group || !guess_bits || n <= guess_bits || (guess_bits & (guess_bits - 1))) return false;
+    return kChunkWords / (guess_bits / 32) >= 2;
+  }
   bool light_round() const {
     return tune.light_cand && (last_active == 0 || last_cand <= tune.light_cand * last_active);
   }
@@ -3819,6 +4212,26 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
           k_expand<0><<<dgrid, kBlock, 0, stream>>>(la);
         S.expand_launches++;
       }
+    } else if (group_rows()) {
+      // rows in groups of up to kGroupRows per wave; the rows that do not
+      // fit their group's window then go to the wave kernel, whose launch
+      // reads their count on the device (no read-back: with none it exits)
+      GG_TRY(grow(mlist, uint64_t(R) * 4, err));
+      ExpandArgs xa = expand_args(false);
+      const uint32_t gwant = uint32_t(std::min<uint64_t>(
+          expand_grid(), (est + uint64_t(kGroupRows) * kWavesPerBlock - 1) / (uint64_t(kGroupRows) * kWavesPerBlock)));
+      const uint32_t ggrid = last_active ? std::max(1u, gwant) : expand_grid();
+      if (pend_nnz)
+        k_expand_group<true><<<ggrid, kBlock, 0, stream>>>(xa, P<uint32_t>(mlist));
+      else
+        k_expand_group<false><<<ggrid, kBlock, 0, stream>>>(xa, P<uint32_t>(mlist));
+      k_reset_work<<<1, kWave, 0, stream>>>(C());
+      xa.defer = P<uint32_t>(mlist);
+      if (pend_nnz)
+        k_expand<0, true><<<expand_grid(), kBlock, 0, stream>>>(xa);
+      else
+        k_expand<0><<<expand_grid(), kBlock, 0, stream>>>(xa);
+      S.expand_launches += 2;
     } else {
       if (pend_nnz)
         k_expand<0, true><<<grid, kBlock, 0, stream>>>(expand_args(false));
