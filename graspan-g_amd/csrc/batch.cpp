@@ -84,6 +84,8 @@ void add_stats(gg_stats& a, const gg_stats& b) {
   a.device_bytes_peak = std::max(a.device_bytes_peak, b.device_bytes_peak);
   a.pair_reruns += b.pair_reruns;
   a.light_rows += b.light_rows;
+  a.group_rows += b.group_rows;
+  a.group_deferred += b.group_deferred;
 }
 
 // Launch groups are planned one at a time, each when a lane is free to

@@ -3477,6 +3477,7 @@ class Engine {
   unsigned long long arena_h_u64 = 0;
   uint32_t expand_grid_cap = 0;  // 0: as many blocks as fit the device
   uint64_t light_rows = 0;       // rows the current round's light pass expanded
+  bool grouped = false;          // the current round's rows went through k_expand_group
   Tuning tune;                   // the tuned parameters of the current run
   int occ_expand = 4, occ_merge = 8, occ_merge_rows = 8, occ_gather = 7;  // blocks per CU the kernels allow
 
@@ -4166,6 +4167,7 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
     EventPair* xp = &ev.mk(ev.expand);
     GG_HIP(rec(xp->a));
     light_rows = 0;
+    grouped = false;
     const bool light = light_round();
     if (light) {
       // one lane per row first; the rows with more candidates than a lane
@@ -4232,6 +4234,7 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
       else
         k_expand<0><<<expand_grid(), kBlock, 0, stream>>>(xa);
       S.expand_launches += 2;
+      grouped = true;
     } else {
       if (pend_nnz)
         k_expand<0, true><<<grid, kBlock, 0, stream>>>(expand_args(false));
@@ -4262,6 +4265,10 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
   }
   N = h_ctr->newe;
   S.light_rows += light_rows;
+  if (grouped) {
+    S.group_rows += inject ? h_ctr->active_base : h_ctr->active;
+    S.group_deferred += h_ctr->deferred;
+  }
   last_cand = h_ctr->cand;
   last_active = h_ctr->active;
   const uint32_t used_guess = guess_bits;

@@ -62,6 +62,7 @@ class Stats(ctypes.Structure):
         ("pairs_run", ctypes.c_uint32), ("pairs_skipped", ctypes.c_uint32), ("spill_bytes", ctypes.c_uint64),
         ("pair_reruns", ctypes.c_uint32), ("uploads_ahead", ctypes.c_uint32),
         ("spills_ahead", ctypes.c_uint32), ("pad0", ctypes.c_uint32), ("light_rows", ctypes.c_uint64),
+        ("group_rows", ctypes.c_uint64), ("group_deferred", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

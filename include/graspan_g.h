@@ -81,6 +81,8 @@ typedef struct gg_stats {
   uint32_t spills_ahead;    /* out-of-core: partitions copied out during a step to make room for the next */
   uint32_t pad0;
   uint64_t light_rows;      /* rows expanded one per lane (rounds with few candidates per row) */
+  uint64_t group_rows;      /* rows expanded four per wave in a shared guessed window */
+  uint64_t group_deferred;  /* of those, rows with a candidate outside the window (expanded again alone) */
 } gg_stats;
 
 /* ---- context ------------------------------------------------------------ */

@@ -109,6 +109,33 @@ def test_gpu_multichunk_rows(built):
     check_against_oracle(g)
 
 
+def test_gpu_row_groups_and_their_deferrals(built, monkeypatch):
+    """Rows expanded four per wave in one shared guessed window
+    (k_expand_group): the closure and the P/K/O/N counters equal the
+    oracle's, also when random far edges send rows to the wave kernel (a
+    candidate outside the group's window), and GG_EXPAND_GROUP=0 (every row
+    through the wave kernel) gives the same."""
+    monkeypatch.setenv("GG_TUNE_LIGHT_CAND", "0")  # no lane-per-row rounds: every round groups its rows
+    say("row groups: alias n=20000")
+    g = gen.alias(20_000, seed=5)
+    _, st, _ = check_against_oracle(g)
+    assert st.group_rows > 0
+    say("row groups: alias n=20000 plus far A edges")
+    rng = np.random.default_rng(9)
+    far = np.flatnonzero(rng.random(g.n) < 0.0005).astype(np.uint32)  # 7 edges: closure 2.5e7
+    fdst = rng.integers(0, g.n, far.shape[0]).astype(np.uint32)
+    a_lab = np.uint8(gen.label_ids(g.rule_lines)["A"])
+    s, d, l = gen._dedup_sorted(np.concatenate([g.src, far]), np.concatenate([g.dst, fdst]),
+                                np.concatenate([g.lab, np.full(far.shape[0], a_lab, np.uint8)]))
+    gf = gen.GraphSpec(g.n, g.rule_lines, s, d, l, "alias-far-n20000")
+    _, st2, r2 = check_against_oracle(gf)
+    assert st2.group_rows > 0 and st2.group_deferred > 0
+    monkeypatch.setenv("GG_EXPAND_GROUP", "0")
+    say("row groups off: alias n=20000 plus far A edges")
+    _, st3, r3 = check_against_oracle(gf, full_triples=False)
+    assert st3.group_rows == 0 and r3.checksum == r2.checksum
+
+
 def _paired_blocks(n, seed):
     """Block dataflow graph plus far edges u -> u + n/2 from 2% of the
     vertices of the lower half: a lower block reaches the blocks half the id
