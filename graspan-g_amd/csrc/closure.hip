@@ -957,6 +957,7 @@ struct ExpandArgs {
   // MODE 0: when set, the rows to expand are the ctr->deferred rows listed
   // here (those the light pass left), not the active list
   const uint32_t* defer;
+  uint32_t group;  // k_expand_group: rows per group at most
 };
 
 // The span of destination region d of a partition operator: global vertex
@@ -1576,7 +1577,7 @@ __device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t spa
 // group. A row with a candidate outside the window goes to the deferred
 // list, which the wave kernel expands afterwards (its own windows); its
 // counts are left to that kernel.
-constexpr uint32_t kGroupRows = 4;
+constexpr uint32_t kGroupRows = 8;  // at most (the tuned group_rows, and as many bitmaps as fit)
 
 struct GroupLds {
   uint32_t seen[kChunkWords];
@@ -1656,13 +1657,13 @@ __device__ inline void for_each_list_group(const ExpandArgs& a, uint32_t G, uint
                                            V&& visit, W&& visitw) {
   const uint32_t n = a.n;
   // a segment descriptor: kind (bits 0-1), packed rows (bit 2), C (bits
-  // 8-15), row of the group (bits 16-17)
+  // 8-15), row of the group (bits 16-18)
   enum { kSegU = 0, kSegD = 1, kSegP = 2 };
   uint32_t s_desc = 0, s_row = 0;
   uint32_t ns = 0;  // segments in the current batch (uniform)
   auto run_group = [&]() {
     unsigned long long b = 0, e = 0;
-    const uint32_t kind_l = s_desc & 3u, tag_l = (s_desc >> 16) & 3u;
+    const uint32_t kind_l = s_desc & 3u, tag_l = (s_desc >> 16) & 7u;
     if (uint32_t(lane) < ns) {
       // pair lists keep two bounds per row (rows are not contiguous)
       const unsigned long long* ptr = kind_l == kSegP ? a.b2.ptr + s_row : a.dl.ptr;
@@ -1693,7 +1694,7 @@ __device__ inline void for_each_list_group(const ExpandArgs& a, uint32_t G, uint
         }
       }
       const uint32_t desc = __shfl(s_desc, j, kWave);
-      const uint32_t kind = desc & 3u, C = (desc >> 8) & 0xFFu, tag = (desc >> 16) & 3u;
+      const uint32_t kind = desc & 3u, C = (desc >> 8) & 0xFFu, tag = (desc >> 16) & 7u;
       const bool pk = (desc & 4u) != 0;
       const unsigned long long bj = __shfl(b, j, kWave);
       unsigned long long ob = 0, db = 0, qb = 0;
@@ -1769,7 +1770,7 @@ k_expand_group(const ExpandArgs a, uint32_t* defer) {
   if (lane < 3) w.acc[lane] = 0;
   const uint32_t n = a.n;
   const uint32_t span = a.guess, words = span >> 5, wshift = 31u - uint32_t(__clz(int(words)));
-  const uint32_t gmax = min(kGroupRows, kChunkWords / words);
+  const uint32_t gmax = min(min(kGroupRows, max(a.group, 1u)), kChunkWords / words);
   unsigned long long blk_pos = 0, blk_end = 0;
   auto reserve = [&](uint32_t cnt) -> unsigned long long {
     if (blk_pos + cnt > blk_end) {
@@ -3243,6 +3244,9 @@ struct Tuning {
   // active row (or the first round) runs the lane-per-row pass first
   // (k_expand_light); 0: never
   uint64_t light_cand = 32;
+  // rows per wave in k_expand_group (at most 8, and as many guessed windows'
+  // bitmaps as fit the wave's LDS: 4 of 4096 ids)
+  uint32_t group_rows = 4;
   uint32_t light_cap = 32;  // candidates a lane holds in the light pass: 16, 32 or 64
 };
 
@@ -3259,6 +3263,7 @@ bool set_tuning(Tuning& t, const std::string& name, const char* v) {
   else if (name == "gather_bpc") t.gather_bpc = std::atoi(v);
   else if (name == "group_mem_share") t.group_mem_share = std::atof(v);
   else if (name == "light_cand") t.light_cand = std::strtoull(v, nullptr, 10);
+  else if (name == "group_rows") t.group_rows = uint32_t(std::strtoul(v, nullptr, 10));
   else if (name == "light_cap") t.light_cap = uint32_t(std::strtoul(v, nullptr, 10));
   else return false;
   return true;
@@ -3266,7 +3271,7 @@ bool set_tuning(Tuning& t, const std::string& name, const char* v) {
 
 const char* const kTuningNames[] = {"grab_light", "grab_heavy",     "heavy_cand", "guess_min",      "defer_ratio",
                                     "expand_bpc", "merge_bpc",      "merge_rows_bpc", "gather_bpc", "group_mem_share",
-                                    "light_cand", "light_cap"};
+                                    "light_cand", "light_cap", "group_rows"};
 
 // The tuning file beside this library (read once), then GG_TUNE_* variables.
 Tuning load_tuning() {
@@ -3826,7 +3831,7 @@ class Engine {
   // for two rows' bitmaps or more (GG_EXPAND_GROUP=0 turns it off)
   bool group_rows() const {
     if (!expand_group || !guess_bits || n <= guess_bits || (guess_bits & (guess_bits - 1))) return false;
-    return kChunkWords / (guess_bits / 32) >= 2;
+    return tune.group_rows >= 2 && kChunkWords / (guess_bits / 32) >= 2;
   }
   bool light_round() const {
     return tune.light_cand && (last_active == 0 || last_cand <= tune.light_cand * last_active);
@@ -4215,13 +4220,14 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
         S.expand_launches++;
       }
     } else if (group_rows()) {
-      // rows in groups of up to kGroupRows per wave; the rows that do not
+      // rows in groups of up to group_rows per wave; the rows that do not
       // fit their group's window then go to the wave kernel, whose launch
       // reads their count on the device (no read-back: with none it exits)
       GG_TRY(grow(mlist, uint64_t(R) * 4, err));
       ExpandArgs xa = expand_args(false);
+      xa.group = std::min({tune.group_rows, kGroupRows, kChunkWords / (guess_bits / 32)});
       const uint32_t gwant = uint32_t(std::min<uint64_t>(
-          expand_grid(), (est + uint64_t(kGroupRows) * kWavesPerBlock - 1) / (uint64_t(kGroupRows) * kWavesPerBlock)));
+          expand_grid(), (est + uint64_t(xa.group) * kWavesPerBlock - 1) / (uint64_t(xa.group) * kWavesPerBlock)));
       const uint32_t ggrid = last_active ? std::max(1u, gwant) : expand_grid();
       if (pend_nnz)
         k_expand_group<true><<<ggrid, kBlock, 0, stream>>>(xa, P<uint32_t>(mlist));
