@@ -16,8 +16,9 @@ rule_table_as_given (extra key) times the same closure with the rule table as
 given (GG_STAR_REWRITE=0).
 
 roofline: the expand stage -- the join/dedup/difference kernels (k_expand_light,
-one lane per row, in rounds with few candidates per row, then k_expand, one
-wave per row, for every other row) -- timed live with HIP events on the
+one lane per row, in rounds with few candidates per row; k_expand_group, four
+rows per wave in one guessed window, in the other rounds; k_expand, one wave
+per row, for the rows either of them leaves) -- timed live with HIP events on the
 engine's stream around each launch. Its VALU instructions and HBM bytes per
 launch come from the committed rocprofv3 counter summary of the same workload
 (profiles/expand_alias1m_counters.*, made by summarize_counters.py). The bound is the resource it uses the
@@ -111,7 +112,8 @@ def cpu_baseline(g):
 
 
 def roofline(stats):
-    """Roofline object for the expand stage (k_expand_light and k_expand). Its
+    """Roofline object for the expand stage (k_expand_light, k_expand_group and
+    k_expand). Its
     time is measured live (HIP events on the engine's stream around every
     launch); its HBM traffic and VALU work
     per launch come from the committed rocprofv3 counter summary of the same
@@ -125,7 +127,8 @@ def roofline(stats):
     fused = sum(4 * s["probe_reads"] + 4 * s["old_reads"] + 4 * s["new_edges"] for s in stats) / max(1, launches)
     survey = sum(4 * s["probe_reads"] + 8 * s["candidates"] + 4 * s["old_reads"] + 8 * s["new_edges"]
                  for s in stats) / max(1, launches)
-    out = {"kernel": "expand stage: k_expand_light + k_expand (join+dedup+difference)", "avg_launch_ms": avg_s * 1e3,
+    out = {"kernel": "expand stage: k_expand_light + k_expand_group + k_expand (join+dedup+difference)",
+           "avg_launch_ms": avg_s * 1e3,
            "launches_per_step": launches / max(1, len(stats)),
            "alg_bytes_per_launch_fused": fused, "alg_bytes_per_launch_4P8K4O8N": survey,
            "alg_gbs_fused": fused / avg_s / 1e9, "alg_gbs_4P8K4O8N": survey / avg_s / 1e9}
