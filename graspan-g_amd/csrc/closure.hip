@@ -3483,6 +3483,7 @@ class Engine {
   uint32_t expand_grid_cap = 0;  // 0: as many blocks as fit the device
   uint64_t light_rows = 0;       // rows the current round's light pass expanded
   bool grouped = false;          // the current round's rows went through k_expand_group
+  uint64_t last_deferred = 0;    // rows the last grouped round deferred to k_expand
   Tuning tune;                   // the tuned parameters of the current run
   int occ_expand = 4, occ_merge = 8, occ_merge_rows = 8, occ_gather = 7;  // blocks per CU the kernels allow
 
@@ -3977,6 +3978,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   pk_ready = false;
   guess_bits = tune.guess_min;
   last_cand = last_active = 0;
+  last_deferred = 0;
   GG_TRY(plan_rounds(rp, err));
   if (pack_any_label()) {
     GG_TRY(alloc(pk_len, uint64_t(R) * 4, err));
@@ -4235,10 +4237,15 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
         k_expand_group<false><<<ggrid, kBlock, 0, stream>>>(xa, P<uint32_t>(mlist));
       k_reset_work<<<1, kWave, 0, stream>>>(C());
       xa.defer = P<uint32_t>(mlist);
+      // grid for twice the last grouped round's deferred rows (at least 8
+      // blocks: a graph with id locality defers few or none, and a round
+      // that defers many moves the guess off for the next one)
+      const uint64_t dper = uint64_t(grab) * kWavesPerBlock;
+      const uint32_t dgrid = uint32_t(std::max<uint64_t>(8, std::min<uint64_t>(expand_grid(), (2 * last_deferred + dper - 1) / dper)));
       if (pend_nnz)
-        k_expand<0, true><<<expand_grid(), kBlock, 0, stream>>>(xa);
+        k_expand<0, true><<<dgrid, kBlock, 0, stream>>>(xa);
       else
-        k_expand<0><<<expand_grid(), kBlock, 0, stream>>>(xa);
+        k_expand<0><<<dgrid, kBlock, 0, stream>>>(xa);
       S.expand_launches += 2;
       grouped = true;
     } else {
@@ -4274,6 +4281,7 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
   if (grouped) {
     S.group_rows += inject ? h_ctr->active_base : h_ctr->active;
     S.group_deferred += h_ctr->deferred;
+    last_deferred = h_ctr->deferred;
   }
   last_cand = h_ctr->cand;
   last_active = h_ctr->active;
@@ -5701,6 +5709,7 @@ int engine_session_begin(Engine* e, const Grammar& gr, std::string& err) {
   if (const char* dr = std::getenv("GG_DEFER_RATIO")) e->defer_ratio = std::strtoull(dr, nullptr, 10);
   e->guess_bits = e->tune.guess_min;
   e->last_cand = e->last_active = 0;
+  e->last_deferred = 0;
   e->pr_arena = 0;
   e->sess_rp = RoundPlan{};
   e->sess_S = gg_stats{};
