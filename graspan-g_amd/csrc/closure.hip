@@ -74,6 +74,7 @@ constexpr uint32_t kGuessMin = 4096;     // smallest guessed window (ids)
 constexpr uint32_t kMergeBits = 16384;   // k_merge: rows spanning fewer ids are merged in an LDS bitmap
 constexpr uint32_t kShortRow = 1024;     // k_merge: unchanged rows up to this length move in the block copy
 constexpr uint32_t kLaneGather = 32;     // k_gather: rows of up to this many ids are packed by one lane each
+constexpr uint32_t kMergeSlot = 4096;    // k_merge_rows: rows spanning fewer ids are merged 4 per wave
 constexpr uint32_t kLaneMerge = 16;      // k_merge_rows: rows of up to this many elements are merged by one lane each
 constexpr int kMaxRules = kMaxRulesPerKind;
 constexpr int kScanItems = 8;
@@ -2681,8 +2682,11 @@ __global__ __launch_bounds__(kBlock, 7) void k_merge_rows(DCsr old, DCsr dl, uin
                                                        const unsigned long long* old_pk, uint32_t* out_col,
                                                        unsigned long long* out_pk, uint32_t* pk_len,
                                                        const uint32_t* mlist, const uint32_t* mcnt, bool ids) {
+  constexpr uint32_t kSlotWords = kMergeSlot / 32, kSlots = kMergeBits / kMergeSlot;
   __shared__ uint32_t s_win[kWavesPerBlock][kWave];
   __shared__ uint32_t s_bits[kWavesPerBlock][kMergeBits / 32];
+  __shared__ unsigned long long s_tab[kWavesPerBlock][kWave], s_tmask[kWavesPerBlock][kPosUnroll];
+  __shared__ uint32_t s_lw[kWavesPerBlock][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wi = threadIdx.x / kWave;
   uint32_t* win = s_win[wi];
@@ -2769,9 +2773,76 @@ __global__ __launch_bounds__(kBlock, 7) void k_merge_rows(DCsr old, DCsr dl, uin
         pk_len[rr_l] = ne;
       }
     }
+    // packed rows with delta (ids not written) whose old and delta ids span
+    // fewer than kMergeSlot ids: merged kSlots at a time, each in its own
+    // kMergeSlot-bit slot of the wave's bitmap, the set's old entries and
+    // delta ids going through two flattened streams
+    bool grp = false;
+    if (uint32_t(lane) < nb && !mine && old_pk && out_pk && !ids && rde_l > rdb_l)
+      grp = packed_label(plab, rr_l / n) && hi_l - (lo_l & ~31u) < kMergeSlot;
+    for (unsigned long long gm = __ballot(grp); gm;) {
+      // lane k < S takes the k-th row of the set
+      int src = 0;
+      uint32_t S = 0;
+      for (; S < kSlots && gm; ++S) {
+        const int p = __ffsll(gm) - 1;
+        gm &= gm - 1;
+        if (uint32_t(lane) == S) src = p;
+      }
+      const bool in = uint32_t(lane) < S;
+      const uint32_t rr_s = uint32_t(__shfl(int(rr_l), src, kWave));
+      const uint32_t pl_s = uint32_t(__shfl(int(pl_l), src, kWave));
+      const uint32_t lo_s = uint32_t(__shfl(int(lo_l), src, kWave)) & ~31u;
+      const unsigned long long rob_s = (unsigned long long)__shfl((long long)rob_l, src, kWave);
+      const unsigned long long rdb_s = (unsigned long long)__shfl((long long)rdb_l, src, kWave);
+      const unsigned long long rde_s = (unsigned long long)__shfl((long long)rde_l, src, kWave);
+      for (uint32_t i = lane; i < S * kSlotWords; i += kWave) bits[i] = 0;
+      // word w of slot k: bits[w + lw], lw = k * kSlotWords - first word
+      s_lw[wi][lane] = uint32_t(lane) * kSlotWords - (lo_s >> 5);
+      wave_sync();
+      flat_tiles<kPosUnroll>(rob_s, in ? pl_s : 0u, s_tab[wi], s_tmask[wi], lane,
+                             [&](const int* own, const unsigned long long* pos, const bool* ok) {
+                               unsigned long long e[kPosUnroll];
+#pragma unroll
+                               for (int q = 0; q < kPosUnroll; ++q) e[q] = ok[q] ? old_pk[pos[q]] : 0ull;
+#pragma unroll
+                               for (int q = 0; q < kPosUnroll; ++q)
+                                 if (ok[q])
+                                   __hip_atomic_fetch_or(&bits[uint32_t(e[q]) + s_lw[wi][own[q]]], uint32_t(e[q] >> 32),
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                             });
+      flat_tiles<kPosUnroll>(rdb_s, in ? uint32_t(rde_s - rdb_s) : 0u, s_tab[wi], s_tmask[wi], lane,
+                             [&](const int* own, const unsigned long long* pos, const bool* ok) {
+                               uint32_t x[kPosUnroll];
+#pragma unroll
+                               for (int q = 0; q < kPosUnroll; ++q) x[q] = ok[q] ? dl.col[pos[q]] : 0u;
+#pragma unroll
+                               for (int q = 0; q < kPosUnroll; ++q)
+                                 if (ok[q])
+                                   __hip_atomic_fetch_or(&bits[(x[q] >> 5) + s_lw[wi][own[q]]], 1u << (x[q] & 31),
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                             });
+      wave_sync();
+      // each row's non-empty words are its merged packed entries
+      const unsigned long long below = (1ull << lane) - 1;
+      for (uint32_t j = 0; j < S; ++j) {
+        const uint32_t rj = uint32_t(__builtin_amdgcn_readlane(int(rr_s), int(j)));
+        const uint32_t lw0 = uint32_t(__builtin_amdgcn_readlane(int(lo_s), int(j))) >> 5;
+        unsigned long long* out = out_pk + lane_u64(rob_s, j) + lane_u64(rdb_s, j);
+        uint32_t pe = 0;
+        for (uint32_t w0 = 0; w0 < kSlotWords; w0 += kWave) {
+          const uint32_t wd = bits[j * kSlotWords + w0 + uint32_t(lane)];
+          const unsigned long long nz = __ballot(wd != 0);
+          if (wd) out[pe + __popcll(nz & below)] = ((unsigned long long)wd << 32) | (lw0 + w0 + uint32_t(lane));
+          pe += uint32_t(__popcll(nz));
+        }
+        if (lane == 0) pk_len[rj] = pe;
+      }
+      wave_sync();
+    }
     // the other rows: one after another, the next one's first old entries
     // and delta ids loading while the current one merges
-    unsigned long long rest = __ballot(uint32_t(lane) < nb && !mine);
+    unsigned long long rest = __ballot(uint32_t(lane) < nb && !mine && !grp);
     int kn = rest ? __ffsll(rest) - 1 : -1;
     if (kn >= 0) prefetch(uint32_t(kn));
     for (int kk = kn; kk >= 0; kk = kn) {
@@ -3483,7 +3554,8 @@ class Engine {
   uint32_t expand_grid_cap = 0;  // 0: as many blocks as fit the device
   uint64_t light_rows = 0;       // rows the current round's light pass expanded
   bool grouped = false;          // the current round's rows went through k_expand_group
-  uint64_t last_deferred = 0;    // rows the last grouped round deferred to k_expand
+  static constexpr uint64_t kUnknown = ~0ull;
+  uint64_t last_deferred = kUnknown;  // rows the last grouped round deferred to k_expand (kUnknown: none yet)
   Tuning tune;                   // the tuned parameters of the current run
   int occ_expand = 4, occ_merge = 8, occ_merge_rows = 8, occ_gather = 7;  // blocks per CU the kernels allow
 
@@ -3978,7 +4050,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
   pk_ready = false;
   guess_bits = tune.guess_min;
   last_cand = last_active = 0;
-  last_deferred = 0;
+  last_deferred = kUnknown;
   GG_TRY(plan_rounds(rp, err));
   if (pack_any_label()) {
     GG_TRY(alloc(pk_len, uint64_t(R) * 4, err));
@@ -4239,9 +4311,13 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
       xa.defer = P<uint32_t>(mlist);
       // grid for twice the last grouped round's deferred rows (at least 8
       // blocks: a graph with id locality defers few or none, and a round
-      // that defers many moves the guess off for the next one)
+      // that defers many moves the guess off for the next one); a full grid
+      // in the run's first grouped round
       const uint64_t dper = uint64_t(grab) * kWavesPerBlock;
-      const uint32_t dgrid = uint32_t(std::max<uint64_t>(8, std::min<uint64_t>(expand_grid(), (2 * last_deferred + dper - 1) / dper)));
+      const uint32_t dgrid =
+          last_deferred == kUnknown
+              ? expand_grid()
+              : uint32_t(std::max<uint64_t>(8, std::min<uint64_t>(expand_grid(), (2 * last_deferred + dper - 1) / dper)));
       if (pend_nnz)
         k_expand<0, true><<<dgrid, kBlock, 0, stream>>>(xa);
       else
@@ -5709,7 +5785,7 @@ int engine_session_begin(Engine* e, const Grammar& gr, std::string& err) {
   if (const char* dr = std::getenv("GG_DEFER_RATIO")) e->defer_ratio = std::strtoull(dr, nullptr, 10);
   e->guess_bits = e->tune.guess_min;
   e->last_cand = e->last_active = 0;
-  e->last_deferred = 0;
+  e->last_deferred = Engine::kUnknown;
   e->pr_arena = 0;
   e->sess_rp = RoundPlan{};
   e->sess_S = gg_stats{};
