@@ -1579,10 +1579,12 @@ __device__ inline void emit_chunk(const ExpandArgs& a, uint32_t c0, uint32_t spa
 // list, which the wave kernel expands afterwards (its own windows); its
 // counts are left to that kernel.
 constexpr uint32_t kGroupRows = 8;  // at most (the tuned group_rows, and as many bitmaps as fit)
+constexpr uint32_t kGroupWords = kChunkWords;  // bitmap words per wave (seen and fresh each)
+constexpr int kGroupWaves = kExpandWaves;      // occupancy target (waves per SIMD)
 
 struct GroupLds {
-  uint32_t seen[kChunkWords];
-  uint32_t fresh[kChunkWords];
+  uint32_t seen[kGroupWords];
+  uint32_t fresh[kGroupWords];
   unsigned long long lbase[kWave];      // stream: base address per non-empty list
   uint32_t lwoff[kWave];                // stream: bitmap offset (words) per non-empty list
   unsigned long long tmask[kUnroll];    // stream: list starts inside the tile
@@ -1761,7 +1763,7 @@ __device__ inline void for_each_list_group(const ExpandArgs& a, uint32_t G, uint
 // Rows that do not fit their group's window are appended to `defer`
 // (ctr->deferred of them, also counted as misses).
 template <bool PEND>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandWaves, kExpandWaves))) void
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGroupWaves, kGroupWaves))) void
 k_expand_group(const ExpandArgs a, uint32_t* defer) {
   __shared__ GroupLds s_lds[kWavesPerBlock];
   const int lane = threadIdx.x & (kWave - 1);
@@ -1771,7 +1773,7 @@ k_expand_group(const ExpandArgs a, uint32_t* defer) {
   if (lane < 3) w.acc[lane] = 0;
   const uint32_t n = a.n;
   const uint32_t span = a.guess, words = span >> 5, wshift = 31u - uint32_t(__clz(int(words)));
-  const uint32_t gmax = min(min(kGroupRows, max(a.group, 1u)), kChunkWords / words);
+  const uint32_t gmax = min(min(kGroupRows, max(a.group, 1u)), kGroupWords / words);
   unsigned long long blk_pos = 0, blk_end = 0;
   auto reserve = [&](uint32_t cnt) -> unsigned long long {
     if (blk_pos + cnt > blk_end) {
@@ -3904,7 +3906,7 @@ class Engine {
   // for two rows' bitmaps or more (GG_EXPAND_GROUP=0 turns it off)
   bool group_rows() const {
     if (!expand_group || !guess_bits || n <= guess_bits || (guess_bits & (guess_bits - 1))) return false;
-    return tune.group_rows >= 2 && kChunkWords / (guess_bits / 32) >= 2;
+    return tune.group_rows >= 2 && kGroupWords / (guess_bits / 32) >= 2;
   }
   bool light_round() const {
     return tune.light_cand && (last_active == 0 || last_cand <= tune.light_cand * last_active);
@@ -4299,7 +4301,7 @@ int Engine::round_step(RoundPlan& rp, gg_stats& S, RunEvents& ev, uint64_t injec
       // reads their count on the device (no read-back: with none it exits)
       GG_TRY(grow(mlist, uint64_t(R) * 4, err));
       ExpandArgs xa = expand_args(false);
-      xa.group = std::min({tune.group_rows, kGroupRows, kChunkWords / (guess_bits / 32)});
+      xa.group = std::min({tune.group_rows, kGroupRows, kGroupWords / (guess_bits / 32)});
       const uint32_t gwant = uint32_t(std::min<uint64_t>(
           expand_grid(), (est + uint64_t(xa.group) * kWavesPerBlock - 1) / (uint64_t(xa.group) * kWavesPerBlock)));
       const uint32_t ggrid = last_active ? std::max(1u, gwant) : expand_grid();
