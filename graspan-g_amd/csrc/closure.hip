@@ -2138,6 +2138,80 @@ __global__ __launch_bounds__(kBlock) void k_expand_light(const ExpandArgs a, uin
   }
 }
 
+// The first round's dedup of the raw input rows, one lane per row: a row of
+// at most kDedupCap raw entries is sorted in the lane's LDS column and its
+// distinct ids are written out (old, delta and pending rows are all empty
+// before the first round). Longer rows go to `defer` (ctr->deferred of them)
+// for k_expand<1>.
+constexpr uint32_t kDedupCap = 16;
+__global__ __launch_bounds__(kBlock) void k_dedup_light(const ExpandArgs a, uint32_t* defer) {
+  __shared__ uint32_t s_buf[kDedupCap][kBlock];
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t t = threadIdx.x;
+  const uint32_t nactive = a.ctr->active;
+  unsigned long long n_sum = 0;
+  constexpr unsigned long long kStage = 1024;
+  unsigned long long blk_pos = 0, blk_end = 0;  // this wave's staging block
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nactive; base += gridDim.x * blockDim.x) {
+    const uint32_t idx = base + t;
+    const bool live = idx < nactive;
+    const uint32_t r = live ? a.active[idx] : 0u;
+    unsigned long long b = 0, e = 0;
+    if (live) {
+      b = a.raw.ptr[r];
+      e = a.raw.ptr[r + 1];
+    }
+    const bool over = live && e - b > kDedupCap;
+    uint32_t c = 0;
+    if (live && !over) {
+      const uint32_t m = uint32_t(e - b);
+      for (uint32_t i = 0; i < m; ++i) {
+        const uint32_t x = a.raw.col[b + i];
+        uint32_t j = i;
+        for (; j > 0 && s_buf[j - 1][t] > x; --j) s_buf[j][t] = s_buf[j - 1][t];
+        s_buf[j][t] = x;
+      }
+      for (uint32_t i = 0; i < m; ++i) {
+        const uint32_t x = s_buf[i][t];
+        if (c == 0 || s_buf[c - 1][t] != x) s_buf[c++][t] = x;
+      }
+    }
+    const uint32_t incl = wave_incl_scan(c, lane);
+    const uint32_t tot = lane63_u32(incl);
+    unsigned long long wb = 0;
+    if (tot) {
+      if (blk_pos + tot > blk_end) {
+        const unsigned long long take = max<unsigned long long>(tot, kStage);
+        unsigned long long b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&a.ctr->cursor, take);
+        b0 = uni_u64(b0);
+        blk_pos = b0;
+        blk_end = b0 + take;
+      }
+      wb = blk_pos;
+      blk_pos += tot;
+    }
+    if (tot && wb + tot > a.staging_cap) {
+      if (lane == 0) atomicExch(&a.ctr->overflow, 1u);
+    } else if (live && !over && c) {
+      const unsigned long long pos = wb + (incl - c);
+      for (uint32_t i = 0; i < c; ++i) a.staging[pos + i] = s_buf[i][t];
+      a.seg_off[r] = pos;
+      a.seg_cnt[r] = c;
+      n_sum += c;
+    }
+    const unsigned long long dm = __ballot(over);
+    if (dm) {
+      uint32_t dpos = 0;
+      if (lane == 0) dpos = atomicAdd(&a.ctr->deferred, uint32_t(__popcll(dm)));
+      dpos = uni_u32(dpos);
+      if (over) defer[dpos + uint32_t(__popcll(dm & ((1ull << lane) - 1)))] = r;
+    }
+  }
+  n_sum = wave_sum_u64(n_sum);
+  if (lane == 0 && n_sum) atomicAdd(&ctr_slot(a.ctr).newe, n_sum);
+}
+
 // PEND: the round reads pending edges beside old (a separate instance, so
 // rounds without them carry none of that code)
 template <int MODE, bool PEND = false>
@@ -2152,7 +2226,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kExpandW
   // the first round's dedup (MODE 1) takes the active rows from active_base:
   // graphs joining a running session append their rows after the join's
   const uint32_t abase = MODE == 1 ? a.ctr->active_base : 0u;
-  const bool from_defer = MODE == 0 && a.defer != nullptr;
+  const bool from_defer = (MODE == 0 || MODE == 1) && a.defer != nullptr;
   const uint32_t* rows_in = from_defer ? a.defer : a.active + abase;
   const uint32_t nactive = from_defer ? a.ctr->deferred : a.ctr->active - abase;
   if (nactive == 0) return;  // (launched before the host knew the count)
@@ -4012,7 +4086,13 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       fill_u32(seg_cnt, 0, R);
       k_activate_raw<<<grid_for(R), kBlock, 0, stream>>>(P<unsigned long long>(raw_ptr), R, P<uint32_t>(active), C());
       GG_HIP(rec(x.a));
-      k_expand<1><<<expand_grid(), kBlock, 0, stream>>>(expand_args(true));
+      // rows of a few raw entries one per lane; the others (read on the
+      // device: no read-back) one per wave
+      GG_TRY(grow(mlist, uint64_t(R) * 4, err));
+      ExpandArgs ra = expand_args(true);
+      k_dedup_light<<<grid_for(R), kBlock, 0, stream>>>(ra, P<uint32_t>(mlist));
+      ra.defer = P<uint32_t>(mlist);
+      k_expand<1><<<expand_grid(), kBlock, 0, stream>>>(ra);
       GG_HIP(rec(x.b));
       GG_HIP(hipGetLastError());
       GG_TRY(read_ctr(err));
