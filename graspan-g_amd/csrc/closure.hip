@@ -48,6 +48,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -4886,39 +4887,119 @@ int Engine::run_r(int filter, const std::vector<std::pair<uint32_t, uint32_t>>& 
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   };
   // initial delta of every partition: its input edges, deduplicated, built
-  // on the host in one pass (global CSR by row, then per-partition slices)
+  // on the host by up to 16 threads. The edges are bucketed by partition (a
+  // count pass and a scatter pass over the same chunks of the edge list),
+  // then each partition is sorted into its rows on its own (counting sort by
+  // row, then sort and unique within each row); the pinned copies follow one
+  // partition after another. Edges whose source is in no partition are left
+  // out, as the rows of the partitions are all there is.
   {
-    const uint64_t R = rows();
-    std::vector<uint64_t> cnt(R + 1, 0);
-    for (uint64_t i = 0; i < m; ++i) cnt[uint64_t(h_lab[i] - 1) * n + h_src[i] + 1]++;
-    for (uint64_t r = 0; r < R; ++r) cnt[r + 1] += cnt[r];
-    std::vector<uint32_t> col(m);
-    {
-      std::vector<uint64_t> fillp(cnt.begin(), cnt.end() - 1);
-      for (uint64_t i = 0; i < m; ++i) col[fillp[uint64_t(h_lab[i] - 1) * n + h_src[i]]++] = h_dst[i];
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::atomic<bool> oom{false};
+    auto par = [&](uint64_t count, auto&& f) {  // f(thread, begin, end) on nt chunks of [0, count)
+      std::vector<std::thread> th;
+      for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t a = count * t / nt, b = count * (t + 1) / nt;
+        if (a < b)
+          th.emplace_back([&f, &oom, t, a, b]() {
+            try {
+              f(t, a, b);
+            } catch (const std::bad_alloc&) {
+              oom = true;
+            }
+          });
+      }
+      for (auto& x : th) x.join();
+    };
+    std::vector<uint32_t> pfirst(static_cast<size_t>(NP));
+    for (int p = 0; p < NP; ++p) pfirst[size_t(p)] = pt[p].first;
+    // partition of vertex v (intervals sorted by first vertex), -1: none
+    auto part_of = [&](uint32_t v) -> int {
+      const int p = int(std::upper_bound(pfirst.begin(), pfirst.end(), v) - pfirst.begin()) - 1;
+      return p >= 0 && v - pt[p].first < pt[p].np ? p : -1;
+    };
+    std::vector<uint64_t> pcnt(size_t(nt) * NP, 0);
+    par(m, [&](unsigned t, uint64_t a, uint64_t b) {
+      uint64_t* c = pcnt.data() + size_t(t) * NP;
+      for (uint64_t i = a; i < b; ++i) {
+        const int p = part_of(h_src[i]);
+        if (p >= 0) c[p]++;
+      }
+    });
+    // bucket offsets: partition-major, then thread
+    std::vector<uint64_t> pbeg(size_t(NP) + 1, 0);
+    uint64_t tot = 0;
+    for (int p = 0; p < NP; ++p) {
+      pbeg[size_t(p)] = tot;
+      for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t c = pcnt[size_t(t) * NP + p];
+        pcnt[size_t(t) * NP + p] = tot;
+        tot += c;
+      }
     }
+    pbeg[size_t(NP)] = tot;
+    std::vector<uint32_t> bsrc(tot), bdst(tot);
+    std::vector<uint8_t> blab(tot);
+    par(m, [&](unsigned t, uint64_t a, uint64_t b) {
+      uint64_t* o = pcnt.data() + size_t(t) * NP;
+      for (uint64_t i = a; i < b; ++i) {
+        const int p = part_of(h_src[i]);
+        if (p < 0) continue;
+        const uint64_t k = o[p]++;
+        bsrc[k] = h_src[i];
+        bdst[k] = h_dst[i];
+        blab[k] = h_lab[i];
+      }
+    });
+    std::vector<std::vector<unsigned long long>> lptrs(static_cast<size_t>(NP));
+    std::vector<std::vector<uint32_t>> lcols(static_cast<size_t>(NP));
+    std::atomic<int> next_p{0};
+    par(nt, [&](unsigned, uint64_t, uint64_t) {
+      for (int p; (p = next_p.fetch_add(1)) < NP && !oom;) {
+        const Part& q = pt[p];
+        const uint64_t rows = rows_of(q);
+        const uint64_t b0 = pbeg[size_t(p)], b1 = pbeg[size_t(p) + 1];
+        std::vector<uint64_t> cnt(rows + 1, 0);
+        for (uint64_t k = b0; k < b1; ++k) cnt[uint64_t(blab[k] - 1) * q.np + (bsrc[k] - q.first) + 1]++;
+        for (uint64_t r = 0; r < rows; ++r) cnt[r + 1] += cnt[r];
+        std::vector<uint32_t> col(b1 - b0);
+        {
+          std::vector<uint64_t> fillp(cnt.begin(), cnt.end() - 1);
+          for (uint64_t k = b0; k < b1; ++k) col[fillp[uint64_t(blab[k] - 1) * q.np + (bsrc[k] - q.first)]++] = bdst[k];
+        }
+        std::vector<unsigned long long>& lptr = lptrs[size_t(p)];
+        std::vector<uint32_t>& lcol = lcols[size_t(p)];
+        lptr.assign(rows + 1, 0);
+        lcol.reserve(col.size());
+        for (uint64_t r = 0; r < rows; ++r) {
+          uint32_t* b = col.data() + cnt[r];
+          uint32_t* e = col.data() + cnt[r + 1];
+          std::sort(b, e);
+          e = std::unique(b, e);
+          lcol.insert(lcol.end(), b, e);
+          lptr[r + 1] = lcol.size();
+        }
+      }
+    });
+    if (oom) {
+      err = "out of host memory building the partitions' input rows";
+      return GG_E_NOMEM;
+    }
+    std::vector<uint32_t>().swap(bsrc);
+    std::vector<uint32_t>().swap(bdst);
+    std::vector<uint8_t>().swap(blab);
     for (int p = 0; p < NP; ++p) {
       Part& q = pt[p];
       const uint64_t rows = rows_of(q);
       Region& d = q.r[kD];
-      std::vector<unsigned long long> lptr(rows + 1, 0);
-      std::vector<uint32_t> lcol;
-      for (int l = 0; l < L; ++l)
-        for (uint32_t k = 0; k < q.np; ++k) {
-          const uint64_t gr = uint64_t(l) * n + q.first + k;
-          uint32_t* b = col.data() + cnt[gr];
-          uint32_t* e = col.data() + cnt[gr + 1];
-          std::sort(b, e);
-          e = std::unique(b, e);
-          lcol.insert(lcol.end(), b, e);
-          lptr[uint64_t(l) * q.np + k + 1] = lcol.size();
-        }
-      d.nnz = lcol.size();
+      d.nnz = lcols[size_t(p)].size();
       S.input_unique += d.nnz;
       GG_TRY(host_alloc(d.hptr, (rows + 1) * 8));
       GG_TRY(host_alloc(d.hcol, d.nnz * 4));
-      std::memcpy(d.hptr.p, lptr.data(), (rows + 1) * 8);
-      if (d.nnz) std::memcpy(d.hcol.p, lcol.data(), d.nnz * 4);
+      std::memcpy(d.hptr.p, lptrs[size_t(p)].data(), (rows + 1) * 8);
+      if (d.nnz) std::memcpy(d.hcol.p, lcols[size_t(p)].data(), d.nnz * 4);
+      std::vector<unsigned long long>().swap(lptrs[size_t(p)]);
+      std::vector<uint32_t>().swap(lcols[size_t(p)]);
       for (int k : {int(kO), int(kT)}) {
         Region& e2 = q.r[k];
         GG_TRY(host_alloc(e2.hptr, (rows + 1) * 8));
