@@ -15,7 +15,8 @@ summed over the timed steps. The engine evaluates the least-fixpoint rule plan
 rule_table_as_given (extra key) times the same closure with the rule table as
 given (GG_STAR_REWRITE=0).
 
-roofline: the expand stage -- the join/dedup/difference kernels (k_expand_light,
+roofline: the expand stage -- the join/dedup/difference kernels (k_dedup_light,
+the first round's dedup of the input, one lane per row; k_expand_light,
 one lane per row, in rounds with few candidates per row; k_expand_group, four
 rows per wave in one guessed window, in the other rounds; k_expand, one wave
 per row, for the rows either of them leaves) -- timed live with HIP events on the
@@ -127,7 +128,7 @@ def roofline(stats):
     fused = sum(4 * s["probe_reads"] + 4 * s["old_reads"] + 4 * s["new_edges"] for s in stats) / max(1, launches)
     survey = sum(4 * s["probe_reads"] + 8 * s["candidates"] + 4 * s["old_reads"] + 8 * s["new_edges"]
                  for s in stats) / max(1, launches)
-    out = {"kernel": "expand stage: k_expand_light + k_expand_group + k_expand (join+dedup+difference)",
+    out = {"kernel": "expand stage: k_dedup_light + k_expand_light + k_expand_group + k_expand (join+dedup+difference)",
            "avg_launch_ms": avg_s * 1e3,
            "launches_per_step": launches / max(1, len(stats)),
            "alg_bytes_per_launch_fused": fused, "alg_bytes_per_launch_4P8K4O8N": survey,

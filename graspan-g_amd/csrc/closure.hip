@@ -4097,7 +4097,7 @@ int Engine::run(int filter, gg_stats* st, std::string& err) {
       GG_HIP(rec(x.b));
       GG_HIP(hipGetLastError());
       GG_TRY(read_ctr(err));
-      S.expand_launches++;
+      S.expand_launches += 2;
       if (!h_ctr->overflow) break;
       S.retries++;
       GG_TRY(grow(staging, size_t(h_ctr->cursor + h_ctr->cursor / 4 + 1024) * 4, err));

@@ -7,8 +7,8 @@ DIR holds the output of the four counter passes and the kernel trace listed
 in profiles/expand_alias1m_counters.md (subfolders p_rd, p_wr, p_sq1, p_sq2
 with <name>_counter_collection.csv, and p_trace with trace_kernel_stats.csv).
 Prints a markdown table per kernel and, with --json, writes the figures that
-bench.py reads for its roofline (the expand stage: every k_expand_light,
-k_expand_group and k_expand launch together).
+bench.py reads for its roofline (the expand stage: every k_dedup_light,
+k_expand_light, k_expand_group and k_expand launch together).
 
 Conversions (MI355X, gfx950):
   HBM read bytes  = 32*TCC_EA0_RDREQ_32B + 64*TCC_EA0_RDREQ_64B + 128*TCC_EA0_RDREQ_128B
@@ -94,8 +94,8 @@ def summarize(d):
                 "lds_bank_conflict_cycles_per_cu_frac": c["SQ_LDS_BANK_CONFLICT"] / CUS / max(1.0, cyc),
                 "mean_waves_per_simd": c["SQ_WAVE_CYCLES"] * 4 / simd_cycles}
 
-    stage = sorted(k for k in trace if k.startswith("k_expand"))
-    res = {"expand stage (k_expand_light + k_expand_group + k_expand, all instances)": group(stage)}
+    stage = sorted(k for k in trace if k.startswith("k_expand") or k.startswith("k_dedup_light"))
+    res = {"expand stage (k_dedup_light + k_expand_light + k_expand_group + k_expand, all instances)": group(stage)}
     for k in stage + ["k_merge_rows", "k_pairs", "k_gather", "k_merge", "k_activate", "k_unpack"]:
         if k in trace:
             res[k] = group([k])
