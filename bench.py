@@ -193,7 +193,7 @@ def main():
                     help="ooc mode: also close the graph in core (no budget) and compare checksums")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU (one process, one device); --gpus must be 1"}))
+        print(json.dumps({"error": "the closure engine runs on one GPU (one process, one device); --gpus must be 1"}))
         sys.exit(2)
 
     import graspan_g as gg
