@@ -76,7 +76,7 @@ constexpr uint32_t kMergeBits = 16384;   // k_merge: rows spanning fewer ids are
 constexpr uint32_t kShortRow = 1024;     // k_merge: unchanged rows up to this length move in the block copy
 constexpr uint32_t kLaneGather = 32;     // k_gather: rows of up to this many ids are packed by one lane each
 constexpr uint32_t kMergeSlot = 2048;    // k_merge_rows: rows spanning fewer ids are merged 8 per wave
-constexpr uint32_t kLaneMerge = 16;      // k_merge_rows: rows of up to this many elements are merged by one lane each
+constexpr uint32_t kLaneMerge = 8;       // k_merge_rows: rows of up to this many elements are merged by one lane each
 constexpr int kMaxRules = kMaxRulesPerKind;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
