@@ -48,7 +48,7 @@ CANDIDATES = {
 }
 # the setting the previous tuning run chose (tuned_gfx950.cfg)
 START = {"grab_light": 4, "grab_heavy": 1, "heavy_cand": 256, "guess_min": 4096, "defer_ratio": 512, "expand_bpc": 7,
-         "merge_rows_bpc": 8, "gather_bpc": 6, "merge_bpc": 8, "group_mem_share": 0.4, "light_cand": 32,
+         "merge_rows_bpc": 8, "gather_bpc": 6, "merge_bpc": 8, "group_mem_share": 0.4, "light_cand": 16,
          "light_cap": 16, "group_rows": 4}
 # parameters that only the batched workload reads
 BATCH_ONLY = {"group_mem_share"}
