@@ -113,8 +113,8 @@ def cpu_baseline(g):
 
 
 def roofline(stats):
-    """Roofline object for the expand stage (k_expand_light, k_expand_group and
-    k_expand). Its
+    """Roofline object for the expand stage (k_dedup_light, k_expand_light,
+    k_expand_group and k_expand). Its
     time is measured live (HIP events on the engine's stream around every
     launch); its HBM traffic and VALU work
     per launch come from the committed rocprofv3 counter summary of the same
@@ -178,6 +178,8 @@ def main():
                          "device budget; dataflow: the 10,000-vertex dataflow closure")
     ap.add_argument("--batch-stride", type=int, default=1,
                     help="batch leg: close every k-th graph of the 4096-graph set (1 = all)")
+    ap.add_argument("--batch-no-solo", action="store_true",
+                    help="batch leg: skip the one-graph-at-a-time comparison (to profile the batched run alone)")
     ap.add_argument("--batch-max-vertices", type=int, default=0,
                     help="batch leg: vertices per launch group (0: planned from device memory by the library)")
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -331,6 +333,13 @@ def bench_batch(args):
     batched = [b.result(i) for i in range(len(graphs))]
     b.close()
     ctx.release()
+    if args.batch_no_solo:
+        return {"workload": f"batched closure of {len(graphs)} graphs of the 4096-graph set "
+                            f"(every {args.batch_stride}th), {nv} vertices, no one-at-a-time comparison",
+                "graphs": len(graphs), "vertices": int(nv), "launch_groups": groups,
+                "closure_edges": int(st.total_edges), "ms": dt_b * 1e3, "graphs_per_s": len(graphs) / dt_b,
+                "closure_edges_per_s": (st.total_edges - st.input_unique) / dt_b,
+                "device_bytes_peak": int(st.device_bytes_peak)}
     one = gg.Context()
     t0 = time.perf_counter()
     single = []
